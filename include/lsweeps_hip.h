@@ -1,0 +1,140 @@
+/*
+ * C-ABI of the MI355X L-sweeps preconditioned Helmholtz solve path.
+ *
+ * Plain pointers and sizes only.  Complex vectors are interleaved
+ * (re, im) doubles.  Host-pointer entry points (suffix _z or none) are
+ * synchronous; _dev entry points take device pointers and a hipStream_t
+ * passed as void* (NULL = the object's own stream) and are asynchronous.
+ * Every function returns 0 on success or a nonzero status (LSW_ERR_*); the
+ * message of the last failure on the calling thread is lsw_last_error().
+ * All pointers are borrowed for the duration of the call.  Calls on one
+ * object must not overlap.
+ *
+ * Global grid vectors: Gx*Gy entries, index gi*Gy + gj (x-major, y fastest),
+ * DOF (gi, gj) at (x0 + (gi+1) hx, y0 + (gj+1) hy) of the PML-extended
+ * domain (the numbering of AssemblerHelmholtz2DFD::getDOFPts,
+ * src/AssemblerHelmholtz2DFD.cpp:147-181, for a 1x1 decomposition).
+ */
+#ifndef LSWEEPS_HIP_H
+#define LSWEEPS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSW_OK 0
+#define LSW_ERR_ARG 1
+#define LSW_ERR_HIP 2
+#define LSW_ERR_STATE 3
+#define LSW_ERR_NUMERIC 4
+#define LSW_ERR_CAPACITY 5
+#define LSW_ERR_UNSUPPORTED 6
+#define LSW_ERR_INTERNAL 7
+
+#define LSW_PREC_FP64 0
+#define LSW_PREC_FP32 1
+
+typedef struct lsw_lu lsw_lu;
+typedef struct lsw_dd lsw_dd;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* lsw_last_error(void);
+/* Library version string. */
+const char* lsw_version(void);
+/* Number of visible HIP devices; selects nothing. */
+int lsw_device_count(int* count);
+
+/* ---- LocalLinearSolver (include/LocalLinearSolver.h:27-40) -------------
+ * Replaces LocalLinearSolverEigenSparseLU (src/LocalLinearSolverEigenSparseLU.cpp:33-76):
+ * setMatrix (:33-38), factorize (:40-62), solve (:65-76). */
+int lsw_lu_create(int precision, lsw_lu** out);
+/* CSC, int32 indices, n x n with nnz entries (Eigen column-major layout,
+ * include/typedef.h:64-73).  The matrix must be the 5-point operator of an
+ * nx x ny grid (index i*ny+j); the arrays are copied. */
+int lsw_lu_set_csc_z(lsw_lu* lu, int n, int nnz, const int* colptr, const int* rowind, const double* vals);
+int lsw_lu_factorize(lsw_lu* lu);
+/* b, x: n x nrhs column-major, interleaved complex (host). */
+int lsw_lu_solve_z(lsw_lu* lu, int nrhs, const double* b, double* x);
+/* nx, ny of the recovered grid, complex factor entries, tree nodes, height. */
+int lsw_lu_info(lsw_lu* lu, int* nx, int* ny, long long* factor_entries, int* nodes, int* height);
+int lsw_lu_destroy(lsw_lu* lu);
+
+/* ---- DD / DDCheckerboard (include/DD.h:40-113, src/DDCheckerboard.cpp:87-645) ----
+ * Parameters of Input::readCheckerboardDD / readAssemblerHelmholtz2DFD
+ * (src/Input.cpp:470-623, 795-857). */
+typedef struct lsw_dd_params {
+  double domain[4];              /* ax, bx, ay, by */
+  int ncells_x, ncells_y;        /* NCellsX (rows, y), NCellsY (columns, x) */
+  int nx, ny;                    /* NX, NY: elements inside one cell */
+  double omega;                  /* Omega */
+  double min_wavelengths_in_pml; /* MinWavelengthsInPML */
+  int order;                     /* Order (2) */
+  int precision;                 /* LSW_PREC_FP64 or LSW_PREC_FP32 local factors/sweeps */
+  int medium_kind;               /* 0: "Constant" (m = 1), 1: "CSV" velocities */
+  const float* velocity;         /* medium_kind 1: vel_rows x vel_cols, row 0 = top */
+  int vel_rows, vel_cols;
+  double vel_domain[4];          /* file domain ax, bx, ay, by */
+} lsw_dd_params;
+
+int lsw_dd_create(const lsw_dd_params* p, lsw_dd** out);
+/* Geometry only (no device memory): sizes, coordinates and cell_info work;
+ * every compute entry point returns LSW_ERR_STATE. */
+int lsw_dd_create_host(const lsw_dd_params* p, lsw_dd** out);
+int lsw_dd_destroy(lsw_dd* dd);
+/* DDCheckerboard::factorize (src/DDCheckerboard.cpp:648-665). */
+int lsw_dd_factorize(lsw_dd* dd);
+/* Global grid size, number of cells, DD mesh size (getMeshSize, :3536-3561). */
+int lsw_dd_sizes(lsw_dd* dd, int* Gx, int* Gy, int* ncells, double* mesh_size);
+/* Global DOF coordinates (xs: Gx entries, ys: Gy entries). */
+int lsw_dd_global_coords(lsw_dd* dd, double* xs, double* ys);
+/* Per-cell geometry: info[0..5] = I, J, nx, ny, ox, oy; info[6..21] = LR
+ * 7-set starts [unshifted 8, shifted 8]; info[22..37] = UD starts. */
+int lsw_dd_cell_info(lsw_dd* dd, int cell, int* info);
+/* Sum of point sources (RHSSrcPts, src/RHSSrcPts.cpp:87-112), global grid. */
+int lsw_dd_assemble_rhs(lsw_dd* dd, int nsrc, const double* src_xy, double* b);
+int lsw_dd_assemble_rhs_dev(lsw_dd* dd, int nsrc, const double* src_xy, void* b, void* stream);
+/* y = M^{-1} x: DDCheckerboard::applyPrecond (src/DDCheckerboard.cpp:687-739). */
+int lsw_dd_apply_precond(lsw_dd* dd, const double* x, double* y);
+int lsw_dd_apply_precond_dev(lsw_dd* dd, const void* x, void* y, void* stream);
+/* y = A x: DDCheckerboard::applySystemMatrix (src/DDCheckerboard.cpp:2611-2635). */
+int lsw_dd_apply_a(lsw_dd* dd, const double* x, double* y);
+int lsw_dd_apply_a_dev(lsw_dd* dd, const void* x, void* y, void* stream);
+/* Local solves per applyPrecond, algorithmic bytes of those solves, factor bytes. */
+int lsw_dd_stats(lsw_dd* dd, long long* solves_per_precond, double* solve_bytes_per_precond,
+                 long long* factor_bytes);
+/* Wait for all work queued on the DD's own stream. */
+int lsw_dd_sync(lsw_dd* dd);
+
+/* ---- GlobalIterativeSolver (include/GlobalIterativeSolver.h:36-48) -------
+ * mode 0: the reference GMRES (src/GlobalIterativeSolverGMRES.cpp:65-348),
+ * tol on its logged estimate, max_iter a hard cap;
+ * mode 1: flexible right-preconditioned GMRES(restart), tol on the true
+ * relative residual. */
+typedef struct lsw_gmres_params {
+  int mode;
+  double tol;
+  int max_iter;
+  int restart;
+  int verbose;
+} lsw_gmres_params;
+
+typedef struct lsw_gmres_result {
+  int iterations;
+  int precond_applies;
+  int matvecs;
+  double estimate;      /* last logged residual estimate */
+  double true_residual; /* ||b - A x|| / ||b|| on the global operator, FP64 */
+} lsw_gmres_result;
+
+/* b, x: global grid vectors on the host; log (optional) receives up to
+ * log_cap logged estimates. */
+int lsw_gmres_solve(lsw_dd* dd, const lsw_gmres_params* p, const double* b, double* x,
+                    lsw_gmres_result* res, double* log, int log_cap);
+int lsw_gmres_solve_dev(lsw_dd* dd, const lsw_gmres_params* p, const void* b, void* x, void* stream,
+                        lsw_gmres_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSWEEPS_HIP_H */

@@ -1,0 +1,376 @@
+// extern "C" boundary of the library (include/lsweeps_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/lsweeps_hip.h"
+#include "lsw_dd.h"
+#include "lsw_gmres.h"
+#include "lsw_model.h"
+#include "lsw_solver.h"
+
+using namespace lsw;
+
+struct lsw_lu {
+  std::unique_ptr<LocalLinearSolverHIP> solver;
+  ComplexSparseMatrix A;
+};
+
+struct lsw_dd {
+  std::unique_ptr<DDCheckerboard> dd;
+  GlobalIterativeSolverGMRESHIP gmres;
+};
+
+namespace {
+thread_local std::string gLastError;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    gLastError.clear();
+    return LSW_OK;
+  } catch (const Error& e) {
+    gLastError = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    gLastError = "out of host memory";
+    return LSW_ERR_CAPACITY;
+  } catch (const std::exception& e) {
+    gLastError = e.what();
+    return LSW_ERR_INTERNAL;
+  } catch (...) {
+    gLastError = "unknown error";
+    return LSW_ERR_INTERNAL;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* lsw_last_error(void) { return gLastError.c_str(); }
+const char* lsw_version(void) { return "lsweeps-mi355x 0.1"; }
+
+int lsw_device_count(int* count) {
+  return guarded([&] {
+    LSW_REQUIRE(count != nullptr, LSW_ERR_ARG, "null pointer");
+    int n = 0;
+    LSW_HIP(hipGetDeviceCount(&n));
+    *count = n;
+  });
+}
+
+// ------------------------------------------------------------------ LU
+int lsw_lu_create(int precision, lsw_lu** out) {
+  return guarded([&] {
+    LSW_REQUIRE(out != nullptr, LSW_ERR_ARG, "null pointer");
+    LSW_REQUIRE(precision == LSW_PREC_FP64 || precision == LSW_PREC_FP32, LSW_ERR_ARG, "unknown precision");
+    auto* h = new lsw_lu();
+    h->solver = std::make_unique<LocalLinearSolverHIP>(precision);
+    *out = h;
+  });
+}
+
+int lsw_lu_set_csc_z(lsw_lu* lu, int n, int nnz, const int* colptr, const int* rowind, const double* vals) {
+  return guarded([&] {
+    LSW_REQUIRE(lu && colptr && rowind && vals, LSW_ERR_ARG, "null pointer");
+    LSW_REQUIRE(n > 0 && nnz >= n, LSW_ERR_ARG, "bad matrix size");
+    LSW_REQUIRE(colptr[0] == 0 && colptr[n] == nnz, LSW_ERR_ARG, "bad column pointer");
+    lu->A.rows = lu->A.cols = n;
+    lu->A.colptr.assign(colptr, colptr + n + 1);
+    lu->A.rowind.assign(rowind, rowind + nnz);
+    lu->A.vals.resize(nnz);
+    for (int k = 0; k < nnz; ++k) lu->A.vals[k] = Complex(vals[2 * k], vals[2 * k + 1]);
+    lu->solver->setMatrix(&lu->A);
+  });
+}
+
+int lsw_lu_factorize(lsw_lu* lu) {
+  return guarded([&] {
+    LSW_REQUIRE(lu, LSW_ERR_ARG, "null pointer");
+    lu->solver->factorize();
+  });
+}
+
+int lsw_lu_solve_z(lsw_lu* lu, int nrhs, const double* b, double* x) {
+  return guarded([&] {
+    LSW_REQUIRE(lu && b && x, LSW_ERR_ARG, "null pointer");
+    lu->solver->solveMany(reinterpret_cast<const Complex*>(b), reinterpret_cast<Complex*>(x), nrhs);
+  });
+}
+
+int lsw_lu_info(lsw_lu* lu, int* nx, int* ny, long long* factor_entries, int* nodes, int* height) {
+  return guarded([&] {
+    LSW_REQUIRE(lu, LSW_ERR_ARG, "null pointer");
+    if (nx) *nx = lu->solver->gridNx();
+    if (ny) *ny = lu->solver->gridNy();
+    const FactorSet* fs = lu->solver->factorSet();
+    if (factor_entries) *factor_entries = fs ? fs->factorEntries() : 0;
+    if (nodes) *nodes = fs ? (int)fs->tree(0).nodes.size() : 0;
+    if (height) *height = fs ? fs->maxHeight() : 0;
+  });
+}
+
+int lsw_lu_destroy(lsw_lu* lu) {
+  return guarded([&] { delete lu; });
+}
+
+// ------------------------------------------------------------------ DD
+static int ddCreate(const lsw_dd_params* p, lsw_dd** out, bool device) {
+  return guarded([&] {
+    LSW_REQUIRE(p && out, LSW_ERR_ARG, "null pointer");
+    LSW_REQUIRE(p->precision == LSW_PREC_FP64 || p->precision == LSW_PREC_FP32, LSW_ERR_ARG, "unknown precision");
+    LSW_REQUIRE(p->omega > 0 && p->min_wavelengths_in_pml >= 0, LSW_ERR_ARG, "bad Omega / MinWavelengthsInPML");
+    LSW_REQUIRE(p->order == 2, LSW_ERR_UNSUPPORTED, "only Order 2 is supported");
+    std::vector<DoubleVector> dom = {{p->domain[0], p->domain[1]}, {p->domain[2], p->domain[3]}};
+    LSW_REQUIRE(dom[0][1] > dom[0][0] && dom[1][1] > dom[1][0], LSW_ERR_ARG, "empty domain");
+    std::unique_ptr<Function> m;
+    if (p->medium_kind == 0) {
+      m.reset(new FunctionConstant(1.0));
+    } else if (p->medium_kind == 1) {
+      LSW_REQUIRE(p->velocity && p->vel_rows > 0 && p->vel_cols > 0, LSW_ERR_ARG, "missing velocity grid");
+      auto* f = new FunctionCSV();
+      f->setData(p->velocity, p->vel_rows, p->vel_cols,
+                 {{p->vel_domain[0], p->vel_domain[1]}, {p->vel_domain[2], p->vel_domain[3]}});
+      m.reset(f);
+    } else {
+      throw Error(LSW_ERR_ARG, "unknown medium kind");
+    }
+    // Input::readAssemblerHelmholtz2DFD (src/Input.cpp:795-857)
+    PMLCubic2D pml;
+    pml.setAbsorptionCoeff(std::log(p->omega));
+    const double wl1 = (dom[0][1] - dom[0][0]) * 2 * kPi / p->omega;
+    const double wl2 = (dom[1][1] - dom[1][0]) * 2 * kPi / p->omega;
+    double wl = std::min(wl1, wl2);
+    wl /= std::sqrt(m->getMinReal());
+    pml.setWidth(p->min_wavelengths_in_pml * wl);
+    AssemblerHelmholtz2DFD as;
+    as.setPML(&pml);
+    as.setOmega(p->omega);
+    m->setDomain(dom);
+    as.setM(m.get());
+    as.setOrder((UInt)p->order);
+    LocalLinearSolverHIP proto(p->precision);
+    auto h = std::make_unique<lsw_dd>();
+    h->dd = std::make_unique<DDCheckerboard>();
+    if (device)
+      h->dd->setUp(&as, &proto, dom, {(UInt)p->ncells_x, (UInt)p->ncells_y}, {(UInt)p->nx, (UInt)p->ny});
+    else
+      h->dd->setUpGeometry(&as, &proto, dom, {(UInt)p->ncells_x, (UInt)p->ncells_y}, {(UInt)p->nx, (UInt)p->ny});
+    h->gmres.setParent(h->dd.get());
+    *out = h.release();
+  });
+}
+
+int lsw_dd_create(const lsw_dd_params* p, lsw_dd** out) { return ddCreate(p, out, true); }
+int lsw_dd_create_host(const lsw_dd_params* p, lsw_dd** out) { return ddCreate(p, out, false); }
+
+int lsw_dd_destroy(lsw_dd* dd) {
+  return guarded([&] { delete dd; });
+}
+
+int lsw_dd_factorize(lsw_dd* dd) {
+  return guarded([&] {
+    LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");
+    LSW_REQUIRE(dd->dd->deviceReady(), LSW_ERR_STATE, "DD was created without device state");
+    dd->dd->factorize();
+  });
+}
+
+int lsw_dd_sizes(lsw_dd* dd, int* Gx, int* Gy, int* ncells, double* mesh_size) {
+  return guarded([&] {
+    LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");
+    if (Gx) *Gx = dd->dd->Gx();
+    if (Gy) *Gy = dd->dd->Gy();
+    if (ncells) *ncells = (int)dd->dd->getNCells();
+    if (mesh_size) *mesh_size = dd->dd->getMeshSize();
+  });
+}
+
+int lsw_dd_global_coords(lsw_dd* dd, double* xs, double* ys) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && xs && ys, LSW_ERR_ARG, "null pointer");
+    for (int i = 0; i < dd->dd->Gx(); ++i) xs[i] = dd->dd->gx(i);
+    for (int j = 0; j < dd->dd->Gy(); ++j) ys[j] = dd->dd->gy(j);
+  });
+}
+
+int lsw_dd_cell_info(lsw_dd* dd, int cell, int* info) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && info, LSW_ERR_ARG, "null pointer");
+    LSW_REQUIRE(cell >= 0 && cell < (int)dd->dd->getNCells(), LSW_ERR_ARG, "cell index out of range");
+    const CellGeom& c = dd->dd->cellGeom(cell);
+    info[0] = c.I;
+    info[1] = c.J;
+    info[2] = c.nx;
+    info[3] = c.ny;
+    info[4] = c.ox;
+    info[5] = c.oy;
+    for (int f = 0; f < 2; ++f)
+      for (int k = 0; k < 8; ++k) {
+        info[6 + f * 8 + k] = c.vlr[f][k];
+        info[22 + f * 8 + k] = c.vud[f][k];
+      }
+  });
+}
+
+int lsw_dd_assemble_rhs(lsw_dd* dd, int nsrc, const double* src_xy, double* b) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && b && (nsrc == 0 || src_xy), LSW_ERR_ARG, "null pointer");
+    std::vector<DoubleVector> s(nsrc);
+    for (int k = 0; k < nsrc; ++k) s[k] = {src_xy[2 * k], src_xy[2 * k + 1]};
+    const long long n = dd->dd->N();
+    double2* d = nullptr;
+    LSW_HIP(hipMalloc(&d, n * sizeof(double2)));
+    try {
+      dd->dd->assembleRHSDev(s, d, dd->dd->stream());
+      LSW_HIP(hipMemcpy(b, d, n * sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (...) {
+      (void)hipFree(d);
+      throw;
+    }
+    (void)hipFree(d);
+  });
+}
+
+int lsw_dd_assemble_rhs_dev(lsw_dd* dd, int nsrc, const double* src_xy, void* b, void* stream) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && b && (nsrc == 0 || src_xy), LSW_ERR_ARG, "null pointer");
+    std::vector<DoubleVector> s(nsrc);
+    for (int k = 0; k < nsrc; ++k) s[k] = {src_xy[2 * k], src_xy[2 * k + 1]};
+    dd->dd->assembleRHSDev(s, (double2*)b, (hipStream_t)stream);
+  });
+}
+
+namespace {
+void hostApply(lsw_dd* dd, const double* x, double* y, bool precond) {
+  const long long n = dd->dd->N();
+  double2 *dx = nullptr, *dy = nullptr;
+  LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+  LSW_HIP(hipMalloc(&dy, n * sizeof(double2)));
+  try {
+    LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
+    if (precond)
+      dd->dd->applyPrecondDev(dx, dy, dd->dd->stream());
+    else
+      dd->dd->applyADev(dx, dy, dd->dd->stream());
+    LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
+    LSW_HIP(hipMemcpy(y, dy, n * sizeof(double2), hipMemcpyDeviceToHost));
+  } catch (...) {
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    throw;
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+}
+}  // namespace
+
+int lsw_dd_apply_precond(lsw_dd* dd, const double* x, double* y) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y, LSW_ERR_ARG, "null pointer");
+    hostApply(dd, x, y, true);
+  });
+}
+
+int lsw_dd_apply_precond_dev(lsw_dd* dd, const void* x, void* y, void* stream) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y, LSW_ERR_ARG, "null pointer");
+    dd->dd->applyPrecondDev((const double2*)x, (double2*)y, (hipStream_t)stream);
+  });
+}
+
+int lsw_dd_apply_a(lsw_dd* dd, const double* x, double* y) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y, LSW_ERR_ARG, "null pointer");
+    hostApply(dd, x, y, false);
+  });
+}
+
+int lsw_dd_apply_a_dev(lsw_dd* dd, const void* x, void* y, void* stream) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y, LSW_ERR_ARG, "null pointer");
+    dd->dd->applyADev((const double2*)x, (double2*)y, (hipStream_t)stream);
+  });
+}
+
+int lsw_dd_stats(lsw_dd* dd, long long* solves, double* bytes, long long* factor_bytes) {
+  return guarded([&] {
+    LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");
+    if (solves) *solves = dd->dd->localSolvesPerPrecond();
+    if (bytes) *bytes = dd->dd->solveBytesPerPrecond();
+    if (factor_bytes) *factor_bytes = dd->dd->factors().factorized() ? dd->dd->factors().factorBytes() : 0;
+  });
+}
+
+int lsw_dd_sync(lsw_dd* dd) {
+  return guarded([&] {
+    LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");
+    LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
+  });
+}
+
+// ------------------------------------------------------------------ GMRES
+namespace {
+void configure(lsw_dd* dd, const lsw_gmres_params* p) {
+  LSW_REQUIRE(p->mode == 0 || p->mode == 1, LSW_ERR_ARG, "unknown GMRES mode");
+  LSW_REQUIRE(p->max_iter >= 0 && p->max_iter <= 1000, LSW_ERR_ARG, "bad MaxIter");
+  LSW_REQUIRE(p->tol >= 0, LSW_ERR_ARG, "bad Tol");
+  dd->gmres.setMode(p->mode);
+  dd->gmres.setTol(p->tol);
+  dd->gmres.setMaxIter(p->max_iter);
+  dd->gmres.setRestart(p->restart > 0 ? p->restart : 30);
+  dd->gmres.setVerbose(p->verbose != 0);
+}
+void fill(const GmresInfo& in, lsw_gmres_result* r) {
+  if (!r) return;
+  r->iterations = in.iterations;
+  r->precond_applies = in.preconditionerApplies;
+  r->matvecs = in.matvecs;
+  r->estimate = in.estimate;
+  r->true_residual = in.trueResidual;
+}
+}  // namespace
+
+int lsw_gmres_solve(lsw_dd* dd, const lsw_gmres_params* p, const double* b, double* x, lsw_gmres_result* res,
+                    double* log, int log_cap) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && p && b && x, LSW_ERR_ARG, "null pointer");
+    configure(dd, p);
+    const long long n = dd->dd->N();
+    double2 *db = nullptr, *dx = nullptr;
+    LSW_HIP(hipMalloc(&db, n * sizeof(double2)));
+    LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+    GmresInfo info;
+    try {
+      LSW_HIP(hipMemcpy(db, b, n * sizeof(double2), hipMemcpyHostToDevice));
+      dd->gmres.solveDev(db, dx, &info, dd->dd->stream());
+      LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
+      LSW_HIP(hipMemcpy(x, dx, n * sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (...) {
+      (void)hipFree(db);
+      (void)hipFree(dx);
+      throw;
+    }
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    fill(info, res);
+    if (log)
+      for (int k = 0; k < log_cap && k < (int)info.log.size(); ++k) log[k] = info.log[k];
+  });
+}
+
+int lsw_gmres_solve_dev(lsw_dd* dd, const lsw_gmres_params* p, const void* b, void* x, void* stream,
+                        lsw_gmres_result* res) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && p && b && x, LSW_ERR_ARG, "null pointer");
+    configure(dd, p);
+    GmresInfo info;
+    dd->gmres.solveDev((const double2*)b, (double2*)x, &info, (hipStream_t)stream);
+    fill(info, res);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// The domain-decomposition layer of the MI355X path: class DD (include/DD.h:
+// 14-114) and DDCheckerboard (include/DDCheckerboard.h), with every
+// checkerboard cell resident on one GPU.
+//
+// Vectors of the global problem are held on the device as one grid function
+// over the global DOF grid (x-major, y fastest: g = gi*Gy + gj).  The
+// reference's per-rank "owned DOF" vectors (mValsToKeep*, reduce/expand and
+// the shifted/unshifted conversions, src/DDCheckerboard.cpp:434-809,
+// 2995-3515) are exact re-layouts of that grid function, so on one device
+// they become index ranges: every trace, owned set and L-trace of a cell is a
+// range of local grid columns or rows.
+//
+// applyPrecond (src/DDCheckerboard.cpp:687-739) runs the four windowed DDs
+// and, per DD, stage 1, the four 1-D sweeps and the four L-sweeps.  Cells on
+// one front (a column, a row or an anti-diagonal) are independent, so every
+// front of every sweep of every DD is one batch of local solves.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "lsw_factor.h"
+#include "lsw_model.h"
+
+namespace lsw {
+
+class LocalLinearSolver;
+class RHS;
+
+// ------------------------------------------------------------------ DD base
+class DD {
+ public:
+  virtual ~DD() = default;
+  virtual void factorize() = 0;
+  // owned-DOF vectors per cell, cells in row-major order (mCells order)
+  virtual void assembleRHS(const RHS* rhs, std::vector<ComplexVector>& res) const = 0;
+  virtual void applyPrecond(const std::vector<ComplexVector>& x, std::vector<ComplexVector>& res) const = 0;
+  virtual void applySystemMatrix(const std::vector<ComplexVector>& x,
+                                 std::vector<ComplexVector>& res) const = 0;
+  virtual UInt getDim() const { return 2; }
+  virtual double getMeshSize() const = 0;
+  virtual UInt getNCells() const = 0;
+};
+
+// Host description of one cell (the reference's SubdomainStructured).
+struct CellGeom {
+  int I = 0, J = 0;
+  AssemblerHelmholtz2DFD asmb;
+  int nx = 0, ny = 0;
+  double lims[2][10] = {};
+  int vlr[2][8] = {}, vud[2][8] = {};
+  int ox = 0, oy = 0;
+  double meshSize = 0;
+};
+
+// Per-slot description of one local solve in a sweep step (device POD).
+struct SlotDev {
+  int d, kind, cell, slot;
+  int setCol[2], setColLo[2], setColHi[2];
+  int setRow[2], setRowLo[2], setRowHi[2];
+  long long colVal[2], rowVal[2];
+  int owCol, owRow;
+  long long owColVal, owRowVal;
+  int nOut;
+  int outIsRow[8], outIdx[8], outAdd[8];
+  long long outDst[8];
+};
+
+struct GroupDev {
+  int d, cell, n, pad;
+  int slot[4];
+};
+
+struct CopyOp {
+  long long dst, src;
+};
+
+class DDCheckerboard : public DD {
+ public:
+  DDCheckerboard();
+  ~DDCheckerboard() override;
+
+  // src/DDCheckerboard.cpp:87-645.  domain = [[ax,bx],[ay,by]];
+  // nCells = (NCellsX, NCellsY) sizing rows and columns; nElsInCell = (NX, NY).
+  // The solver prototype selects the factor precision (LocalLinearSolverHIP).
+  void setUp(const Assembler* assembler, const LocalLinearSolver* solver,
+             std::vector<DoubleVector> domain, std::vector<UInt> nCells, std::vector<UInt> nElsInCell);
+  // The host half of setUp: geometry, index sets and ownership only (no
+  // device work).  setUp = setUpGeometry + device initialisation.
+  void setUpGeometry(const Assembler* assembler, const LocalLinearSolver* solver,
+                     std::vector<DoubleVector> domain, std::vector<UInt> nCells,
+                     std::vector<UInt> nElsInCell);
+  bool deviceReady() const { return mFactors != nullptr; }
+
+  void factorize() override;
+  void assembleRHS(const RHS* rhs, std::vector<ComplexVector>& res) const override;
+  void applyPrecond(const std::vector<ComplexVector>& x, std::vector<ComplexVector>& res) const override;
+  void applySystemMatrix(const std::vector<ComplexVector>& x,
+                         std::vector<ComplexVector>& res) const override;
+  double getMeshSize() const override { return mMeshSize; }
+  UInt getNCells() const override { return (UInt)mCells.size(); }
+
+  // ---- device API (global grid vectors, complex double interleaved) ----
+  int Gx() const { return mGx; }
+  int Gy() const { return mGy; }
+  long long N() const { return (long long)mGx * mGy; }
+  // y = M^{-1} x
+  void applyPrecondDev(const double2* x, double2* y, hipStream_t s) const;
+  // y = A x (global operator)
+  void applyADev(const double2* x, double2* y, hipStream_t s) const;
+  // global RHS of point sources (summed, RHSSrcPts semantics)
+  void assembleRHSDev(const std::vector<DoubleVector>& srcs, double2* b, hipStream_t s) const;
+  // global DOF coordinates
+  double gx(int i) const { return mGlobal.asmb.xCoord(i); }
+  double gy(int j) const { return mGlobal.asmb.yCoord(j); }
+  hipStream_t stream() const { return mStream; }
+  int precision() const { return mPrecision; }
+  const FactorSet& factors() const { return *mFactors; }
+  const CellGeom& cellGeom(int i) const { return mCells[i]; }
+  int nRows() const { return mNR; }
+  int nCols() const { return mNC; }
+  long long localSolvesPerPrecond() const { return mSolvesPerPrecond; }
+  // algorithmic bytes moved by the local solves of one applyPrecond
+  double solveBytesPerPrecond() const { return mSolveBytesPerPrecond; }
+  // owned global indices of cell c (unshifted), in local DOF order
+  void ownedGlobalIndices(int c, std::vector<long long>& idx) const;
+  // global vector <-> per-cell owned vectors
+  void gatherOwned(const std::vector<Complex>& g, std::vector<ComplexVector>& res) const;
+  void scatterOwned(const std::vector<ComplexVector>& x, std::vector<Complex>& g) const;
+
+ private:
+  void buildSchedule();
+  void allocBuffers();
+  void initDevice();
+  std::vector<unsigned char> mColT, mRowT;
+
+  std::vector<DoubleVector> mDomain;
+  int mNR = 0, mNC = 0;
+  int mGx = 0, mGy = 0;
+  double mMeshSize = 0;
+  int mPrecision = LSW_FP64;
+  std::vector<CellGeom> mCells;
+  CellGeom mGlobal;
+  std::unique_ptr<FactorSet> mFactors;
+  hipStream_t mStream = nullptr;
+
+  // global operator
+  CellDev mGlobalDev{};
+  double2* dGlobalCoef = nullptr;
+  // window classification of global columns / rows (1 = trace column/row)
+  unsigned char* dColTrace = nullptr;
+  unsigned char* dRowTrace = nullptr;
+
+  // schedule
+  struct Step {
+    SlotDev* dSlots = nullptr;
+    int nSlots = 0;
+    SolveTask* dTasks = nullptr;
+    int nTasks = 0, maxNrhs = 1;
+    GroupDev* dGroups = nullptr;
+    int nGroups = 0;
+    CopyOp* dCopies = nullptr;  // trace copies to run after this step
+    int nCopies = 0;
+  };
+  std::vector<Step> mSteps;
+  int mMaxSlots = 0;
+  int mTmax = 0;
+  long long mTraceEntries = 0;
+  long long mSolvesPerPrecond = 0;
+  double mSolveBytesPerPrecond = 0;
+
+  // buffers (element type follows precision)
+  void* dX = nullptr;
+  void* dR = nullptr;
+  void* dG = nullptr;
+  void* dT = nullptr;
+  double2* dY = nullptr;  // 4 accumulators of N entries
+  long long mXs = 0, mGs = 0;
+};
+
+}  // namespace lsw

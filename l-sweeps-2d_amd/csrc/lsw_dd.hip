@@ -1,0 +1,943 @@
+// DDCheckerboard on one GPU: geometry (src/DDCheckerboard.cpp:87-645), the
+// L-sweeps preconditioner (:687-2608) and the global operator (:2611-2635).
+#include "lsw_dd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lsw_cplx.h"
+#include "lsw_solver.h"
+
+namespace lsw {
+
+double2* assembleStencilDev(const CellDev& c, const std::vector<double>& m, hipStream_t s);
+
+// --------------------------------------------------------------- constants
+enum Ctx { kS1 = 0, kR = 1, kL = 2, kU = 3, kD = 4, kBL = 5, kTR = 6, kBR = 7, kTL = 8, kNumCtx = 9 };
+
+// ------------------------------------------------------------------ kernels
+
+// Right-hand side of every local solve of one sweep step.
+//   stage 1: the windowed input restricted to the cell's owned set of its DD
+//            (getSeparatedRHS + reduce/expand, src/DDCheckerboard.cpp:3564-3674)
+//   sweeps : equivalent sources res0 = A01 v1 on set 0, res1 = -A10 v0 on set 1
+//            (SubdomainStructured::getEquivalentSources/extractLTraces,
+//            src/SubdomainStructured.cpp:1304-1539)
+template <typename T2>
+__global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots,
+                            const double2* __restrict__ xin, const unsigned char* __restrict__ colT,
+                            const unsigned char* __restrict__ rowT, int Gy,
+                            const double2* __restrict__ coef, const T2* __restrict__ T,
+                            T2* __restrict__ X, long long xs) {
+  const SlotDev& sl = slots[blockIdx.y];
+  const CellDev& cd = cells[sl.cell];
+  const int nx = cd.nx, ny = cd.ny, N = nx * ny;
+  const int slr = sl.d & 1, sud = sl.d >> 1;
+  T2* x = X + (long long)sl.slot * xs;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+    const int i = p / ny, j = p - i * ny;
+    T2 out = CTraits<T2>::make(0, 0);
+    if (sl.kind == 0) {
+      const bool own = i >= cd.vlr[slr][2] && i < cd.vlr[slr][5] && j >= cd.vud[sud][2] && j < cd.vud[sud][5];
+      if (own) {
+        const int gi = i + cd.ox, gj = j + cd.oy;
+        const bool ct = colT[gi] != 0, rt = rowT[gj] != 0;
+        const bool keep = (sl.d == 0) ? (!ct && !rt) : (sl.d == 1) ? (ct && !rt) : (sl.d == 2) ? (!ct && rt) : (ct && rt);
+        if (keep) {
+          const double2 v = xin[(long long)gi * Gy + gj];
+          out = CTraits<T2>::make(v.x, v.y);
+        }
+      }
+    } else {
+      auto inSet = [&](int s, int ii, int jj) -> bool {
+        return (ii == sl.setCol[s] && jj >= sl.setColLo[s] && jj < sl.setColHi[s]) ||
+               (jj == sl.setRow[s] && ii >= sl.setRowLo[s] && ii < sl.setRowHi[s]);
+      };
+      auto val = [&](int s, int ii, int jj) -> T2 {
+        if (sl.rowVal[s] >= 0 && jj == sl.setRow[s] && ii >= sl.setRowLo[s] && ii < sl.setRowHi[s])
+          return T[sl.rowVal[s] + ii];
+        return T[sl.colVal[s] + jj];
+      };
+      const bool in0 = inSet(0, i, j);
+      const bool in1 = !in0 && inSet(1, i, j);
+      if (in0 || in1) {
+        const int other = in0 ? 1 : 0;
+        const int ni[4] = {i - 1, i + 1, i, i};
+        const int nj[4] = {j, j, j - 1, j + 1};
+        double2 acc = make_double2(0.0, 0.0);
+        for (int q = 0; q < 4; ++q) {
+          if (ni[q] < 0 || ni[q] >= nx || nj[q] < 0 || nj[q] >= ny) continue;
+          if (!inSet(other, ni[q], nj[q])) continue;
+          const double2 a = coef[cd.coefOff + (long long)(q + 1) * N + p];
+          const T2 vv = val(other, ni[q], nj[q]);
+          acc = cfma(a, make_double2((double)vv.x, (double)vv.y), acc);
+        }
+        if (in1) acc = cneg(acc);
+        out = CTraits<T2>::make(acc.x, acc.y);
+      }
+    }
+    x[p] = out;
+  }
+}
+
+// Trace extraction after the local solves of one step (assign or accumulate
+// into the slot's own trace context), with the stage-3 overwrite of the
+// incoming outer traces (src/DDCheckerboard.cpp:1861-1864,2085-2088,
+// 2305-2308,2520-2526).
+template <typename T2>
+__global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots,
+                       const T2* __restrict__ X, long long xs, T2* __restrict__ T) {
+  const SlotDev& sl = slots[blockIdx.y];
+  const CellDev& cd = cells[sl.cell];
+  const int nx = cd.nx, ny = cd.ny;
+  const T2* x = X + (long long)sl.slot * xs;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int o = 0; o < sl.nOut; ++o) {
+    const bool isRow = sl.outIsRow[o] != 0;
+    const int len = isRow ? nx : ny;
+    if (t >= len) continue;
+    const int i = isRow ? t : sl.outIdx[o];
+    const int j = isRow ? sl.outIdx[o] : t;
+    T2 v;
+    if (j == sl.owRow)
+      v = T[sl.owRowVal + i];
+    else if (i == sl.owCol)
+      v = T[sl.owColVal + j];
+    else
+      v = x[(long long)i * ny + j];
+    T2* d = T + sl.outDst[o] + t;
+    *d = sl.outAdd[o] ? cadd(*d, v) : v;
+  }
+}
+
+// Add the owned part of the step's solutions of each (DD, cell) to the DD's
+// global accumulator.
+template <typename T2>
+__global__ void k_accumulate(const CellDev* __restrict__ cells, const GroupDev* __restrict__ groups,
+                             const T2* __restrict__ X, long long xs, double2* __restrict__ Y,
+                             long long N, int Gy) {
+  const GroupDev& g = groups[blockIdx.y];
+  const CellDev& cd = cells[g.cell];
+  const int slr = g.d & 1, sud = g.d >> 1;
+  const int i0 = cd.vlr[slr][2], i1 = cd.vlr[slr][5];
+  const int j0 = cd.vud[sud][2], j1 = cd.vud[sud][5];
+  const int w = j1 - j0;
+  const long long cnt = (long long)(i1 - i0) * w;
+  double2* y = Y + (long long)g.d * N;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < cnt;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = i0 + (int)(q / w), j = j0 + (int)(q - (q / w) * w);
+    const long long p = (long long)i * cd.ny + j;
+    double2 acc = y[(long long)(i + cd.ox) * Gy + (j + cd.oy)];
+    for (int s = 0; s < g.n; ++s) {
+      const T2 v = X[(long long)g.slot[s] * xs + p];
+      acc.x += (double)v.x;
+      acc.y += (double)v.y;
+    }
+    y[(long long)(i + cd.ox) * Gy + (j + cd.oy)] = acc;
+  }
+}
+
+template <typename T2>
+__global__ void k_copy_traces(const CopyOp* __restrict__ ops, T2* __restrict__ T, int tmax) {
+  const CopyOp op = ops[blockIdx.y];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tmax; t += gridDim.x * blockDim.x)
+    T[op.dst + t] = T[op.src + t];
+}
+
+// res = res1 + (res2 + res3 + res4) (src/DDCheckerboard.cpp:733-735)
+__global__ void k_combine(const double2* __restrict__ Y, double2* __restrict__ out, long long N) {
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < N;
+       g += (long long)gridDim.x * blockDim.x) {
+    const double2 a = Y[g], b = Y[N + g], c = Y[2 * N + g], d = Y[3 * N + g];
+    const double2 s = cadd(cadd(b, c), d);
+    out[g] = cadd(a, s);
+  }
+}
+
+// Global 5-point operator y = A x.
+__global__ void k_matvec(const double2* __restrict__ coef, const double2* __restrict__ x,
+                         double2* __restrict__ y, int Gx, int Gy) {
+  const long long N = (long long)Gx * Gy;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < N;
+       g += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(g / Gy), j = (int)(g - (long long)i * Gy);
+    double2 acc = cmul(coef[g], x[g]);
+    if (i > 0) acc = cfma(coef[N + g], x[g - Gy], acc);
+    if (i < Gx - 1) acc = cfma(coef[2 * N + g], x[g + Gy], acc);
+    if (j > 0) acc = cfma(coef[3 * N + g], x[g - 1], acc);
+    if (j < Gy - 1) acc = cfma(coef[4 * N + g], x[g + 1], acc);
+    y[g] = acc;
+  }
+}
+
+// Point-source RHS on the global grid (RHSSrcPts::eval, src/RHSSrcPts.cpp:87-112).
+__global__ void k_point_sources(const double* __restrict__ src, int nsrc, double eps, double x0, double hx,
+                                double y0, double hy, int Gx, int Gy, double2* __restrict__ b) {
+  const long long N = (long long)Gx * Gy;
+  const double pi = 3.14159265358979323846;
+  const double scale = 1.0 / (pi * (eps * eps));
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < N;
+       g += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(g / Gy), j = (int)(g - (long long)i * Gy);
+    const double x = x0 + (double)(i + 1) * hx, y = y0 + (double)(j + 1) * hy;
+    double acc = 0.0;
+    for (int s = 0; s < nsrc; ++s) {
+      const double dx = src[2 * s] - x, dy = src[2 * s + 1] - y;
+      acc += scale * exp(-(dx * dx + dy * dy) / (eps * eps));
+    }
+    b[g] = make_double2(acc, 0.0);
+  }
+}
+
+// ------------------------------------------------------------------ setup
+DDCheckerboard::DDCheckerboard() = default;
+
+DDCheckerboard::~DDCheckerboard() {
+  for (auto& st : mSteps) {
+    if (st.dSlots) (void)hipFree(st.dSlots);
+    if (st.dTasks) (void)hipFree(st.dTasks);
+    if (st.dGroups) (void)hipFree(st.dGroups);
+    if (st.dCopies) (void)hipFree(st.dCopies);
+  }
+  for (void* p : {dX, dR, dG, dT}) if (p) (void)hipFree(p);
+  if (dY) (void)hipFree(dY);
+  if (dGlobalCoef) (void)hipFree(dGlobalCoef);
+  if (dColTrace) (void)hipFree(dColTrace);
+  if (dRowTrace) (void)hipFree(dRowTrace);
+  mFactors.reset();
+  if (mStream) (void)hipStreamDestroy(mStream);
+}
+
+namespace {
+// assignIndexSets: first limit the coordinate lies below, else 10
+// (src/SubdomainStructured.cpp:1019-1116).  Classes are monotone along the
+// axis, so each raw set is a range; s[k] = number of DOFs with class < k.
+void rawStarts(const std::vector<double>& coords, const double lims[10], int s[12]) {
+  std::vector<int> cls(coords.size());
+  for (size_t i = 0; i < coords.size(); ++i) {
+    int c = 10;
+    for (int k = 0; k < 10; ++k)
+      if (coords[i] < lims[k]) {
+        c = k;
+        break;
+      }
+    cls[i] = c;
+  }
+  for (size_t i = 1; i < cls.size(); ++i)
+    LSW_REQUIRE(cls[i] >= cls[i - 1], LSW_ERR_ARG, "index sets are not monotone along the axis");
+  for (int k = 0; k <= 11; ++k) {
+    int n = 0;
+    for (int c : cls) n += (c < k);
+    s[k] = n;
+  }
+}
+// 7-set views (src/SubdomainStructured.cpp:749-801).
+void sevenView(const int s[12], int n, int v[2][8]) {
+  const int un[8] = {0, s[1], s[2], s[3], s[6], s[7], s[8], n};
+  const int sh[8] = {0, s[3], s[4], s[5], s[8], s[9], s[10], n};
+  for (int k = 0; k < 8; ++k) {
+    v[0][k] = un[k];
+    v[1][k] = sh[k];
+  }
+}
+}  // namespace
+
+void DDCheckerboard::setUp(const Assembler* inputAssembler, const LocalLinearSolver* solver,
+                           std::vector<DoubleVector> domain, std::vector<UInt> nCells,
+                           std::vector<UInt> nElsInCell) {
+  setUpGeometry(inputAssembler, solver, domain, nCells, nElsInCell);
+  initDevice();
+}
+
+void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalLinearSolver* solver,
+                                   std::vector<DoubleVector> domain, std::vector<UInt> nCells,
+                                   std::vector<UInt> nElsInCell) {
+  const auto* a0 = dynamic_cast<const AssemblerHelmholtz2DFD*>(inputAssembler);
+  LSW_REQUIRE(a0 != nullptr, LSW_ERR_ARG, "DDCheckerboard needs an AssemblerHelmholtz2DFD");
+  LSW_REQUIRE(domain.size() == 2 && nCells.size() == 2 && nElsInCell.size() == 2, LSW_ERR_ARG,
+              "DDCheckerboard: dimension must be 2");
+  const auto* hs = dynamic_cast<const LocalLinearSolverHIP*>(solver);
+  LSW_REQUIRE(hs != nullptr, LSW_ERR_ARG, "DDCheckerboard (HIP) needs a LocalLinearSolverHIP");
+  mPrecision = hs->precision();
+  const UInt order = a0->getOrder();
+  LSW_REQUIRE(order == 2, LSW_ERR_UNSUPPORTED, "only Order 2 is supported");
+  const UInt tt = (UInt)std::ceil(order / 2.0);
+  LSW_REQUIRE(nElsInCell[0] > 4 * tt && nElsInCell[1] > 4 * tt, LSW_ERR_ARG, "Not enough elements in cell!");
+  mDomain = domain;
+  mNR = (int)nCells[0];
+  mNC = (int)nCells[1];
+  LSW_REQUIRE(mNR >= 1 && mNC >= 1, LSW_ERR_ARG, "DDCheckerboard: empty board");
+  const UInt n1 = nCells[0] * nElsInCell[0] + nCells[0] - 1;
+  const UInt n2 = nCells[1] * nElsInCell[1] + nCells[1] - 1;
+  const double h1 = (domain[0][1] - domain[0][0]) / n1;
+  const double h2 = (domain[1][1] - domain[1][0]) / n2;
+  const UInt NX = nElsInCell[0], NY = nElsInCell[1];
+  mCells.clear();
+  mCells.resize((size_t)mNR * mNC);
+  for (int I = 0; I < mNR; ++I)
+    for (int J = 0; J < mNC; ++J) {
+      CellGeom& cg = mCells[(size_t)I * mNC + J];
+      cg.I = I;
+      cg.J = J;
+      double lims1[10] = {}, lims2[10] = {};
+      std::vector<DoubleVector> intDom(2, DoubleVector(2));
+      double a = domain[0][0] + (NY + 1) * (UInt)J * h1;
+      if (J > 0) {
+        lims1[0] = a - tt * h1 - 0.5 * h1;
+        lims1[1] = a - 0.5 * h1;
+        lims1[2] = a + tt * h1 - 0.5 * h1;
+        lims1[3] = a + 2 * tt * h1 - 0.5 * h1;
+        lims1[4] = a + 3 * tt * h1 - 0.5 * h1;
+        a -= 2 * tt * h1;
+        if (a < domain[0][0]) a = domain[0][0];
+      }
+      double b = domain[0][0] + (NY + 1) * (UInt)J * h1 + NY * h1;
+      if (J < mNC - 1) {
+        lims1[5] = b - tt * h1 + 0.5 * h1;
+        lims1[6] = b + 0.5 * h1;
+        lims1[7] = b + tt * h1 + 0.5 * h1;
+        lims1[8] = b + 2 * tt * h1 + 0.5 * h1;
+        lims1[9] = b + 3 * tt * h1 + 0.5 * h1;
+        b += 4 * tt * h1;
+        if (b > domain[0][1]) b = domain[0][1];
+      }
+      intDom[0][0] = a;
+      intDom[0][1] = b;
+      a = domain[1][0] + (NX + 1) * (UInt)I * h2;
+      if (I > 0) {
+        lims2[0] = a - tt * h2 - 0.5 * h2;
+        lims2[1] = a - 0.5 * h2;
+        lims2[2] = a + tt * h2 - 0.5 * h2;
+        lims2[3] = a + 2 * tt * h2 - 0.5 * h2;
+        lims2[4] = a + 3 * tt * h2 - 0.5 * h2;
+        a -= 2 * tt * h2;
+        if (a < domain[1][0]) a = domain[1][0];
+      }
+      b = domain[1][0] + (NX + 1) * (UInt)I * h2 + NX * h2;
+      if (I < mNR - 1) {
+        lims2[5] = b - tt * h2 + 0.5 * h2;
+        lims2[6] = b + 0.5 * h2;
+        lims2[7] = b + tt * h2 + 0.5 * h2;
+        lims2[8] = b + 2 * tt * h2 + 0.5 * h2;
+        lims2[9] = b + 3 * tt * h2 + 0.5 * h2;
+        b += 4 * tt * h2;
+        if (b > domain[1][1]) b = domain[1][1];
+      }
+      intDom[1][0] = a;
+      intDom[1][1] = b;
+      std::vector<UInt> n(2);
+      n[0] = (UInt)((intDom[0][1] - intDom[0][0] + 0.5 * h1) / h1);
+      n[1] = (UInt)((intDom[1][1] - intDom[1][0] + 0.5 * h2) / h2);
+      cg.asmb = *a0;
+      cg.asmb.setDomain(intDom);
+      cg.asmb.setN(n);
+      cg.asmb.setOrder(order);
+      cg.nx = cg.asmb.nxDofs();
+      cg.ny = cg.asmb.nyDofs();
+      // quirky min/max scan over the DOF points (src/DDCheckerboard.cpp:347-361)
+      double minX = 0.0, maxX = 0.0, minY = 0.0, maxY = 0.0;
+      for (int i = 0; i < cg.nx; ++i)
+        for (int j = 0; j < cg.ny; ++j) {
+          const double px = cg.asmb.xCoord(i), py = cg.asmb.yCoord(j);
+          if (px < minX) minX = px;
+          else if (px > maxX) maxX = px;
+          if (py < minY) minY = py;
+          else if (py > maxY) maxY = py;
+        }
+      if (J == 0)
+        for (int k = 0; k < 5; ++k) lims1[k] = minX - h1;
+      if (J == mNC - 1)
+        for (int k = 5; k < 10; ++k) lims1[k] = maxX + h1;
+      if (I == 0)
+        for (int k = 0; k < 5; ++k) lims2[k] = minY - h2;
+      if (I == mNR - 1)
+        for (int k = 5; k < 10; ++k) lims2[k] = maxY + h2;
+      std::memcpy(cg.lims[0], lims1, sizeof(lims1));
+      std::memcpy(cg.lims[1], lims2, sizeof(lims2));
+      std::vector<double> xs(cg.nx), ys(cg.ny);
+      for (int i = 0; i < cg.nx; ++i) xs[i] = cg.asmb.xCoord(i);
+      for (int j = 0; j < cg.ny; ++j) ys[j] = cg.asmb.yCoord(j);
+      int sx[12], sy[12];
+      rawStarts(xs, lims1, sx);
+      rawStarts(ys, lims2, sy);
+      sevenView(sx, cg.nx, cg.vlr);
+      sevenView(sy, cg.ny, cg.vud);
+      for (int f = 0; f < 2; ++f)
+        for (int k = 1; k < 6; ++k) {
+          if (k == 3) continue;
+          LSW_REQUIRE(cg.vlr[f][k + 1] - cg.vlr[f][k] <= 1 && cg.vud[f][k + 1] - cg.vud[f][k] <= 1,
+                      LSW_ERR_UNSUPPORTED, "trace sets thicker than one grid line");
+        }
+      cg.meshSize = cg.asmb.getMeshSize();
+    }
+  // global grid = the local grid of a 1x1 DD over the whole domain
+  mGlobal = CellGeom();
+  mGlobal.asmb = *a0;
+  mGlobal.asmb.setDomain(domain);
+  mGlobal.asmb.setN({n1, n2});
+  mGlobal.asmb.setOrder(order);
+  mGx = mGlobal.asmb.nxDofs();
+  mGy = mGlobal.asmb.nyDofs();
+  mGlobal.nx = mGx;
+  mGlobal.ny = mGy;
+  mMeshSize = mCells[0].meshSize;
+  for (auto& c : mCells) mMeshSize = std::min(mMeshSize, c.meshSize);
+  // local -> global offsets; owned DOFs must sit on global grid points
+  std::vector<int> colOwner(mGx, -1), rowOwner(mGy, -1);
+  std::vector<unsigned char> colT(mGx, 0), rowT(mGy, 0);
+  for (auto& c : mCells) {
+    c.ox = (int)std::lround((c.asmb.xCoord(0) - mGlobal.asmb.xCoord(0)) / h1);
+    c.oy = (int)std::lround((c.asmb.yCoord(0) - mGlobal.asmb.yCoord(0)) / h2);
+    for (int i = c.vlr[0][2]; i < c.vlr[0][5]; ++i) {
+      const int gi = i + c.ox;
+      LSW_REQUIRE(gi >= 0 && gi < mGx, LSW_ERR_ARG, "owned column outside the global grid");
+      LSW_REQUIRE(std::fabs(c.asmb.xCoord(i) - mGlobal.asmb.xCoord(gi)) < 1e-6 * h1, LSW_ERR_ARG,
+                  "local and global grids are not aligned");
+      if (c.I == 0) {
+        LSW_REQUIRE(colOwner[gi] < 0, LSW_ERR_ARG, "owned columns overlap");
+        colOwner[gi] = c.J;
+        colT[gi] = (i < c.vlr[0][3] || i >= c.vlr[0][4]) ? 1 : 0;
+      }
+    }
+    for (int j = c.vud[0][2]; j < c.vud[0][5]; ++j) {
+      const int gj = j + c.oy;
+      LSW_REQUIRE(gj >= 0 && gj < mGy, LSW_ERR_ARG, "owned row outside the global grid");
+      LSW_REQUIRE(std::fabs(c.asmb.yCoord(j) - mGlobal.asmb.yCoord(gj)) < 1e-6 * h2, LSW_ERR_ARG,
+                  "local and global grids are not aligned");
+      if (c.J == 0) {
+        LSW_REQUIRE(rowOwner[gj] < 0, LSW_ERR_ARG, "owned rows overlap");
+        rowOwner[gj] = c.I;
+        rowT[gj] = (j < c.vud[0][3] || j >= c.vud[0][4]) ? 1 : 0;
+      }
+    }
+  }
+  for (int gi = 0; gi < mGx; ++gi) LSW_REQUIRE(colOwner[gi] >= 0, LSW_ERR_ARG, "owned columns do not cover the grid");
+  for (int gj = 0; gj < mGy; ++gj) LSW_REQUIRE(rowOwner[gj] >= 0, LSW_ERR_ARG, "owned rows do not cover the grid");
+  // neighbours share the perpendicular grid
+  for (auto& c : mCells) {
+    if (c.J + 1 < mNC) {
+      const auto& r = mCells[(size_t)c.I * mNC + c.J + 1];
+      LSW_REQUIRE(r.ny == c.ny && r.oy == c.oy, LSW_ERR_ARG, "row neighbours have different y grids");
+    }
+    if (c.I + 1 < mNR) {
+      const auto& t = mCells[(size_t)(c.I + 1) * mNC + c.J];
+      LSW_REQUIRE(t.nx == c.nx && t.ox == c.ox, LSW_ERR_ARG, "column neighbours have different x grids");
+    }
+  }
+  mColT = colT;
+  mRowT = rowT;
+}
+
+void DDCheckerboard::initDevice() {
+  if (!mStream) LSW_HIP(hipStreamCreate(&mStream));
+  LSW_HIP(hipMalloc(&dColTrace, mGx));
+  LSW_HIP(hipMalloc(&dRowTrace, mGy));
+  LSW_HIP(hipMemcpy(dColTrace, mColT.data(), mGx, hipMemcpyHostToDevice));
+  LSW_HIP(hipMemcpy(dRowTrace, mRowT.data(), mGy, hipMemcpyHostToDevice));
+
+  auto toDev = [&](const CellGeom& c) {
+    CellDev d;
+    std::memset(&d, 0, sizeof(d));
+    d.nx = c.nx;
+    d.ny = c.ny;
+    d.ox = c.ox;
+    d.oy = c.oy;
+    std::memcpy(d.vlr, c.vlr, sizeof(d.vlr));
+    std::memcpy(d.vud, c.vud, sizeof(d.vud));
+    d.x0 = c.asmb.x0();
+    d.y0 = c.asmb.y0();
+    d.hx = c.asmb.hx();
+    d.hy = c.asmb.hy();
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) {
+        d.intDom[a][b] = c.asmb.intDomain()[a][b];
+        d.compDom[a][b] = c.asmb.compDomain()[a][b];
+      }
+    d.C = c.asmb.getPML()->getAbsorptionCoeff();
+    d.omega = c.asmb.getOmega();
+    return d;
+  };
+  mFactors = std::make_unique<FactorSet>();
+  for (auto& c : mCells) {
+    std::vector<double> m;
+    c.asmb.squaredSlowness(m);
+    mFactors->addCell(toDev(c), m);
+  }
+  mFactors->setup(mStream);
+  {
+    std::vector<double> m;
+    mGlobal.asmb.squaredSlowness(m);
+    mGlobalDev = toDev(mGlobal);
+    mGlobalDev.mOff = 0;
+    mGlobalDev.coefOff = 0;
+    dGlobalCoef = assembleStencilDev(mGlobalDev, m, mStream);
+  }
+  buildSchedule();
+  allocBuffers();
+}
+
+// ------------------------------------------------------------ the schedule
+void DDCheckerboard::buildSchedule() {
+  const int nc = (int)mCells.size();
+  mTmax = 0;
+  for (auto& c : mCells) mTmax = std::max(mTmax, std::max(c.nx, c.ny));
+  mTraceEntries = 4LL * nc * kNumCtx * 8 * mTmax;
+  auto tOff = [&](int d, int c, int ctx, int k) -> long long {
+    return ((((long long)d * nc + c) * kNumCtx + ctx) * 8 + k) * (long long)mTmax;
+  };
+  auto cidx = [&](int I, int J) { return I * mNC + J; };
+
+  struct HostStep {
+    std::vector<SlotDev> slots;
+    std::vector<CopyOp> copies;
+  };
+  std::vector<HostStep> steps;
+
+  auto baseSlot = [&](int d, int kind, int c) {
+    SlotDev s;
+    std::memset(&s, 0, sizeof(s));
+    s.d = d;
+    s.kind = kind;
+    s.cell = c;
+    for (int k = 0; k < 2; ++k) {
+      s.setCol[k] = s.setRow[k] = -1;
+      s.colVal[k] = s.rowVal[k] = -1;
+    }
+    s.owCol = s.owRow = -1;
+    s.owColVal = s.owRowVal = 0;
+    return s;
+  };
+  auto addOut = [&](SlotDev& s, bool isRow, int idx, int lo, int hi, long long dst, bool add) {
+    if (hi <= lo) return;  // empty trace set
+    LSW_REQUIRE(s.nOut < 8, LSW_ERR_ARG, "too many trace outputs");
+    s.outIsRow[s.nOut] = isRow ? 1 : 0;
+    s.outIdx[s.nOut] = idx;
+    s.outAdd[s.nOut] = add ? 1 : 0;
+    s.outDst[s.nOut] = dst;
+    ++s.nOut;
+  };
+  // set the four traces of one kind (LR: columns, UD: rows) from v (7-view)
+  auto outFour = [&](SlotDev& s, bool isRow, const int v[8], int d, int c, int ctx, bool add) {
+    const int sets[4] = {1, 2, 4, 5};
+    for (int k = 0; k < 4; ++k)
+      addOut(s, isRow, v[sets[k]], v[sets[k]], v[sets[k] + 1], tOff(d, c, ctx, (isRow ? 4 : 0) + k), add);
+  };
+
+  // ---- stage 1
+  {
+    HostStep st;
+    for (int d = 0; d < 4; ++d)
+      for (int c = 0; c < nc; ++c) {
+        const auto& cg = mCells[c];
+        const int slr = d & 1, sud = d >> 1;
+        SlotDev s = baseSlot(d, 0, c);
+        outFour(s, false, cg.vlr[slr], d, c, kS1, false);
+        outFour(s, true, cg.vud[sud], d, c, kS1, false);
+        st.slots.push_back(s);
+        // copies for stage 2: S1.LR -> R.LR, L.LR ; S1.UD -> U.UD, D.UD
+        for (int k = 0; k < 4; ++k) {
+          st.copies.push_back({tOff(d, c, kR, k), tOff(d, c, kS1, k)});
+          st.copies.push_back({tOff(d, c, kL, k), tOff(d, c, kS1, k)});
+          st.copies.push_back({tOff(d, c, kU, 4 + k), tOff(d, c, kS1, 4 + k)});
+          st.copies.push_back({tOff(d, c, kD, 4 + k), tOff(d, c, kS1, 4 + k)});
+        }
+      }
+    steps.push_back(st);
+  }
+  // ---- stage 2
+  const int nStage2 = std::max(mNR, mNC) - 1;
+  for (int t = 1; t <= nStage2; ++t) {
+    HostStep st;
+    for (int d = 0; d < 4; ++d) {
+      const int slr = d & 1, sud = d >> 1;
+      // right sweep: column t (src/DDCheckerboard.cpp:1168-1350)
+      if (t < mNC)
+        for (int I = 0; I < mNR; ++I) {
+          const int c = cidx(I, t), l = cidx(I, t - 1);
+          const auto& v = mCells[c].vlr[slr];
+          SlotDev s = baseSlot(d, 1, c);
+          s.setCol[0] = v[1]; s.setColLo[0] = 0; s.setColHi[0] = mCells[c].ny;
+          s.setCol[1] = v[2]; s.setColLo[1] = 0; s.setColHi[1] = mCells[c].ny;
+          s.colVal[0] = tOff(d, l, kR, 2);
+          s.colVal[1] = tOff(d, l, kR, 3);
+          outFour(s, true, mCells[c].vud[sud], d, c, kR, false);
+          addOut(s, false, v[4], v[4], v[5], tOff(d, c, kR, 2), true);
+          addOut(s, false, v[5], v[5], v[6], tOff(d, c, kR, 3), true);
+          st.slots.push_back(s);
+        }
+      // left sweep: column nC-1-t (src/DDCheckerboard.cpp:976-1164)
+      if (t < mNC)
+        for (int I = 0; I < mNR; ++I) {
+          const int J = mNC - 1 - t;
+          const int c = cidx(I, J), r = cidx(I, J + 1);
+          const auto& v = mCells[c].vlr[slr];
+          SlotDev s = baseSlot(d, 2, c);
+          s.setCol[0] = v[5]; s.setColLo[0] = 0; s.setColHi[0] = mCells[c].ny;
+          s.setCol[1] = v[4]; s.setColLo[1] = 0; s.setColHi[1] = mCells[c].ny;
+          s.colVal[0] = tOff(d, r, kL, 1);
+          s.colVal[1] = tOff(d, r, kL, 0);
+          outFour(s, true, mCells[c].vud[sud], d, c, kL, false);
+          addOut(s, false, v[1], v[1], v[2], tOff(d, c, kL, 0), true);
+          addOut(s, false, v[2], v[2], v[3], tOff(d, c, kL, 1), true);
+          st.slots.push_back(s);
+        }
+      // up sweep: row t (src/DDCheckerboard.cpp:1544-1724)
+      if (t < mNR)
+        for (int J = 0; J < mNC; ++J) {
+          const int c = cidx(t, J), bo = cidx(t - 1, J);
+          const auto& v = mCells[c].vud[sud];
+          SlotDev s = baseSlot(d, 3, c);
+          s.setRow[0] = v[1]; s.setRowLo[0] = 0; s.setRowHi[0] = mCells[c].nx;
+          s.setRow[1] = v[2]; s.setRowLo[1] = 0; s.setRowHi[1] = mCells[c].nx;
+          s.rowVal[0] = tOff(d, bo, kU, 4 + 2);
+          s.rowVal[1] = tOff(d, bo, kU, 4 + 3);
+          outFour(s, false, mCells[c].vlr[slr], d, c, kU, false);
+          addOut(s, true, v[4], v[4], v[5], tOff(d, c, kU, 4 + 2), true);
+          addOut(s, true, v[5], v[5], v[6], tOff(d, c, kU, 4 + 3), true);
+          st.slots.push_back(s);
+        }
+      // down sweep: row nR-1-t (src/DDCheckerboard.cpp:1353-1540)
+      if (t < mNR)
+        for (int J = 0; J < mNC; ++J) {
+          const int I = mNR - 1 - t;
+          const int c = cidx(I, J), to = cidx(I + 1, J);
+          const auto& v = mCells[c].vud[sud];
+          SlotDev s = baseSlot(d, 4, c);
+          s.setRow[0] = v[5]; s.setRowLo[0] = 0; s.setRowHi[0] = mCells[c].nx;
+          s.setRow[1] = v[4]; s.setRowLo[1] = 0; s.setRowHi[1] = mCells[c].nx;
+          s.rowVal[0] = tOff(d, to, kD, 4 + 1);
+          s.rowVal[1] = tOff(d, to, kD, 4 + 0);
+          outFour(s, false, mCells[c].vlr[slr], d, c, kD, false);
+          addOut(s, true, v[1], v[1], v[2], tOff(d, c, kD, 4 + 0), true);
+          addOut(s, true, v[2], v[2], v[3], tOff(d, c, kD, 4 + 1), true);
+          st.slots.push_back(s);
+        }
+    }
+    if (t == nStage2) {
+      // copies for stage 3 (the L-sweeps take their traces by value,
+      // include/DDCheckerboard.h:209-239)
+      for (int d = 0; d < 4; ++d)
+        for (int c = 0; c < nc; ++c)
+          for (int k = 0; k < 4; ++k) {
+            st.copies.push_back({tOff(d, c, kBL, 4 + k), tOff(d, c, kR, 4 + k)});
+            st.copies.push_back({tOff(d, c, kTL, 4 + k), tOff(d, c, kR, 4 + k)});
+            st.copies.push_back({tOff(d, c, kTR, 4 + k), tOff(d, c, kL, 4 + k)});
+            st.copies.push_back({tOff(d, c, kBR, 4 + k), tOff(d, c, kL, 4 + k)});
+            st.copies.push_back({tOff(d, c, kBL, k), tOff(d, c, kU, k)});
+            st.copies.push_back({tOff(d, c, kBR, k), tOff(d, c, kU, k)});
+            st.copies.push_back({tOff(d, c, kTR, k), tOff(d, c, kD, k)});
+            st.copies.push_back({tOff(d, c, kTL, k), tOff(d, c, kD, k)});
+          }
+    }
+    steps.push_back(st);
+  }
+  if (nStage2 == 0) {
+    // 1 x 1 board: no sweeps at all
+  }
+  // ---- stage 3
+  const int nStage3 = (mNR >= 2 && mNC >= 2) ? (mNR - 2) + (mNC - 2) + 1 : 0;
+  for (int t = 0; t < nStage3; ++t) {
+    HostStep st;
+    for (int d = 0; d < 4; ++d) {
+      const int slr = d & 1, sud = d >> 1;
+      for (int kind = 5; kind <= 8; ++kind) {
+        const bool fromLeft = (kind == 5 || kind == 8);    // BL, TL
+        const bool fromBottom = (kind == 5 || kind == 7);  // BL, BR
+        const int ctx = kind;                               // kBL..kTL == 5..8
+        for (int a = 0; a <= t; ++a) {
+          const int b = t - a;
+          const int I = fromBottom ? 1 + a : mNR - 2 - a;
+          const int J = fromLeft ? 1 + b : mNC - 2 - b;
+          if (I < 0 || I >= mNR || J < 0 || J >= mNC) continue;
+          if (fromBottom ? I < 1 : I > mNR - 2) continue;
+          if (fromLeft ? J < 1 : J > mNC - 2) continue;
+          const int c = cidx(I, J);
+          const auto& cg = mCells[c];
+          const int* vl = cg.vlr[slr];
+          const int* vu = cg.vud[sud];
+          const int nb = fromLeft ? cidx(I, J - 1) : cidx(I, J + 1);
+          const int nv = fromBottom ? cidx(I - 1, J) : cidx(I + 1, J);
+          SlotDev s = baseSlot(d, kind, c);
+          const int oc = fromLeft ? vl[1] : vl[5], ic = fromLeft ? vl[2] : vl[4];
+          const int orr = fromBottom ? vu[1] : vu[5], ir = fromBottom ? vu[2] : vu[4];
+          s.setCol[0] = oc;
+          s.setColLo[0] = fromBottom ? vu[1] : 0;
+          s.setColHi[0] = fromBottom ? cg.ny : vu[6];
+          s.setRow[0] = orr;
+          s.setRowLo[0] = fromLeft ? vl[1] : 0;
+          s.setRowHi[0] = fromLeft ? cg.nx : vl[6];
+          s.setCol[1] = ic;
+          s.setColLo[1] = fromBottom ? vu[2] : 0;
+          s.setColHi[1] = fromBottom ? cg.ny : vu[5];
+          s.setRow[1] = ir;
+          s.setRowLo[1] = fromLeft ? vl[2] : 0;
+          s.setRowHi[1] = fromLeft ? cg.nx : vl[5];
+          // incoming: from the left neighbour's traceN/NP (LR 2,3) or the
+          // right neighbour's trace0/1 (LR 0,1); bottom's UD 2,3 or top's UD 0,1
+          const long long colOuter = fromLeft ? tOff(d, nb, ctx, 2) : tOff(d, nb, ctx, 1);
+          const long long colInner = fromLeft ? tOff(d, nb, ctx, 3) : tOff(d, nb, ctx, 0);
+          const long long rowOuter = fromBottom ? tOff(d, nv, ctx, 4 + 2) : tOff(d, nv, ctx, 4 + 1);
+          const long long rowInner = fromBottom ? tOff(d, nv, ctx, 4 + 3) : tOff(d, nv, ctx, 4 + 0);
+          s.colVal[0] = colOuter;
+          s.rowVal[0] = rowOuter;
+          s.colVal[1] = colInner;
+          s.rowVal[1] = rowInner;
+          s.owCol = oc;
+          s.owColVal = colOuter;
+          s.owRow = orr;
+          s.owRowVal = rowOuter;
+          if (fromLeft) {
+            addOut(s, false, vl[4], vl[4], vl[5], tOff(d, c, ctx, 2), true);
+            addOut(s, false, vl[5], vl[5], vl[6], tOff(d, c, ctx, 3), true);
+          } else {
+            addOut(s, false, vl[1], vl[1], vl[2], tOff(d, c, ctx, 0), true);
+            addOut(s, false, vl[2], vl[2], vl[3], tOff(d, c, ctx, 1), true);
+          }
+          if (fromBottom) {
+            addOut(s, true, vu[4], vu[4], vu[5], tOff(d, c, ctx, 4 + 2), true);
+            addOut(s, true, vu[5], vu[5], vu[6], tOff(d, c, ctx, 4 + 3), true);
+          } else {
+            addOut(s, true, vu[1], vu[1], vu[2], tOff(d, c, ctx, 4 + 0), true);
+            addOut(s, true, vu[2], vu[2], vu[3], tOff(d, c, ctx, 4 + 1), true);
+          }
+          st.slots.push_back(s);
+        }
+      }
+    }
+    steps.push_back(st);
+  }
+
+  // slot buffers, tasks (same cell -> one task of up to kMaxRhs), groups
+  mMaxSlots = 0;
+  mSolvesPerPrecond = 0;
+  mSolveBytesPerPrecond = 0;
+  const double S = (mPrecision == LSW_FP64) ? 16.0 : 8.0;
+  for (auto& st : steps) mMaxSlots = std::max(mMaxSlots, (int)st.slots.size());
+  mSteps.clear();
+  for (auto& hst : steps) {
+    Step st;
+    for (size_t k = 0; k < hst.slots.size(); ++k) hst.slots[k].slot = (int)k;
+    // tasks
+    std::vector<int> order(hst.slots.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return hst.slots[a].cell < hst.slots[b].cell; });
+    std::vector<SolveTask> tasks;
+    for (size_t k = 0; k < order.size();) {
+      SolveTask tk;
+      std::memset(&tk, 0, sizeof(tk));
+      tk.cell = hst.slots[order[k]].cell;
+      tk.nrhs = 0;
+      while (k < order.size() && hst.slots[order[k]].cell == tk.cell && tk.nrhs < kMaxRhs)
+        tk.slot[tk.nrhs++] = hst.slots[order[k++]].slot;
+      tasks.push_back(tk);
+      st.maxNrhs = std::max(st.maxNrhs, tk.nrhs);
+      const NdTree& tr = mFactors->tree(mFactors->cell(tk.cell).shape);
+      mSolveBytesPerPrecond += (double)tr.factorSize * S + 4.0 * tr.n * S * tk.nrhs;
+    }
+    mSolvesPerPrecond += (long long)hst.slots.size();
+    // groups: (d, cell)
+    std::vector<GroupDev> groups;
+    for (auto& s : hst.slots) {
+      bool found = false;
+      for (auto& g : groups)
+        if (g.d == s.d && g.cell == s.cell) {
+          LSW_REQUIRE(g.n < 4, LSW_ERR_ARG, "too many solves of one cell in a step");
+          g.slot[g.n++] = s.slot;
+          found = true;
+          break;
+        }
+      if (!found) {
+        GroupDev g;
+        std::memset(&g, 0, sizeof(g));
+        g.d = s.d;
+        g.cell = s.cell;
+        g.n = 1;
+        g.slot[0] = s.slot;
+        groups.push_back(g);
+      }
+    }
+    st.nSlots = (int)hst.slots.size();
+    st.nTasks = (int)tasks.size();
+    st.nGroups = (int)groups.size();
+    st.nCopies = (int)hst.copies.size();
+    LSW_REQUIRE(st.nSlots <= 65535 && st.nCopies <= 65535 * 8, LSW_ERR_CAPACITY, "board too large for one launch");
+    LSW_HIP(hipMalloc(&st.dSlots, std::max<size_t>(1, hst.slots.size()) * sizeof(SlotDev)));
+    LSW_HIP(hipMemcpy(st.dSlots, hst.slots.data(), hst.slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice));
+    LSW_HIP(hipMalloc(&st.dTasks, std::max<size_t>(1, tasks.size()) * sizeof(SolveTask)));
+    LSW_HIP(hipMemcpy(st.dTasks, tasks.data(), tasks.size() * sizeof(SolveTask), hipMemcpyHostToDevice));
+    LSW_HIP(hipMalloc(&st.dGroups, std::max<size_t>(1, groups.size()) * sizeof(GroupDev)));
+    LSW_HIP(hipMemcpy(st.dGroups, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice));
+    if (!hst.copies.empty()) {
+      LSW_HIP(hipMalloc(&st.dCopies, hst.copies.size() * sizeof(CopyOp)));
+      LSW_HIP(hipMemcpy(st.dCopies, hst.copies.data(), hst.copies.size() * sizeof(CopyOp), hipMemcpyHostToDevice));
+    }
+    mSteps.push_back(st);
+  }
+}
+
+void DDCheckerboard::allocBuffers() {
+  const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
+  mXs = mFactors->maxDofs();
+  mGs = std::max<long long>(mFactors->maxG(), 1);
+  LSW_HIP(hipMalloc(&dX, (size_t)mMaxSlots * mXs * es));
+  LSW_HIP(hipMalloc(&dR, (size_t)mMaxSlots * mXs * es));
+  LSW_HIP(hipMalloc(&dG, (size_t)mMaxSlots * mGs * es));
+  LSW_HIP(hipMalloc(&dT, (size_t)mTraceEntries * es));
+  LSW_HIP(hipMalloc(&dY, (size_t)4 * N() * sizeof(double2)));
+}
+
+void DDCheckerboard::factorize() {
+  LSW_REQUIRE(mFactors != nullptr, LSW_ERR_STATE, "setUp must be called before factorize");
+  if (!mFactors->factorized()) mFactors->factorize(mPrecision, mStream);
+}
+
+// ------------------------------------------------------------- operations
+
+void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s) const {
+  LSW_REQUIRE(mFactors && mFactors->factorized(), LSW_ERR_STATE, "applyPrecond before factorize");
+  if (!s) s = mStream;
+  const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
+  const long long n = N();
+  LSW_HIP(hipMemsetAsync(dY, 0, (size_t)4 * n * sizeof(double2), s));
+  LSW_HIP(hipMemsetAsync(dT, 0, (size_t)mTraceEntries * es, s));
+  const CellDev* dCells = mFactors->deviceCells();
+  const double2* dCoef = mFactors->deviceCoef();
+  const int tb = 256;
+  const unsigned gxN = (unsigned)std::min<long long>((mFactors->maxDofs() + tb - 1) / tb, 1024);
+  for (const auto& st : mSteps) {
+    if (mPrecision == LSW_FP64) {
+      hipLaunchKernelGGL(k_build_rhs<double2>, dim3(gxN, st.nSlots), dim3(tb), 0, s, dCells, st.dSlots, x,
+                         dColTrace, dRowTrace, mGy, dCoef, (const double2*)dT, (double2*)dX, mXs);
+    } else {
+      hipLaunchKernelGGL(k_build_rhs<float2>, dim3(gxN, st.nSlots), dim3(tb), 0, s, dCells, st.dSlots, x,
+                         dColTrace, dRowTrace, mGy, dCoef, (const float2*)dT, (float2*)dX, mXs);
+    }
+    mFactors->solve(st.dTasks, st.nTasks, st.maxNrhs, dX, dR, dG, mXs, mGs, s);
+    const unsigned gxT = (unsigned)((mTmax + 63) / 64);
+    if (mPrecision == LSW_FP64) {
+      hipLaunchKernelGGL(k_post<double2>, dim3(gxT, st.nSlots), dim3(64), 0, s, dCells, st.dSlots,
+                         (const double2*)dX, mXs, (double2*)dT);
+      hipLaunchKernelGGL(k_accumulate<double2>, dim3(gxN, st.nGroups), dim3(tb), 0, s, dCells, st.dGroups,
+                         (const double2*)dX, mXs, dY, n, mGy);
+      if (st.nCopies)
+        hipLaunchKernelGGL(k_copy_traces<double2>, dim3(gxT, st.nCopies), dim3(64), 0, s, st.dCopies,
+                           (double2*)dT, mTmax);
+    } else {
+      hipLaunchKernelGGL(k_post<float2>, dim3(gxT, st.nSlots), dim3(64), 0, s, dCells, st.dSlots,
+                         (const float2*)dX, mXs, (float2*)dT);
+      hipLaunchKernelGGL(k_accumulate<float2>, dim3(gxN, st.nGroups), dim3(tb), 0, s, dCells, st.dGroups,
+                         (const float2*)dX, mXs, dY, n, mGy);
+      if (st.nCopies)
+        hipLaunchKernelGGL(k_copy_traces<float2>, dim3(gxT, st.nCopies), dim3(64), 0, s, st.dCopies,
+                           (float2*)dT, mTmax);
+    }
+  }
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, s,
+                     dY, y, n);
+  LSW_HIP(hipGetLastError());
+}
+
+void DDCheckerboard::applyADev(const double2* x, double2* y, hipStream_t s) const {
+  LSW_REQUIRE(dGlobalCoef != nullptr, LSW_ERR_STATE, "DD has no device state");
+  if (!s) s = mStream;
+  const long long n = N();
+  hipLaunchKernelGGL(k_matvec, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s,
+                     dGlobalCoef, x, y, mGx, mGy);
+  LSW_HIP(hipGetLastError());
+}
+
+void DDCheckerboard::assembleRHSDev(const std::vector<DoubleVector>& srcs, double2* b, hipStream_t s) const {
+  LSW_REQUIRE(deviceReady(), LSW_ERR_STATE, "DD has no device state");
+  if (!s) s = mStream;
+  std::vector<double> xy;
+  for (auto& p : srcs) {
+    xy.push_back(p[0]);
+    xy.push_back(p[1]);
+  }
+  double* dsrc = nullptr;
+  LSW_HIP(hipMalloc(&dsrc, std::max<size_t>(1, xy.size()) * sizeof(double)));
+  LSW_HIP(hipMemcpyAsync(dsrc, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  const long long n = N();
+  hipLaunchKernelGGL(k_point_sources, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0,
+                     s, dsrc, (int)srcs.size(), mMeshSize, mGlobal.asmb.x0(), mGlobal.asmb.hx(), mGlobal.asmb.y0(),
+                     mGlobal.asmb.hy(), mGx, mGy, b);
+  LSW_HIP(hipGetLastError());
+  LSW_HIP(hipStreamSynchronize(s));
+  (void)hipFree(dsrc);
+}
+
+void DDCheckerboard::ownedGlobalIndices(int c, std::vector<long long>& idx) const {
+  const CellGeom& cg = mCells[c];
+  idx.clear();
+  for (int i = cg.vlr[0][2]; i < cg.vlr[0][5]; ++i)
+    for (int j = cg.vud[0][2]; j < cg.vud[0][5]; ++j)
+      idx.push_back((long long)(i + cg.ox) * mGy + (j + cg.oy));
+}
+
+void DDCheckerboard::gatherOwned(const std::vector<Complex>& g, std::vector<ComplexVector>& res) const {
+  res.resize(mCells.size());
+  std::vector<long long> idx;
+  for (size_t c = 0; c < mCells.size(); ++c) {
+    ownedGlobalIndices((int)c, idx);
+    res[c].resize(idx.size());
+    for (size_t k = 0; k < idx.size(); ++k) res[c][k] = g[idx[k]];
+  }
+}
+
+void DDCheckerboard::scatterOwned(const std::vector<ComplexVector>& x, std::vector<Complex>& g) const {
+  LSW_REQUIRE(x.size() == mCells.size(), LSW_ERR_ARG, "wrong number of cell vectors");
+  g.assign((size_t)N(), 0.0);
+  std::vector<long long> idx;
+  for (size_t c = 0; c < mCells.size(); ++c) {
+    ownedGlobalIndices((int)c, idx);
+    LSW_REQUIRE(x[c].size() == idx.size(), LSW_ERR_ARG, "wrong owned vector length");
+    for (size_t k = 0; k < idx.size(); ++k) g[idx[k]] = x[c][k];
+  }
+}
+
+namespace {
+struct DevBuf {
+  double2* p = nullptr;
+  explicit DevBuf(long long n) { LSW_HIP(hipMalloc(&p, std::max<long long>(n, 1) * sizeof(double2))); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
+void DDCheckerboard::applyPrecond(const std::vector<ComplexVector>& x, std::vector<ComplexVector>& res) const {
+  std::vector<Complex> g;
+  scatterOwned(x, g);
+  DevBuf dx(N()), dy(N());
+  LSW_HIP(hipMemcpy(dx.p, g.data(), N() * sizeof(double2), hipMemcpyHostToDevice));
+  applyPrecondDev(dx.p, dy.p, mStream);
+  LSW_HIP(hipStreamSynchronize(mStream));
+  LSW_HIP(hipMemcpy(g.data(), dy.p, N() * sizeof(double2), hipMemcpyDeviceToHost));
+  gatherOwned(g, res);
+}
+
+void DDCheckerboard::applySystemMatrix(const std::vector<ComplexVector>& x, std::vector<ComplexVector>& res) const {
+  std::vector<Complex> g;
+  scatterOwned(x, g);
+  DevBuf dx(N()), dy(N());
+  LSW_HIP(hipMemcpy(dx.p, g.data(), N() * sizeof(double2), hipMemcpyHostToDevice));
+  applyADev(dx.p, dy.p, mStream);
+  LSW_HIP(hipStreamSynchronize(mStream));
+  LSW_HIP(hipMemcpy(g.data(), dy.p, N() * sizeof(double2), hipMemcpyDeviceToHost));
+  gatherOwned(g, res);
+}
+
+void DDCheckerboard::assembleRHS(const RHS* rhs, std::vector<ComplexVector>& res) const {
+  const auto* sp = dynamic_cast<const RHSSrcPts*>(rhs);
+  LSW_REQUIRE(sp != nullptr, LSW_ERR_ARG, "assembleRHS supports RHSSrcPts");
+  DevBuf db(N());
+  assembleRHSDev(sp->getSrcPts(), db.p, mStream);
+  std::vector<Complex> g((size_t)N());
+  LSW_HIP(hipMemcpy(g.data(), db.p, N() * sizeof(double2), hipMemcpyDeviceToHost));
+  gatherOwned(g, res);
+}
+
+}  // namespace lsw

@@ -1,0 +1,67 @@
+// Device-side descriptors and HIP helpers shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "lsw_common.h"
+#include "lsw_nd.h"
+
+namespace lsw {
+
+#define LSW_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess)                                                               \
+      throw ::lsw::Error(LSW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr int kMaxHeight = 40;
+
+// One local problem (a checkerboard cell, or the single matrix of a
+// LocalLinearSolverHIP) as the kernels see it.
+struct CellDev {
+  int nx, ny, ox, oy;      // local DOF grid, offset of local (0,0) in the global grid
+  int vlr[2][8];           // LR 7-set starts [shifted][set]; [.][7] = nx
+  int vud[2][8];           // UD 7-set starts; [.][7] = ny
+  int shape, pad;
+  long long coefOff;       // 5*nx*ny complex stencil entries (dir-major)
+  long long factorOff;     // factor blob offset (complex entries)
+  long long schurOff;      // Schur workspace offset (factorisation only)
+  long long mOff;          // squared slowness per DOF
+  double x0, y0, hx, hy;   // DOF (i,j) at (x0 + (i+1) hx, y0 + (j+1) hy)
+  double intDom[2][2], compDom[2][2];
+  double C, omega;
+};
+
+// ND structure of one shape class (device pointers).
+struct ShapeDev {
+  const NdNodeDev* nodes;
+  const int* E;
+  const int* B;
+  const int* cmap;
+  const int* up;
+  const AsmEntry* asmList;
+  const SolveItem* fwdItems;
+  const SolveItem* bwdItems;
+  int fwdStart[kMaxHeight], fwdCount[kMaxHeight];
+  int bwdStart[kMaxHeight], bwdCount[kMaxHeight];
+  int maxHeight, nodeCount;
+};
+
+// A batched solve task: up to kMaxRhs right-hand sides of one cell.
+constexpr int kMaxRhs = 4;
+struct SolveTask {
+  int cell;
+  int nrhs;
+  int slot[kMaxRhs];
+};
+
+// One front of the multifrontal factorisation.
+struct FrontTask {
+  int cell, node;
+  long long wsOff;  // dense workspace offset (complex entries)
+};
+
+}  // namespace lsw

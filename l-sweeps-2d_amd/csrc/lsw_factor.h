@@ -1,0 +1,91 @@
+// FactorSet: a batch of structured-grid local problems (5-point operator),
+// assembled, factorised and solved on the GPU.
+//
+// This is the LocalLinearSolver of the MI355X path (the reference uses one
+// Eigen SparseLU per subdomain, src/LocalLinearSolverEigenSparseLU.cpp:33-76):
+//   * assembly      -- one thread per DOF evaluates the cubic-PML stencil;
+//   * factorisation -- multifrontal LU over a geometric nested-dissection
+//                      tree, one workgroup per dense front, fronts of equal
+//                      height in one launch; partial pivoting inside F11;
+//   * solve         -- per tree height one launch of batched dense GEMVs
+//                      (forward: W = F21 F11^{-1}; backward:
+//                      K = [F11^{-1} | -F11^{-1} F12]) over many cells and
+//                      right-hand sides at once.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "lsw_device.h"
+#include "lsw_nd.h"
+
+namespace lsw {
+
+class FactorSet {
+ public:
+  FactorSet() = default;
+  ~FactorSet();
+  FactorSet(const FactorSet&) = delete;
+  FactorSet& operator=(const FactorSet&) = delete;
+
+  // Add a cell described by its grid and PML geometry plus squared slowness
+  // per DOF; stencils are assembled on the device.
+  int addCell(const CellDev& c, const std::vector<double>& m);
+  // Add a cell whose stencil (5*nx*ny complex, dir-major) is given on the host.
+  int addCellWithStencil(const CellDev& c, const std::vector<double2>& coef);
+
+  // Build ND trees, upload descriptors, allocate, assemble stencils.
+  void setup(hipStream_t s);
+  // Multifrontal factorisation (FP64 on the device); FP32 copies of the
+  // factors are made when precision == LSW_FP32.
+  void factorize(int precision, hipStream_t s);
+  bool factorized() const { return mFactorized; }
+  int precision() const { return mPrecision; }
+
+  // Solve every task (dtasks: device array of ntasks SolveTask).  X holds the
+  // right-hand sides on entry and the solutions on exit (slot-major, stride
+  // xStride complex entries); R and G are scratch with the same slot layout
+  // (G stride gStride).  Element type follows precision(): double2 or float2.
+  // maxNrhs: the largest SolveTask::nrhs among the tasks (1..kMaxRhs).
+  void solve(const SolveTask* dtasks, int ntasks, int maxNrhs, void* X, void* R, void* G,
+             long long xStride, long long gStride, hipStream_t s) const;
+
+  int numCells() const { return (int)mCells.size(); }
+  const CellDev& cell(int i) const { return mCells[i]; }
+  const CellDev* deviceCells() const { return dCells; }
+  const double2* deviceCoef() const { return dCoef; }
+  long long maxDofs() const { return mMaxDofs; }
+  long long maxG() const { return mMaxG; }
+  long long factorEntries() const { return mFactorEntries; }
+  long long factorBytes() const;
+  const NdTree& tree(int shape) const { return *mTrees[shape]; }
+  int numShapes() const { return (int)mTrees.size(); }
+  int maxHeight() const { return mMaxHeight; }
+  int maxFront() const { return mMaxFront; }
+  // pivot/breakdown status of the last factorisation (0 = ok)
+  int lastInfo() const { return mInfo; }
+
+ private:
+  std::vector<CellDev> mCells;
+  std::vector<std::vector<double>> mM;
+  std::vector<std::vector<double2>> mCoefHost;
+  std::vector<std::unique_ptr<NdTree>> mTrees;
+  std::map<std::pair<int, int>, int> mShapeOf;
+  CellDev* dCells = nullptr;
+  ShapeDev* dShapes = nullptr;
+  std::vector<void*> mShapeAllocs;
+  double2* dCoef = nullptr;
+  double* dM = nullptr;
+  double2* dFactor = nullptr;
+  float2* dFactor32 = nullptr;
+  long long mMaxDofs = 0, mMaxG = 0, mFactorEntries = 0;
+  int mMaxHeight = 0, mMaxFront = 0;
+  int mPrecision = LSW_FP64;
+  bool mFactorized = false;
+  int mInfo = 0;
+};
+
+}  // namespace lsw

@@ -1,0 +1,75 @@
+// Nested-dissection symbolic analysis of the 5-point operator on an nx x ny
+// grid (DOF index i*ny + j, y fastest -- the numbering of
+// AssemblerHelmholtz2DFD::getDOFPts, src/AssemblerHelmholtz2DFD.cpp:147-181).
+//
+// The tree is geometric: a region is split by one grid line across its longer
+// side until it holds at most kLeafDofs DOFs.  Node p eliminates E_p (its
+// separator line, or the whole region for a leaf); its boundary B_p is the
+// set of grid points adjacent to the region, which all lie on ancestor
+// separators.  The front of p is F_p = [E_p, B_p].
+//
+// Factor storage per node (complex, column-major):
+//   W_p = F21 F11^{-1}                    b x e   (forward solve)
+//   K_p = [F11^{-1} | -F11^{-1} F12]      e x (e+b) (backward solve)
+// so both triangular solves become dense matrix-vector products.
+#pragma once
+
+#include <vector>
+
+namespace lsw {
+
+constexpr int kLeafDofs = 16;
+constexpr int kSolveRows = 64;  // rows per solve work item (one wavefront)
+
+struct NdNode {
+  int e = 0, b = 0;
+  int eOff = 0, bOff = 0;   // offsets into E / B pools
+  int c1 = -1, c2 = -1, parent = -1;
+  int height = 0;
+  int mapOff = 0;           // m1 then m2, each e+b ints: front pos -> child g index or -1
+  int upOff = 0;            // b ints: position of B_p[t] in the parent's front
+  int gOff = 0;             // offset of g_p (b entries) in a task's update buffer
+  int asmOff = 0, asmCnt = 0;
+  long long fwdOff = 0, bwdOff = 0;  // W_p and K_p offsets in the cell's factor blob
+  long long sOff = 0;                // Schur complement (b*b) offset in a cell's workspace
+  long long wsOff = 0;               // dense front workspace offset (within a height batch)
+};
+
+// Device-side mirror (POD, 16-byte aligned fields first).
+struct NdNodeDev {
+  long long fwdOff, bwdOff, sOff, wsOff;
+  int e, b, eOff, bOff, c1, c2, mapOff, upOff, gOff, asmOff, asmCnt, pad;
+};
+
+// One original matrix entry scattered into a front: F[fr][fc] += A[dof][dir].
+struct AsmEntry {
+  int fr, fc, dof, dir;
+};
+
+// Solve work item: rows [r0, r0+rows) of node `node` (forward: rows of W,
+// backward: rows of K).
+struct SolveItem {
+  int node, r0, rows, pad;
+};
+
+struct NdTree {
+  int nx = 0, ny = 0, n = 0;
+  std::vector<NdNode> nodes;
+  std::vector<int> E, B, cmap, up;
+  std::vector<AsmEntry> asmList;
+  int maxHeight = 0;
+  long long factorSize = 0;  // complex entries per cell
+  long long gSize = 0;       // update-vector entries per right-hand side
+  long long schurSize = 0;   // Schur workspace entries per cell
+  int maxFront = 0;          // max e+b
+  std::vector<std::vector<int>> byHeight;
+  std::vector<std::vector<SolveItem>> fwdItems, bwdItems;  // per height
+
+  void build(int nx_, int ny_);
+  std::vector<NdNodeDev> deviceNodes() const;
+
+ private:
+  int rec(int i0, int i1, int j0, int j1, int parent);
+};
+
+}  // namespace lsw

@@ -1,0 +1,60 @@
+// LocalLinearSolver (include/LocalLinearSolver.h:8-41) and its MI355X
+// implementation LocalLinearSolverHIP, the drop-in beside the reference's
+// LocalLinearSolverEigenSparseLU (include/LocalLinearSolverEigenSparseLU.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+
+#include "lsw_common.h"
+#include "lsw_factor.h"
+
+namespace lsw {
+
+class LocalLinearSolver {
+ public:
+  virtual ~LocalLinearSolver() = default;
+  virtual LocalLinearSolver* copy() const = 0;
+  // The matrix is borrowed and must outlive the solver
+  // (src/SubdomainStructured.cpp:28-29,549-562).
+  virtual void setMatrix(ComplexSparseMatrix* A) = 0;
+  virtual void factorize() = 0;
+  virtual void solve(ComplexVector& b, ComplexVector& res) const = 0;
+};
+
+// Multifrontal LU of a 5-point structured-grid matrix on the GPU.  The grid
+// (nx, ny with index i*ny+j) is recovered from the sparsity pattern; other
+// patterns are rejected with LSW_ERR_UNSUPPORTED.  Copies made before
+// factorize() share nothing (the reference copies unfactorised prototypes,
+// src/LocalLinearSolverEigenSparseLU.cpp:9-12).
+class LocalLinearSolverHIP : public LocalLinearSolver {
+ public:
+  explicit LocalLinearSolverHIP(int precision = LSW_FP64) : mPrecision(precision) {}
+  ~LocalLinearSolverHIP() override;
+  LocalLinearSolver* copy() const override { return new LocalLinearSolverHIP(mPrecision); }
+  void setMatrix(ComplexSparseMatrix* A) override;
+  void factorize() override;
+  void solve(ComplexVector& b, ComplexVector& res) const override;
+  // Multiple right-hand sides: b and x are n x nrhs column-major (ld = n).
+  void solveMany(const Complex* b, Complex* x, int nrhs) const;
+  int precision() const { return mPrecision; }
+  void setPrecision(int p) { mPrecision = p; }
+  int gridNx() const { return mNx; }
+  int gridNy() const { return mNy; }
+  const FactorSet* factorSet() const { return mFS.get(); }
+
+ private:
+  int mPrecision;
+  ComplexSparseMatrix* mA = nullptr;
+  int mNx = 0, mNy = 0;
+  std::unique_ptr<FactorSet> mFS;
+  hipStream_t mStream = nullptr;
+  mutable void* dX = nullptr;
+  mutable void* dR = nullptr;
+  mutable void* dG = nullptr;
+  mutable SolveTask* dTasks = nullptr;
+  mutable int mCap = 0;
+};
+
+}  // namespace lsw

@@ -1,0 +1,124 @@
+// LocalLinearSolverHIP (see lsw_solver.h).
+#include "lsw_solver.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace lsw {
+
+LocalLinearSolverHIP::~LocalLinearSolverHIP() {
+  for (void* p : {dX, dR, dG}) if (p) (void)hipFree(p);
+  if (dTasks) (void)hipFree(dTasks);
+  mFS.reset();
+  if (mStream) (void)hipStreamDestroy(mStream);
+}
+
+// Recover the nx x ny grid of a 5-point matrix with index i*ny+j and check the
+// whole pattern (LocalLinearSolverEigenSparseLU accepts any sparse matrix; the
+// MI355X solver is specialised to the structured operators the assemblers
+// produce).
+void LocalLinearSolverHIP::setMatrix(ComplexSparseMatrix* A) {
+  LSW_REQUIRE(A != nullptr, LSW_ERR_ARG, "setMatrix: null matrix");
+  LSW_REQUIRE(A->rows == A->cols && A->rows > 0, LSW_ERR_ARG, "setMatrix: matrix must be square");
+  LSW_REQUIRE((int)A->colptr.size() == A->cols + 1, LSW_ERR_ARG, "setMatrix: bad column pointer");
+  const int N = A->rows;
+  int ny = N;  // 1 x N grid unless column 0 couples to a far row
+  for (int k = A->colptr[0]; k < A->colptr[1]; ++k) {
+    const int r = A->rowind[k];
+    if (r > 1) ny = std::min(ny == N ? r : ny, r);
+  }
+  LSW_REQUIRE(N % ny == 0, LSW_ERR_UNSUPPORTED, "matrix is not a 5-point structured-grid operator");
+  const int nx = N / ny;
+  for (int d = 0; d < N; ++d) {
+    const int i = d / ny, j = d % ny;
+    for (int k = A->colptr[d]; k < A->colptr[d + 1]; ++k) {
+      const int q = A->rowind[k];
+      LSW_REQUIRE(q >= 0 && q < N, LSW_ERR_ARG, "setMatrix: row index out of range");
+      const int diff = d - q;
+      const int qi = q / ny, qj = q % ny;
+      const bool ok = diff == 0 || ((diff == 1 || diff == -1) && qi == i && std::abs(qj - j) == 1) ||
+                      ((diff == ny || diff == -ny) && qj == j && ny < N);
+      LSW_REQUIRE(ok, LSW_ERR_UNSUPPORTED, "matrix is not a 5-point structured-grid operator");
+    }
+  }
+  mA = A;
+  mNx = nx;
+  mNy = ny;
+  mFS.reset();
+}
+
+void LocalLinearSolverHIP::factorize() {
+  LSW_REQUIRE(mA != nullptr, LSW_ERR_STATE, "factorize called before setMatrix");
+  if (mFS && mFS->factorized()) return;  // idempotent (src/LocalLinearSolverEigenSparseLU.cpp:42-45)
+  if (!mStream) LSW_HIP(hipStreamCreate(&mStream));
+  const int N = mA->rows, ny = mNy;
+  std::vector<double2> coef((size_t)5 * N, make_double2(0.0, 0.0));
+  for (int d = 0; d < N; ++d)
+    for (int k = mA->colptr[d]; k < mA->colptr[d + 1]; ++k) {
+      const int q = mA->rowind[k];
+      const int diff = d - q;  // entry A[q][d]: coefficient of row q towards d
+      const int dir = diff == 0 ? 0 : diff == -ny ? 1 : diff == ny ? 2 : diff == -1 ? 3 : 4;
+      coef[(size_t)dir * N + q] = make_double2(mA->vals[k].real(), mA->vals[k].imag());
+    }
+  CellDev c;
+  std::memset(&c, 0, sizeof(c));
+  c.nx = mNx;
+  c.ny = mNy;
+  mFS = std::make_unique<FactorSet>();
+  mFS->addCellWithStencil(c, coef);
+  mFS->setup(mStream);
+  mFS->factorize(mPrecision, mStream);
+}
+
+void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) const {
+  LSW_REQUIRE(mFS && mFS->factorized(), LSW_ERR_STATE, "Solve called before factorization");
+  LSW_REQUIRE(nrhs >= 0, LSW_ERR_ARG, "negative right-hand-side count");
+  if (nrhs == 0) return;
+  const long long N = mA->rows;
+  const long long G = std::max<long long>(mFS->maxG(), 1);
+  const size_t es = mPrecision == LSW_FP64 ? sizeof(double2) : sizeof(float2);
+  const int ntasks = (nrhs + kMaxRhs - 1) / kMaxRhs;
+  if (nrhs > mCap) {
+    for (void* p : {dX, dR, dG}) if (p) (void)hipFree(p);
+    if (dTasks) (void)hipFree(dTasks);
+    LSW_HIP(hipMalloc(&dX, (size_t)nrhs * N * es));
+    LSW_HIP(hipMalloc(&dR, (size_t)nrhs * N * es));
+    LSW_HIP(hipMalloc(&dG, (size_t)nrhs * G * es));
+    LSW_HIP(hipMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
+    mCap = nrhs;
+  }
+  std::vector<SolveTask> tasks(ntasks);
+  for (int t = 0; t < ntasks; ++t) {
+    std::memset(&tasks[t], 0, sizeof(SolveTask));
+    tasks[t].cell = 0;
+    tasks[t].nrhs = std::min(kMaxRhs, nrhs - t * kMaxRhs);
+    for (int r = 0; r < tasks[t].nrhs; ++r) tasks[t].slot[r] = t * kMaxRhs + r;
+  }
+  LSW_HIP(hipMemcpyAsync(dTasks, tasks.data(), ntasks * sizeof(SolveTask), hipMemcpyHostToDevice, mStream));
+  if (mPrecision == LSW_FP64) {
+    LSW_HIP(hipMemcpyAsync(dX, b, (size_t)nrhs * N * sizeof(double2), hipMemcpyHostToDevice, mStream));
+  } else {
+    std::vector<float2> bf((size_t)nrhs * N);
+    for (size_t k = 0; k < bf.size(); ++k) bf[k] = make_float2((float)b[k].real(), (float)b[k].imag());
+    LSW_HIP(hipMemcpyAsync(dX, bf.data(), bf.size() * sizeof(float2), hipMemcpyHostToDevice, mStream));
+    LSW_HIP(hipStreamSynchronize(mStream));
+  }
+  mFS->solve(dTasks, ntasks, std::min(nrhs, kMaxRhs), dX, dR, dG, N, G, mStream);
+  if (mPrecision == LSW_FP64) {
+    LSW_HIP(hipMemcpyAsync(x, dX, (size_t)nrhs * N * sizeof(double2), hipMemcpyDeviceToHost, mStream));
+    LSW_HIP(hipStreamSynchronize(mStream));
+  } else {
+    std::vector<float2> xf((size_t)nrhs * N);
+    LSW_HIP(hipMemcpyAsync(xf.data(), dX, xf.size() * sizeof(float2), hipMemcpyDeviceToHost, mStream));
+    LSW_HIP(hipStreamSynchronize(mStream));
+    for (size_t k = 0; k < xf.size(); ++k) x[k] = Complex(xf[k].x, xf[k].y);
+  }
+}
+
+void LocalLinearSolverHIP::solve(ComplexVector& b, ComplexVector& res) const {
+  LSW_REQUIRE(mA && (int)b.size() == mA->rows, LSW_ERR_ARG, "solve: right-hand side has the wrong length");
+  res.resize(b.size());
+  solveMany(b.data(), res.data(), 1);
+}
+
+}  // namespace lsw

@@ -1,0 +1,288 @@
+"""Python binding of the MI355X L-sweeps library (ctypes over include/lsweeps_hip.h).
+
+The HIP library ``liblsweeps_hip.so`` must have been built in-tree
+(``__graft_entry__.build()`` or ``python l-sweeps-2d_amd/lsweeps_amd/build.py``).
+There is no CPU fallback: every call goes to the HIP library, and importing
+this module raises if the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblsweeps_hip.so")
+if not os.path.exists(_LIB_PATH):
+    raise ImportError(f"HIP library not built: {_LIB_PATH} (run __graft_entry__.build())")
+_lib = C.CDLL(_LIB_PATH)
+
+LSW_PREC_FP64 = 0
+LSW_PREC_FP32 = 1
+ERRORS = {1: "ARG", 2: "HIP", 3: "STATE", 4: "NUMERIC", 5: "CAPACITY", 6: "UNSUPPORTED", 7: "INTERNAL"}
+
+EXPORTED = [
+    "lsw_last_error", "lsw_version", "lsw_device_count",
+    "lsw_lu_create", "lsw_lu_set_csc_z", "lsw_lu_factorize", "lsw_lu_solve_z", "lsw_lu_info",
+    "lsw_lu_destroy",
+    "lsw_dd_create", "lsw_dd_create_host", "lsw_dd_destroy", "lsw_dd_factorize", "lsw_dd_sizes",
+    "lsw_dd_global_coords", "lsw_dd_cell_info", "lsw_dd_assemble_rhs", "lsw_dd_assemble_rhs_dev",
+    "lsw_dd_apply_precond", "lsw_dd_apply_precond_dev", "lsw_dd_apply_a", "lsw_dd_apply_a_dev",
+    "lsw_dd_stats", "lsw_dd_sync",
+    "lsw_gmres_solve", "lsw_gmres_solve_dev",
+]
+
+
+class LswError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{ERRORS.get(code, code)}] {msg}")
+        self.code = code
+
+
+class DDParams(C.Structure):
+    _fields_ = [
+        ("domain", C.c_double * 4),
+        ("ncells_x", C.c_int), ("ncells_y", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("omega", C.c_double),
+        ("min_wavelengths_in_pml", C.c_double),
+        ("order", C.c_int),
+        ("precision", C.c_int),
+        ("medium_kind", C.c_int),
+        ("velocity", C.POINTER(C.c_float)),
+        ("vel_rows", C.c_int), ("vel_cols", C.c_int),
+        ("vel_domain", C.c_double * 4),
+    ]
+
+
+class GmresParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
+                ("restart", C.c_int), ("verbose", C.c_int)]
+
+
+class GmresResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("precond_applies", C.c_int), ("matvecs", C.c_int),
+                ("estimate", C.c_double), ("true_residual", C.c_double)]
+
+
+_lib.lsw_last_error.restype = C.c_char_p
+_lib.lsw_version.restype = C.c_char_p
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+
+
+def _sig(name, *args):
+    f = getattr(_lib, name)
+    f.argtypes = list(args)
+    f.restype = C.c_int
+
+
+_sig("lsw_device_count", C.POINTER(C.c_int))
+_sig("lsw_lu_create", C.c_int, C.POINTER(_P))
+_sig("lsw_lu_set_csc_z", _P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _D)
+_sig("lsw_lu_factorize", _P)
+_sig("lsw_lu_solve_z", _P, C.c_int, _D, _D)
+_sig("lsw_lu_info", _P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+     C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("lsw_lu_destroy", _P)
+_sig("lsw_dd_create", C.POINTER(DDParams), C.POINTER(_P))
+_sig("lsw_dd_create_host", C.POINTER(DDParams), C.POINTER(_P))
+_sig("lsw_dd_destroy", _P)
+_sig("lsw_dd_factorize", _P)
+_sig("lsw_dd_sizes", _P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _D)
+_sig("lsw_dd_global_coords", _P, _D, _D)
+_sig("lsw_dd_cell_info", _P, C.c_int, C.POINTER(C.c_int))
+_sig("lsw_dd_assemble_rhs", _P, C.c_int, _D, _D)
+_sig("lsw_dd_assemble_rhs_dev", _P, C.c_int, _D, _P, _P)
+_sig("lsw_dd_apply_precond", _P, _D, _D)
+_sig("lsw_dd_apply_precond_dev", _P, _P, _P, _P)
+_sig("lsw_dd_apply_a", _P, _D, _D)
+_sig("lsw_dd_apply_a_dev", _P, _P, _P, _P)
+_sig("lsw_dd_stats", _P, C.POINTER(C.c_longlong), _D, C.POINTER(C.c_longlong))
+_sig("lsw_dd_sync", _P)
+_sig("lsw_gmres_solve", _P, C.POINTER(GmresParams), _D, _D, C.POINTER(GmresResult), _D, C.c_int)
+_sig("lsw_gmres_solve_dev", _P, C.POINTER(GmresParams), _P, _P, _P, C.POINTER(GmresResult))
+
+
+def _check(rc):
+    if rc != 0:
+        raise LswError(rc, _lib.lsw_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def lib():
+    return _lib
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(_lib.lsw_device_count(C.byref(n)))
+    return n.value
+
+
+# --------------------------------------------------------------------------
+class LocalLinearSolverHIP:
+    """setMatrix / factorize / solve of one 5-point structured-grid matrix."""
+
+    def __init__(self, precision=LSW_PREC_FP64):
+        self._h = _P()
+        _check(_lib.lsw_lu_create(precision, C.byref(self._h)))
+
+    def set_matrix(self, A):
+        import scipy.sparse as sp
+        A = sp.csc_matrix(A)
+        A.sort_indices()
+        n = A.shape[0]
+        self.n = n
+        colptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        rowind = np.ascontiguousarray(A.indices, dtype=np.int32)
+        vals = np.ascontiguousarray(A.data, dtype=np.complex128).view(np.float64)
+        _check(_lib.lsw_lu_set_csc_z(self._h, n, int(A.nnz), colptr.ctypes.data_as(C.POINTER(C.c_int)),
+                                     rowind.ctypes.data_as(C.POINTER(C.c_int)), _dptr(vals)))
+
+    def factorize(self):
+        _check(_lib.lsw_lu_factorize(self._h))
+
+    def solve(self, b):
+        b = np.asarray(b, dtype=np.complex128)
+        one = b.ndim == 1
+        # n x nrhs column-major == C-contiguous (nrhs, n)
+        Bt = np.ascontiguousarray(b.reshape(self.n, -1).T)
+        Xt = np.zeros_like(Bt)
+        _check(_lib.lsw_lu_solve_z(self._h, Bt.shape[0], _dptr(Bt.view(np.float64)),
+                                   _dptr(Xt.view(np.float64))))
+        return Xt[0] if one else Xt.T
+
+    def info(self):
+        nx, ny, nodes, h = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        fe = C.c_longlong()
+        _check(_lib.lsw_lu_info(self._h, C.byref(nx), C.byref(ny), C.byref(fe), C.byref(nodes), C.byref(h)))
+        return dict(nx=nx.value, ny=ny.value, factor_entries=fe.value, nodes=nodes.value, height=h.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lsw_lu_destroy(self._h)
+            self._h = None
+
+
+# --------------------------------------------------------------------------
+def dd_params(domain=((0.0, 1.0), (0.0, 1.0)), ncells=(4, 4), nels=(9, 9), omega=2 * np.pi * 4.1,
+              min_wl_in_pml=2.0, precision=LSW_PREC_FP64, velocity=None, vel_domain=None, order=2):
+    p = DDParams()
+    p.domain[:] = [domain[0][0], domain[0][1], domain[1][0], domain[1][1]]
+    p.ncells_x, p.ncells_y = int(ncells[0]), int(ncells[1])
+    p.nx, p.ny = int(nels[0]), int(nels[1])
+    p.omega = float(omega)
+    p.min_wavelengths_in_pml = float(min_wl_in_pml)
+    p.order = order
+    p.precision = precision
+    if velocity is None:
+        p.medium_kind = 0
+        p._vel_keep = None
+    else:
+        v = np.ascontiguousarray(velocity, dtype=np.float32)
+        p.medium_kind = 1
+        p._vel_keep = v
+        p.velocity = v.ctypes.data_as(C.POINTER(C.c_float))
+        p.vel_rows, p.vel_cols = v.shape
+        vd = vel_domain if vel_domain is not None else domain
+        p.vel_domain[:] = [vd[0][0], vd[0][1], vd[1][0], vd[1][1]]
+    return p
+
+
+class DDCheckerboard:
+    """A checkerboard DD with every cell resident on the GPU."""
+
+    def __init__(self, params, device=True):
+        self.params = params
+        self._h = _P()
+        fn = _lib.lsw_dd_create if device else _lib.lsw_dd_create_host
+        _check(fn(C.byref(params), C.byref(self._h)))
+        gx, gy, nc = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        _check(_lib.lsw_dd_sizes(self._h, C.byref(gx), C.byref(gy), C.byref(nc), C.byref(ms)))
+        self.Gx, self.Gy, self.ncells, self.mesh_size = gx.value, gy.value, nc.value, ms.value
+        self.N = self.Gx * self.Gy
+
+    @property
+    def handle(self):
+        return self._h
+
+    def coords(self):
+        xs = np.zeros(self.Gx)
+        ys = np.zeros(self.Gy)
+        _check(_lib.lsw_dd_global_coords(self._h, _dptr(xs), _dptr(ys)))
+        return xs, ys
+
+    def cell_info(self, c):
+        info = (C.c_int * 38)()
+        _check(_lib.lsw_dd_cell_info(self._h, c, info))
+        a = list(info)
+        return dict(I=a[0], J=a[1], nx=a[2], ny=a[3], ox=a[4], oy=a[5],
+                    vlr=[a[6:14], a[14:22]], vud=[a[22:30], a[30:38]])
+
+    def factorize(self):
+        _check(_lib.lsw_dd_factorize(self._h))
+
+    def assemble_rhs(self, sources):
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 2))
+        b = np.zeros(self.N, dtype=np.complex128)
+        _check(_lib.lsw_dd_assemble_rhs(self._h, src.shape[0], _dptr(src), _dptr(b.view(np.float64))))
+        return b
+
+    def apply_precond(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        _check(_lib.lsw_dd_apply_precond(self._h, _dptr(x.view(np.float64)), _dptr(y.view(np.float64))))
+        return y
+
+    def apply_a(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        _check(_lib.lsw_dd_apply_a(self._h, _dptr(x.view(np.float64)), _dptr(y.view(np.float64))))
+        return y
+
+    def stats(self):
+        s, fb = C.c_longlong(), C.c_longlong()
+        b = C.c_double()
+        _check(_lib.lsw_dd_stats(self._h, C.byref(s), C.byref(b), C.byref(fb)))
+        return dict(solves_per_precond=s.value, solve_bytes_per_precond=b.value, factor_bytes=fb.value)
+
+    def gmres(self, b, mode=0, tol=1e-8, max_iter=30, restart=30, verbose=False):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros_like(b)
+        p = GmresParams(mode, tol, max_iter, restart, int(verbose))
+        r = GmresResult()
+        log = np.zeros(max_iter + 2 + (max_iter // max(restart, 1) + 2) * 2)
+        _check(_lib.lsw_gmres_solve(self._h, C.byref(p), _dptr(b.view(np.float64)),
+                                    _dptr(x.view(np.float64)), C.byref(r), _dptr(log), len(log)))
+        n = r.iterations + 1 if mode == 0 else len(log)
+        return x, dict(iterations=r.iterations, precond_applies=r.precond_applies, matvecs=r.matvecs,
+                       estimate=r.estimate, true_residual=r.true_residual, log=log[:n].copy())
+
+    # device-pointer variants (pointers are integers, e.g. torch tensor.data_ptr())
+    def apply_precond_dev(self, x_ptr, y_ptr, stream=0):
+        _check(_lib.lsw_dd_apply_precond_dev(self._h, x_ptr, y_ptr, stream or None))
+
+    def apply_a_dev(self, x_ptr, y_ptr, stream=0):
+        _check(_lib.lsw_dd_apply_a_dev(self._h, x_ptr, y_ptr, stream or None))
+
+    def assemble_rhs_dev(self, sources, b_ptr, stream=0):
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 2))
+        _check(_lib.lsw_dd_assemble_rhs_dev(self._h, src.shape[0], _dptr(src), b_ptr, stream or None))
+
+    def gmres_dev(self, b_ptr, x_ptr, mode=0, tol=1e-8, max_iter=30, restart=30, stream=0):
+        p = GmresParams(mode, tol, max_iter, restart, 0)
+        r = GmresResult()
+        _check(_lib.lsw_gmres_solve_dev(self._h, C.byref(p), b_ptr, x_ptr, stream or None, C.byref(r)))
+        return dict(iterations=r.iterations, precond_applies=r.precond_applies, matvecs=r.matvecs,
+                    estimate=r.estimate, true_residual=r.true_residual)
+
+    def sync(self):
+        _check(_lib.lsw_dd_sync(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lsw_dd_destroy(self._h)
+            self._h = None

@@ -498,13 +498,11 @@ class Checkerboard:
         if ratio - pml_els > 0.5:
             pml_els += 1
         self.pml_els = pml_els
-        self.Gx = n1 + 2 * pml_els - 1
-        self.Gy = n2 + 2 * pml_els - 1
-        self.N = self.Gx * self.Gy
-        # global DOF coordinates (a 1x1 DD's local grid, src/AssemblerHelmholtz2DFD.cpp:166-177)
+        # global DOF grid = the local grid of a 1x1 DD (src/AssemblerHelmholtz2DFD.cpp:166-177)
         g = Cell(0, 0, dom, [n1, n2], [0.0] * 10, [0.0] * 10, self.pml_width_in,
                  self.omega, self.C, self.medium)
-        assert g.nx == self.Gx and g.ny == self.Gy
+        self.Gx, self.Gy = g.nx, g.ny
+        self.N = self.Gx * self.Gy
         self.gcell = g
         self.gxs, self.gys = g.xs, g.ys
         # local -> global offsets (grid-aligned)

@@ -1,0 +1,72 @@
+"""The DD preconditioner, operator and GMRES on the GPU against the oracle."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle.gmres import gmres_reference
+from oracle.sweeps import Operator, Preconditioner
+from tests._common import lens_velocity, make_pair, relerr
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ((1, 1), (30, 30), 2 * math.pi * 4.1),
+    ((2, 2), (19, 19), 2 * math.pi * 4.1),
+    ((3, 3), (12, 12), 2 * math.pi * 4.1),
+    ((4, 4), (9, 9), 2 * math.pi * 4.1),
+    ((2, 4), (19, 9), 2 * math.pi * 4.1),
+]
+
+
+@pytest.mark.parametrize("ncells,nels,omega", CASES)
+def test_operator_and_precond_match_oracle(ncells, nels, omega):
+    cb, dd = make_pair(ncells, nels, omega)
+    assert (dd.Gx, dd.Gy) == (cb.Gx, cb.Gy)
+    cb.factorize()
+    dd.factorize()
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(cb.N) + 1j * rng.standard_normal(cb.N)
+    A = cb.global_operator()
+    assert relerr(dd.apply_a(x), A @ x) < 1e-13
+    P = Preconditioner(cb)
+    ref = P.apply_global(x)
+    got = dd.apply_precond(x)
+    assert relerr(got, ref) < 1e-9, relerr(got, ref)
+
+
+def test_rhs_matches_oracle():
+    cb, dd = make_pair((3, 3), (12, 12), 2 * math.pi * 4.1)
+    srcs = [(0.5, 0.5), (0.31, 0.72)]
+    assert relerr(dd.assemble_rhs(srcs), cb.global_rhs(srcs)) < 1e-12
+
+
+@pytest.mark.parametrize("ncells,nels,omega", CASES[1:4])
+def test_gmres_reference_mode_matches_oracle(ncells, nels, omega):
+    cb, dd = make_pair(ncells, nels, omega)
+    cb.factorize()
+    dd.factorize()
+    b = cb.global_rhs([(0.5, 0.5)])
+    P = Preconditioner(cb)
+    A = cb.global_operator()
+    xo, log = gmres_reference(lambda v: A @ v, P.apply_global, b, tol=1e-8, max_iter=30)
+    x, info = dd.gmres(b, mode=0, tol=1e-8, max_iter=30)
+    assert abs(info["iterations"] - (len(log) - 1)) <= 1
+    u = spla.spsolve(A.tocsc(), b)
+    assert relerr(x, xo) < 1e-6
+    assert relerr(x, u) < 1e-6
+    assert info["true_residual"] < 1e-6
+
+
+def test_gmres_lens_medium_flexible_fp32():
+    vel = lens_velocity(60)
+    omega = 2 * math.pi * 0.6 * 3.0
+    cb, dd = make_pair((3, 3), (19, 19), omega, velocity=vel, precision=1)
+    dd.factorize()
+    b = dd.assemble_rhs([(0.5, 0.1)])
+    A = cb.global_operator()
+    x, info = dd.gmres(b, mode=1, tol=1e-8, max_iter=60, restart=30)
+    u = spla.spsolve(A.tocsc(), b)
+    assert info["true_residual"] < 1e-8
+    assert relerr(x, u) < 1e-6

@@ -1,0 +1,46 @@
+"""Pin the GMRES restatements on small dense systems."""
+import numpy as np
+
+from oracle.gmres import fgmres, gmres_reference
+
+
+def _sys(n=40, seed=0):
+    rng = np.random.default_rng(seed)
+    A = np.eye(n) * 4 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    return A, b
+
+
+def test_reference_mode_exact_preconditioner_returns_at_iteration_zero():
+    A, b = _sys()
+    Ainv = np.linalg.inv(A)
+    x, log = gmres_reference(lambda v: A @ v, lambda v: Ainv @ v, b, tol=1e-8, max_iter=30)
+    assert len(log) == 1 and log[0] < 1e-12
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(x) < 1e-12
+
+
+def test_reference_mode_first_estimate_is_preconditioned_residual():
+    A, b = _sys(seed=1)
+    D = np.diag(1 / np.diag(A))
+    x, log = gmres_reference(lambda v: A @ v, lambda v: D @ v, b, tol=1e-30, max_iter=1)
+    x0 = D @ b
+    r0 = D @ b - D @ (A @ x0)
+    assert abs(log[0] - np.linalg.norm(r0) / np.linalg.norm(D @ b)) < 1e-14
+    # after one step the estimate is |s0| * ||r0|| / ||M^-1 b|| with the
+    # complex-symmetric rotation of (h00, h10)
+    w = D @ (A @ (r0 / np.linalg.norm(r0)))
+    q0 = r0 / np.linalg.norm(r0)
+    h0 = np.vdot(w, q0)                    # conj(w) . q0
+    h1 = np.linalg.norm(w - h0 * q0)
+    t = np.sqrt(h0 ** 2 + h1 ** 2)
+    c = abs(h0) / t
+    s = c * h1 / h0
+    assert abs(log[1] - abs(s) * np.linalg.norm(r0) / np.linalg.norm(D @ b)) < 1e-12
+
+
+def test_flexible_gmres_solves_dense_system():
+    A, b = _sys(seed=2)
+    D = np.diag(1 / np.diag(A))
+    x, it, hist = fgmres(lambda v: A @ v, lambda v: D @ v, b, tol=1e-12, restart=10, max_iter=200)
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-11
+    assert hist[-1] <= 1e-12
