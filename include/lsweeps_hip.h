@@ -104,6 +104,12 @@ int lsw_dd_stats(lsw_dd* dd, long long* solves_per_precond, double* solve_bytes_
                  long long* factor_bytes);
 /* Wait for all work queued on the DD's own stream. */
 int lsw_dd_sync(lsw_dd* dd);
+/* One applyPrecond on device vectors with every batched local-solve launch
+ * timed by HIP events on the DD's stream (synchronous; not for timed
+ * regions).  out[0..5] = forward: summed ms, launches, algorithmic bytes;
+ * backward: summed ms, launches, algorithmic bytes.  No counterpart in the
+ * reference: it serves the roofline measurement. */
+int lsw_dd_profile_precond(lsw_dd* dd, const void* x, void* y, double* out);
 
 /* ---- GlobalIterativeSolver (include/GlobalIterativeSolver.h:36-48) -------
  * mode 0: the reference GMRES (src/GlobalIterativeSolverGMRES.cpp:65-348),

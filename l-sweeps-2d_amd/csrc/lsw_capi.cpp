@@ -305,6 +305,20 @@ int lsw_dd_stats(lsw_dd* dd, long long* solves, double* bytes, long long* factor
   });
 }
 
+int lsw_dd_profile_precond(lsw_dd* dd, const void* x, void* y, double* out) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y && out, LSW_ERR_ARG, "null pointer");
+    SolveProfile prof;
+    dd->dd->applyPrecondProfiled((const double2*)x, (double2*)y, prof);
+    out[0] = prof.fwdMs;
+    out[1] = prof.fwdLaunches;
+    out[2] = prof.fwdBytes;
+    out[3] = prof.bwdMs;
+    out[4] = prof.bwdLaunches;
+    out[5] = prof.bwdBytes;
+  });
+}
+
 int lsw_dd_sync(lsw_dd* dd) {
   return guarded([&] {
     LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");

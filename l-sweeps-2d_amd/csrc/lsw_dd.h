@@ -108,6 +108,9 @@ class DDCheckerboard : public DD {
   long long N() const { return (long long)mGx * mGy; }
   // y = M^{-1} x
   void applyPrecondDev(const double2* x, double2* y, hipStream_t s) const;
+  // Same, timing every batched-solve launch with HIP events (synchronises
+  // after each sweep step; use outside timed regions).
+  void applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof) const;
   // y = A x (global operator)
   void applyADev(const double2* x, double2* y, hipStream_t s) const;
   // global RHS of point sources (summed, RHSSrcPts semantics)
@@ -133,6 +136,7 @@ class DDCheckerboard : public DD {
  private:
   void buildSchedule();
   void allocBuffers();
+  void applyPrecondImpl(const double2* x, double2* y, hipStream_t s, SolveProfile* prof) const;
   void initDevice();
   std::vector<unsigned char> mColT, mRowT;
 
@@ -159,6 +163,7 @@ class DDCheckerboard : public DD {
     int nSlots = 0;
     SolveTask* dTasks = nullptr;
     int nTasks = 0, maxNrhs = 1;
+    std::vector<SolveTask> hTasks;
     GroupDev* dGroups = nullptr;
     int nGroups = 0;
     CopyOp* dCopies = nullptr;  // trace copies to run after this step

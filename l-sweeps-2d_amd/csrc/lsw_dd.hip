@@ -761,6 +761,7 @@ void DDCheckerboard::buildSchedule() {
     }
     st.nSlots = (int)hst.slots.size();
     st.nTasks = (int)tasks.size();
+    st.hTasks = tasks;
     st.nGroups = (int)groups.size();
     st.nCopies = (int)hst.copies.size();
     LSW_REQUIRE(st.nSlots <= 65535 && st.nCopies <= 65535 * 8, LSW_ERR_CAPACITY, "board too large for one launch");
@@ -797,6 +798,15 @@ void DDCheckerboard::factorize() {
 // ------------------------------------------------------------- operations
 
 void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s) const {
+  applyPrecondImpl(x, y, s, nullptr);
+}
+
+void DDCheckerboard::applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof) const {
+  applyPrecondImpl(x, y, mStream, &prof);
+  LSW_HIP(hipStreamSynchronize(mStream));
+}
+
+void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t s, SolveProfile* prof) const {
   LSW_REQUIRE(mFactors && mFactors->factorized(), LSW_ERR_STATE, "applyPrecond before factorize");
   if (!s) s = mStream;
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
@@ -815,7 +825,10 @@ void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s
       hipLaunchKernelGGL(k_build_rhs<float2>, dim3(gxN, st.nSlots), dim3(tb), 0, s, dCells, st.dSlots, x,
                          dColTrace, dRowTrace, mGy, dCoef, (const float2*)dT, (float2*)dX, mXs);
     }
-    mFactors->solve(st.dTasks, st.nTasks, st.maxNrhs, dX, dR, dG, mXs, mGs, s);
+    if (prof)
+      mFactors->solveProfiled(st.dTasks, st.hTasks, st.maxNrhs, dX, dR, dG, mXs, mGs, s, *prof);
+    else
+      mFactors->solve(st.dTasks, st.nTasks, st.maxNrhs, dX, dR, dG, mXs, mGs, s);
     const unsigned gxT = (unsigned)((mTmax + 63) / 64);
     if (mPrecision == LSW_FP64) {
       hipLaunchKernelGGL(k_post<double2>, dim3(gxT, st.nSlots), dim3(64), 0, s, dCells, st.dSlots,

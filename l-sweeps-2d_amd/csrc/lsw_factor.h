@@ -1,3 +1,11 @@
+// Per-launch timing of the batched solve kernels (HIP events on the launch
+// stream) and their algorithmic bytes.
+struct SolveProfile {
+  double fwdMs = 0, bwdMs = 0;        // summed launch durations
+  int fwdLaunches = 0, bwdLaunches = 0;
+  double fwdBytes = 0, bwdBytes = 0;  // summed algorithmic bytes
+};
+
 // FactorSet: a batch of structured-grid local problems (5-point operator),
 // assembled, factorised and solved on the GPU.
 //
@@ -52,6 +60,13 @@ class FactorSet {
   // maxNrhs: the largest SolveTask::nrhs among the tasks (1..kMaxRhs).
   void solve(const SolveTask* dtasks, int ntasks, int maxNrhs, void* X, void* R, void* G,
              long long xStride, long long gStride, hipStream_t s) const;
+  // Same, timing every launch with HIP events (synchronises); htasks is the
+  // host copy of dtasks, used to count algorithmic bytes.
+  void solveProfiled(const SolveTask* dtasks, const std::vector<SolveTask>& htasks, int maxNrhs,
+                     void* X, void* R, void* G, long long xStride, long long gStride, hipStream_t s,
+                     SolveProfile& prof) const;
+  // Algorithmic bytes of the forward / backward launch at height h.
+  double launchBytes(const std::vector<SolveTask>& htasks, int h, bool forward) const;
 
   int numCells() const { return (int)mCells.size(); }
   const CellDev& cell(int i) const { return mCells[i]; }

@@ -546,34 +546,102 @@ void FactorSet::factorize(int precision, hipStream_t s) {
 }
 
 namespace {
+struct LaunchTimer {
+  SolveProfile* prof;
+  hipStream_t s;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;  // 0 fwd, 1 bwd
+  std::vector<double> bytes;
+  void begin() {
+    if (!prof) return;
+    hipEvent_t e;
+    LSW_HIP(hipEventCreate(&e));
+    LSW_HIP(hipEventRecord(e, s));
+    ev.push_back(e);
+  }
+  void end(int k, double b) {
+    if (!prof) return;
+    hipEvent_t e;
+    LSW_HIP(hipEventCreate(&e));
+    LSW_HIP(hipEventRecord(e, s));
+    ev.push_back(e);
+    kind.push_back(k);
+    bytes.push_back(b);
+  }
+  void finish() {
+    if (!prof) return;
+    LSW_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < kind.size(); ++i) {
+      float ms = 0;
+      LSW_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+      if (kind[i] == 0) {
+        prof->fwdMs += ms;
+        prof->fwdLaunches++;
+        prof->fwdBytes += bytes[i];
+      } else {
+        prof->bwdMs += ms;
+        prof->bwdLaunches++;
+        prof->bwdBytes += bytes[i];
+      }
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
 template <typename T2, int NR>
 void launchSolve(const FactorSet& fs, const std::vector<const NdTree*>& trees, const CellDev* dCells,
                  const ShapeDev* dShapes, const SolveTask* dtasks, int ntasks, const T2* factor, T2* X,
-                 T2* R, T2* G, long long xs, long long gs, int maxH, int maxFront, hipStream_t s) {
+                 T2* R, T2* G, long long xs, long long gs, int maxH, int maxFront, hipStream_t s,
+                 SolveProfile* prof, const std::vector<SolveTask>* htasks) {
   const size_t shm = (size_t)NR * maxFront * sizeof(T2);
+  LaunchTimer tm{prof, s, {}, {}, {}};
   for (int h = 0; h <= maxH; ++h) {
     int gx = 0;
     for (auto* t : trees)
       if (h <= t->maxHeight) gx = std::max(gx, (int)t->fwdItems[h].size());
     if (gx == 0) continue;
+    tm.begin();
     hipLaunchKernelGGL((k_nd_fwd<T2, NR>), dim3(gx, ntasks), dim3(64), shm, s, dCells, dShapes, dtasks, h,
                        factor, X, R, G, xs, gs);
+    tm.end(0, prof ? fs.launchBytes(*htasks, h, true) : 0.0);
   }
   for (int h = maxH; h >= 0; --h) {
     int gx = 0;
     for (auto* t : trees)
       if (h <= t->maxHeight) gx = std::max(gx, (int)t->bwdItems[h].size());
     if (gx == 0) continue;
+    tm.begin();
     hipLaunchKernelGGL((k_nd_bwd<T2, NR>), dim3(gx, ntasks), dim3(64), shm, s, dCells, dShapes, dtasks, h,
                        factor, X, R, xs);
+    tm.end(1, prof ? fs.launchBytes(*htasks, h, false) : 0.0);
   }
   LSW_HIP(hipGetLastError());
-  (void)fs;
+  tm.finish();
 }
+
 }  // namespace
+
+double FactorSet::launchBytes(const std::vector<SolveTask>& htasks, int h, bool forward) const {
+  const double S = mPrecision == LSW_FP64 ? 16.0 : 8.0;
+  double bytes = 0;
+  for (const auto& tk : htasks) {
+    const NdTree& t = *mTrees[mCells[tk.cell].shape];
+    if (h > t.maxHeight) continue;
+    for (int p : t.byHeight[h]) {
+      const NdNode& nd = t.nodes[p];
+      const double e = nd.e, b = nd.b, r = tk.nrhs;
+      if (forward)  // W, read b_E + child updates, write r_E and g_p
+        bytes += S * (b * e + r * (e + e + b + b));
+      else          // K, gather [r_E; x_B], write x_E
+        bytes += S * (e * (e + b) + r * (e + b + e));
+    }
+  }
+  return bytes;
+}
 
 void FactorSet::solve(const SolveTask* dtasks, int ntasks, int maxNrhs, void* X, void* R, void* G,
                       long long xs, long long gs, hipStream_t s) const {
+  static const std::vector<SolveTask> none;
   LSW_REQUIRE(mFactorized, LSW_ERR_STATE, "solve called before factorize");
   if (ntasks <= 0) return;
   LSW_REQUIRE(ntasks <= 65535, LSW_ERR_ARG, "too many solve tasks in one launch");
@@ -583,17 +651,42 @@ void FactorSet::solve(const SolveTask* dtasks, int ntasks, int maxNrhs, void* X,
   if (mPrecision == LSW_FP64) {
     if (maxNrhs == 1)
       launchSolve<double2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
-                              (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s);
+                              (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
     else
       launchSolve<double2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
-                                    (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s);
+                                    (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
   } else {
     if (maxNrhs == 1)
       launchSolve<float2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
-                             (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s);
+                             (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
     else
       launchSolve<float2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
-                                   (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s);
+                                   (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
+  }
+}
+
+void FactorSet::solveProfiled(const SolveTask* dtasks, const std::vector<SolveTask>& htasks, int maxNrhs,
+                              void* X, void* R, void* G, long long xs, long long gs, hipStream_t s,
+                              SolveProfile& prof) const {
+  LSW_REQUIRE(mFactorized, LSW_ERR_STATE, "solve called before factorize");
+  const int ntasks = (int)htasks.size();
+  if (ntasks <= 0) return;
+  std::vector<const NdTree*> trees;
+  for (auto& t : mTrees) trees.push_back(t.get());
+  if (mPrecision == LSW_FP64) {
+    if (maxNrhs == 1)
+      launchSolve<double2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
+                              (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
+    else
+      launchSolve<double2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
+                                    (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
+  } else {
+    if (maxNrhs == 1)
+      launchSolve<float2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
+                             (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
+    else
+      launchSolve<float2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
+                                   (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
   }
 }
 
