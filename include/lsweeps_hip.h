@@ -96,6 +96,10 @@ int lsw_dd_assemble_rhs_dev(lsw_dd* dd, int nsrc, const double* src_xy, void* b,
 /* y = M^{-1} x: DDCheckerboard::applyPrecond (src/DDCheckerboard.cpp:687-739). */
 int lsw_dd_apply_precond(lsw_dd* dd, const double* x, double* y);
 int lsw_dd_apply_precond_dev(lsw_dd* dd, const void* x, void* y, void* stream);
+/* Block form: x, y hold nrhs consecutive global vectors.  The local solves of
+ * each front share one read of the cell's factors among the right-hand sides. */
+int lsw_dd_apply_precond_block(lsw_dd* dd, int nrhs, const double* x, double* y);
+int lsw_dd_apply_precond_block_dev(lsw_dd* dd, int nrhs, const void* x, void* y, void* stream);
 /* y = A x: DDCheckerboard::applySystemMatrix (src/DDCheckerboard.cpp:2611-2635). */
 int lsw_dd_apply_a(lsw_dd* dd, const double* x, double* y);
 int lsw_dd_apply_a_dev(lsw_dd* dd, const void* x, void* y, void* stream);
@@ -138,6 +142,50 @@ int lsw_gmres_solve(lsw_dd* dd, const lsw_gmres_params* p, const double* b, doub
                     lsw_gmres_result* res, double* log, int log_cap);
 int lsw_gmres_solve_dev(lsw_dd* dd, const lsw_gmres_params* p, const void* b, void* x, void* stream,
                         lsw_gmres_result* res);
+/* Flexible GMRES for nrhs right-hand sides sharing the factors (B, X: nrhs
+ * consecutive global vectors; res: nrhs results).  Every Arnoldi step applies
+ * the preconditioner to the whole block of active right-hand sides.  No
+ * counterpart in the reference, which sums all sources into one right-hand
+ * side (src/RHSSrcPts.cpp:93-94). */
+int lsw_gmres_solve_block(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const double* B, double* X,
+                          lsw_gmres_result* res);
+int lsw_gmres_solve_block_dev(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const void* B, void* X,
+                              void* stream, lsw_gmres_result* res);
+/* Point-source right-hand sides, one per source: b holds nsrc global vectors. */
+int lsw_dd_assemble_rhs_block_dev(lsw_dd* dd, int nsrc, const double* src_xy, void* b, void* stream);
+
+/* ---- Input / Problem (include/Input.h:44-76, include/Problem.h:44-47) -----
+ * The reference's libconfig input file (src/Input.cpp) and driver
+ * (src/Problem.cpp:94-158). */
+typedef struct lsw_problem_spec {
+  lsw_dd_params dd;        /* velocity is NULL for CSV media (file named in the config) */
+  int csv_medium;          /* 1 when SquaredSlowness Type = "CSV" */
+  int solver_mode;         /* 0 reference GMRES ("GMRES"/"GMRESHIP"), 1 flexible ("FGMRESHIP") */
+  double tol;
+  int max_iter;
+  int restart;
+  int verbose;
+  int nsources;
+} lsw_problem_spec;
+
+typedef struct lsw_problem_result {
+  double setup_seconds;
+  double factorize_seconds; /* "Time to factorize system" (src/Problem.cpp:143) */
+  double gmres_seconds;     /* "Time to run the GMRES" (src/Problem.cpp:144) */
+  int iterations;
+  double estimate;
+  double true_residual;
+  long long dofs;
+  int Gx, Gy;
+} lsw_problem_result;
+
+/* Number of problems in a config file (Input::getNumProblems). */
+int lsw_config_num_problems(const char* path, int* n);
+/* Parsed parameters of problem `id`; src_xy receives up to src_cap sources. */
+int lsw_config_problem(const char* path, int id, lsw_problem_spec* spec, double* src_xy, int src_cap);
+/* Problem::read + Problem::solve on the GPU.  x (optional, x_cap complex
+ * entries) receives the solution on the global grid. */
+int lsw_problem_run(const char* path, int id, int verbose, double* x, long long x_cap, lsw_problem_result* res);
 
 #ifdef __cplusplus
 }

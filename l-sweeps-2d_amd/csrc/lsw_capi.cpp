@@ -8,6 +8,7 @@
 #include "../../include/lsweeps_hip.h"
 #include "lsw_dd.h"
 #include "lsw_gmres.h"
+#include "lsw_input.h"
 #include "lsw_model.h"
 #include "lsw_solver.h"
 
@@ -282,6 +283,35 @@ int lsw_dd_apply_precond_dev(lsw_dd* dd, const void* x, void* y, void* stream) {
   });
 }
 
+int lsw_dd_apply_precond_block(lsw_dd* dd, int nrhs, const double* x, double* y) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y && nrhs > 0, LSW_ERR_ARG, "null pointer or bad nrhs");
+    const long long n = dd->dd->N() * nrhs;
+    double2 *dx = nullptr, *dy = nullptr;
+    LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+    LSW_HIP(hipMalloc(&dy, n * sizeof(double2)));
+    try {
+      LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
+      dd->dd->applyPrecondDev(dx, dy, dd->dd->stream(), nrhs);
+      LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
+      LSW_HIP(hipMemcpy(y, dy, n * sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (...) {
+      (void)hipFree(dx);
+      (void)hipFree(dy);
+      throw;
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+  });
+}
+
+int lsw_dd_apply_precond_block_dev(lsw_dd* dd, int nrhs, const void* x, void* y, void* stream) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y && nrhs > 0, LSW_ERR_ARG, "null pointer or bad nrhs");
+    dd->dd->applyPrecondDev((const double2*)x, (double2*)y, (hipStream_t)stream, nrhs);
+  });
+}
+
 int lsw_dd_apply_a(lsw_dd* dd, const double* x, double* y) {
   return guarded([&] {
     LSW_REQUIRE(dd && x && y, LSW_ERR_ARG, "null pointer");
@@ -384,6 +414,133 @@ int lsw_gmres_solve_dev(lsw_dd* dd, const lsw_gmres_params* p, const void* b, vo
     GmresInfo info;
     dd->gmres.solveDev((const double2*)b, (double2*)x, &info, (hipStream_t)stream);
     fill(info, res);
+  });
+}
+
+int lsw_gmres_solve_block_dev(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const void* B, void* X, void* stream,
+                              lsw_gmres_result* res) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && p && B && X && nrhs > 0, LSW_ERR_ARG, "null pointer or bad nrhs");
+    configure(dd, p);
+    std::vector<GmresInfo> infos;
+    dd->gmres.solveBlockDev((const double2*)B, (double2*)X, nrhs, &infos, (hipStream_t)stream);
+    if (res)
+      for (int r = 0; r < nrhs; ++r) fill(infos[r], res + r);
+  });
+}
+
+int lsw_gmres_solve_block(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const double* B, double* X,
+                          lsw_gmres_result* res) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && p && B && X && nrhs > 0, LSW_ERR_ARG, "null pointer or bad nrhs");
+    configure(dd, p);
+    const long long n = dd->dd->N() * nrhs;
+    double2 *db = nullptr, *dx = nullptr;
+    LSW_HIP(hipMalloc(&db, n * sizeof(double2)));
+    LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+    std::vector<GmresInfo> infos;
+    try {
+      LSW_HIP(hipMemcpy(db, B, n * sizeof(double2), hipMemcpyHostToDevice));
+      dd->gmres.solveBlockDev(db, dx, nrhs, &infos, dd->dd->stream());
+      LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
+      LSW_HIP(hipMemcpy(X, dx, n * sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (...) {
+      (void)hipFree(db);
+      (void)hipFree(dx);
+      throw;
+    }
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    if (res)
+      for (int r = 0; r < nrhs; ++r) fill(infos[r], res + r);
+  });
+}
+
+int lsw_dd_assemble_rhs_block_dev(lsw_dd* dd, int nsrc, const double* src_xy, void* b, void* stream) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && b && src_xy && nsrc > 0, LSW_ERR_ARG, "null pointer or bad nsrc");
+    const long long n = dd->dd->N();
+    for (int k = 0; k < nsrc; ++k)
+      dd->dd->assembleRHSDev({{src_xy[2 * k], src_xy[2 * k + 1]}}, (double2*)b + k * n, (hipStream_t)stream);
+  });
+}
+
+// ------------------------------------------------------------ Input/Problem
+int lsw_config_num_problems(const char* path, int* n) {
+  return guarded([&] {
+    LSW_REQUIRE(path && n, LSW_ERR_ARG, "null pointer");
+    Input in;
+    in.openFile(path);
+    *n = (int)in.getNumProblems();
+  });
+}
+
+int lsw_config_problem(const char* path, int id, lsw_problem_spec* spec, double* src_xy, int src_cap) {
+  return guarded([&] {
+    LSW_REQUIRE(path && spec, LSW_ERR_ARG, "null pointer");
+    LSW_REQUIRE(id >= 0, LSW_ERR_ARG, "Invalid problem ID");
+    Input in;
+    in.openFile(path);
+    const ProblemSpec sp = in.readProblem((UInt)id);
+    std::memset(spec, 0, sizeof(*spec));
+    lsw_dd_params& p = spec->dd;
+    p.domain[0] = sp.domain[0][0];
+    p.domain[1] = sp.domain[0][1];
+    p.domain[2] = sp.domain[1][0];
+    p.domain[3] = sp.domain[1][1];
+    p.ncells_x = (int)sp.nCellsX;
+    p.ncells_y = (int)sp.nCellsY;
+    p.nx = (int)sp.NX;
+    p.ny = (int)sp.NY;
+    p.omega = sp.omega;
+    p.min_wavelengths_in_pml = sp.minWavelengthsInPML;
+    p.order = (int)sp.order;
+    p.precision = sp.precision;
+    p.medium_kind = sp.mediumType == "CSV" ? 1 : 0;
+    if (sp.mediumType == "CSV") {
+      p.vel_domain[0] = sp.csvDomain[0][0];
+      p.vel_domain[1] = sp.csvDomain[0][1];
+      p.vel_domain[2] = sp.csvDomain[1][0];
+      p.vel_domain[3] = sp.csvDomain[1][1];
+    }
+    spec->csv_medium = p.medium_kind;
+    spec->solver_mode = sp.solverType == "FGMRESHIP" ? 1 : 0;
+    spec->tol = sp.tol;
+    spec->max_iter = (int)sp.maxIter;
+    spec->restart = (int)sp.restart;
+    spec->verbose = sp.verbose ? 1 : 0;
+    spec->nsources = (int)sp.sources.size();
+    if (src_xy)
+      for (int k = 0; k < src_cap && k < (int)sp.sources.size(); ++k) {
+        src_xy[2 * k] = sp.sources[k][0];
+        src_xy[2 * k + 1] = sp.sources[k][1];
+      }
+  });
+}
+
+int lsw_problem_run(const char* path, int id, int verbose, double* x, long long x_cap, lsw_problem_result* res) {
+  return guarded([&] {
+    LSW_REQUIRE(path, LSW_ERR_ARG, "null pointer");
+    Input in;
+    in.openFile(path);
+    Problem pb;
+    pb.read(in, (UInt)id);
+    const ProblemResult r = pb.solve(verbose != 0);
+    if (res) {
+      res->setup_seconds = r.setupSeconds;
+      res->factorize_seconds = r.factorizeSeconds;
+      res->gmres_seconds = r.gmresSeconds;
+      res->iterations = r.iterations;
+      res->estimate = r.estimate;
+      res->true_residual = r.trueResidual;
+      res->dofs = r.dofs;
+      res->Gx = pb.dd().Gx();
+      res->Gy = pb.dd().Gy();
+    }
+    if (x) {
+      LSW_REQUIRE(x_cap >= (long long)pb.solution().size(), LSW_ERR_CAPACITY, "solution buffer too small");
+      std::memcpy(x, pb.solution().data(), pb.solution().size() * sizeof(Complex));
+    }
   });
 }
 

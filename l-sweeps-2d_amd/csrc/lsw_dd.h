@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -106,11 +107,15 @@ class DDCheckerboard : public DD {
   int Gx() const { return mGx; }
   int Gy() const { return mGy; }
   long long N() const { return (long long)mGx * mGy; }
-  // y = M^{-1} x
-  void applyPrecondDev(const double2* x, double2* y, hipStream_t s) const;
+  // y = M^{-1} x for nrhs right-hand sides (x, y: nrhs consecutive global
+  // vectors).  Every local solve of a front step shares one read of the
+  // cell's factors among up to kMaxRhs right-hand sides.
+  void applyPrecondDev(const double2* x, double2* y, hipStream_t s, int nrhs = 1) const;
   // Same, timing every batched-solve launch with HIP events (synchronises
   // after each sweep step; use outside timed regions).
-  void applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof) const;
+  void applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof, int nrhs = 1) const;
+  // y = A x for nrhs vectors
+  void applyADevBlock(const double2* x, double2* y, int nrhs, hipStream_t s) const;
   // y = A x (global operator)
   void applyADev(const double2* x, double2* y, hipStream_t s) const;
   // global RHS of point sources (summed, RHSSrcPts semantics)
@@ -136,7 +141,9 @@ class DDCheckerboard : public DD {
  private:
   void buildSchedule();
   void allocBuffers();
-  void applyPrecondImpl(const double2* x, double2* y, hipStream_t s, SolveProfile* prof) const;
+  void applyPrecondImpl(const double2* x, double2* y, hipStream_t s, SolveProfile* prof, int nrhs) const;
+  void ensureBuffers(int nrhs) const;
+  std::vector<Rect> slotSupport(const SlotDev& s) const;
   void initDevice();
   std::vector<unsigned char> mColT, mRowT;
 
@@ -161,27 +168,31 @@ class DDCheckerboard : public DD {
   struct Step {
     SlotDev* dSlots = nullptr;
     int nSlots = 0;
-    SolveTask* dTasks = nullptr;
     int nTasks = 0, maxNrhs = 1;
-    std::vector<SolveTask> hTasks;
+    std::vector<SlotDev> hSlots;               // host copy (single right-hand side)
+    std::vector<std::vector<Rect>> hSupport;   // right-hand-side support per slot
+    // solve plans per block width (built on first use)
+    mutable std::map<int, std::unique_ptr<SolvePlan>> plans;
     GroupDev* dGroups = nullptr;
     int nGroups = 0;
     CopyOp* dCopies = nullptr;  // trace copies to run after this step
     int nCopies = 0;
   };
   std::vector<Step> mSteps;
+  const SolvePlan& planFor(const Step& st, int nrhs) const;
   int mMaxSlots = 0;
   int mTmax = 0;
   long long mTraceEntries = 0;
   long long mSolvesPerPrecond = 0;
   double mSolveBytesPerPrecond = 0;
 
-  // buffers (element type follows precision)
-  void* dX = nullptr;
-  void* dR = nullptr;
-  void* dG = nullptr;
-  void* dT = nullptr;
-  double2* dY = nullptr;  // 4 accumulators of N entries
+  // buffers (element type follows precision), sized for mBufRhs right-hand sides
+  mutable void* dX = nullptr;
+  mutable void* dR = nullptr;
+  mutable void* dG = nullptr;
+  mutable void* dT = nullptr;
+  mutable double2* dY = nullptr;  // 4 accumulators of N entries per right-hand side
+  mutable int mBufRhs = 0;
   long long mXs = 0, mGs = 0;
 };
 

@@ -24,17 +24,24 @@ enum Ctx { kS1 = 0, kR = 1, kL = 2, kU = 3, kD = 4, kBL = 5, kTR = 6, kBR = 7, k
 //   sweeps : equivalent sources res0 = A01 v1 on set 0, res1 = -A10 v0 on set 1
 //            (SubdomainStructured::getEquivalentSources/extractLTraces,
 //            src/SubdomainStructured.cpp:1304-1539)
+// Slot s of a block of right-hand sides is (slots[s % nSlots1], rhs = s / nSlots1);
+// its buffer is X + s*xs, its trace storage T + rhs*traceStride and its input
+// vector xin + rhs*Nglob.
 template <typename T2>
-__global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots,
-                            const double2* __restrict__ xin, const unsigned char* __restrict__ colT,
-                            const unsigned char* __restrict__ rowT, int Gy,
-                            const double2* __restrict__ coef, const T2* __restrict__ T,
+__global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
+                            long long traceStride, long long Nglob, const double2* __restrict__ xin,
+                            const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT,
+                            int Gy, const double2* __restrict__ coef, const T2* __restrict__ T,
                             T2* __restrict__ X, long long xs) {
-  const SlotDev& sl = slots[blockIdx.y];
+  const int sidx = blockIdx.y;
+  const SlotDev& sl = slots[sidx % nSlots1];
+  const int rhs = sidx / nSlots1;
+  xin += (long long)rhs * Nglob;
+  T += (long long)rhs * traceStride;
   const CellDev& cd = cells[sl.cell];
   const int nx = cd.nx, ny = cd.ny, N = nx * ny;
   const int slr = sl.d & 1, sud = sl.d >> 1;
-  T2* x = X + (long long)sl.slot * xs;
+  T2* x = X + (long long)sidx * xs;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
     const int i = p / ny, j = p - i * ny;
     T2 out = CTraits<T2>::make(0, 0);
@@ -86,12 +93,14 @@ __global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __
 // incoming outer traces (src/DDCheckerboard.cpp:1861-1864,2085-2088,
 // 2305-2308,2520-2526).
 template <typename T2>
-__global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots,
-                       const T2* __restrict__ X, long long xs, T2* __restrict__ T) {
-  const SlotDev& sl = slots[blockIdx.y];
+__global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
+                       long long traceStride, const T2* __restrict__ X, long long xs, T2* __restrict__ T) {
+  const int sidx = blockIdx.y;
+  const SlotDev& sl = slots[sidx % nSlots1];
+  T += (long long)(sidx / nSlots1) * traceStride;
   const CellDev& cd = cells[sl.cell];
   const int nx = cd.nx, ny = cd.ny;
-  const T2* x = X + (long long)sl.slot * xs;
+  const T2* x = X + (long long)sidx * xs;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   for (int o = 0; o < sl.nOut; ++o) {
     const bool isRow = sl.outIsRow[o] != 0;
@@ -115,9 +124,12 @@ __global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restr
 // global accumulator.
 template <typename T2>
 __global__ void k_accumulate(const CellDev* __restrict__ cells, const GroupDev* __restrict__ groups,
-                             const T2* __restrict__ X, long long xs, double2* __restrict__ Y,
-                             long long N, int Gy) {
-  const GroupDev& g = groups[blockIdx.y];
+                             int nGroups1, int nSlots1, const T2* __restrict__ X, long long xs,
+                             double2* __restrict__ Y, long long N, int Gy) {
+  const GroupDev& g = groups[blockIdx.y % nGroups1];
+  const int rhs = blockIdx.y / nGroups1;
+  Y += (long long)rhs * 4 * N;
+  X += (long long)rhs * nSlots1 * xs;
   const CellDev& cd = cells[g.cell];
   const int slr = g.d & 1, sud = g.d >> 1;
   const int i0 = cd.vlr[slr][2], i1 = cd.vlr[slr][5];
@@ -140,19 +152,24 @@ __global__ void k_accumulate(const CellDev* __restrict__ cells, const GroupDev* 
 }
 
 template <typename T2>
-__global__ void k_copy_traces(const CopyOp* __restrict__ ops, T2* __restrict__ T, int tmax) {
-  const CopyOp op = ops[blockIdx.y];
+__global__ void k_copy_traces(const CopyOp* __restrict__ ops, int nOps1, long long traceStride,
+                              T2* __restrict__ T, int tmax) {
+  const CopyOp op = ops[blockIdx.y % nOps1];
+  T += (long long)(blockIdx.y / nOps1) * traceStride;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tmax; t += gridDim.x * blockDim.x)
     T[op.dst + t] = T[op.src + t];
 }
 
-// res = res1 + (res2 + res3 + res4) (src/DDCheckerboard.cpp:733-735)
-__global__ void k_combine(const double2* __restrict__ Y, double2* __restrict__ out, long long N) {
-  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < N;
-       g += (long long)gridDim.x * blockDim.x) {
-    const double2 a = Y[g], b = Y[N + g], c = Y[2 * N + g], d = Y[3 * N + g];
+// res = res1 + (res2 + res3 + res4) (src/DDCheckerboard.cpp:733-735), per right-hand side
+__global__ void k_combine(const double2* __restrict__ Y, double2* __restrict__ out, long long N, int nrhs) {
+  const long long tot = N * nrhs;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long r = q / N, g = q - r * N;
+    const double2* y = Y + r * 4 * N;
+    const double2 a = y[g], b = y[N + g], c = y[2 * N + g], d = y[3 * N + g];
     const double2 s = cadd(cadd(b, c), d);
-    out[g] = cadd(a, s);
+    out[q] = cadd(a, s);
   }
 }
 
@@ -197,7 +214,6 @@ DDCheckerboard::DDCheckerboard() = default;
 DDCheckerboard::~DDCheckerboard() {
   for (auto& st : mSteps) {
     if (st.dSlots) (void)hipFree(st.dSlots);
-    if (st.dTasks) (void)hipFree(st.dTasks);
     if (st.dGroups) (void)hipFree(st.dGroups);
     if (st.dCopies) (void)hipFree(st.dCopies);
   }
@@ -719,24 +735,9 @@ void DDCheckerboard::buildSchedule() {
   for (auto& hst : steps) {
     Step st;
     for (size_t k = 0; k < hst.slots.size(); ++k) hst.slots[k].slot = (int)k;
-    // tasks
-    std::vector<int> order(hst.slots.size());
-    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return hst.slots[a].cell < hst.slots[b].cell; });
-    std::vector<SolveTask> tasks;
-    for (size_t k = 0; k < order.size();) {
-      SolveTask tk;
-      std::memset(&tk, 0, sizeof(tk));
-      tk.cell = hst.slots[order[k]].cell;
-      tk.nrhs = 0;
-      while (k < order.size() && hst.slots[order[k]].cell == tk.cell && tk.nrhs < kMaxRhs)
-        tk.slot[tk.nrhs++] = hst.slots[order[k++]].slot;
-      tasks.push_back(tk);
-      st.maxNrhs = std::max(st.maxNrhs, tk.nrhs);
-      const NdTree& tr = mFactors->tree(mFactors->cell(tk.cell).shape);
-      mSolveBytesPerPrecond += (double)tr.factorSize * S + 4.0 * tr.n * S * tk.nrhs;
-    }
+    st.hSlots = hst.slots;
+    for (const auto& sl : hst.slots) st.hSupport.push_back(slotSupport(sl));
+    (void)S;
     mSolvesPerPrecond += (long long)hst.slots.size();
     // groups: (d, cell)
     std::vector<GroupDev> groups;
@@ -760,34 +761,121 @@ void DDCheckerboard::buildSchedule() {
       }
     }
     st.nSlots = (int)hst.slots.size();
-    st.nTasks = (int)tasks.size();
-    st.hTasks = tasks;
     st.nGroups = (int)groups.size();
     st.nCopies = (int)hst.copies.size();
     LSW_REQUIRE(st.nSlots <= 65535 && st.nCopies <= 65535 * 8, LSW_ERR_CAPACITY, "board too large for one launch");
     LSW_HIP(hipMalloc(&st.dSlots, std::max<size_t>(1, hst.slots.size()) * sizeof(SlotDev)));
     LSW_HIP(hipMemcpy(st.dSlots, hst.slots.data(), hst.slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice));
-    LSW_HIP(hipMalloc(&st.dTasks, std::max<size_t>(1, tasks.size()) * sizeof(SolveTask)));
-    LSW_HIP(hipMemcpy(st.dTasks, tasks.data(), tasks.size() * sizeof(SolveTask), hipMemcpyHostToDevice));
+    {
+      const SolvePlan& pl = planFor(st, 1);
+      mSolveBytesPerPrecond += pl.fwdBytes + pl.bwdBytes;
+      st.nTasks = pl.ntasks;
+      st.maxNrhs = pl.maxNrhs;
+    }
     LSW_HIP(hipMalloc(&st.dGroups, std::max<size_t>(1, groups.size()) * sizeof(GroupDev)));
     LSW_HIP(hipMemcpy(st.dGroups, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice));
     if (!hst.copies.empty()) {
       LSW_HIP(hipMalloc(&st.dCopies, hst.copies.size() * sizeof(CopyOp)));
       LSW_HIP(hipMemcpy(st.dCopies, hst.copies.data(), hst.copies.size() * sizeof(CopyOp), hipMemcpyHostToDevice));
     }
-    mSteps.push_back(st);
+    mSteps.push_back(std::move(st));
   }
 }
 
-void DDCheckerboard::allocBuffers() {
+// Solve plan of one step for a block of nrhs right-hand sides.  Slots are
+// ordered by (cell, DD, sweep, right-hand side) and cut into tasks of up to
+// kMaxRhs: with one right-hand side a task carries the four DDs of a cell
+// (one factor read for all of them); with several, a task carries one DD and
+// sweep for several right-hand sides, so its support stays a few grid lines.
+const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
+  auto it = st.plans.find(nrhs);
+  if (it != st.plans.end()) return *it->second;
+  const int n1 = (int)st.hSlots.size();
+  std::vector<int> order((size_t)n1 * nrhs);
+  for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+  auto key = [&](int s) {
+    const SlotDev& a = st.hSlots[s % n1];
+    return std::make_tuple(a.cell, a.d, a.kind, s / n1, s % n1);
+  };
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  std::vector<SolveTask> tasks;
+  std::vector<std::vector<Rect>> supports;
+  for (size_t k = 0; k < order.size();) {
+    SolveTask tk;
+    std::memset(&tk, 0, sizeof(tk));
+    tk.cell = st.hSlots[order[k] % n1].cell;
+    std::vector<Rect> sup;
+    while (k < order.size() && st.hSlots[order[k] % n1].cell == tk.cell && tk.nrhs < kMaxRhs) {
+      const int s = order[k++];
+      tk.slot[tk.nrhs++] = s;
+      const auto& r = st.hSupport[s % n1];
+      sup.insert(sup.end(), r.begin(), r.end());
+    }
+    tasks.push_back(tk);
+    supports.push_back(sup);
+  }
+  auto plan = std::make_unique<SolvePlan>();
+  mFactors->buildPlan(tasks, supports, *plan);
+  const SolvePlan& ref = *plan;
+  st.plans[nrhs] = std::move(plan);
+  return ref;
+}
+
+void DDCheckerboard::ensureBuffers(int nrhs) const {
+  if (nrhs <= mBufRhs) return;
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
+  for (void* p : {dX, dR, dG, dT}) if (p) (void)hipFree(p);
+  if (dY) (void)hipFree(dY);
+  dX = dR = dG = dT = nullptr;
+  dY = nullptr;
+  mBufRhs = 0;
+  const size_t slots = (size_t)mMaxSlots * nrhs;
+  LSW_HIP(hipMalloc(&dX, slots * mXs * es));
+  LSW_HIP(hipMalloc(&dR, slots * mXs * es));
+  LSW_HIP(hipMalloc(&dG, slots * mGs * es));
+  LSW_HIP(hipMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
+  LSW_HIP(hipMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
+  mBufRhs = nrhs;
+}
+
+// Rectangles of local grid points where the right-hand side of a slot can be
+// nonzero: the windowed owned set (stage 1) or the two trace sets.
+std::vector<Rect> DDCheckerboard::slotSupport(const SlotDev& s) const {
+  std::vector<Rect> out;
+  const CellGeom& c = mCells[s.cell];
+  if (s.kind == 0) {
+    const int slr = s.d & 1, sud = s.d >> 1;
+    const bool wantColTrace = (s.d == 1 || s.d == 3), wantRowTrace = (s.d == 2 || s.d == 3);
+    std::vector<std::pair<int, int>> cr, rr;
+    for (int i = c.vlr[slr][2]; i < c.vlr[slr][5]; ++i) {
+      if ((mColT[i + c.ox] != 0) != wantColTrace) continue;
+      if (!cr.empty() && cr.back().second == i) cr.back().second = i + 1;
+      else cr.push_back({i, i + 1});
+    }
+    for (int j = c.vud[sud][2]; j < c.vud[sud][5]; ++j) {
+      if ((mRowT[j + c.oy] != 0) != wantRowTrace) continue;
+      if (!rr.empty() && rr.back().second == j) rr.back().second = j + 1;
+      else rr.push_back({j, j + 1});
+    }
+    for (auto& a : cr)
+      for (auto& b : rr) out.push_back({a.first, a.second, b.first, b.second});
+    if (out.empty()) out.push_back({0, 0, 0, 0});  // zero right-hand side: visit nothing
+    return out;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (s.setCol[k] >= 0 && s.setColHi[k] > s.setColLo[k])
+      out.push_back({s.setCol[k], s.setCol[k] + 1, s.setColLo[k], s.setColHi[k]});
+    if (s.setRow[k] >= 0 && s.setRowHi[k] > s.setRowLo[k])
+      out.push_back({s.setRowLo[k], s.setRowHi[k], s.setRow[k], s.setRow[k] + 1});
+  }
+  if (out.empty()) out.push_back({0, 0, 0, 0});
+  return out;
+}
+
+void DDCheckerboard::allocBuffers() {
   mXs = mFactors->maxDofs();
   mGs = std::max<long long>(mFactors->maxG(), 1);
-  LSW_HIP(hipMalloc(&dX, (size_t)mMaxSlots * mXs * es));
-  LSW_HIP(hipMalloc(&dR, (size_t)mMaxSlots * mXs * es));
-  LSW_HIP(hipMalloc(&dG, (size_t)mMaxSlots * mGs * es));
-  LSW_HIP(hipMalloc(&dT, (size_t)mTraceEntries * es));
-  LSW_HIP(hipMalloc(&dY, (size_t)4 * N() * sizeof(double2)));
+  ensureBuffers(1);
 }
 
 void DDCheckerboard::factorize() {
@@ -797,68 +885,78 @@ void DDCheckerboard::factorize() {
 
 // ------------------------------------------------------------- operations
 
-void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s) const {
-  applyPrecondImpl(x, y, s, nullptr);
+void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s, int nrhs) const {
+  applyPrecondImpl(x, y, s, nullptr, nrhs);
 }
 
-void DDCheckerboard::applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof) const {
-  applyPrecondImpl(x, y, mStream, &prof);
+void DDCheckerboard::applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof, int nrhs) const {
+  applyPrecondImpl(x, y, mStream, &prof, nrhs);
   LSW_HIP(hipStreamSynchronize(mStream));
 }
 
-void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t s, SolveProfile* prof) const {
+
+void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t s, SolveProfile* prof,
+                                      int nrhs) const {
   LSW_REQUIRE(mFactors && mFactors->factorized(), LSW_ERR_STATE, "applyPrecond before factorize");
+  LSW_REQUIRE(nrhs >= 1, LSW_ERR_ARG, "applyPrecond: nrhs must be positive");
+  LSW_REQUIRE((long long)mMaxSlots * nrhs <= 65535, LSW_ERR_CAPACITY, "too many right-hand sides in one block");
   if (!s) s = mStream;
+  ensureBuffers(nrhs);
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
   const long long n = N();
-  LSW_HIP(hipMemsetAsync(dY, 0, (size_t)4 * n * sizeof(double2), s));
-  LSW_HIP(hipMemsetAsync(dT, 0, (size_t)mTraceEntries * es, s));
+  LSW_HIP(hipMemsetAsync(dY, 0, (size_t)4 * nrhs * n * sizeof(double2), s));
+  LSW_HIP(hipMemsetAsync(dT, 0, (size_t)nrhs * mTraceEntries * es, s));
   const CellDev* dCells = mFactors->deviceCells();
   const double2* dCoef = mFactors->deviceCoef();
   const int tb = 256;
   const unsigned gxN = (unsigned)std::min<long long>((mFactors->maxDofs() + tb - 1) / tb, 1024);
+  const unsigned gxT = (unsigned)((mTmax + 63) / 64);
   for (const auto& st : mSteps) {
-    if (mPrecision == LSW_FP64) {
-      hipLaunchKernelGGL(k_build_rhs<double2>, dim3(gxN, st.nSlots), dim3(tb), 0, s, dCells, st.dSlots, x,
-                         dColTrace, dRowTrace, mGy, dCoef, (const double2*)dT, (double2*)dX, mXs);
-    } else {
-      hipLaunchKernelGGL(k_build_rhs<float2>, dim3(gxN, st.nSlots), dim3(tb), 0, s, dCells, st.dSlots, x,
-                         dColTrace, dRowTrace, mGy, dCoef, (const float2*)dT, (float2*)dX, mXs);
-    }
-    if (prof)
-      mFactors->solveProfiled(st.dTasks, st.hTasks, st.maxNrhs, dX, dR, dG, mXs, mGs, s, *prof);
+    const unsigned ns = (unsigned)st.nSlots * nrhs, ng = (unsigned)st.nGroups * nrhs;
+    const unsigned nc = (unsigned)st.nCopies * nrhs;
+    if (mPrecision == LSW_FP64)
+      hipLaunchKernelGGL(k_build_rhs<double2>, dim3(gxN, ns), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
+                         mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const double2*)dT, (double2*)dX,
+                         mXs);
     else
-      mFactors->solve(st.dTasks, st.nTasks, st.maxNrhs, dX, dR, dG, mXs, mGs, s);
-    const unsigned gxT = (unsigned)((mTmax + 63) / 64);
+      hipLaunchKernelGGL(k_build_rhs<float2>, dim3(gxN, ns), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
+                         mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const float2*)dT, (float2*)dX,
+                         mXs);
+    mFactors->solve(planFor(st, nrhs), dX, dR, dG, mXs, mGs, s, prof);
     if (mPrecision == LSW_FP64) {
-      hipLaunchKernelGGL(k_post<double2>, dim3(gxT, st.nSlots), dim3(64), 0, s, dCells, st.dSlots,
-                         (const double2*)dX, mXs, (double2*)dT);
-      hipLaunchKernelGGL(k_accumulate<double2>, dim3(gxN, st.nGroups), dim3(tb), 0, s, dCells, st.dGroups,
-                         (const double2*)dX, mXs, dY, n, mGy);
+      hipLaunchKernelGGL(k_post<double2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots,
+                         mTraceEntries, (const double2*)dX, mXs, (double2*)dT);
+      hipLaunchKernelGGL(k_accumulate<double2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
+                         st.nSlots, (const double2*)dX, mXs, dY, n, mGy);
       if (st.nCopies)
-        hipLaunchKernelGGL(k_copy_traces<double2>, dim3(gxT, st.nCopies), dim3(64), 0, s, st.dCopies,
-                           (double2*)dT, mTmax);
+        hipLaunchKernelGGL(k_copy_traces<double2>, dim3(gxT, nc), dim3(64), 0, s, st.dCopies, st.nCopies,
+                           mTraceEntries, (double2*)dT, mTmax);
     } else {
-      hipLaunchKernelGGL(k_post<float2>, dim3(gxT, st.nSlots), dim3(64), 0, s, dCells, st.dSlots,
-                         (const float2*)dX, mXs, (float2*)dT);
-      hipLaunchKernelGGL(k_accumulate<float2>, dim3(gxN, st.nGroups), dim3(tb), 0, s, dCells, st.dGroups,
-                         (const float2*)dX, mXs, dY, n, mGy);
+      hipLaunchKernelGGL(k_post<float2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots,
+                         mTraceEntries, (const float2*)dX, mXs, (float2*)dT);
+      hipLaunchKernelGGL(k_accumulate<float2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
+                         st.nSlots, (const float2*)dX, mXs, dY, n, mGy);
       if (st.nCopies)
-        hipLaunchKernelGGL(k_copy_traces<float2>, dim3(gxT, st.nCopies), dim3(64), 0, s, st.dCopies,
-                           (float2*)dT, mTmax);
+        hipLaunchKernelGGL(k_copy_traces<float2>, dim3(gxT, nc), dim3(64), 0, s, st.dCopies, st.nCopies,
+                           mTraceEntries, (float2*)dT, mTmax);
     }
   }
-  hipLaunchKernelGGL(k_combine, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, s,
-                     dY, y, n);
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)std::min<long long>((n * nrhs + 255) / 256, 8192)), dim3(256), 0,
+                     s, dY, y, n, nrhs);
   LSW_HIP(hipGetLastError());
 }
 
 void DDCheckerboard::applyADev(const double2* x, double2* y, hipStream_t s) const {
+  applyADevBlock(x, y, 1, s);
+}
+
+void DDCheckerboard::applyADevBlock(const double2* x, double2* y, int nrhs, hipStream_t s) const {
   LSW_REQUIRE(dGlobalCoef != nullptr, LSW_ERR_STATE, "DD has no device state");
   if (!s) s = mStream;
   const long long n = N();
-  hipLaunchKernelGGL(k_matvec, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s,
-                     dGlobalCoef, x, y, mGx, mGy);
+  for (int r = 0; r < nrhs; ++r)
+    hipLaunchKernelGGL(k_matvec, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s,
+                       dGlobalCoef, x + r * n, y + r * n, mGx, mGy);
   LSW_HIP(hipGetLastError());
 }
 

@@ -1,11 +1,3 @@
-// Per-launch timing of the batched solve kernels (HIP events on the launch
-// stream) and their algorithmic bytes.
-struct SolveProfile {
-  double fwdMs = 0, bwdMs = 0;        // summed launch durations
-  int fwdLaunches = 0, bwdLaunches = 0;
-  double fwdBytes = 0, bwdBytes = 0;  // summed algorithmic bytes
-};
-
 // FactorSet: a batch of structured-grid local problems (5-point operator),
 // assembled, factorised and solved on the GPU.
 //
@@ -32,6 +24,40 @@ struct SolveProfile {
 
 namespace lsw {
 
+// Per-launch timing of the batched solve kernels (HIP events on the launch
+// stream) and their algorithmic bytes.
+struct SolveProfile {
+  double fwdMs = 0, bwdMs = 0;        // summed launch durations
+  int fwdLaunches = 0, bwdLaunches = 0;
+  double fwdBytes = 0, bwdBytes = 0;  // summed algorithmic bytes
+};
+
+struct Rect {
+  int i0, i1, j0, j1;
+};
+
+// Launch plan of one batch of solves (device work lists per tree height).
+struct SolvePlan {
+  struct Launch {
+    long long off = 0;  // first entry in the plan's entry array
+    int n = 0;          // entries
+    int ipb = 1;        // entries per workgroup
+    int cols = 0;       // largest column count (sizes the LDS)
+    double bytes = 0;   // algorithmic bytes
+  };
+  int ntasks = 0, maxNrhs = 1, activeWords = 0;
+  std::vector<SolveTask> hTasks;
+  SolveTask* dTasks = nullptr;
+  int4* dEntries = nullptr;        // {task, node, r0, rows}
+  unsigned* dActive = nullptr;     // forward-visited nodes, activeWords per task
+  std::vector<Launch> fwd, bwd;    // in launch order
+  double fwdBytes = 0, bwdBytes = 0;
+  SolvePlan() = default;
+  SolvePlan(const SolvePlan&) = delete;
+  SolvePlan& operator=(const SolvePlan&) = delete;
+  ~SolvePlan();
+};
+
 class FactorSet {
  public:
   FactorSet() = default;
@@ -57,16 +83,20 @@ class FactorSet {
   // right-hand sides on entry and the solutions on exit (slot-major, stride
   // xStride complex entries); R and G are scratch with the same slot layout
   // (G stride gStride).  Element type follows precision(): double2 or float2.
-  // maxNrhs: the largest SolveTask::nrhs among the tasks (1..kMaxRhs).
-  void solve(const SolveTask* dtasks, int ntasks, int maxNrhs, void* X, void* R, void* G,
-             long long xStride, long long gStride, hipStream_t s) const;
-  // Same, timing every launch with HIP events (synchronises); htasks is the
-  // host copy of dtasks, used to count algorithmic bytes.
-  void solveProfiled(const SolveTask* dtasks, const std::vector<SolveTask>& htasks, int maxNrhs,
-                     void* X, void* R, void* G, long long xStride, long long gStride, hipStream_t s,
-                     SolveProfile& prof) const;
-  // Algorithmic bytes of the forward / backward launch at height h.
-  double launchBytes(const std::vector<SolveTask>& htasks, int h, bool forward) const;
+  // Build the launch plan of a batch of tasks.  supports[t] lists rectangles
+  // (i0, i1, j0, j1) of local grid points outside which every right-hand side
+  // of task t is zero (empty list = anywhere).  The forward solve visits only
+  // tree nodes whose region meets the support; the backward solve skips the
+  // F11^{-1} r_E product of the nodes it did not visit.
+  void buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
+                 SolvePlan& plan) const;
+  // Run a plan.  X holds the right-hand sides on entry and the solutions on
+  // exit (slot-major, stride xStride complex entries); R and G are scratch
+  // with the same slot layout (G stride gStride).  Element type follows
+  // precision(): double2 or float2.  With prof, every launch is timed with
+  // HIP events (synchronises at the end).
+  void solve(const SolvePlan& plan, void* X, void* R, void* G, long long xStride, long long gStride,
+             hipStream_t s, SolveProfile* prof = nullptr) const;
 
   int numCells() const { return (int)mCells.size(); }
   const CellDev& cell(int i) const { return mCells[i]; }

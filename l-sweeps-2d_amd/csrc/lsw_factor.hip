@@ -174,143 +174,6 @@ __global__ void k_to_f32(const double2* __restrict__ a, float2* __restrict__ o, 
     o[i] = make_float2((float)a[i].x, (float)a[i].y);
 }
 
-// Forward solve at one tree height: for each work item (node rows r0..r0+rows
-// of one task), gather r_E = b_E - sum(children updates), store it, and form
-// the update g_p = W r_E + (children updates on B_p).
-template <typename T2, int NR>
-__global__ __launch_bounds__(64) void k_nd_fwd(const CellDev* __restrict__ cells,
-                                               const ShapeDev* __restrict__ shapes,
-                                               const SolveTask* __restrict__ tasks, int h,
-                                               const T2* __restrict__ factor, T2* __restrict__ X,
-                                               T2* __restrict__ Rb, T2* __restrict__ G,
-                                               long long xs, long long gs) {
-  extern __shared__ char smem_raw[];
-  T2* v = reinterpret_cast<T2*>(smem_raw);
-  const SolveTask tk = tasks[blockIdx.y];
-  const CellDev& cd = cells[tk.cell];
-  const ShapeDev& sh = shapes[cd.shape];
-  if (h > sh.maxHeight || (int)blockIdx.x >= sh.fwdCount[h]) return;
-  const SolveItem it = sh.fwdItems[sh.fwdStart[h] + blockIdx.x];
-  const NdNodeDev nd = sh.nodes[it.node];
-  const int e = nd.e, b = nd.b, F = e + b;
-  const int lane = threadIdx.x;
-  const int* E = sh.E + nd.eOff;
-  const int* m1 = sh.cmap + nd.mapOff;
-  const int* m2 = m1 + F;
-  const int g1 = nd.c1 >= 0 ? sh.nodes[nd.c1].gOff : 0;
-  const int g2 = nd.c2 >= 0 ? sh.nodes[nd.c2].gOff : 0;
-  const int nr = tk.nrhs;
-  for (int k = lane; k < e; k += 64) {
-    const int a1 = m1[k], a2 = m2[k], dof = E[k];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      if (r >= nr) break;
-      const long long sl = tk.slot[r];
-      T2 val = X[sl * xs + dof];
-      if (a1 >= 0) val = csub(val, G[sl * gs + g1 + a1]);
-      if (a2 >= 0) val = csub(val, G[sl * gs + g2 + a2]);
-      v[r * e + k] = val;
-      if (it.r0 == 0) Rb[sl * xs + dof] = val;
-    }
-  }
-  __syncthreads();
-  const int rows = it.rows;
-  if (rows == 0) return;
-  int RP = 1;
-  while (RP < rows) RP <<= 1;
-  const int CG = 64 / RP;
-  const int r = lane & (RP - 1), cg = lane / RP;
-  T2 acc[NR];
-#pragma unroll
-  for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-  if (r < rows) {
-    const T2* W = factor + cd.factorOff + nd.fwdOff + it.r0 + r;
-#pragma unroll 4
-    for (int k = cg; k < e; k += CG) {
-      const T2 w = W[(long long)k * b];
-#pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = cfma(w, v[q * e + k], acc[q]);
-    }
-  }
-  for (int off = RP; off < 64; off <<= 1) {
-#pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
-  }
-  if (cg == 0 && r < rows) {
-    const int f = e + it.r0 + r;
-    const int a1 = m1[f], a2 = m2[f];
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      if (q >= nr) break;
-      const long long sl = tk.slot[q];
-      T2 g = acc[q];
-      if (a1 >= 0) g = cadd(g, G[sl * gs + g1 + a1]);
-      if (a2 >= 0) g = cadd(g, G[sl * gs + g2 + a2]);
-      G[sl * gs + nd.gOff + it.r0 + r] = g;
-    }
-  }
-}
-
-// Backward solve at one tree height: x_E = K [r_E; x_B].
-template <typename T2, int NR>
-__global__ __launch_bounds__(64) void k_nd_bwd(const CellDev* __restrict__ cells,
-                                               const ShapeDev* __restrict__ shapes,
-                                               const SolveTask* __restrict__ tasks, int h,
-                                               const T2* __restrict__ factor, T2* __restrict__ X,
-                                               const T2* __restrict__ Rb, long long xs) {
-  extern __shared__ char smem_raw[];
-  T2* v = reinterpret_cast<T2*>(smem_raw);
-  const SolveTask tk = tasks[blockIdx.y];
-  const CellDev& cd = cells[tk.cell];
-  const ShapeDev& sh = shapes[cd.shape];
-  if (h > sh.maxHeight || (int)blockIdx.x >= sh.bwdCount[h]) return;
-  const SolveItem it = sh.bwdItems[sh.bwdStart[h] + blockIdx.x];
-  const NdNodeDev nd = sh.nodes[it.node];
-  const int e = nd.e, b = nd.b, F = e + b;
-  const int lane = threadIdx.x;
-  const int* E = sh.E + nd.eOff;
-  const int* Bp = sh.B + nd.bOff;
-  const int nr = tk.nrhs;
-  for (int k = lane; k < F; k += 64) {
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      if (q >= nr) break;
-      const long long sl = tk.slot[q];
-      v[q * F + k] = (k < e) ? Rb[sl * xs + E[k]] : X[sl * xs + Bp[k - e]];
-    }
-  }
-  __syncthreads();
-  const int rows = it.rows;
-  int RP = 1;
-  while (RP < rows) RP <<= 1;
-  const int CG = 64 / RP;
-  const int r = lane & (RP - 1), cg = lane / RP;
-  T2 acc[NR];
-#pragma unroll
-  for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-  if (r < rows) {
-    const T2* K = factor + cd.factorOff + nd.bwdOff + it.r0 + r;
-#pragma unroll 4
-    for (int k = cg; k < F; k += CG) {
-      const T2 w = K[(long long)k * e];
-#pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = cfma(w, v[q * F + k], acc[q]);
-    }
-  }
-  for (int off = RP; off < 64; off <<= 1) {
-#pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
-  }
-  if (cg == 0 && r < rows) {
-    const int dof = E[it.r0 + r];
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      if (q >= nr) break;
-      X[(long long)tk.slot[q] * xs + dof] = acc[q];
-    }
-  }
-}
-
 // --------------------------------------------------------------- host side
 namespace {
 template <typename T>
@@ -543,151 +406,6 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     dFactor = nullptr;
   }
   mFactorized = true;
-}
-
-namespace {
-struct LaunchTimer {
-  SolveProfile* prof;
-  hipStream_t s;
-  std::vector<hipEvent_t> ev;
-  std::vector<int> kind;  // 0 fwd, 1 bwd
-  std::vector<double> bytes;
-  void begin() {
-    if (!prof) return;
-    hipEvent_t e;
-    LSW_HIP(hipEventCreate(&e));
-    LSW_HIP(hipEventRecord(e, s));
-    ev.push_back(e);
-  }
-  void end(int k, double b) {
-    if (!prof) return;
-    hipEvent_t e;
-    LSW_HIP(hipEventCreate(&e));
-    LSW_HIP(hipEventRecord(e, s));
-    ev.push_back(e);
-    kind.push_back(k);
-    bytes.push_back(b);
-  }
-  void finish() {
-    if (!prof) return;
-    LSW_HIP(hipStreamSynchronize(s));
-    for (size_t i = 0; i < kind.size(); ++i) {
-      float ms = 0;
-      LSW_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
-      if (kind[i] == 0) {
-        prof->fwdMs += ms;
-        prof->fwdLaunches++;
-        prof->fwdBytes += bytes[i];
-      } else {
-        prof->bwdMs += ms;
-        prof->bwdLaunches++;
-        prof->bwdBytes += bytes[i];
-      }
-    }
-    for (auto e : ev) (void)hipEventDestroy(e);
-  }
-};
-
-template <typename T2, int NR>
-void launchSolve(const FactorSet& fs, const std::vector<const NdTree*>& trees, const CellDev* dCells,
-                 const ShapeDev* dShapes, const SolveTask* dtasks, int ntasks, const T2* factor, T2* X,
-                 T2* R, T2* G, long long xs, long long gs, int maxH, int maxFront, hipStream_t s,
-                 SolveProfile* prof, const std::vector<SolveTask>* htasks) {
-  const size_t shm = (size_t)NR * maxFront * sizeof(T2);
-  LaunchTimer tm{prof, s, {}, {}, {}};
-  for (int h = 0; h <= maxH; ++h) {
-    int gx = 0;
-    for (auto* t : trees)
-      if (h <= t->maxHeight) gx = std::max(gx, (int)t->fwdItems[h].size());
-    if (gx == 0) continue;
-    tm.begin();
-    hipLaunchKernelGGL((k_nd_fwd<T2, NR>), dim3(gx, ntasks), dim3(64), shm, s, dCells, dShapes, dtasks, h,
-                       factor, X, R, G, xs, gs);
-    tm.end(0, prof ? fs.launchBytes(*htasks, h, true) : 0.0);
-  }
-  for (int h = maxH; h >= 0; --h) {
-    int gx = 0;
-    for (auto* t : trees)
-      if (h <= t->maxHeight) gx = std::max(gx, (int)t->bwdItems[h].size());
-    if (gx == 0) continue;
-    tm.begin();
-    hipLaunchKernelGGL((k_nd_bwd<T2, NR>), dim3(gx, ntasks), dim3(64), shm, s, dCells, dShapes, dtasks, h,
-                       factor, X, R, xs);
-    tm.end(1, prof ? fs.launchBytes(*htasks, h, false) : 0.0);
-  }
-  LSW_HIP(hipGetLastError());
-  tm.finish();
-}
-
-}  // namespace
-
-double FactorSet::launchBytes(const std::vector<SolveTask>& htasks, int h, bool forward) const {
-  const double S = mPrecision == LSW_FP64 ? 16.0 : 8.0;
-  double bytes = 0;
-  for (const auto& tk : htasks) {
-    const NdTree& t = *mTrees[mCells[tk.cell].shape];
-    if (h > t.maxHeight) continue;
-    for (int p : t.byHeight[h]) {
-      const NdNode& nd = t.nodes[p];
-      const double e = nd.e, b = nd.b, r = tk.nrhs;
-      if (forward)  // W, read b_E + child updates, write r_E and g_p
-        bytes += S * (b * e + r * (e + e + b + b));
-      else          // K, gather [r_E; x_B], write x_E
-        bytes += S * (e * (e + b) + r * (e + b + e));
-    }
-  }
-  return bytes;
-}
-
-void FactorSet::solve(const SolveTask* dtasks, int ntasks, int maxNrhs, void* X, void* R, void* G,
-                      long long xs, long long gs, hipStream_t s) const {
-  static const std::vector<SolveTask> none;
-  LSW_REQUIRE(mFactorized, LSW_ERR_STATE, "solve called before factorize");
-  if (ntasks <= 0) return;
-  LSW_REQUIRE(ntasks <= 65535, LSW_ERR_ARG, "too many solve tasks in one launch");
-  std::vector<const NdTree*> trees;
-  for (auto& t : mTrees) trees.push_back(t.get());
-  LSW_REQUIRE(maxNrhs >= 1 && maxNrhs <= kMaxRhs, LSW_ERR_ARG, "bad right-hand-side count");
-  if (mPrecision == LSW_FP64) {
-    if (maxNrhs == 1)
-      launchSolve<double2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
-                              (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
-    else
-      launchSolve<double2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
-                                    (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
-  } else {
-    if (maxNrhs == 1)
-      launchSolve<float2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
-                             (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
-    else
-      launchSolve<float2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
-                                   (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, nullptr, &none);
-  }
-}
-
-void FactorSet::solveProfiled(const SolveTask* dtasks, const std::vector<SolveTask>& htasks, int maxNrhs,
-                              void* X, void* R, void* G, long long xs, long long gs, hipStream_t s,
-                              SolveProfile& prof) const {
-  LSW_REQUIRE(mFactorized, LSW_ERR_STATE, "solve called before factorize");
-  const int ntasks = (int)htasks.size();
-  if (ntasks <= 0) return;
-  std::vector<const NdTree*> trees;
-  for (auto& t : mTrees) trees.push_back(t.get());
-  if (mPrecision == LSW_FP64) {
-    if (maxNrhs == 1)
-      launchSolve<double2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
-                              (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
-    else
-      launchSolve<double2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor, (double2*)X,
-                                    (double2*)R, (double2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
-  } else {
-    if (maxNrhs == 1)
-      launchSolve<float2, 1>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
-                             (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
-    else
-      launchSolve<float2, kMaxRhs>(*this, trees, dCells, dShapes, dtasks, ntasks, dFactor32, (float2*)X,
-                                   (float2*)R, (float2*)G, xs, gs, mMaxHeight, mMaxFront, s, &prof, &htasks);
-  }
 }
 
 }  // namespace lsw

@@ -64,6 +64,13 @@ class GlobalIterativeSolverGMRESHIP : public GlobalIterativeSolver {
   void solve(const std::vector<ComplexVector>& rhs, std::vector<ComplexVector>& res) const override;
   // Device API: b and x are global grid vectors on the device.
   void solveDev(const double2* b, double2* x, GmresInfo* info, hipStream_t s = nullptr) const;
+  // Flexible GMRES(restart) for nrhs right-hand sides (B, X: nrhs consecutive
+  // global vectors).  The active right-hand sides advance in lockstep so that
+  // every Arnoldi step applies the preconditioner once to the whole block;
+  // orthogonalisation is classical Gram-Schmidt with one reorthogonalisation
+  // (two batched passes), and each right-hand side stops at its own tolerance.
+  void solveBlockDev(const double2* B, double2* X, int nrhs, std::vector<GmresInfo>* infos,
+                     hipStream_t s = nullptr) const;
   const GmresInfo& lastInfo() const { return mInfo; }
 
  private:
@@ -87,6 +94,14 @@ class GlobalIterativeSolverGMRESHIP : public GlobalIterativeSolver {
   mutable double2* dScal = nullptr;
   mutable long long mN = 0;
   mutable int mVecs = 0;
+  // block workspace
+  mutable double2* dBV = nullptr;
+  mutable double2* dBZ = nullptr;
+  mutable double2* dBT = nullptr;
+  mutable double2* dBPart = nullptr;
+  mutable double2* dBH = nullptr;
+  mutable long long mBN = 0;
+  mutable int mBRhs = 0, mBM = 0;
 };
 
 // device BLAS-1 helpers (FP64 complex), exposed for tests and benchmarks

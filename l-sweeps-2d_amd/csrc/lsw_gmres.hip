@@ -110,7 +110,7 @@ void norm2sq(const double2* a, long long n, double2* part, double2* out, hipStre
 }  // namespace blas
 
 GlobalIterativeSolverGMRESHIP::~GlobalIterativeSolverGMRESHIP() {
-  for (double2* p : {dV, dZ, dT, dPart, dScal}) if (p) (void)hipFree(p);
+  for (double2* p : {dV, dZ, dT, dPart, dScal, dBV, dBZ, dBT, dBPart, dBH}) if (p) (void)hipFree(p);
 }
 
 GlobalIterativeSolver* GlobalIterativeSolverGMRESHIP::copy() const {

@@ -46,6 +46,10 @@ int NdTree::rec(int i0, int i1, int j0, int j1, int parent) {
     for (int i = i0; i < i1; ++i) Bloc.push_back(i * ny + j1);
   std::sort(Bloc.begin(), Bloc.end());
   NdNode& nd = nodes[id];
+  nd.i0 = i0;
+  nd.i1 = i1;
+  nd.j0 = j0;
+  nd.j1 = j1;
   nd.c1 = c1;
   nd.c2 = c2;
   nd.e = (int)Eloc.size();

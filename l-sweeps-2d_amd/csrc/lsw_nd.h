@@ -22,6 +22,7 @@ constexpr int kLeafDofs = 16;
 constexpr int kSolveRows = 64;  // rows per solve work item (one wavefront)
 
 struct NdNode {
+  int i0 = 0, i1 = 0, j0 = 0, j1 = 0;  // region [i0,i1) x [j0,j1) of the subtree
   int e = 0, b = 0;
   int eOff = 0, bOff = 0;   // offsets into E / B pools
   int c1 = -1, c2 = -1, parent = -1;

@@ -94,7 +94,6 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
     tasks[t].nrhs = std::min(kMaxRhs, nrhs - t * kMaxRhs);
     for (int r = 0; r < tasks[t].nrhs; ++r) tasks[t].slot[r] = t * kMaxRhs + r;
   }
-  LSW_HIP(hipMemcpyAsync(dTasks, tasks.data(), ntasks * sizeof(SolveTask), hipMemcpyHostToDevice, mStream));
   if (mPrecision == LSW_FP64) {
     LSW_HIP(hipMemcpyAsync(dX, b, (size_t)nrhs * N * sizeof(double2), hipMemcpyHostToDevice, mStream));
   } else {
@@ -103,7 +102,9 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
     LSW_HIP(hipMemcpyAsync(dX, bf.data(), bf.size() * sizeof(float2), hipMemcpyHostToDevice, mStream));
     LSW_HIP(hipStreamSynchronize(mStream));
   }
-  mFS->solve(dTasks, ntasks, std::min(nrhs, kMaxRhs), dX, dR, dG, N, G, mStream);
+  SolvePlan plan;
+  mFS->buildPlan(tasks, {}, plan);
+  mFS->solve(plan, dX, dR, dG, N, G, mStream);
   if (mPrecision == LSW_FP64) {
     LSW_HIP(hipMemcpyAsync(x, dX, (size_t)nrhs * N * sizeof(double2), hipMemcpyDeviceToHost, mStream));
     LSW_HIP(hipStreamSynchronize(mStream));

@@ -27,7 +27,10 @@ EXPORTED = [
     "lsw_dd_global_coords", "lsw_dd_cell_info", "lsw_dd_assemble_rhs", "lsw_dd_assemble_rhs_dev",
     "lsw_dd_apply_precond", "lsw_dd_apply_precond_dev", "lsw_dd_apply_a", "lsw_dd_apply_a_dev",
     "lsw_dd_stats", "lsw_dd_sync", "lsw_dd_profile_precond",
+    "lsw_dd_apply_precond_block", "lsw_dd_apply_precond_block_dev",
     "lsw_gmres_solve", "lsw_gmres_solve_dev",
+    "lsw_config_num_problems", "lsw_config_problem", "lsw_problem_run",
+    "lsw_gmres_solve_block", "lsw_gmres_solve_block_dev", "lsw_dd_assemble_rhs_block_dev",
 ]
 
 
@@ -61,6 +64,17 @@ class GmresParams(C.Structure):
 class GmresResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("precond_applies", C.c_int), ("matvecs", C.c_int),
                 ("estimate", C.c_double), ("true_residual", C.c_double)]
+
+
+class ProblemSpec(C.Structure):
+    _fields_ = [("dd", DDParams), ("csv_medium", C.c_int), ("solver_mode", C.c_int), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("restart", C.c_int), ("verbose", C.c_int), ("nsources", C.c_int)]
+
+
+class ProblemResult(C.Structure):
+    _fields_ = [("setup_seconds", C.c_double), ("factorize_seconds", C.c_double),
+                ("gmres_seconds", C.c_double), ("iterations", C.c_int), ("estimate", C.c_double),
+                ("true_residual", C.c_double), ("dofs", C.c_longlong), ("Gx", C.c_int), ("Gy", C.c_int)]
 
 
 _lib.lsw_last_error.restype = C.c_char_p
@@ -99,8 +113,61 @@ _sig("lsw_dd_apply_a_dev", _P, _P, _P, _P)
 _sig("lsw_dd_stats", _P, C.POINTER(C.c_longlong), _D, C.POINTER(C.c_longlong))
 _sig("lsw_dd_sync", _P)
 _sig("lsw_dd_profile_precond", _P, _P, _P, _D)
+_sig("lsw_dd_apply_precond_block", _P, C.c_int, _D, _D)
+_sig("lsw_dd_apply_precond_block_dev", _P, C.c_int, _P, _P, _P)
 _sig("lsw_gmres_solve", _P, C.POINTER(GmresParams), _D, _D, C.POINTER(GmresResult), _D, C.c_int)
 _sig("lsw_gmres_solve_dev", _P, C.POINTER(GmresParams), _P, _P, _P, C.POINTER(GmresResult))
+
+
+_sig("lsw_gmres_solve_block", _P, C.POINTER(GmresParams), C.c_int, _D, _D, C.POINTER(GmresResult))
+_sig("lsw_gmres_solve_block_dev", _P, C.POINTER(GmresParams), C.c_int, _P, _P, _P, C.POINTER(GmresResult))
+_sig("lsw_dd_assemble_rhs_block_dev", _P, C.c_int, _D, _P, _P)
+_sig("lsw_config_num_problems", C.c_char_p, C.POINTER(C.c_int))
+_sig("lsw_config_problem", C.c_char_p, C.c_int, C.POINTER(ProblemSpec), _D, C.c_int)
+_sig("lsw_problem_run", C.c_char_p, C.c_int, C.c_int, _D, C.c_longlong, C.POINTER(ProblemResult))
+
+
+def config_num_problems(path):
+    n = C.c_int()
+    _check(_lib.lsw_config_num_problems(path.encode(), C.byref(n)))
+    return n.value
+
+
+def config_problem(path, pid=0, max_sources=4096):
+    """Parse problem `pid` of a libconfig file (host only)."""
+    spec = ProblemSpec()
+    src = np.zeros(2 * max_sources)
+    _check(_lib.lsw_config_problem(path.encode(), pid, C.byref(spec), _dptr(src), max_sources))
+    d = spec.dd
+    return dict(domain=((d.domain[0], d.domain[1]), (d.domain[2], d.domain[3])),
+                ncells=(d.ncells_x, d.ncells_y), nels=(d.nx, d.ny), omega=d.omega,
+                min_wl_in_pml=d.min_wavelengths_in_pml, order=d.order, precision=d.precision,
+                csv_medium=bool(spec.csv_medium), solver_mode=spec.solver_mode, tol=spec.tol,
+                max_iter=spec.max_iter, restart=spec.restart, verbose=bool(spec.verbose),
+                sources=src[:2 * min(spec.nsources, max_sources)].reshape(-1, 2).copy())
+
+
+def problem_run(path, pid=0, verbose=False, want_solution=True):
+    """Problem::read + Problem::solve of a config file on the GPU."""
+    r = ProblemResult()
+    x = None
+    if want_solution:
+        spec = ProblemSpec()
+        _check(_lib.lsw_config_problem(path.encode(), pid, C.byref(spec), None, 0))
+        n = 0
+        if not spec.csv_medium:  # geometry only depends on the PML width, known without the file
+            g = DDCheckerboard(spec.dd, device=False)
+            n = g.N
+        x = np.zeros(max(n, 1) if n else 1 << 26, dtype=np.complex128)
+    _check(_lib.lsw_problem_run(path.encode(), pid, int(verbose),
+                                _dptr(x.view(np.float64)) if x is not None else None,
+                                len(x) if x is not None else 0, C.byref(r)))
+    out = dict(setup_s=r.setup_seconds, factorize_s=r.factorize_seconds, gmres_s=r.gmres_seconds,
+               iterations=r.iterations, estimate=r.estimate, true_residual=r.true_residual, dofs=r.dofs,
+               Gx=r.Gx, Gy=r.Gy)
+    if x is not None:
+        out["x"] = x[:r.dofs].copy()
+    return out
 
 
 def _check(rc):
@@ -238,6 +305,17 @@ class DDCheckerboard:
         _check(_lib.lsw_dd_apply_precond(self._h, _dptr(x.view(np.float64)), _dptr(y.view(np.float64))))
         return y
 
+    def apply_precond_block(self, X):
+        """X: (nrhs, N) complex array; returns M^{-1} applied to every row."""
+        X = np.ascontiguousarray(X, dtype=np.complex128)
+        Y = np.zeros_like(X)
+        _check(_lib.lsw_dd_apply_precond_block(self._h, X.shape[0], _dptr(X.view(np.float64)),
+                                               _dptr(Y.view(np.float64))))
+        return Y
+
+    def apply_precond_block_dev(self, nrhs, x_ptr, y_ptr, stream=0):
+        _check(_lib.lsw_dd_apply_precond_block_dev(self._h, nrhs, x_ptr, y_ptr, stream or None))
+
     def apply_a(self, x):
         x = np.ascontiguousarray(x, dtype=np.complex128)
         y = np.zeros_like(x)
@@ -261,6 +339,28 @@ class DDCheckerboard:
         n = r.iterations + 1 if mode == 0 else len(log)
         return x, dict(iterations=r.iterations, precond_applies=r.precond_applies, matvecs=r.matvecs,
                        estimate=r.estimate, true_residual=r.true_residual, log=log[:n].copy())
+
+    def gmres_block(self, B, tol=1e-6, max_iter=60, restart=30, verbose=False):
+        """Flexible GMRES for the rows of B (nrhs, N) with shared block preconditioning."""
+        B = np.ascontiguousarray(B, dtype=np.complex128)
+        X = np.zeros_like(B)
+        p = GmresParams(1, tol, max_iter, restart, int(verbose))
+        res = (GmresResult * B.shape[0])()
+        _check(_lib.lsw_gmres_solve_block(self._h, C.byref(p), B.shape[0], _dptr(B.view(np.float64)),
+                                          _dptr(X.view(np.float64)), res))
+        return X, [dict(iterations=r.iterations, precond_applies=r.precond_applies, matvecs=r.matvecs,
+                        estimate=r.estimate, true_residual=r.true_residual) for r in res]
+
+    def gmres_block_dev(self, nrhs, b_ptr, x_ptr, tol=1e-6, max_iter=60, restart=30, stream=0):
+        p = GmresParams(1, tol, max_iter, restart, 0)
+        res = (GmresResult * nrhs)()
+        _check(_lib.lsw_gmres_solve_block_dev(self._h, C.byref(p), nrhs, b_ptr, x_ptr, stream or None, res))
+        return [dict(iterations=r.iterations, precond_applies=r.precond_applies, matvecs=r.matvecs,
+                     estimate=r.estimate, true_residual=r.true_residual) for r in res]
+
+    def assemble_rhs_block_dev(self, sources, b_ptr, stream=0):
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 2))
+        _check(_lib.lsw_dd_assemble_rhs_block_dev(self._h, src.shape[0], _dptr(src), b_ptr, stream or None))
 
     # device-pointer variants (pointers are integers, e.g. torch tensor.data_ptr())
     def apply_precond_dev(self, x_ptr, y_ptr, stream=0):

@@ -20,9 +20,11 @@ SOURCES = [
     "lsw_nd.cpp",
     "lsw_model.cpp",
     "lsw_factor.hip",
+    "lsw_ndsolve.hip",
     "lsw_dd.hip",
     "lsw_solver.hip",
     "lsw_gmres.hip",
+    "lsw_gmres_block.hip",
     "lsw_input.cpp",
     "lsw_capi.cpp",
 ]
