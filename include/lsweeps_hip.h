@@ -74,6 +74,8 @@ typedef struct lsw_dd_params {
   const float* velocity;         /* medium_kind 1: vel_rows x vel_cols, row 0 = top */
   int vel_rows, vel_cols;
   double vel_domain[4];          /* file domain ax, bx, ay, by */
+  int geometry;                  /* 0: the reference's spacing rule (needs NCellsX*(NX+1) ==
+                                    NCellsY*(NY+1)); 1: per-axis spacing (rectangular domains) */
 } lsw_dd_params;
 
 int lsw_dd_create(const lsw_dd_params* p, lsw_dd** out);

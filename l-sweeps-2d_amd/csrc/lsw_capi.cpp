@@ -156,6 +156,8 @@ static int ddCreate(const lsw_dd_params* p, lsw_dd** out, bool device) {
     LocalLinearSolverHIP proto(p->precision);
     auto h = std::make_unique<lsw_dd>();
     h->dd = std::make_unique<DDCheckerboard>();
+    LSW_REQUIRE(p->geometry == 0 || p->geometry == 1, LSW_ERR_ARG, "unknown geometry mode");
+    h->dd->setGeneralGeometry(p->geometry == 1);
     if (device)
       h->dd->setUp(&as, &proto, dom, {(UInt)p->ncells_x, (UInt)p->ncells_y}, {(UInt)p->nx, (UInt)p->ny});
     else

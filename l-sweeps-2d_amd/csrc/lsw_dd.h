@@ -94,6 +94,9 @@ class DDCheckerboard : public DD {
                      std::vector<DoubleVector> domain, std::vector<UInt> nCells,
                      std::vector<UInt> nElsInCell);
   bool deviceReady() const { return mFactors != nullptr; }
+  // Geometry mode, set before setUp: false = the reference's spacing rule,
+  // true = spacing per axis from that axis' cell count (rectangular domains).
+  void setGeneralGeometry(bool g) { mGeneral = g; }
 
   void factorize() override;
   void assembleRHS(const RHS* rhs, std::vector<ComplexVector>& res) const override;
@@ -148,6 +151,7 @@ class DDCheckerboard : public DD {
   std::vector<unsigned char> mColT, mRowT;
 
   std::vector<DoubleVector> mDomain;
+  bool mGeneral = false;
   int mNR = 0, mNC = 0;
   int mGx = 0, mGy = 0;
   double mMeshSize = 0;

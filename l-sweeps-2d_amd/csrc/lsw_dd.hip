@@ -285,8 +285,16 @@ void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalL
   mNR = (int)nCells[0];
   mNC = (int)nCells[1];
   LSW_REQUIRE(mNR >= 1 && mNC >= 1, LSW_ERR_ARG, "DDCheckerboard: empty board");
-  const UInt n1 = nCells[0] * nElsInCell[0] + nCells[0] - 1;
-  const UInt n2 = nCells[1] * nElsInCell[1] + nCells[1] - 1;
+  // Reference mode (src/DDCheckerboard.cpp:119-124): the x spacing comes from
+  // the row count and NX.  It tiles consistently only when
+  // NCellsX*(NX+1) == NCellsY*(NY+1); general mode takes the x spacing from the
+  // column count and NY (the same grid whenever the reference's is consistent).
+  const UInt n1 = mGeneral ? nCells[1] * nElsInCell[1] + nCells[1] - 1
+                           : nCells[0] * nElsInCell[0] + nCells[0] - 1;
+  const UInt n2 = mGeneral ? nCells[0] * nElsInCell[0] + nCells[0] - 1
+                           : nCells[1] * nElsInCell[1] + nCells[1] - 1;
+  LSW_REQUIRE(mGeneral || nCells[0] * (nElsInCell[0] + 1) == nCells[1] * (nElsInCell[1] + 1), LSW_ERR_UNSUPPORTED,
+              "inconsistent tiling: NCellsX*(NX+1) != NCellsY*(NY+1) (use the general geometry mode)");
   const double h1 = (domain[0][1] - domain[0][0]) / n1;
   const double h2 = (domain[1][1] - domain[1][0]) / n2;
   const UInt NX = nElsInCell[0], NY = nElsInCell[1];

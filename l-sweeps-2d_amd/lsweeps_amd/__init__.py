@@ -53,6 +53,7 @@ class DDParams(C.Structure):
         ("velocity", C.POINTER(C.c_float)),
         ("vel_rows", C.c_int), ("vel_cols", C.c_int),
         ("vel_domain", C.c_double * 4),
+        ("geometry", C.c_int),
     ]
 
 
@@ -236,8 +237,10 @@ class LocalLinearSolverHIP:
 
 # --------------------------------------------------------------------------
 def dd_params(domain=((0.0, 1.0), (0.0, 1.0)), ncells=(4, 4), nels=(9, 9), omega=2 * np.pi * 4.1,
-              min_wl_in_pml=2.0, precision=LSW_PREC_FP64, velocity=None, vel_domain=None, order=2):
+              min_wl_in_pml=2.0, precision=LSW_PREC_FP64, velocity=None, vel_domain=None, order=2,
+              general=False):
     p = DDParams()
+    p.geometry = 1 if general else 0
     p.domain[:] = [domain[0][0], domain[0][1], domain[1][0], domain[1][1]]
     p.ncells_x, p.ncells_y = int(ncells[0]), int(ncells[1])
     p.nx, p.ny = int(nels[0]), int(nels[1])

@@ -53,4 +53,65 @@ def config_const_400():
                 sources=[(0.5, 0.5)], tol=1e-8, max_iter=30, grid=(400, 400))
 
 
-CONFIGS = {"lens1600": config_lens_1600, "const400": config_const_400}
+def layered_velocity(nx, ny, Lx, Ly, seed, n_layers=10, vmin=1.5, vmax=4.5, smooth_cells=5.0,
+                     field_std=0.05):
+    """Seeded layered model on nx x ny cells over [0,Lx]x[0,Ly] (row 0 = top).
+
+    n_layers-1 interface depths drawn uniform in (0, Ly) and sorted; each
+    interface is perturbed by 3 sinusoids of random amplitude (at most 4% of
+    its depth), frequency and phase.  Layer velocities rise linearly from vmin
+    to vmax plus U(-0.3, 0.3); a Gaussian-smoothed (sigma = smooth_cells)
+    random field scales the result by (1 + field) with field std field_std.
+    """
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    depths = np.sort(rng.uniform(0.0, Ly, n_layers - 1))
+    x = (np.arange(nx) + 0.5) * (Lx / nx)
+    z = (np.arange(ny) + 0.5) * (Ly / ny)           # depth below the top, row order
+    ifaces = []
+    for d in depths:
+        zk = np.full(nx, d)
+        for _ in range(3):
+            amp = rng.uniform(0.0, 0.04 * d)
+            freq = rng.uniform(0.5, 3.0)
+            ph = rng.uniform(0.0, 2 * math.pi)
+            zk = zk + amp * np.sin(2 * math.pi * freq * x / Lx + ph)
+        ifaces.append(zk)
+    ifaces = np.array(ifaces)                        # (n_layers-1, nx)
+    layer = (z[:, None, None] > ifaces.T[None, :, :]).sum(axis=2)  # (ny, nx)
+    v = vmin + (vmax - vmin) * np.arange(n_layers) / (n_layers - 1) + rng.uniform(-0.3, 0.3, n_layers)
+    c = v[layer]
+    f = gaussian_filter(rng.standard_normal((ny, nx)), smooth_cells, mode="nearest")
+    f *= field_std / max(f.std(), 1e-30)
+    c = c * (1.0 + f)
+    return c.astype(np.float32)
+
+
+def config_layered_3x1(seed=0, nsrc=128, ppw=8.0, cells=(4, 12), cell_nodes=200):
+    """The layered 3x1 workload: 12 x 4 cells of 200 x 200 nodes.
+
+    The reference tiling needs n = cells*(nodes per cell) - 1 intervals per
+    axis, so 2401 x 801 nodes at h = 1/800 cannot be tiled by 12 x 4 equal
+    cells; the closest tiling is 2400 x 800 nodes at h = 1/799 on
+    [0, 2399/799] x [0, 1] (general geometry mode).  omega gives `ppw` points
+    per wavelength at the minimum velocity, 20 PML cells, `nsrc` sources at
+    depth 10 cells.
+    """
+    nR, nC = cells
+    nyn, nxn = nR * cell_nodes, nC * cell_nodes
+    h = 1.0 / (nyn - 1)
+    Lx, Ly = (nxn - 1) * h, 1.0
+    vel = layered_velocity(nxn, nyn, Lx + h, Ly + h, seed)   # cells centred on the nodes
+    vdom = ((-h / 2, Lx + h / 2), (-h / 2, Ly + h / 2))
+    cmin, cmax = float(vel.min()), float(vel.max())
+    omega = 2 * math.pi * cmin / (ppw * h)
+    wl = 20 * h * omega / (2 * math.pi * cmax)
+    xs = (np.arange(nsrc) + 0.5) * Lx / nsrc
+    srcs = [(float(x), Ly - 10 * h) for x in xs]
+    return dict(name=f"layered{nxn}x{nyn}_s{seed}", domain=((0.0, Lx), (0.0, Ly)), ncells=(nR, nC),
+                nels=(cell_nodes - 1, cell_nodes - 1), omega=omega, min_wl_in_pml=wl, velocity=vel,
+                vel_domain=vdom, sources=srcs, tol=1e-6, max_iter=60, grid=(nxn, nyn), general=True,
+                ppw=ppw, seed=seed)
+
+
+CONFIGS = {"lens1600": config_lens_1600, "const400": config_const_400, "layered": config_layered_3x1}

@@ -352,7 +352,7 @@ class Checkerboard:
     """
 
     def __init__(self, domain, ncells, nels, omega, min_wl_in_pml, medium, order=2,
-                 assemble=True):
+                 assemble=True, general=False):
         if order != 2:
             raise ValueError("only Order 2 is supported")
         tt = int(math.ceil(order / 2.0))
@@ -371,8 +371,14 @@ class Checkerboard:
         wl = min(wl1, wl2)
         wl /= math.sqrt(medium.min_real())
         self.pml_width_in = min_wl_in_pml * wl
-        n1 = self.nR * self.nels[0] + self.nR - 1
-        n2 = self.nC * self.nels[1] + self.nC - 1
+        if general:
+            # per-axis spacing: x from the columns and NY, y from the rows and NX
+            # (coincides with the reference's rule when nR(NX+1) == nC(NY+1))
+            n1 = self.nC * self.nels[1] + self.nC - 1
+            n2 = self.nR * self.nels[0] + self.nR - 1
+        else:
+            n1 = self.nR * self.nels[0] + self.nR - 1
+            n2 = self.nC * self.nels[1] + self.nC - 1
         self.n_el = (n1, n2)
         h1 = (dom[0][1] - dom[0][0]) / n1
         h2 = (dom[1][1] - dom[1][0]) / n2

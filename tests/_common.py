@@ -16,16 +16,16 @@ def lens_velocity(n, amp=0.4, width=0.02, cx=0.5, cy=0.5, dom=((0.0, 1.0), (0.0,
 
 
 def make_pair(ncells, nels, omega, wl=2.0, velocity=None, precision=0, device=True,
-              dom=((0.0, 1.0), (0.0, 1.0))):
+              dom=((0.0, 1.0), (0.0, 1.0)), general=False):
     """(oracle Checkerboard, lsweeps_amd.DDCheckerboard) on the same inputs."""
     import lsweeps_amd as L
     if velocity is None:
         med = ConstantMedium()
     else:
         med = CSVMedium(velocity, [list(dom[0]), list(dom[1])])
-    cb = Checkerboard([list(dom[0]), list(dom[1])], ncells, nels, omega, wl, med)
+    cb = Checkerboard([list(dom[0]), list(dom[1])], ncells, nels, omega, wl, med, general=general)
     p = L.dd_params(domain=dom, ncells=ncells, nels=nels, omega=omega, min_wl_in_pml=wl,
-                    precision=precision, velocity=velocity, vel_domain=dom)
+                    precision=precision, velocity=velocity, vel_domain=dom, general=general)
     dd = L.DDCheckerboard(p, device=device)
     return cb, dd
 

@@ -80,3 +80,25 @@ def test_error_paths():
     with pytest.raises(L.LswError) as e:
         s.set_matrix(sp.random(20, 20, density=0.3, format="csc", random_state=1) + sp.eye(20))
     assert e.value.code == 6
+
+
+def test_general_geometry_rectangular_domain():
+    """3x1 domain, isotropic h = 1/29: 2 rows of NX=14 and 4 columns of NY=21."""
+    dom = ((0.0, 3.0), (0.0, 1.0))
+    cb, dd = make_pair((2, 4), (14, 21), 2 * math.pi * 2.2, device=False, dom=dom, general=True)
+    assert (dd.Gx, dd.Gy) == (cb.Gx, cb.Gy)
+    assert cb.h[0] == pytest.approx(cb.h[1], rel=1e-12)
+    for k, c in enumerate(cb.cells):
+        inf = dd.cell_info(k)
+        assert (inf["nx"], inf["ny"], inf["ox"], inf["oy"]) == (c.nx, c.ny, c.ox, c.oy)
+    for shift in ((False, False), (True, True)):
+        owned = np.concatenate([c.gidx[c.keep[shift]] for c in cb.cells])
+        assert len(np.unique(owned)) == cb.N == len(owned)
+
+
+def test_reference_geometry_rejects_inconsistent_tiling():
+    import lsweeps_amd as L
+    with pytest.raises(L.LswError) as e:
+        L.DDCheckerboard(L.dd_params(domain=((0.0, 3.0), (0.0, 1.0)), ncells=(2, 4), nels=(14, 21)),
+                         device=False)
+    assert e.value.code == 6
