@@ -189,6 +189,61 @@ int lsw_config_problem(const char* path, int id, lsw_problem_spec* spec, double*
  * entries) receives the solution on the global grid. */
 int lsw_problem_run(const char* path, int id, int verbose, double* x, long long x_cap, lsw_problem_result* res);
 
+/* ---- Serving ------------------------------------------------------------
+ * No counterpart in the reference, whose driver reads, factorises and solves
+ * one problem at a time (src/Problem.cpp:70-158).  A server keeps factorised
+ * problems on the device (cache keyed by the registered problem = model and
+ * omega), admits a problem when its estimated device bytes fit budget_bytes
+ * (evicting idle problems, least recently used first), runs every source of
+ * every request as one right-hand side of flexible GMRES, and batches the
+ * right-hand sides of a problem per Arnoldi step: they join free slots between
+ * steps and leave when their true relative residual is <= tol.  `workers`
+ * host threads step different problems at once on their own streams.
+ * submit/wait may be called from any thread. */
+typedef struct lsw_server lsw_server;
+
+typedef struct lsw_server_options {
+  double budget_bytes; /* device bytes the cache may reserve */
+  int slots;           /* right-hand sides in flight per problem (1..64) */
+  int restart;         /* GMRES restart length (1..32) */
+  double tol;          /* true relative residual target */
+  int max_iter;        /* Arnoldi steps per right-hand side */
+  int precision;       /* LSW_PREC_FP32 (mixed) or LSW_PREC_FP64 */
+  int workers;         /* host threads (1..16) */
+} lsw_server_options;
+
+typedef struct lsw_request_result {
+  int status;          /* LSW_OK or the error that failed the request */
+  int problem;
+  int nsrc;
+  int max_iterations;  /* over the request's right-hand sides */
+  double max_true_residual;
+  double submit_s, start_s, end_s; /* seconds since lsw_server_create */
+  char message[160];
+} lsw_request_result;
+
+typedef struct lsw_server_stats {
+  long long requests, completed, solves, loads, evictions, steps;
+  double reserved_bytes, peak_reserved_bytes;
+  int resident;
+} lsw_server_stats;
+
+int lsw_server_create(const lsw_server_options* opt, lsw_server** out);
+/* Stops the workers after their current step; pending requests are dropped. */
+int lsw_server_destroy(lsw_server* s);
+/* Registers a problem (parameters and velocity grid copied; precision taken
+ * from the options); Gx, Gy: global grid of its solutions. */
+int lsw_server_add_problem(lsw_server* s, const lsw_dd_params* p, int* problem, int* Gx, int* Gy);
+/* Estimated device bytes of a registered problem: steady (factors, sweep and
+ * Krylov buffers) and peak (while factorising); admission reserves these. */
+int lsw_server_problem_bytes(lsw_server* s, int problem, double* steady, double* peak);
+/* Queues nsrc point sources (src_xy: x0, y0, x1, y1, ...); x (optional, host)
+ * receives nsrc global vectors and must stay valid until the wait returns. */
+int lsw_server_submit(lsw_server* s, int problem, int nsrc, const double* src_xy, double* x, long long* request);
+/* Blocks until the request is done (each request id is waited once). */
+int lsw_server_wait(lsw_server* s, long long request, lsw_request_result* res);
+int lsw_server_get_stats(lsw_server* s, lsw_server_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

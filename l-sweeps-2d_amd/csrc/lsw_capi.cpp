@@ -10,6 +10,7 @@
 #include "lsw_gmres.h"
 #include "lsw_input.h"
 #include "lsw_model.h"
+#include "lsw_serve.h"
 #include "lsw_solver.h"
 
 using namespace lsw;
@@ -22,6 +23,10 @@ struct lsw_lu {
 struct lsw_dd {
   std::unique_ptr<DDCheckerboard> dd;
   GlobalIterativeSolverGMRESHIP gmres;
+};
+
+struct lsw_server {
+  std::unique_ptr<Server> srv;
 };
 
 namespace {
@@ -122,46 +127,8 @@ int lsw_lu_destroy(lsw_lu* lu) {
 static int ddCreate(const lsw_dd_params* p, lsw_dd** out, bool device) {
   return guarded([&] {
     LSW_REQUIRE(p && out, LSW_ERR_ARG, "null pointer");
-    LSW_REQUIRE(p->precision == LSW_PREC_FP64 || p->precision == LSW_PREC_FP32, LSW_ERR_ARG, "unknown precision");
-    LSW_REQUIRE(p->omega > 0 && p->min_wavelengths_in_pml >= 0, LSW_ERR_ARG, "bad Omega / MinWavelengthsInPML");
-    LSW_REQUIRE(p->order == 2, LSW_ERR_UNSUPPORTED, "only Order 2 is supported");
-    std::vector<DoubleVector> dom = {{p->domain[0], p->domain[1]}, {p->domain[2], p->domain[3]}};
-    LSW_REQUIRE(dom[0][1] > dom[0][0] && dom[1][1] > dom[1][0], LSW_ERR_ARG, "empty domain");
-    std::unique_ptr<Function> m;
-    if (p->medium_kind == 0) {
-      m.reset(new FunctionConstant(1.0));
-    } else if (p->medium_kind == 1) {
-      LSW_REQUIRE(p->velocity && p->vel_rows > 0 && p->vel_cols > 0, LSW_ERR_ARG, "missing velocity grid");
-      auto* f = new FunctionCSV();
-      f->setData(p->velocity, p->vel_rows, p->vel_cols,
-                 {{p->vel_domain[0], p->vel_domain[1]}, {p->vel_domain[2], p->vel_domain[3]}});
-      m.reset(f);
-    } else {
-      throw Error(LSW_ERR_ARG, "unknown medium kind");
-    }
-    // Input::readAssemblerHelmholtz2DFD (src/Input.cpp:795-857)
-    PMLCubic2D pml;
-    pml.setAbsorptionCoeff(std::log(p->omega));
-    const double wl1 = (dom[0][1] - dom[0][0]) * 2 * kPi / p->omega;
-    const double wl2 = (dom[1][1] - dom[1][0]) * 2 * kPi / p->omega;
-    double wl = std::min(wl1, wl2);
-    wl /= std::sqrt(m->getMinReal());
-    pml.setWidth(p->min_wavelengths_in_pml * wl);
-    AssemblerHelmholtz2DFD as;
-    as.setPML(&pml);
-    as.setOmega(p->omega);
-    m->setDomain(dom);
-    as.setM(m.get());
-    as.setOrder((UInt)p->order);
-    LocalLinearSolverHIP proto(p->precision);
     auto h = std::make_unique<lsw_dd>();
-    h->dd = std::make_unique<DDCheckerboard>();
-    LSW_REQUIRE(p->geometry == 0 || p->geometry == 1, LSW_ERR_ARG, "unknown geometry mode");
-    h->dd->setGeneralGeometry(p->geometry == 1);
-    if (device)
-      h->dd->setUp(&as, &proto, dom, {(UInt)p->ncells_x, (UInt)p->ncells_y}, {(UInt)p->nx, (UInt)p->ny});
-    else
-      h->dd->setUpGeometry(&as, &proto, dom, {(UInt)p->ncells_x, (UInt)p->ncells_y}, {(UInt)p->nx, (UInt)p->ny});
+    h->dd = makeCheckerboard(*p, device);
     h->gmres.setParent(h->dd.get());
     *out = h.release();
   });
@@ -543,6 +510,82 @@ int lsw_problem_run(const char* path, int id, int verbose, double* x, long long 
       LSW_REQUIRE(x_cap >= (long long)pb.solution().size(), LSW_ERR_CAPACITY, "solution buffer too small");
       std::memcpy(x, pb.solution().data(), pb.solution().size() * sizeof(Complex));
     }
+  });
+}
+
+// ------------------------------------------------------------------ serving
+int lsw_server_create(const lsw_server_options* o, lsw_server** out) {
+  return guarded([&] {
+    LSW_REQUIRE(o && out, LSW_ERR_ARG, "null pointer");
+    ServeOptions so;
+    so.budgetBytes = o->budget_bytes;
+    so.slots = o->slots;
+    so.restart = o->restart;
+    so.tol = o->tol;
+    so.maxIter = o->max_iter;
+    so.precision = o->precision;
+    so.workers = o->workers;
+    auto h = std::make_unique<lsw_server>();
+    h->srv = std::make_unique<Server>(so);
+    *out = h.release();
+  });
+}
+
+int lsw_server_destroy(lsw_server* s) {
+  return guarded([&] { delete s; });
+}
+
+int lsw_server_add_problem(lsw_server* s, const lsw_dd_params* p, int* problem, int* Gx, int* Gy) {
+  return guarded([&] {
+    LSW_REQUIRE(s && p && problem, LSW_ERR_ARG, "null pointer");
+    *problem = s->srv->addProblem(*p, Gx, Gy);
+  });
+}
+
+int lsw_server_problem_bytes(lsw_server* s, int problem, double* steady, double* peak) {
+  return guarded([&] {
+    LSW_REQUIRE(s, LSW_ERR_ARG, "null pointer");
+    s->srv->problemBytes(problem, steady, peak);
+  });
+}
+
+int lsw_server_submit(lsw_server* s, int problem, int nsrc, const double* src_xy, double* x, long long* request) {
+  return guarded([&] {
+    LSW_REQUIRE(s && src_xy && request, LSW_ERR_ARG, "null pointer");
+    *request = s->srv->submit(problem, nsrc, src_xy, x);
+  });
+}
+
+int lsw_server_wait(lsw_server* s, long long request, lsw_request_result* res) {
+  return guarded([&] {
+    LSW_REQUIRE(s && res, LSW_ERR_ARG, "null pointer");
+    const ServeResult r = s->srv->wait(request);
+    res->status = r.status;
+    res->problem = r.problem;
+    res->nsrc = r.nsrc;
+    res->max_iterations = r.maxIterations;
+    res->max_true_residual = r.maxTrueResidual;
+    res->submit_s = r.submitS;
+    res->start_s = r.startS;
+    res->end_s = r.endS;
+    std::memset(res->message, 0, sizeof(res->message));
+    std::strncpy(res->message, r.message.c_str(), sizeof(res->message) - 1);
+  });
+}
+
+int lsw_server_get_stats(lsw_server* s, lsw_server_stats* st) {
+  return guarded([&] {
+    LSW_REQUIRE(s && st, LSW_ERR_ARG, "null pointer");
+    const ServeStats v = s->srv->stats();
+    st->requests = v.requests;
+    st->completed = v.completed;
+    st->solves = v.solves;
+    st->loads = v.loads;
+    st->evictions = v.evictions;
+    st->steps = v.steps;
+    st->reserved_bytes = v.reservedBytes;
+    st->peak_reserved_bytes = v.peakReservedBytes;
+    st->resident = v.resident;
   });
 }
 

@@ -117,6 +117,15 @@ class DDCheckerboard : public DD {
   // Same, timing every batched-solve launch with HIP events (synchronises
   // after each sweep step; use outside timed regions).
   void applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof, int nrhs = 1) const;
+  // Sizes the sweep buffers for blocks of up to nrhs right-hand sides now
+  // (otherwise they grow on first use).
+  void reserveBlock(int nrhs) const { ensureBuffers(nrhs); }
+  // Device bytes of this DD once factorised, serving blocks of nrhs
+  // right-hand sides (a host estimate from the nested-dissection trees of the
+  // cell shapes; valid after setUpGeometry): steady = factors in the solve
+  // precision + stencils + sweep buffers; peak adds what factorisation holds
+  // for a while (FP64 factors before the FP32 copy, front workspace).
+  void deviceBytesEstimate(int precision, int nrhs, double* steady, double* peak) const;
   // y = A x for nrhs vectors
   void applyADevBlock(const double2* x, double2* y, int nrhs, hipStream_t s) const;
   // y = A x (global operator)

@@ -886,6 +886,39 @@ void DDCheckerboard::allocBuffers() {
   ensureBuffers(1);
 }
 
+void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady, double* peak) const {
+  LSW_REQUIRE(!mCells.empty(), LSW_ERR_STATE, "deviceBytesEstimate before setUp");
+  std::map<std::pair<int, int>, std::unique_ptr<NdTree>> trees;
+  double entries = 0, schur = 0, local = 0;
+  std::vector<double> wsPerHeight(kMaxHeight + 1, 0.0);
+  for (const auto& c : mCells) {
+    auto& t = trees[{c.nx, c.ny}];
+    if (!t) {
+      t = std::make_unique<NdTree>();
+      t->build(c.nx, c.ny);
+    }
+    entries += (double)t->factorSize;
+    schur += (double)t->schurSize;
+    local += (double)c.nx * c.ny;
+    for (const auto& nd : t->nodes)
+      if (nd.height <= kMaxHeight) wsPerHeight[nd.height] += (double)(nd.e + nd.b) * (2 * nd.e + nd.b);
+  }
+  const double es = precision == LSW_FP32 ? 8.0 : 16.0;
+  const double n = (double)N();
+  const double r = std::max(1, nrhs);
+  // stencils (local + global), squared slowness, sweep buffers (X, R, G and
+  // traces per block column, 4 accumulators of global vectors)
+  const double fixed = 80.0 * local + 8.0 * local + 80.0 * n;
+  const double sweep = r * (3.0 * es * local + 64.0 * n);
+  const double s = entries * es + fixed + sweep;
+  double ws = 0;
+  for (double w : wsPerHeight) ws = std::max(ws, w);
+  ws = std::min(ws, (double)(512LL << 20)) * 16.0;
+  const double f = fixed + entries * 16.0 + schur * 16.0 + ws + (precision == LSW_FP32 ? entries * 8.0 : 0.0);
+  if (steady) *steady = s;
+  if (peak) *peak = std::max(s, f);
+}
+
 void DDCheckerboard::factorize() {
   LSW_REQUIRE(mFactors != nullptr, LSW_ERR_STATE, "setUp must be called before factorize");
   if (!mFactors->factorized()) mFactors->factorize(mPrecision, mStream);

@@ -1,0 +1,361 @@
+// Server: continuous-batching request loop over the L-sweeps solve path
+// (see lsw_serve.h).
+#include "lsw_serve.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lsw_model.h"
+#include "lsw_solver.h"
+
+namespace lsw {
+
+std::unique_ptr<DDCheckerboard> makeCheckerboard(const lsw_dd_params& p, bool device) {
+  LSW_REQUIRE(p.precision == LSW_PREC_FP64 || p.precision == LSW_PREC_FP32, LSW_ERR_ARG, "unknown precision");
+  LSW_REQUIRE(p.omega > 0 && p.min_wavelengths_in_pml >= 0, LSW_ERR_ARG, "bad Omega / MinWavelengthsInPML");
+  LSW_REQUIRE(p.order == 2, LSW_ERR_UNSUPPORTED, "only Order 2 is supported");
+  LSW_REQUIRE(p.ncells_x >= 1 && p.ncells_y >= 1 && p.nx >= 1 && p.ny >= 1, LSW_ERR_ARG, "bad cell counts");
+  std::vector<DoubleVector> dom = {{p.domain[0], p.domain[1]}, {p.domain[2], p.domain[3]}};
+  LSW_REQUIRE(dom[0][1] > dom[0][0] && dom[1][1] > dom[1][0], LSW_ERR_ARG, "empty domain");
+  std::unique_ptr<Function> m;
+  if (p.medium_kind == 0) {
+    m.reset(new FunctionConstant(1.0));
+  } else if (p.medium_kind == 1) {
+    LSW_REQUIRE(p.velocity && p.vel_rows > 0 && p.vel_cols > 0, LSW_ERR_ARG, "missing velocity grid");
+    auto* f = new FunctionCSV();
+    f->setData(p.velocity, p.vel_rows, p.vel_cols,
+               {{p.vel_domain[0], p.vel_domain[1]}, {p.vel_domain[2], p.vel_domain[3]}});
+    m.reset(f);
+  } else {
+    throw Error(LSW_ERR_ARG, "unknown medium kind");
+  }
+  // Input::readAssemblerHelmholtz2DFD (src/Input.cpp:795-857): C = ln(omega),
+  // width = MinWavelengthsInPML * min(Lx, Ly) * 2 pi / omega / sqrt(min m)
+  PMLCubic2D pml;
+  pml.setAbsorptionCoeff(std::log(p.omega));
+  const double wl1 = (dom[0][1] - dom[0][0]) * 2 * kPi / p.omega;
+  const double wl2 = (dom[1][1] - dom[1][0]) * 2 * kPi / p.omega;
+  double wl = std::min(wl1, wl2);
+  wl /= std::sqrt(m->getMinReal());
+  pml.setWidth(p.min_wavelengths_in_pml * wl);
+  AssemblerHelmholtz2DFD as;
+  as.setPML(&pml);
+  as.setOmega(p.omega);
+  m->setDomain(dom);
+  as.setM(m.get());
+  as.setOrder((UInt)p.order);
+  LocalLinearSolverHIP proto(p.precision);
+  auto dd = std::make_unique<DDCheckerboard>();
+  LSW_REQUIRE(p.geometry == 0 || p.geometry == 1, LSW_ERR_ARG, "unknown geometry mode");
+  dd->setGeneralGeometry(p.geometry == 1);
+  if (device)
+    dd->setUp(&as, &proto, dom, {(UInt)p.ncells_x, (UInt)p.ncells_y}, {(UInt)p.nx, (UInt)p.ny});
+  else
+    dd->setUpGeometry(&as, &proto, dom, {(UInt)p.ncells_x, (UInt)p.ncells_y}, {(UInt)p.nx, (UInt)p.ny});
+  return dd;
+}
+
+Server::Server(const ServeOptions& opt) : mOpt(opt), mT0(std::chrono::steady_clock::now()) {
+  LSW_REQUIRE(opt.budgetBytes > 0, LSW_ERR_ARG, "serve: budget must be positive");
+  LSW_REQUIRE(opt.slots >= 1 && opt.slots <= kMaxSlots, LSW_ERR_ARG, "serve: slots must be in [1, 64]");
+  LSW_REQUIRE(opt.workers >= 1 && opt.workers <= 16, LSW_ERR_ARG, "serve: workers must be in [1, 16]");
+  LSW_REQUIRE(opt.precision == LSW_PREC_FP64 || opt.precision == LSW_PREC_FP32, LSW_ERR_ARG,
+              "serve: unknown precision");
+  for (int w = 0; w < opt.workers; ++w) mWorkers.emplace_back([this] { workerLoop(); });
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lk(mMu);
+    mStop = true;
+  }
+  mCv.notify_all();
+  for (auto& t : mWorkers) t.join();
+  for (auto& e : mEntries) {
+    e->slots.reset();
+    e->dd.reset();
+  }
+}
+
+double Server::now() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - mT0).count();
+}
+
+int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
+  auto e = std::make_unique<Entry>();
+  e->params = p;
+  e->params.precision = mOpt.precision;
+  if (p.medium_kind == 1) {
+    LSW_REQUIRE(p.velocity && p.vel_rows > 0 && p.vel_cols > 0, LSW_ERR_ARG, "missing velocity grid");
+    e->velocity.assign(p.velocity, p.velocity + (size_t)p.vel_rows * p.vel_cols);
+    e->params.velocity = e->velocity.data();
+  }
+  // host geometry: grid size and the device-bytes estimate
+  auto geo = makeCheckerboard(e->params, false);
+  e->Gx = geo->Gx();
+  e->Gy = geo->Gy();
+  geo->deviceBytesEstimate(mOpt.precision, mOpt.slots, &e->steadyBytes, &e->peakBytes);
+  e->steadyBytes += (double)GmresSlots::bytesFor(geo->N(), mOpt.slots, mOpt.restart);
+  e->peakBytes = std::max(e->peakBytes, e->steadyBytes);
+  LSW_REQUIRE(e->peakBytes <= mOpt.budgetBytes, LSW_ERR_CAPACITY,
+              "serve: problem needs more device memory than the budget");
+  if (Gx) *Gx = e->Gx;
+  if (Gy) *Gy = e->Gy;
+  std::lock_guard<std::mutex> lk(mMu);
+  mEntries.push_back(std::move(e));
+  return (int)mEntries.size() - 1;
+}
+
+void Server::problemBytes(int problem, double* steady, double* peak) const {
+  std::lock_guard<std::mutex> lk(mMu);
+  LSW_REQUIRE(problem >= 0 && problem < (int)mEntries.size(), LSW_ERR_ARG, "serve: unknown problem");
+  if (steady) *steady = mEntries[problem]->steadyBytes;
+  if (peak) *peak = mEntries[problem]->peakBytes;
+}
+
+long long Server::submit(int problem, int nsrc, const double* xy, double* xOut) {
+  LSW_REQUIRE(nsrc >= 1 && xy != nullptr, LSW_ERR_ARG, "serve: a request needs at least one source");
+  std::unique_lock<std::mutex> lk(mMu);
+  LSW_REQUIRE(problem >= 0 && problem < (int)mEntries.size(), LSW_ERR_ARG, "serve: unknown problem");
+  Entry& e = *mEntries[problem];
+  const long long id = mNextId++;
+  Request& r = mRequests[id];
+  r.id = id;
+  r.problem = problem;
+  r.nsrc = r.remaining = nsrc;
+  r.xy.assign(xy, xy + 2 * nsrc);
+  r.xOut = xOut;
+  r.res.problem = problem;
+  r.res.nsrc = nsrc;
+  r.res.submitS = now();
+  r.res.startS = -1;
+  mStats.requests++;
+  if (e.failed) {
+    r.done = true;
+    r.res.status = LSW_ERR_STATE;
+    r.res.message = "serve: problem failed earlier";
+    r.res.endS = r.res.submitS;
+    mStats.completed++;
+    mDoneCv.notify_all();
+    return id;
+  }
+  for (int c = 0; c < nsrc; ++c) e.pending.push_back({id, c});
+  lk.unlock();
+  mCv.notify_all();
+  return id;
+}
+
+ServeResult Server::wait(long long id) {
+  std::unique_lock<std::mutex> lk(mMu);
+  auto it = mRequests.find(id);
+  LSW_REQUIRE(it != mRequests.end(), LSW_ERR_ARG, "serve: unknown request id");
+  mDoneCv.wait(lk, [&] { return it->second.done; });
+  ServeResult res = it->second.res;
+  mRequests.erase(it);
+  return res;
+}
+
+ServeStats Server::stats() const {
+  std::lock_guard<std::mutex> lk(mMu);
+  ServeStats s = mStats;
+  s.resident = 0;
+  for (auto& e : mEntries) s.resident += e->dd != nullptr;
+  return s;
+}
+
+void Server::evict(Entry& e) {
+  e.slots.reset();
+  e.dd.reset();
+  mStats.reservedBytes -= e.reserved;
+  e.reserved = 0;
+  mStats.evictions++;
+}
+
+bool Server::admit(Entry& e) {
+  if (mStats.reservedBytes + e.peakBytes <= mOpt.budgetBytes) {
+    e.reserved = e.peakBytes;
+    mStats.reservedBytes += e.reserved;
+    mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
+    return true;
+  }
+  // evict idle resident problems, least recently used first
+  std::vector<Entry*> idle;
+  double freeable = 0;
+  for (auto& o : mEntries)
+    if (o.get() != &e && o->dd && !o->busy && o->active == 0 && o->pending.empty()) {
+      idle.push_back(o.get());
+      freeable += o->reserved;
+    }
+  if (mStats.reservedBytes - freeable + e.peakBytes > mOpt.budgetBytes) return false;
+  std::sort(idle.begin(), idle.end(), [](Entry* a, Entry* b) { return a->lastUse < b->lastUse; });
+  for (Entry* o : idle) {
+    if (mStats.reservedBytes + e.peakBytes <= mOpt.budgetBytes) break;
+    evict(*o);
+  }
+  e.reserved = e.peakBytes;
+  mStats.reservedBytes += e.reserved;
+  mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
+  return true;
+}
+
+Server::Entry* Server::pick() {
+  // the problem whose oldest outstanding right-hand side is the oldest
+  Entry* best = nullptr;
+  double bestT = 0;
+  for (auto& up : mEntries) {
+    Entry& e = *up;
+    if (e.busy || e.failed) continue;
+    if (e.active == 0 && e.pending.empty()) continue;
+    double t = 1e300;
+    for (auto& kv : e.inFlight) t = std::min(t, mRequests[kv.second.req].res.submitS);
+    if (!e.pending.empty()) t = std::min(t, mRequests[e.pending.front().req].res.submitS);
+    if (!e.dd && e.reserved == 0 && !admit(e)) continue;
+    if (!best || t < bestT) {
+      best = &e;
+      bestT = t;
+    }
+  }
+  return best;
+}
+
+void Server::failEntry(Entry& e, int code, const std::string& msg) {
+  e.failed = true;
+  auto failReq = [&](long long id) {
+    auto it = mRequests.find(id);
+    if (it == mRequests.end() || it->second.done) return;
+    it->second.done = true;
+    it->second.res.status = code;
+    it->second.res.message = msg;
+    it->second.res.endS = now();
+    mStats.completed++;
+  };
+  for (auto& c : e.pending) failReq(c.req);
+  for (auto& kv : e.inFlight) failReq(kv.second.req);
+  e.pending.clear();
+  e.inFlight.clear();
+  e.active = 0;
+  e.slots.reset();
+  e.dd.reset();
+  mStats.reservedBytes -= e.reserved;
+  e.reserved = 0;
+  mDoneCv.notify_all();
+}
+
+void Server::workerLoop() {
+  (void)hipSetDevice(0);
+  std::vector<GmresSlots::Done> done;
+  std::unique_lock<std::mutex> lk(mMu);
+  while (true) {
+    Entry* ep = nullptr;
+    mCv.wait(lk, [&] { return mStop || (ep = pick()) != nullptr; });
+    if (mStop) return;
+    Entry& e = *ep;
+    e.busy = true;
+    const bool load = !e.dd;
+    // right-hand sides to admit in this step
+    std::vector<std::pair<long long, Column>> take;
+    const int room = load ? mOpt.slots : e.freeSlots;
+    while ((int)take.size() < room && !e.pending.empty()) {
+      take.push_back({mNextTag++, e.pending.front()});
+      e.pending.pop_front();
+    }
+    std::vector<std::vector<double>> srcs;
+    for (auto& t : take) {
+      Request& r = mRequests[t.second.req];
+      if (r.res.startS < 0) r.res.startS = now();
+      srcs.push_back({r.xy[2 * t.second.col], r.xy[2 * t.second.col + 1]});
+    }
+    lk.unlock();
+    int err = LSW_OK;
+    std::string msg;
+    std::vector<std::pair<Column, GmresSlots::Done>> finished;
+    std::vector<std::pair<long long, Column>> admitted;
+    try {
+      if (load) {
+        auto dd = makeCheckerboard(e.params, true);
+        dd->factorize();
+        dd->reserveBlock(mOpt.slots);
+        e.slots = std::make_unique<GmresSlots>(dd.get(), mOpt.slots, mOpt.restart, mOpt.tol, mOpt.maxIter);
+        e.dd = std::move(dd);
+      }
+      DDCheckerboard& dd = *e.dd;
+      GmresSlots& gs = *e.slots;
+      for (size_t k = 0; k < take.size(); ++k) {
+        dd.assembleRHSDev({srcs[k]}, gs.staging(), dd.stream());
+        gs.admit(gs.staging(), take[k].first);
+        admitted.push_back(take[k]);
+      }
+      done.clear();
+      gs.step(done);
+      for (const auto& d : done) {
+        Column c{-1, -1};
+        {
+          std::lock_guard<std::mutex> g(mMu);
+          auto it = e.inFlight.find(d.tag);
+          if (it != e.inFlight.end()) {
+            c = it->second;
+          } else {
+            for (auto& a : admitted)
+              if (a.first == d.tag) c = a.second;
+          }
+        }
+        double* xo = nullptr;
+        {
+          std::lock_guard<std::mutex> g(mMu);
+          auto rit = mRequests.find(c.req);
+          if (rit != mRequests.end() && rit->second.xOut) xo = rit->second.xOut + 2 * (size_t)c.col * dd.N();
+        }
+        if (xo) {
+          LSW_HIP(hipMemcpyAsync(xo, gs.solution(d.slot), (size_t)dd.N() * sizeof(double2),
+                                 hipMemcpyDeviceToHost, dd.stream()));
+          LSW_HIP(hipStreamSynchronize(dd.stream()));
+        }
+        gs.release(d.slot);
+        finished.push_back({c, d});
+      }
+    } catch (const Error& ex) {
+      err = ex.code;
+      msg = ex.what();
+    } catch (const std::exception& ex) {
+      err = LSW_ERR_INTERNAL;
+      msg = ex.what();
+    }
+    lk.lock();
+    e.busy = false;
+    e.lastUse = ++mClock;
+    if (load && err == LSW_OK) {
+      mStats.loads++;
+      mStats.reservedBytes += e.steadyBytes - e.reserved;  // factorisation workspace is gone
+      e.reserved = e.steadyBytes;
+    }
+    if (err != LSW_OK) {
+      for (auto& t : take) e.pending.push_front(t.second);  // failed with the entry below
+      failEntry(e, err, msg);
+      mCv.notify_all();
+      continue;
+    }
+    for (auto& a : admitted) e.inFlight[a.first] = a.second;
+    mStats.steps++;
+    for (auto& f : finished) {
+      e.inFlight.erase(f.second.tag);
+      auto it = mRequests.find(f.first.req);
+      if (it == mRequests.end()) continue;
+      Request& r = it->second;
+      r.res.maxIterations = std::max(r.res.maxIterations, f.second.iterations);
+      r.res.maxTrueResidual = std::max(r.res.maxTrueResidual, f.second.trueResidual);
+      mStats.solves++;
+      if (--r.remaining == 0) {
+        r.done = true;
+        r.res.endS = now();
+        mStats.completed++;
+        mDoneCv.notify_all();
+      }
+    }
+    e.active = e.slots ? e.slots->active() : 0;
+    e.freeSlots = e.slots ? e.slots->freeSlots() : 0;
+    mCv.notify_all();
+  }
+}
+
+}  // namespace lsw

@@ -31,6 +31,8 @@ EXPORTED = [
     "lsw_gmres_solve", "lsw_gmres_solve_dev",
     "lsw_config_num_problems", "lsw_config_problem", "lsw_problem_run",
     "lsw_gmres_solve_block", "lsw_gmres_solve_block_dev", "lsw_dd_assemble_rhs_block_dev",
+    "lsw_server_create", "lsw_server_destroy", "lsw_server_add_problem", "lsw_server_submit",
+    "lsw_server_wait", "lsw_server_get_stats", "lsw_server_problem_bytes",
 ]
 
 
@@ -76,6 +78,23 @@ class ProblemResult(C.Structure):
     _fields_ = [("setup_seconds", C.c_double), ("factorize_seconds", C.c_double),
                 ("gmres_seconds", C.c_double), ("iterations", C.c_int), ("estimate", C.c_double),
                 ("true_residual", C.c_double), ("dofs", C.c_longlong), ("Gx", C.c_int), ("Gy", C.c_int)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("budget_bytes", C.c_double), ("slots", C.c_int), ("restart", C.c_int), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("precision", C.c_int), ("workers", C.c_int)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("problem", C.c_int), ("nsrc", C.c_int), ("max_iterations", C.c_int),
+                ("max_true_residual", C.c_double), ("submit_s", C.c_double), ("start_s", C.c_double),
+                ("end_s", C.c_double), ("message", C.c_char * 160)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_longlong), ("completed", C.c_longlong), ("solves", C.c_longlong),
+                ("loads", C.c_longlong), ("evictions", C.c_longlong), ("steps", C.c_longlong),
+                ("reserved_bytes", C.c_double), ("peak_reserved_bytes", C.c_double), ("resident", C.c_int)]
 
 
 _lib.lsw_last_error.restype = C.c_char_p
@@ -126,6 +145,14 @@ _sig("lsw_dd_assemble_rhs_block_dev", _P, C.c_int, _D, _P, _P)
 _sig("lsw_config_num_problems", C.c_char_p, C.POINTER(C.c_int))
 _sig("lsw_config_problem", C.c_char_p, C.c_int, C.POINTER(ProblemSpec), _D, C.c_int)
 _sig("lsw_problem_run", C.c_char_p, C.c_int, C.c_int, _D, C.c_longlong, C.POINTER(ProblemResult))
+_sig("lsw_server_create", C.POINTER(ServerOptions), C.POINTER(_P))
+_sig("lsw_server_destroy", _P)
+_sig("lsw_server_add_problem", _P, C.POINTER(DDParams), C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_int))
+_sig("lsw_server_submit", _P, C.c_int, C.c_int, _D, _D, C.POINTER(C.c_longlong))
+_sig("lsw_server_wait", _P, C.c_longlong, C.POINTER(RequestResult))
+_sig("lsw_server_problem_bytes", _P, C.c_int, _D, _D)
+_sig("lsw_server_get_stats", _P, C.POINTER(ServerStats))
 
 
 def config_num_problems(path):
@@ -397,3 +424,72 @@ class DDCheckerboard:
         if getattr(self, "_h", None):
             _lib.lsw_dd_destroy(self._h)
             self._h = None
+
+
+# --------------------------------------------------------------------------
+class Server:
+    """Continuous-batching solve server (lsw_server_*): a device factor cache
+    keyed by registered problem, admission by device bytes, and per-Arnoldi-step
+    batching of every right-hand side of a problem.  submit() returns at once;
+    wait() blocks.  Solutions land in host arrays that the server owns until
+    the request is waited for."""
+
+    def __init__(self, budget_bytes=250e9, slots=16, restart=16, tol=1e-6, max_iter=200,
+                 precision=LSW_PREC_FP32, workers=4):
+        o = ServerOptions(float(budget_bytes), int(slots), int(restart), float(tol), int(max_iter),
+                          int(precision), int(workers))
+        self._h = _P()
+        _check(_lib.lsw_server_create(C.byref(o), C.byref(self._h)))
+        self._grids = {}
+        self._out = {}
+
+    def add_problem(self, params):
+        pid, gx, gy = C.c_int(), C.c_int(), C.c_int()
+        _check(_lib.lsw_server_add_problem(self._h, C.byref(params), C.byref(pid), C.byref(gx), C.byref(gy)))
+        self._grids[pid.value] = (gx.value, gy.value)
+        return pid.value
+
+    def grid(self, pid):
+        return self._grids[pid]
+
+    def problem_bytes(self, pid):
+        st, pk = C.c_double(), C.c_double()
+        _check(_lib.lsw_server_problem_bytes(self._h, int(pid), C.byref(st), C.byref(pk)))
+        return st.value, pk.value
+
+    def submit(self, pid, sources, want_solution=True):
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 2))
+        x = None
+        if want_solution:
+            gx, gy = self._grids[pid]
+            x = np.zeros((len(src), gx * gy), dtype=np.complex128)
+        rid = C.c_longlong()
+        _check(_lib.lsw_server_submit(self._h, int(pid), len(src), _dptr(src.reshape(-1)),
+                                      _dptr(x.view(np.float64)) if x is not None else None, C.byref(rid)))
+        self._out[rid.value] = (x, src)
+        return rid.value
+
+    def wait(self, rid):
+        r = RequestResult()
+        _check(_lib.lsw_server_wait(self._h, C.c_longlong(rid), C.byref(r)))
+        x, _ = self._out.pop(rid)
+        return dict(status=r.status, message=r.message.decode(), problem=r.problem, nsrc=r.nsrc,
+                    max_iterations=r.max_iterations, max_true_residual=r.max_true_residual,
+                    submit_s=r.submit_s, start_s=r.start_s, end_s=r.end_s, latency_s=r.end_s - r.submit_s,
+                    x=x)
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.lsw_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            _check(_lib.lsw_server_destroy(self._h))
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

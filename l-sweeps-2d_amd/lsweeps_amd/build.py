@@ -25,6 +25,8 @@ SOURCES = [
     "lsw_solver.hip",
     "lsw_gmres.hip",
     "lsw_gmres_block.hip",
+    "lsw_slots.hip",
+    "lsw_serve.cpp",
     "lsw_input.cpp",
     "lsw_capi.cpp",
 ]
@@ -52,7 +54,7 @@ def build(verbose=True, jobs=None):
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

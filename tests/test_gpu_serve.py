@@ -1,0 +1,131 @@
+"""Serving on the GPU: continuous batching over a factor cache.
+
+Each request's solutions are compared between a run where it is served alone
+(one request at a time, one worker) and a run where all requests are in
+flight together (several workers, fewer slots than right-hand sides, so
+right-hand sides join and leave between Arnoldi steps).  Tolerances:
+  * every right-hand side reaches the true relative residual 1e-6, checked
+    again on the CPU with the oracle's global operator;
+  * alone vs mix: rel. L2 <= 1e-10 with FP64 factors (only the grouping of
+    the block solves differs), <= 1e-5 with FP32 factors;
+  * against the FP64 direct solve of the oracle's global operator: <= 1e-4.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import lsweeps_amd as L
+from oracle.geometry import Checkerboard, CSVMedium, ConstantMedium
+from tests._common import lens_velocity, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _problems(precision):
+    vel = lens_velocity(64)
+    specs = [
+        dict(ncells=(3, 3), nels=(19, 19), omega=2 * math.pi * 3.3, velocity=None, general=False,
+             dom=((0.0, 1.0), (0.0, 1.0))),
+        dict(ncells=(2, 2), nels=(24, 24), omega=2 * math.pi * 2.5, velocity=vel, general=False,
+             dom=((0.0, 1.0), (0.0, 1.0))),
+        dict(ncells=(2, 4), nels=(14, 21), omega=2 * math.pi * 2.2, velocity=None, general=True,
+             dom=((0.0, 3.0), (0.0, 1.0))),
+    ]
+    out = []
+    for s in specs:
+        p = L.dd_params(domain=s["dom"], ncells=s["ncells"], nels=s["nels"], omega=s["omega"], min_wl_in_pml=2.0,
+                        precision=precision, velocity=s["velocity"], vel_domain=s["dom"], general=s["general"])
+        med = ConstantMedium() if s["velocity"] is None else CSVMedium(s["velocity"], [list(s["dom"][0]),
+                                                                                      list(s["dom"][1])])
+        cb = Checkerboard([list(s["dom"][0]), list(s["dom"][1])], s["ncells"], s["nels"], s["omega"], 2.0, med,
+                          general=s["general"])
+        out.append((p, cb, s))
+    return out
+
+
+def _requests(seed=5):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(9):
+        pid = k % 3
+        ns = int(rng.integers(1, 5))
+        lx = 3.0 if pid == 2 else 1.0
+        src = np.stack([rng.uniform(0.15, 0.85, ns) * lx, rng.uniform(0.15, 0.85, ns)], axis=1)
+        reqs.append((pid, src))
+    return reqs
+
+
+def _serve(probs, reqs, slots, workers, together, budget=64e9):
+    srv = L.Server(budget_bytes=budget, slots=slots, restart=12, tol=TOL, max_iter=120,
+                   precision=probs[0][0].precision, workers=workers)
+    try:
+        pids = [srv.add_problem(p) for p, _, _ in probs]
+        out = [None] * len(reqs)
+        if together:
+            ids = []
+            for k, (pid, src) in enumerate(reqs):
+                ids.append(srv.submit(pids[pid], src))
+            for k, i in enumerate(ids):
+                out[k] = srv.wait(i)
+        else:
+            for k, (pid, src) in enumerate(reqs):
+                out[k] = srv.wait(srv.submit(pids[pid], src))
+        return out, srv.stats()
+    finally:
+        srv.close()
+
+
+def _check_against_oracle(probs, reqs, res):
+    lus = {}
+    for k, (pid, src) in enumerate(reqs):
+        r = res[k]
+        assert r["status"] == 0, r["message"]
+        assert r["max_true_residual"] <= TOL
+        cb = probs[pid][1]
+        if pid not in lus:
+            lus[pid] = (cb.global_operator().tocsc(), None)
+            lus[pid] = (lus[pid][0], spla.splu(lus[pid][0]))
+        A, lu = lus[pid]
+        for c in range(len(src)):
+            b = cb.global_rhs([tuple(src[c])])
+            x = r["x"][c]
+            assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1.05 * TOL
+            assert relerr(x, lu.solve(b)) <= 1e-4
+
+
+@pytest.mark.parametrize("precision", [L.LSW_PREC_FP64, L.LSW_PREC_FP32])
+def test_each_result_same_alone_and_in_mix(precision):
+    probs = _problems(precision)
+    reqs = _requests()
+    alone, st1 = _serve(probs, reqs, slots=8, workers=1, together=False)
+    mix, st2 = _serve(probs, reqs, slots=3, workers=3, together=True)
+    _check_against_oracle(probs, reqs, alone)
+    _check_against_oracle(probs, reqs, mix)
+    lim = 1e-10 if precision == L.LSW_PREC_FP64 else 1e-5
+    for k in range(len(reqs)):
+        for c in range(len(reqs[k][1])):
+            assert relerr(mix[k]["x"][c], alone[k]["x"][c]) <= lim
+    total = sum(len(s) for _, s in reqs)
+    assert st1["solves"] == total and st2["solves"] == total
+    assert st2["loads"] == 3 and st2["evictions"] == 0
+    # right-hand sides of several requests shared Arnoldi steps in the mix
+    assert st2["steps"] < st1["steps"]
+
+
+def test_admission_evicts_idle_problems_under_a_small_budget():
+    probs = _problems(L.LSW_PREC_FP32)
+    srv = L.Server(budget_bytes=64e9, slots=4, precision=L.LSW_PREC_FP32, workers=1)
+    pids = [srv.add_problem(p) for p, _, _ in probs]
+    peaks = [srv.problem_bytes(p)[1] for p in pids]
+    srv.close()
+    budget = 1.05 * max(peaks)  # one problem at a time
+    reqs = _requests(seed=9)
+    res, st = _serve(probs, reqs, slots=4, workers=2, together=True, budget=budget)
+    _check_against_oracle(probs, reqs, res)
+    assert st["evictions"] >= 2
+    assert st["loads"] == st["evictions"] + st["resident"]
+    assert st["peak_reserved_bytes"] <= budget
