@@ -17,6 +17,8 @@ on the device.  Factorisation is timed separately, as the reference does
 ("Time to factorize system" / "Time to run the GMRES", src/Problem.cpp:143-144).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lens1600|const400]
+       python bench.py --workload serve    (request stream through the server)
+       python bench.py --workload pack48   (48 uneven problems at once)
 """
 import argparse
 import json
@@ -158,6 +160,189 @@ def cpu_baseline(cfg, iterations, seconds=20.0):
                 local_solve_ms=round(t_loc * 1e3, 3), local_factor_s=round(t_fact, 3))
 
 
+def _problem_params(L, p, precision):
+    return L.dd_params(domain=p["domain"], ncells=p["ncells"], nels=p["nels"], omega=p["omega"],
+                       min_wl_in_pml=p["min_wl_in_pml"], precision=precision, velocity=p["velocity"],
+                       vel_domain=p["vel_domain"], general=p.get("general", False))
+
+
+def plain_request(L, torch, params, sources, tol=1e-6, max_iter=200):
+    """The plain FP64 path for one request, as the reference runs a problem:
+    set up and factorise (no reuse across requests, src/Problem.cpp:70-91),
+    then each source solved on its own (FP64 flexible GMRES to the true
+    residual).  Returns (seconds, worst true residual, max iterations)."""
+    t0 = time.perf_counter()
+    dd = L.DDCheckerboard(params)
+    dd.factorize()
+    b = torch.zeros(dd.N, dtype=torch.complex128, device="cuda:0")
+    x = torch.zeros_like(b)
+    worst, iters = 0.0, 0
+    for s in sources:
+        dd.assemble_rhs_dev([s], b.data_ptr())
+        info = dd.gmres_dev(b.data_ptr(), x.data_ptr(), mode=1, tol=tol, max_iter=max_iter, restart=30)
+        worst = max(worst, info["true_residual"])
+        iters = max(iters, info["iterations"])
+    dd.sync()
+    t = time.perf_counter() - t0
+    del dd
+    return t, worst, iters
+
+
+def run_stream(L, srv, pids, stream, t_scale=1.0):
+    """Submits the requests of `stream` at their arrival times (scaled) from
+    this thread; returns the per-request results and the wall time."""
+    import threading
+    ids = [None] * len(stream)
+    t0 = time.perf_counter()
+    res = [None] * len(stream)
+
+    def waiter():
+        for k in range(len(stream)):
+            while ids[k] is None:
+                time.sleep(0.001)
+            res[k] = srv.wait(ids[k])
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    for k, (ta, key, srcs) in enumerate(stream):
+        dt = ta * t_scale - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids[k] = srv.submit(pids[key], srcs, want_solution=False)
+    th.join()
+    return res, time.perf_counter() - t0
+
+
+def serve_bench(args):
+    """Serving workload: a Poisson stream over layered models (seeds 0..M-1)
+    x 4 frequencies (6, 8, 10, 12 points per wavelength), 1-64 sources per
+    request near the surface.  Measures the plain FP64 path one request at a
+    time on a sample of the stream, then the server (FP32 factors, FP64
+    GMRES, continuous batching) at 0.8x the plain capacity and at saturation."""
+    import numpy as np
+    import torch
+    import lsweeps_amd as L
+    from lsweeps_amd.workloads import config_serve_problem, serve_requests, SERVE_PPW
+    M, F = args.serve_models, len(SERVE_PPW)
+    probs = {}
+    t0 = time.perf_counter()
+    for m in range(M):
+        for f in range(F):
+            probs[(m, f)] = config_serve_problem(m, f)
+    log(f"serve: {len(probs)} problems generated in {time.perf_counter() - t0:.1f} s")
+
+    def sources(key, xf):
+        p = probs[key]
+        (ax, bx), (ay, by) = p["domain"]
+        h = (by - ay) / (p["grid"][1] - 1)
+        return [(ax + v * (bx - ax), by - 10 * h) for v in xf]
+
+    raw = serve_requests(args.serve_requests, rate=1.0, seed=0, models=M, freqs=F)
+    stream = [(t, (m, f), sources((m, f), xf)) for (t, m, f, xf) in raw]
+    # plain path on a bounded sample
+    rng = np.random.default_rng(1)
+    sample = rng.choice(len(stream), size=min(args.plain_sample, len(stream)), replace=False)
+    pt, pn, pres = 0.0, 0, 0.0
+    for k in sample:
+        _, key, srcs = stream[k]
+        t, worst, it = plain_request(L, torch, _problem_params(L, probs[key], L.LSW_PREC_FP64), srcs)
+        pt += t
+        pn += len(srcs)
+        pres = max(pres, worst)
+        log(f"  plain request {k}: {probs[key]['name']} {len(srcs)} sources {t:.2f} s (iters {it})")
+        torch.cuda.empty_cache()
+    plain_req_s = pt / len(sample)
+    plain = dict(solves_per_s=pn / pt, s_per_request=plain_req_s, requests=len(sample), solves=pn,
+                 worst_true_residual=pres)
+    log("plain", plain)
+    out = {}
+    for label, scale in (("load0.8", plain_req_s / 0.8), ("saturated", 0.0)):
+        srv = L.Server(budget_bytes=args.serve_budget, slots=args.serve_slots, restart=args.serve_restart,
+                       tol=1e-6, max_iter=200, precision=L.LSW_PREC_FP32, workers=args.serve_workers)
+        pids = {key: srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)) for key, p in probs.items()}
+        st = stream if label == "saturated" else stream[:args.serve_load_requests]
+        res, wall = run_stream(L, srv, pids, st, t_scale=scale)
+        stats = srv.stats()
+        srv.close()
+        lat = [r["latency_s"] for r in res]
+        bad = [r for r in res if r["status"] != 0]
+        nsol = sum(r["nsrc"] for r in res)
+        out[label] = dict(requests=len(res), solves=nsol, wall_s=wall, solves_per_s=nsol / wall,
+                          p50_s=pct(lat, 50), p99_s=pct(lat, 99), failed=len(bad),
+                          worst_true_residual=max(r["max_true_residual"] for r in res),
+                          mean_interarrival_s=scale, **{k: stats[k] for k in ("loads", "evictions", "steps")},
+                          peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
+        log(label, out[label])
+        torch.cuda.empty_cache()
+    line = {
+        "metric": METRIC, "value": round(out["saturated"]["solves_per_s"], 4), "unit": "solves/s",
+        "n_gpus": 1, "steps": len(stream), "warmup": 0,
+        "ms_per_step": round(1e3 * out["saturated"]["wall_s"] / len(stream), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 factors/sweeps + c128 Krylov", "data": "synthetic",
+        "config": {"workload": f"serve_{M}models_x{F}freq_{len(stream)}req", "budget_gb": args.serve_budget / 1e9,
+                   "slots": args.serve_slots, "restart": args.serve_restart, "workers": args.serve_workers},
+        "p50_ms": round(1e3 * out["saturated"]["p50_s"], 1), "p99_ms": round(1e3 * out["saturated"]["p99_s"], 1),
+        "plain": plain, "served": out,
+        "speedup_served_vs_plain": round(out["saturated"]["solves_per_s"] / plain["solves_per_s"], 2),
+    }
+    print(json.dumps(line))
+
+
+def pack_bench(args):
+    """The uneven batch of 48 problems: every problem submitted at once to
+    the server (FP32 factors, FP64 GMRES, continuous batching across
+    problems) against the plain FP64 path one problem at a time on a
+    sample."""
+    import numpy as np
+    import torch
+    import lsweeps_amd as L
+    from lsweeps_amd.workloads import config_pack48
+    t0 = time.perf_counter()
+    probs = config_pack48(seed=0, count=args.pack_count)
+    log(f"pack: {len(probs)} problems generated in {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(2)
+    sample = rng.choice(len(probs), size=min(args.plain_sample, len(probs)), replace=False)
+    pt, pn, pres = 0.0, 0, 0.0
+    for k in sample:
+        p = probs[k]
+        t, worst, it = plain_request(L, torch, _problem_params(L, p, L.LSW_PREC_FP64), p["sources"])
+        pt += t
+        pn += len(p["sources"])
+        pres = max(pres, worst)
+        log(f"  plain {p['name']}: {len(p['sources'])} sources {t:.2f} s (iters {it})")
+        torch.cuda.empty_cache()
+    plain = dict(solves_per_s=pn / pt, problems=len(sample), solves=pn, worst_true_residual=pres)
+    log("plain", plain)
+    srv = L.Server(budget_bytes=args.serve_budget, slots=16, restart=args.serve_restart, tol=1e-6, max_iter=200,
+                   precision=L.LSW_PREC_FP32, workers=args.serve_workers)
+    pids = [srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)) for p in probs]
+    stream = [(0.0, k, p["sources"]) for k, p in enumerate(probs)]
+    res, wall = run_stream(L, srv, pids, stream)
+    stats = srv.stats()
+    srv.close()
+    nsol = sum(r["nsrc"] for r in res)
+    lat = [r["latency_s"] for r in res]
+    packed = dict(problems=len(res), solves=nsol, wall_s=wall, solves_per_s=nsol / wall,
+                  p50_s=pct(lat, 50), p99_s=pct(lat, 99), failed=sum(r["status"] != 0 for r in res),
+                  worst_true_residual=max(r["max_true_residual"] for r in res),
+                  dofs=int(sum(p["grid"][0] * p["grid"][1] for p in probs)),
+                  **{k: stats[k] for k in ("loads", "evictions", "steps")},
+                  peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
+    log("packed", packed)
+    line = {
+        "metric": METRIC, "value": round(packed["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": len(probs), "warmup": 0, "ms_per_step": round(1e3 * wall / len(probs), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 factors/sweeps + c128 Krylov", "data": "synthetic",
+        "config": {"workload": f"pack{len(probs)}_seed0", "budget_gb": args.serve_budget / 1e9,
+                   "workers": args.serve_workers, "restart": args.serve_restart},
+        "p50_ms": round(1e3 * packed["p50_s"], 1), "p99_ms": round(1e3 * packed["p99_s"], 1),
+        "plain": plain, "packed": packed,
+        "speedup_packed_vs_plain": round(packed["solves_per_s"] / plain["solves_per_s"], 2),
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,11 +351,26 @@ def main():
     ap.add_argument("--workload", default="lens1600")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
+    ap.add_argument("--serve-models", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-load-requests", type=int, default=12)
+    ap.add_argument("--serve-budget", type=float, default=250e9)
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-restart", type=int, default=8)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--pack-count", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(sharding across GPUs is design-only in this build)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serve":
+        serve_bench(args)
+        return
+    if args.workload == "pack48":
+        pack_bench(args)
+        return
     import numpy as np
     import torch
     import lsweeps_amd as L

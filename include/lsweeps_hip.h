@@ -189,6 +189,14 @@ int lsw_config_problem(const char* path, int id, lsw_problem_spec* spec, double*
  * entries) receives the solution on the global grid. */
 int lsw_problem_run(const char* path, int id, int verbose, double* x, long long x_cap, lsw_problem_result* res);
 
+/* ---- Output (OutputVTK::print2D, src/OutputVTK.cpp:82-366) ---------------
+ * Writes <path>.vtk (the reference's legacy-ASCII quad layout over the
+ * global DOF grid) and <path>.lswgrid (the same field in full precision:
+ * "LSWGRID1", int32 Gx, Gy, Gx x-coordinates, Gy y-coordinates, Gx*Gy
+ * complex doubles).  x: global grid vector on the host.  Works on a
+ * geometry-only DD. */
+int lsw_dd_write_vtk(lsw_dd* dd, const double* x, const char* path);
+
 /* ---- Serving ------------------------------------------------------------
  * No counterpart in the reference, whose driver reads, factorises and solves
  * one problem at a time (src/Problem.cpp:70-158).  A server keeps factorised

@@ -10,6 +10,7 @@
 #include "lsw_gmres.h"
 #include "lsw_input.h"
 #include "lsw_model.h"
+#include "lsw_output.h"
 #include "lsw_serve.h"
 #include "lsw_solver.h"
 
@@ -510,6 +511,17 @@ int lsw_problem_run(const char* path, int id, int verbose, double* x, long long 
       LSW_REQUIRE(x_cap >= (long long)pb.solution().size(), LSW_ERR_CAPACITY, "solution buffer too small");
       std::memcpy(x, pb.solution().data(), pb.solution().size() * sizeof(Complex));
     }
+  });
+}
+
+// ------------------------------------------------------------------ output
+int lsw_dd_write_vtk(lsw_dd* dd, const double* x, const char* path) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && path, LSW_ERR_ARG, "null pointer");
+    const size_t n = (size_t)dd->dd->Gx() * dd->dd->Gy();
+    std::vector<Complex> v(reinterpret_cast<const Complex*>(x), reinterpret_cast<const Complex*>(x) + n);
+    writeVTK(std::string(path) + ".vtk", *dd->dd, v);
+    writeGrid(std::string(path) + ".lswgrid", *dd->dd, v);
   });
 }
 

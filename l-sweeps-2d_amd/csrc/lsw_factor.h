@@ -42,6 +42,7 @@ struct SolvePlan {
     long long off = 0;  // first entry in the plan's entry array
     int n = 0;          // entries
     int ipb = 1;        // entries per workgroup
+    int waves = 1;      // wavefronts per workgroup (they split one entry's columns)
     int cols = 0;       // largest column count (sizes the LDS)
     double bytes = 0;   // algorithmic bytes
   };

@@ -1,5 +1,6 @@
 // Input / Problem (see lsw_input.h).
 #include "lsw_input.h"
+#include "lsw_output.h"
 
 #include <hip/hip_runtime.h>
 
@@ -357,14 +358,19 @@ ProblemSpec Input::readProblem(UInt probID) const {
   sp.verbose = verbose;
   UInt restart = 0;
   if (gs->lookupValue("Restart", restart) && restart > 0) sp.restart = restart;
-  // ---- output (writing is not part of this build; the name is checked)
+  // ---- output (Input::readOutput, src/Input.cpp:402-467): "" = none
   std::string outName;
   LSW_REQUIRE(prob.lookupValue("Output", outName), LSW_ERR_ARG, "No Output prescribed in problem!");
   sp.output = outName;
   if (!outName.empty()) {
     const CfgNode* outs = mRoot.find("Outputs");
     LSW_REQUIRE(outs != nullptr, LSW_ERR_ARG, "No Outputs defined in input file");
-    LSW_REQUIRE(outs->find(outName) != nullptr, LSW_ERR_ARG, "Invalid output type prescribed!");
+    const CfgNode* o = outs->find(outName);
+    LSW_REQUIRE(o != nullptr, LSW_ERR_ARG, "Invalid output type prescribed!");
+    std::string type;
+    LSW_REQUIRE(o->lookupValue("Type", type), LSW_ERR_ARG, "No Type prescribed for Output");
+    LSW_REQUIRE(type == "VTK", LSW_ERR_ARG, "Unknown output type!");
+    LSW_REQUIRE(o->lookupValue("Filename", sp.vtkFile), LSW_ERR_ARG, "No Filename prescribed!");
   }
   return sp;
 }
@@ -451,6 +457,11 @@ ProblemResult Problem::solve(bool verbose) {
   if (verbose) {
     std::printf("Time to factorize system: %g\n", r.factorizeSeconds);
     std::printf("Time to run the GMRES: %g\n", r.gmresSeconds);
+  }
+  // Problem::solve writes the output after the timers (src/Problem.cpp:147-156)
+  if (!mSpec.vtkFile.empty()) {
+    writeVTK(mSpec.vtkFile + ".vtk", *mDD, mSolution);
+    writeGrid(mSpec.vtkFile + ".lswgrid", *mDD, mSolution);
   }
   return r;
 }

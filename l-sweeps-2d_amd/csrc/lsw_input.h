@@ -69,6 +69,7 @@ struct ProblemSpec {
   UInt restart = 30;
   bool verbose = true;
   std::string output;
+  std::string vtkFile;  // Outputs.<output>.Filename when Type = "VTK" (no ".vtk")
 };
 
 class Input {

@@ -9,11 +9,17 @@
 //     right-hand side support (sweep right-hand sides live on 2-4 grid lines),
 //   * the backward list holds every node, and skips the F11^{-1} r_E columns of
 //     nodes the forward pass did not visit (their r_E is zero),
-//   * a workgroup (one 64-lane wavefront) walks `ipb` consecutive entries so
-//     that tiny leaf fronts do not pay a workgroup launch each.
+//   * a workgroup walks `ipb` consecutive entries so that tiny leaf fronts do
+//     not pay a workgroup launch each;
+//   * heights with few, large entries (the separators near the root, where a
+//     front step holds only a few dozen of them) give each entry W = 2..8
+//     wavefronts that split its columns, so the launch still keeps enough
+//     loads in flight to approach the HBM bandwidth.
 // Factors are column-major, so the lanes of a wavefront read consecutive rows
 // of one column (coalesced); right-hand-side values are broadcast from LDS.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "lsw_cplx.h"
@@ -26,60 +32,84 @@ __device__ __forceinline__ bool bitSet(const unsigned* __restrict__ a, int node)
   return (a[node >> 5] >> (node & 31)) & 1u;
 }
 
-// y[row] = sum_k M[k*ld + row] * v[k] over k in [k0, K) for rows [0, rows),
-// with lanes = rows-per-pass x column groups; result valid in cg == 0 lanes.
-template <typename T2, int NR>
-__device__ __forceinline__ void gemvCols(const T2* __restrict__ M, long long ld, int rows, int k0, int K,
-                                         const T2* v, int vstride, int lane, T2 (&acc)[NR], int& r, int& cg) {
+// y[row] = sum_k M[k*ld + row] * v[k] over k in [k0, K) for rows [0, rows).
+// A workgroup of W wavefronts works on one product: a wavefront splits into
+// RP rows x CG column groups, the W*CG column groups stride over the columns
+// (8 independent loads in flight per lane), partial sums are reduced across
+// the column groups of a wavefront by lane shuffles and across wavefronts in
+// a fixed order through LDS (`red`: W*64*NR entries).  On return the owner
+// lanes (wavefront 0, first column group, r < rows) hold the result.
+template <typename T2, int NR, int W>
+__device__ __forceinline__ bool gemvCols(const T2* __restrict__ M, long long ld, int rows, int k0, int K,
+                                         const T2* v, int vstride, T2* red, T2 (&acc)[NR], int& r) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int RP = 1;
   while (RP < rows) RP <<= 1;
   const int CG = 64 / RP;
   r = lane & (RP - 1);
-  cg = lane / RP;
+  const int cg = lane / RP;
+  const int CGA = W * CG;
 #pragma unroll
   for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
   if (r < rows) {
     const T2* p = M + r;
-    int k = k0 + cg;
-    // 4 independent loads in flight per lane
-    for (; k + 3 * CG < K; k += 4 * CG) {
-      const T2 w0 = p[(long long)k * ld];
-      const T2 w1 = p[(long long)(k + CG) * ld];
-      const T2 w2 = p[(long long)(k + 2 * CG) * ld];
-      const T2 w3 = p[(long long)(k + 3 * CG) * ld];
+    int k = k0 + w * CG + cg;
+    for (; k + 7 * CGA < K; k += 8 * CGA) {
+      T2 m[8];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        acc[q] = cfma(w0, v[q * vstride + k], acc[q]);
-        acc[q] = cfma(w1, v[q * vstride + k + CG], acc[q]);
-        acc[q] = cfma(w2, v[q * vstride + k + 2 * CG], acc[q]);
-        acc[q] = cfma(w3, v[q * vstride + k + 3 * CG], acc[q]);
-      }
+      for (int u = 0; u < 8; ++u) m[u] = p[(long long)(k + u * CGA) * ld];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vstride + k + u * CGA], acc[q]);
     }
-    for (; k < K; k += CG) {
-      const T2 w = p[(long long)k * ld];
+    for (; k < K; k += CGA) {
+      const T2 m0 = p[(long long)k * ld];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = cfma(w, v[q * vstride + k], acc[q]);
+      for (int q = 0; q < NR; ++q) acc[q] = cfma(m0, v[q * vstride + k], acc[q]);
     }
   }
   for (int off = RP; off < 64; off <<= 1) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
   }
+  if constexpr (W == 1) {
+    return cg == 0 && r < rows;
+  } else {
+    if (cg == 0 && r < rows) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) red[(w * 64 + r) * NR + q] = acc[q];
+    }
+    __syncthreads();
+    const bool own = w == 0 && cg == 0 && r < rows;
+    if (own) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        T2 s = red[r * NR + q];
+        for (int ww = 1; ww < W; ++ww) s = cadd(s, red[(ww * 64 + r) * NR + q]);
+        acc[q] = s;
+      }
+    }
+    return own;
+  }
 }
 }  // namespace
 
-template <typename T2, int NR>
-__global__ __launch_bounds__(64) void k_nd_fwd(const CellDev* __restrict__ cells,
-                                               const ShapeDev* __restrict__ shapes,
-                                               const SolveTask* __restrict__ tasks,
-                                               const int4* __restrict__ ent, int nent, int ipb,
-                                               const unsigned* __restrict__ active, int aw,
-                                               const T2* __restrict__ factor, T2* __restrict__ X,
-                                               T2* __restrict__ Rb, T2* __restrict__ G, long long xs,
-                                               long long gs) {
+// LDS layout of both kernels: v (NR x cols right-hand-side values), then the
+// cross-wavefront reduction buffer (W x 64 x NR) when W > 1.
+template <typename T2, int NR, int W>
+__global__ __launch_bounds__(64 * W) void k_nd_fwd(const CellDev* __restrict__ cells,
+                                                   const ShapeDev* __restrict__ shapes,
+                                                   const SolveTask* __restrict__ tasks,
+                                                   const int4* __restrict__ ent, int nent, int ipb, int cols,
+                                                   const unsigned* __restrict__ active, int aw,
+                                                   const T2* __restrict__ factor, T2* __restrict__ X,
+                                                   T2* __restrict__ Rb, T2* __restrict__ G, long long xs,
+                                                   long long gs) {
   extern __shared__ char smem_raw[];
   T2* v = reinterpret_cast<T2*>(smem_raw);
-  const int lane = threadIdx.x;
+  T2* red = v + (size_t)NR * cols;
+  const int tid = threadIdx.x;
   for (int q = 0; q < ipb; ++q) {
     const int idx = blockIdx.x * ipb + q;
     if (idx >= nent) break;
@@ -98,7 +128,7 @@ __global__ __launch_bounds__(64) void k_nd_fwd(const CellDev* __restrict__ cells
     const int g1 = a1ok ? sh.nodes[nd.c1].gOff : 0;
     const int g2 = a2ok ? sh.nodes[nd.c2].gOff : 0;
     const int nr = tk.nrhs;
-    for (int k = lane; k < e; k += 64) {
+    for (int k = tid; k < e; k += 64 * W) {
       const int a1 = a1ok ? m1[k] : -1, a2 = a2ok ? m2[k] : -1, dof = E[k];
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
@@ -115,9 +145,10 @@ __global__ __launch_bounds__(64) void k_nd_fwd(const CellDev* __restrict__ cells
     const int rows = en.w;
     if (rows > 0) {
       T2 acc[NR];
-      int r, cg;
-      gemvCols<T2, NR>(factor + cd.factorOff + nd.fwdOff + en.z, b, rows, 0, e, v, e, lane, acc, r, cg);
-      if (cg == 0 && r < rows) {
+      int r;
+      const bool own = gemvCols<T2, NR, W>(factor + cd.factorOff + nd.fwdOff + en.z, b, rows, 0, e, v, e, red,
+                                           acc, r);
+      if (own) {
         const int f = e + en.z + r;
         const int a1 = a1ok ? m1[f] : -1, a2 = a2ok ? m2[f] : -1;
 #pragma unroll
@@ -135,17 +166,18 @@ __global__ __launch_bounds__(64) void k_nd_fwd(const CellDev* __restrict__ cells
   }
 }
 
-template <typename T2, int NR>
-__global__ __launch_bounds__(64) void k_nd_bwd(const CellDev* __restrict__ cells,
-                                               const ShapeDev* __restrict__ shapes,
-                                               const SolveTask* __restrict__ tasks,
-                                               const int4* __restrict__ ent, int nent, int ipb,
-                                               const unsigned* __restrict__ active, int aw,
-                                               const T2* __restrict__ factor, T2* __restrict__ X,
-                                               const T2* __restrict__ Rb, long long xs) {
+template <typename T2, int NR, int W>
+__global__ __launch_bounds__(64 * W) void k_nd_bwd(const CellDev* __restrict__ cells,
+                                                   const ShapeDev* __restrict__ shapes,
+                                                   const SolveTask* __restrict__ tasks,
+                                                   const int4* __restrict__ ent, int nent, int ipb, int cols,
+                                                   const unsigned* __restrict__ active, int aw,
+                                                   const T2* __restrict__ factor, T2* __restrict__ X,
+                                                   const T2* __restrict__ Rb, long long xs) {
   extern __shared__ char smem_raw[];
   T2* v = reinterpret_cast<T2*>(smem_raw);
-  const int lane = threadIdx.x;
+  T2* red = v + (size_t)NR * cols;
+  const int tid = threadIdx.x;
   for (int q = 0; q < ipb; ++q) {
     const int idx = blockIdx.x * ipb + q;
     if (idx >= nent) break;
@@ -160,7 +192,7 @@ __global__ __launch_bounds__(64) void k_nd_bwd(const CellDev* __restrict__ cells
     const int* E = sh.E + nd.eOff;
     const int* Bp = sh.B + nd.bOff;
     const int nr = tk.nrhs;
-    for (int k = k0 + lane; k < F; k += 64) {
+    for (int k = k0 + tid; k < F; k += 64 * W) {
 #pragma unroll
       for (int qq = 0; qq < NR; ++qq) {
         if (qq >= nr) break;
@@ -170,10 +202,11 @@ __global__ __launch_bounds__(64) void k_nd_bwd(const CellDev* __restrict__ cells
     }
     __syncthreads();
     T2 acc[NR];
-    int r, cg;
+    int r;
     const int rows = en.w;
-    gemvCols<T2, NR>(factor + cd.factorOff + nd.bwdOff + en.z, e, rows, k0, F, v, F, lane, acc, r, cg);
-    if (cg == 0 && r < rows) {
+    const bool own = gemvCols<T2, NR, W>(factor + cd.factorOff + nd.bwdOff + en.z, e, rows, k0, F, v, F, red,
+                                         acc, r);
+    if (own) {
       const int dof = E[en.z + r];
 #pragma unroll
       for (int qq = 0; qq < NR; ++qq) {
@@ -193,6 +226,8 @@ SolvePlan::~SolvePlan() {
 }
 
 namespace {
+constexpr double kMaxLds = 65536.0;  // bytes of dynamic LDS per workgroup
+
 bool meets(const NdNode& nd, const std::vector<Rect>& sup) {
   if (sup.empty()) return true;
   for (const Rect& r : sup)
@@ -270,6 +305,16 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     int ipb = (int)std::max(1.0, std::min(16.0, 2048.0 / std::max(per, 1.0)));
     while (ipb > 1 && (v.size() + ipb - 1) / ipb < 4096) ipb /= 2;
     L.ipb = ipb;
+    // few, large entries: several wavefronts per entry split the columns
+    int W = 1;
+    while (W < 8 && (double)v.size() * W * 2 <= 4096.0 && per >= 1024.0 * W * 2) W *= 2;
+    // LDS: NR x cols right-hand-side values + W x 64 x NR reduction entries
+    const double nrk = plan.maxNrhs == 1 ? 1.0 : (double)kMaxRhs;
+    auto lds = [&](int w) { return S * nrk * ((double)cols + (w > 1 ? 64.0 * w : 0.0)); };
+    while (W > 1 && lds(W) > kMaxLds) W /= 2;
+    LSW_REQUIRE(lds(1) <= kMaxLds, LSW_ERR_CAPACITY, "front too wide for the solve kernel's LDS staging");
+    L.waves = W;
+    if (W > 1) L.ipb = 1;
     all.insert(all.end(), v.begin(), v.end());
     out.push_back(L);
   };
@@ -283,6 +328,12 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   for (int h = mMaxHeight; h >= 0; --h) {
     addLaunch(plan.bwd, bw[h], bcols[h], bbytes[h], bwork[h]);
     plan.bwdBytes += bbytes[h];
+  }
+  if (std::getenv("LSW_PLAN_LOG")) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (const auto& L : pass ? plan.bwd : plan.fwd)
+        std::fprintf(stderr, "plan %s tasks %d nrhs %d: entries %d ipb %d waves %d cols %d bytes %.0f\n",
+                     pass ? "bwd" : "fwd", nt, plan.maxNrhs, L.n, L.ipb, L.waves, L.cols, L.bytes);
   }
   if (plan.dTasks) (void)hipFree(plan.dTasks);
   if (plan.dEntries) (void)hipFree(plan.dEntries);
@@ -338,25 +389,47 @@ struct LaunchTimer {
   }
 };
 
+template <typename T2, int NR, int W>
+void launchFwd(const SolvePlan& plan, const SolvePlan::Launch& L, const CellDev* dCells, const ShapeDev* dShapes,
+               const T2* factor, T2* X, T2* R, T2* G, long long xs, long long gs, hipStream_t s) {
+  const unsigned grid = (unsigned)((L.n + L.ipb - 1) / L.ipb);
+  const size_t shm = ((size_t)NR * L.cols + (W > 1 ? (size_t)W * 64 * NR : 0)) * sizeof(T2);
+  hipLaunchKernelGGL((k_nd_fwd<T2, NR, W>), dim3(grid), dim3(64 * W), shm, s, dCells, dShapes, plan.dTasks,
+                     plan.dEntries + L.off, L.n, L.ipb, L.cols, plan.dActive, plan.activeWords, factor, X, R, G, xs,
+                     gs);
+}
+
+template <typename T2, int NR, int W>
+void launchBwd(const SolvePlan& plan, const SolvePlan::Launch& L, const CellDev* dCells, const ShapeDev* dShapes,
+               const T2* factor, T2* X, T2* R, long long xs, hipStream_t s) {
+  const unsigned grid = (unsigned)((L.n + L.ipb - 1) / L.ipb);
+  const size_t shm = ((size_t)NR * L.cols + (W > 1 ? (size_t)W * 64 * NR : 0)) * sizeof(T2);
+  hipLaunchKernelGGL((k_nd_bwd<T2, NR, W>), dim3(grid), dim3(64 * W), shm, s, dCells, dShapes, plan.dTasks,
+                     plan.dEntries + L.off, L.n, L.ipb, L.cols, plan.dActive, plan.activeWords, factor, X, R, xs);
+}
+
 template <typename T2, int NR>
 void runPlan(const SolvePlan& plan, const CellDev* dCells, const ShapeDev* dShapes, const T2* factor, T2* X, T2* R,
              T2* G, long long xs, long long gs, hipStream_t s, SolveProfile* prof) {
   LaunchTimer tm{prof, s, {}, {}, {}};
   for (const auto& L : plan.fwd) {
-    const unsigned grid = (unsigned)((L.n + L.ipb - 1) / L.ipb);
-    const size_t shm = (size_t)NR * L.cols * sizeof(T2);
     tm.begin();
-    hipLaunchKernelGGL((k_nd_fwd<T2, NR>), dim3(grid), dim3(64), shm, s, dCells, dShapes, plan.dTasks,
-                       plan.dEntries + L.off, L.n, L.ipb, plan.dActive, plan.activeWords, factor, X, R, G, xs,
-                       gs);
+    switch (L.waves) {
+      case 1: launchFwd<T2, NR, 1>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
+      case 2: launchFwd<T2, NR, 2>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
+      case 4: launchFwd<T2, NR, 4>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
+      default: launchFwd<T2, NR, 8>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
+    }
     tm.end(0, L.bytes);
   }
   for (const auto& L : plan.bwd) {
-    const unsigned grid = (unsigned)((L.n + L.ipb - 1) / L.ipb);
-    const size_t shm = (size_t)NR * L.cols * sizeof(T2);
     tm.begin();
-    hipLaunchKernelGGL((k_nd_bwd<T2, NR>), dim3(grid), dim3(64), shm, s, dCells, dShapes, plan.dTasks,
-                       plan.dEntries + L.off, L.n, L.ipb, plan.dActive, plan.activeWords, factor, X, R, xs);
+    switch (L.waves) {
+      case 1: launchBwd<T2, NR, 1>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
+      case 2: launchBwd<T2, NR, 2>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
+      case 4: launchBwd<T2, NR, 4>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
+      default: launchBwd<T2, NR, 8>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
+    }
     tm.end(1, L.bytes);
   }
   LSW_HIP(hipGetLastError());

@@ -32,7 +32,7 @@ EXPORTED = [
     "lsw_config_num_problems", "lsw_config_problem", "lsw_problem_run",
     "lsw_gmres_solve_block", "lsw_gmres_solve_block_dev", "lsw_dd_assemble_rhs_block_dev",
     "lsw_server_create", "lsw_server_destroy", "lsw_server_add_problem", "lsw_server_submit",
-    "lsw_server_wait", "lsw_server_get_stats", "lsw_server_problem_bytes",
+    "lsw_server_wait", "lsw_server_get_stats", "lsw_server_problem_bytes", "lsw_dd_write_vtk",
 ]
 
 
@@ -145,6 +145,7 @@ _sig("lsw_dd_assemble_rhs_block_dev", _P, C.c_int, _D, _P, _P)
 _sig("lsw_config_num_problems", C.c_char_p, C.POINTER(C.c_int))
 _sig("lsw_config_problem", C.c_char_p, C.c_int, C.POINTER(ProblemSpec), _D, C.c_int)
 _sig("lsw_problem_run", C.c_char_p, C.c_int, C.c_int, _D, C.c_longlong, C.POINTER(ProblemResult))
+_sig("lsw_dd_write_vtk", _P, _D, C.c_char_p)
 _sig("lsw_server_create", C.POINTER(ServerOptions), C.POINTER(_P))
 _sig("lsw_server_destroy", _P)
 _sig("lsw_server_add_problem", _P, C.POINTER(DDParams), C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -312,6 +313,12 @@ class DDCheckerboard:
         ys = np.zeros(self.Gy)
         _check(_lib.lsw_dd_global_coords(self._h, _dptr(xs), _dptr(ys)))
         return xs, ys
+
+    def write_vtk(self, x, path):
+        """<path>.vtk (OutputVTK layout) and <path>.lswgrid (full precision)."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        assert x.size == self.N
+        _check(_lib.lsw_dd_write_vtk(self._h, _dptr(x.view(np.float64)), str(path).encode()))
 
     def cell_info(self, c):
         info = (C.c_int * 38)()

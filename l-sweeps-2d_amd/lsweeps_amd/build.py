@@ -27,6 +27,7 @@ SOURCES = [
     "lsw_gmres_block.hip",
     "lsw_slots.hip",
     "lsw_serve.cpp",
+    "lsw_output.cpp",
     "lsw_input.cpp",
     "lsw_capi.cpp",
 ]

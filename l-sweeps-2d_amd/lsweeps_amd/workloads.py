@@ -114,4 +114,108 @@ def config_layered_3x1(seed=0, nsrc=128, ppw=8.0, cells=(4, 12), cell_nodes=200)
                 ppw=ppw, seed=seed)
 
 
+def _rect_problem(nxn, nyn, cells, vel_fn, ppw, pml_cells, sources_fn, name, tol=1e-6, max_iter=200):
+    """A general-geometry problem on nxn x nyn nodes with h = 1/(nyn-1):
+    domain [0, (nxn-1)h] x [0, 1], cells = (rows, columns) equal cells."""
+    nR, nC = cells
+    assert nxn % nC == 0 and nyn % nR == 0
+    h = 1.0 / (nyn - 1)
+    Lx = (nxn - 1) * h
+    vel = vel_fn(nxn, nyn, Lx + h, 1.0 + h)
+    cmin, cmax = float(vel.min()), float(vel.max())
+    omega = 2 * math.pi * cmin / (ppw * h)
+    wl = pml_cells * h * omega / (2 * math.pi * cmax)
+    return dict(name=name, domain=((0.0, Lx), (0.0, 1.0)), ncells=(nR, nC),
+                nels=(nyn // nR - 1, nxn // nC - 1), omega=omega, min_wl_in_pml=wl, velocity=vel,
+                vel_domain=((-h / 2, Lx + h / 2), (-h / 2, 1.0 + h / 2)), sources=sources_fn(Lx, h),
+                tol=tol, max_iter=max_iter, grid=(nxn, nyn), general=True, ppw=ppw)
+
+
+def lens_velocity_rect(nx, ny, Lx, Ly, cx, cy, amp, width=0.02):
+    """Gaussian lens c = 1 - amp exp(-r^2/width) on nx x ny cells over
+    [0,Lx]x[0,Ly] (row 0 = top), centre (cx, cy) in domain units."""
+    xs = (np.arange(nx) + 0.5) * (Lx / nx)
+    ys = Ly - (np.arange(ny) + 0.5) * (Ly / ny)
+    X, Y = np.meshgrid(xs, ys)
+    return (1.0 - amp * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / width)).astype(np.float32)
+
+
+def config_pack48(seed=0, count=48):
+    """The uneven batch: `count` independent problems drawn from one seed.
+
+    Each problem: the long grid side drawn from {256, 384, ..., 1536}, the
+    short side from the same set within aspect ratio 1..3, orientation
+    random; cells of about 128-200 nodes per side (2..10 per axis); either a
+    Gaussian lens (random centre and amplitude) or a layered model (the
+    layered generator with a per-problem seed); 8-12 points per wavelength;
+    20 PML cells; 1-16 point sources at random interior positions.
+    """
+    rng = np.random.default_rng(seed)
+    sides = np.arange(256, 1537, 128)
+    out = []
+    for k in range(count):
+        L = int(rng.choice(sides))
+        shorts = sides[(sides >= L / 3.0) & (sides <= L)]
+        S = int(rng.choice(shorts))
+        wide = bool(rng.integers(0, 2))
+        nxn, nyn = (L, S) if wide else (S, L)
+        cpx = int(np.clip(round(nxn / rng.uniform(128, 200)), 2, 10))
+        cpy = int(np.clip(round(nyn / rng.uniform(128, 200)), 2, 10))
+        nxn -= nxn % cpx
+        nyn -= nyn % cpy
+        ppw = float(rng.uniform(8.0, 12.0))
+        ns = int(rng.integers(1, 17))
+        lens = bool(rng.integers(0, 2))
+        pseed = int(rng.integers(0, 2**31 - 1))
+        cx, cy, amp = float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.2, 0.5))
+        sx, sy = rng.uniform(0.1, 0.9, ns), rng.uniform(0.1, 0.9, ns)
+        if lens:
+            vel_fn = (lambda nx, ny, Lx, Ly, cx=cx, cy=cy, amp=amp:
+                      lens_velocity_rect(nx, ny, Lx, Ly, cx * Lx, cy * Ly, amp))
+        else:
+            vel_fn = lambda nx, ny, Lx, Ly, s=pseed: layered_velocity(nx, ny, Lx, Ly, s)
+        src_fn = lambda Lx, h, sx=sx, sy=sy: [(float(a * Lx), float(b)) for a, b in zip(sx, sy)]
+        out.append(_rect_problem(nxn, nyn, (cpy, cpx), vel_fn, ppw, 20, src_fn,
+                                 f"p{k}_{'lens' if lens else 'layer'}_{nxn}x{nyn}"))
+    return out
+
+
+SERVE_GRIDS = [(512, 512), (1024, 512), (1024, 1024), (1536, 768), (1600, 800), (2048, 1024), (2400, 800),
+               (3000, 1000)]
+SERVE_PPW = [6.0, 8.0, 10.0, 12.0]
+
+
+def config_serve_problem(model, ppw_index, cell_nodes=200):
+    """Problem (model, omega) of the serving workload: layered model `model`
+    (seed = model) on grid SERVE_GRIDS[model] with cells of about
+    `cell_nodes` nodes, SERVE_PPW[ppw_index] points per wavelength, 20 PML
+    cells."""
+    nxn, nyn = SERVE_GRIDS[model]
+    cpx, cpy = max(1, round(nxn / cell_nodes)), max(1, round(nyn / cell_nodes))
+    nxn -= nxn % cpx
+    nyn -= nyn % cpy
+    ppw = SERVE_PPW[ppw_index]
+    vel_fn = lambda nx, ny, Lx, Ly, s=model: layered_velocity(nx, ny, Lx, Ly, s)
+    return _rect_problem(nxn, nyn, (cpy, cpx), vel_fn, ppw, 20, lambda Lx, h: [],
+                         f"m{model}_ppw{int(ppw)}_{nxn}x{nyn}")
+
+
+def serve_requests(count, rate, seed=0, models=8, freqs=4, max_sources=64):
+    """Poisson request stream: exponential inter-arrival times of mean 1/rate
+    seconds; each request picks (model, frequency) uniformly and carries 1-64
+    sources at random surface positions (x uniform in the interior, 10 nodes
+    below the top).  Returns [(t_arrival, model, ppw_index, [(x_frac, depth_nodes)])]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        m = int(rng.integers(0, models))
+        f = int(rng.integers(0, freqs))
+        ns = int(rng.integers(1, max_sources + 1))
+        xf = rng.uniform(0.05, 0.95, ns)
+        out.append((t, m, f, [float(a) for a in xf]))
+    return out
+
+
 CONFIGS = {"lens1600": config_lens_1600, "const400": config_const_400, "layered": config_layered_3x1}

@@ -56,3 +56,22 @@ def test_block_gmres_matches_direct_and_single(prec):
     # superposition: the summed source (the reference's single RHS) equals the sum
     xs, _ = dd.gmres(B.sum(axis=0), mode=1, tol=1e-10, max_iter=60, restart=20)
     assert relerr(X.sum(axis=0), xs) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_block_precond_wide_fronts(prec):
+    """Cells of 100 x 100 nodes: the separators near the root are solved by
+    several wavefronts per front (single and block right-hand sides)."""
+    cb, dd = make_pair((2, 2), (59, 59), 2 * math.pi * 6.0, precision=prec)
+    dd.factorize()
+    rng = np.random.default_rng(13)
+    X = rng.standard_normal((6, dd.N)) + 1j * rng.standard_normal((6, dd.N))
+    Y = dd.apply_precond_block(X)
+    if prec == 0:
+        cb.factorize()
+        P = Preconditioner(cb)
+    for k in range(6):
+        y1 = dd.apply_precond(X[k])
+        assert relerr(Y[k], y1) < (1e-12 if prec == 0 else 1e-6)
+        if prec == 0 and k < 2:
+            assert relerr(Y[k], P.apply_global(X[k])) < 1e-9

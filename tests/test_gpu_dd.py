@@ -17,6 +17,7 @@ CASES = [
     ((3, 3), (12, 12), 2 * math.pi * 4.1),
     ((4, 4), (9, 9), 2 * math.pi * 4.1),
     ((2, 4), (19, 9), 2 * math.pi * 4.1),
+    ((2, 2), (59, 59), 2 * math.pi * 6.0),   # fronts wide enough for multi-wavefront solves
 ]
 
 
