@@ -234,6 +234,8 @@ typedef struct lsw_server_stats {
   long long requests, completed, solves, loads, evictions, steps;
   double reserved_bytes, peak_reserved_bytes;
   int resident;
+  long long merged_steps; /* steps whose right-hand sides came from >= 2 requests */
+  int max_active;         /* most right-hand sides in flight in one step */
 } lsw_server_stats;
 
 int lsw_server_create(const lsw_server_options* opt, lsw_server** out);

@@ -224,7 +224,7 @@ void hostApply(lsw_dd* dd, const double* x, double* y, bool precond) {
   try {
     LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
     if (precond)
-      dd->dd->applyPrecondDev(dx, dy, dd->dd->stream());
+      dd->dd->applyPrecondDev(dx, dy, dd->dd->stream(), 1, false);
     else
       dd->dd->applyADev(dx, dy, dd->dd->stream());
     LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
@@ -262,7 +262,7 @@ int lsw_dd_apply_precond_block(lsw_dd* dd, int nrhs, const double* x, double* y)
     LSW_HIP(hipMalloc(&dy, n * sizeof(double2)));
     try {
       LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
-      dd->dd->applyPrecondDev(dx, dy, dd->dd->stream(), nrhs);
+      dd->dd->applyPrecondDev(dx, dy, dd->dd->stream(), nrhs, false);
       LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
       LSW_HIP(hipMemcpy(y, dy, n * sizeof(double2), hipMemcpyDeviceToHost));
     } catch (...) {
@@ -598,6 +598,8 @@ int lsw_server_get_stats(lsw_server* s, lsw_server_stats* st) {
     st->reserved_bytes = v.reservedBytes;
     st->peak_reserved_bytes = v.peakReservedBytes;
     st->resident = v.resident;
+    st->merged_steps = v.mergedSteps;
+    st->max_active = v.maxActive;
   });
 }
 

@@ -18,7 +18,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <vector>
 
@@ -94,6 +96,9 @@ class DDCheckerboard : public DD {
                      std::vector<DoubleVector> domain, std::vector<UInt> nCells,
                      std::vector<UInt> nElsInCell);
   bool deviceReady() const { return mFactors != nullptr; }
+  // Replay applyPrecondDev from captured HIP graphs (default on; the
+  // LSW_NO_GRAPHS environment variable turns it off for A/B timing).
+  void setGraphs(bool on) { mUseGraphs = on; if (!on) clearGraphs(); }
   // Geometry mode, set before setUp: false = the reference's spacing rule,
   // true = spacing per axis from that axis' cell count (rectangular domains).
   void setGeneralGeometry(bool g) { mGeneral = g; }
@@ -113,7 +118,9 @@ class DDCheckerboard : public DD {
   // y = M^{-1} x for nrhs right-hand sides (x, y: nrhs consecutive global
   // vectors).  Every local solve of a front step shares one read of the
   // cell's factors among up to kMaxRhs right-hand sides.
-  void applyPrecondDev(const double2* x, double2* y, hipStream_t s, int nrhs = 1) const;
+  // replay = false runs the launches directly (for one-shot buffers whose
+  // addresses will not recur).
+  void applyPrecondDev(const double2* x, double2* y, hipStream_t s, int nrhs = 1, bool replay = true) const;
   // Same, timing every batched-solve launch with HIP events (synchronises
   // after each sweep step; use outside timed regions).
   void applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof, int nrhs = 1) const;
@@ -207,6 +214,21 @@ class DDCheckerboard : public DD {
   mutable double2* dY = nullptr;  // 4 accumulators of N entries per right-hand side
   mutable int mBufRhs = 0;
   long long mXs = 0, mGs = 0;
+
+  // captured applyPrecondDev graphs per (x, y, nrhs, stream)
+  struct GraphKey {
+    const void* x;
+    const void* y;
+    int nrhs;
+    hipStream_t s;
+    bool operator<(const GraphKey& o) const {
+      return std::tie(x, y, nrhs, s) < std::tie(o.x, o.y, o.nrhs, o.s);
+    }
+  };
+  static constexpr size_t kMaxGraphs = 96;
+  bool mUseGraphs = std::getenv("LSW_NO_GRAPHS") == nullptr;
+  mutable std::map<GraphKey, hipGraphExec_t> mGraphs;
+  void clearGraphs() const;
 };
 
 }  // namespace lsw

@@ -161,6 +161,13 @@ __global__ void k_copy_traces(const CopyOp* __restrict__ ops, int nOps1, long lo
 }
 
 // res = res1 + (res2 + res3 + res4) (src/DDCheckerboard.cpp:733-735), per right-hand side
+// zero n 8-byte words (a kernel rather than a memset node, so that the
+// sequence can be captured into a graph)
+__global__ void k_zero_words(unsigned long long* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
 __global__ void k_combine(const double2* __restrict__ Y, double2* __restrict__ out, long long N, int nrhs) {
   const long long tot = N * nrhs;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
@@ -212,6 +219,9 @@ __global__ void k_point_sources(const double* __restrict__ src, int nsrc, double
 DDCheckerboard::DDCheckerboard() = default;
 
 DDCheckerboard::~DDCheckerboard() {
+  if (mStream) (void)hipStreamSynchronize(mStream);
+  for (auto& kv : mGraphs) (void)hipGraphExecDestroy(kv.second);
+  mGraphs.clear();
   for (auto& st : mSteps) {
     if (st.dSlots) (void)hipFree(st.dSlots);
     if (st.dGroups) (void)hipFree(st.dGroups);
@@ -484,6 +494,7 @@ void DDCheckerboard::initDevice() {
     return d;
   };
   mFactors = std::make_unique<FactorSet>();
+  mFactors->setSolvePrecision(mPrecision);
   for (auto& c : mCells) {
     std::vector<double> m;
     c.asmb.squaredSlowness(m);
@@ -926,8 +937,52 @@ void DDCheckerboard::factorize() {
 
 // ------------------------------------------------------------- operations
 
-void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s, int nrhs) const {
-  applyPrecondImpl(x, y, s, nullptr, nrhs);
+void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s, int nrhs, bool replay) const {
+  if (!s) s = mStream;
+  if (!mUseGraphs || !replay) {
+    applyPrecondImpl(x, y, s, nullptr, nrhs);
+    return;
+  }
+  // One M^{-1} application is ~30 launches per sweep step.  The sequence for
+  // a given (x, y, nrhs) is captured once into a graph and replayed, which
+  // takes the host launch cost off small problems.  Plans and buffers are
+  // prepared eagerly first: nothing inside the capture allocates or waits.
+  LSW_REQUIRE(mFactors && mFactors->factorized(), LSW_ERR_STATE, "applyPrecond before factorize");
+  LSW_REQUIRE(nrhs >= 1, LSW_ERR_ARG, "applyPrecond: nrhs must be positive");
+  const int before = mBufRhs;
+  ensureBuffers(nrhs);
+  if (mBufRhs != before) clearGraphs();  // captured graphs point at the old buffers
+  for (const auto& st : mSteps) (void)planFor(st, nrhs);
+  const GraphKey key{x, y, nrhs, s};
+  auto it = mGraphs.find(key);
+  if (it == mGraphs.end()) {
+    if (mGraphs.size() >= kMaxGraphs) clearGraphs();
+    hipGraph_t g = nullptr;
+    LSW_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      applyPrecondImpl(x, y, s, nullptr, nrhs);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    LSW_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    LSW_HIP(e);
+    it = mGraphs.emplace(key, ex).first;
+  }
+  LSW_HIP(hipGraphLaunch(it->second, s));
+}
+
+void DDCheckerboard::clearGraphs() const {
+  if (mGraphs.empty()) return;
+  hipStream_t last = nullptr;
+  for (auto& kv : mGraphs)
+    if (kv.first.s != last) (void)hipStreamSynchronize(last = kv.first.s);
+  for (auto& kv : mGraphs) (void)hipGraphExecDestroy(kv.second);
+  mGraphs.clear();
 }
 
 void DDCheckerboard::applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof, int nrhs) const {
@@ -945,8 +1000,14 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   ensureBuffers(nrhs);
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
   const long long n = N();
-  LSW_HIP(hipMemsetAsync(dY, 0, (size_t)4 * nrhs * n * sizeof(double2), s));
-  LSW_HIP(hipMemsetAsync(dT, 0, (size_t)nrhs * mTraceEntries * es, s));
+  {
+    const long long wy = 4LL * nrhs * n * 2, wt = (long long)(nrhs * mTraceEntries * es / 8);
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>((wy + 255) / 256, 8192)), dim3(256), 0, s,
+                       (unsigned long long*)dY, wy);
+    if (wt > 0)
+      hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>((wt + 255) / 256, 8192)), dim3(256), 0,
+                         s, (unsigned long long*)dT, wt);
+  }
   const CellDev* dCells = mFactors->deviceCells();
   const double2* dCoef = mFactors->deviceCoef();
   const int tb = 256;
@@ -1065,7 +1126,7 @@ void DDCheckerboard::applyPrecond(const std::vector<ComplexVector>& x, std::vect
   scatterOwned(x, g);
   DevBuf dx(N()), dy(N());
   LSW_HIP(hipMemcpy(dx.p, g.data(), N() * sizeof(double2), hipMemcpyHostToDevice));
-  applyPrecondDev(dx.p, dy.p, mStream);
+  applyPrecondDev(dx.p, dy.p, mStream, 1, false);
   LSW_HIP(hipStreamSynchronize(mStream));
   LSW_HIP(hipMemcpy(g.data(), dy.p, N() * sizeof(double2), hipMemcpyDeviceToHost));
   gatherOwned(g, res);

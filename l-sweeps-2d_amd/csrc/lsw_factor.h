@@ -72,6 +72,8 @@ class FactorSet {
   // Add a cell whose stencil (5*nx*ny complex, dir-major) is given on the host.
   int addCellWithStencil(const CellDev& c, const std::vector<double2>& coef);
 
+  // Precision the solves will run in (sizes plans before factorisation).
+  void setSolvePrecision(int precision) { mPrecision = precision; }
   // Build ND trees, upload descriptors, allocate, assemble stencils.
   void setup(hipStream_t s);
   // Multifrontal factorisation (FP64 on the device); FP32 copies of the

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <set>
 
 #include "lsw_model.h"
 #include "lsw_solver.h"
@@ -337,6 +338,12 @@ void Server::workerLoop() {
     }
     for (auto& a : admitted) e.inFlight[a.first] = a.second;
     mStats.steps++;
+    {
+      std::set<long long> reqs;
+      for (auto& kv : e.inFlight) reqs.insert(kv.second.req);
+      if (reqs.size() >= 2) mStats.mergedSteps++;
+      mStats.maxActive = std::max(mStats.maxActive, (int)e.inFlight.size());
+    }
     for (auto& f : finished) {
       e.inFlight.erase(f.second.tag);
       auto it = mRequests.find(f.first.req);

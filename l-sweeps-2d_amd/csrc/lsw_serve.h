@@ -72,6 +72,8 @@ struct ServeStats {
   long long loads = 0, evictions = 0, steps = 0;
   double reservedBytes = 0, peakReservedBytes = 0;
   int resident = 0;
+  long long mergedSteps = 0;  // steps whose right-hand sides came from >= 2 requests
+  int maxActive = 0;          // most right-hand sides in one step
 };
 
 class Server {

@@ -1,5 +1,12 @@
 """Block right-hand sides: M^{-1} applied to several vectors at once equals
-the single-vector application, for both precisions."""
+the single-vector application, for both precisions.
+
+With FP64 factors the two differ only by the order of the additions (1e-12).
+With FP32 factors the grouping of right-hand sides into solve tasks changes
+which tree nodes a task visits and so the order of FP32 additions; through
+the 21 sweep steps the two results then differ at the FP32 level of the
+preconditioner itself (1e-6 .. 1e-4 relative), which FP64 GMRES absorbs.  An
+indexing mistake shows as an O(1) difference."""
 import math
 
 import numpy as np
@@ -21,7 +28,7 @@ def test_block_precond_equals_single(prec, nrhs):
     Y = dd.apply_precond_block(X)
     for k in range(nrhs):
         y1 = dd.apply_precond(X[k])
-        assert relerr(Y[k], y1) < (1e-12 if prec == 0 else 1e-6)
+        assert relerr(Y[k], y1) < (1e-12 if prec == 0 else 1e-4)
 
 
 def test_block_precond_fp64_matches_oracle():
@@ -72,6 +79,6 @@ def test_block_precond_wide_fronts(prec):
         P = Preconditioner(cb)
     for k in range(6):
         y1 = dd.apply_precond(X[k])
-        assert relerr(Y[k], y1) < (1e-12 if prec == 0 else 1e-6)
+        assert relerr(Y[k], y1) < (1e-12 if prec == 0 else 1e-4)
         if prec == 0 and k < 2:
             assert relerr(Y[k], P.apply_global(X[k])) < 1e-9

