@@ -108,6 +108,11 @@ int lsw_dd_apply_a_dev(lsw_dd* dd, const void* x, void* y, void* stream);
 /* Local solves per applyPrecond, algorithmic bytes of those solves, factor bytes. */
 int lsw_dd_stats(lsw_dd* dd, long long* solves_per_precond, double* solve_bytes_per_precond,
                  long long* factor_bytes);
+/* on != 0: the _dev preconditioner entry points capture their launch
+ * sequence per (x, y, nrhs, stream) into a HIP graph once and replay it
+ * (off by default).  While a graph is being captured no other thread may
+ * issue legacy-stream work.  No counterpart in the reference. */
+int lsw_dd_set_graphs(lsw_dd* dd, int on);
 /* Wait for all work queued on the DD's own stream. */
 int lsw_dd_sync(lsw_dd* dd);
 /* One applyPrecond on device vectors with every batched local-solve launch

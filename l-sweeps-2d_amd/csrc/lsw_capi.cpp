@@ -319,6 +319,13 @@ int lsw_dd_profile_precond(lsw_dd* dd, const void* x, void* y, double* out) {
   });
 }
 
+int lsw_dd_set_graphs(lsw_dd* dd, int on) {
+  return guarded([&] {
+    LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");
+    dd->dd->setGraphs(on != 0);
+  });
+}
+
 int lsw_dd_sync(lsw_dd* dd) {
   return guarded([&] {
     LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");

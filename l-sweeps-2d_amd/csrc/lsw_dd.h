@@ -69,6 +69,8 @@ struct SlotDev {
   int nOut;
   int outIsRow[8], outIdx[8], outAdd[8];
   long long outDst[8];
+  int grp;       // (d, cell) index d*ncells + cell of the summed right-hand sides
+  int deferred;  // 1: the final solve of (d, cell): windowed RHS + summed sweep RHS
 };
 
 struct GroupDev {
@@ -96,9 +98,13 @@ class DDCheckerboard : public DD {
                      std::vector<DoubleVector> domain, std::vector<UInt> nCells,
                      std::vector<UInt> nElsInCell);
   bool deviceReady() const { return mFactors != nullptr; }
-  // Replay applyPrecondDev from captured HIP graphs (default on; the
-  // LSW_NO_GRAPHS environment variable turns it off for A/B timing).
+  // Replay applyPrecondDev from captured HIP graphs (off by default; the
+  // LSW_GRAPHS environment variable turns it on).  Capture needs the stream
+  // to itself: no other thread may issue legacy-stream work (synchronous
+  // copies, allocation with implicit synchronisation) while a graph is being
+  // captured, which is why the server keeps this off.
   void setGraphs(bool on) { mUseGraphs = on; if (!on) clearGraphs(); }
+  bool graphs() const { return mUseGraphs; }
   // Geometry mode, set before setUp: false = the reference's spacing rule,
   // true = spacing per axis from that axis' cell count (rectangular domains).
   void setGeneralGeometry(bool g) { mGeneral = g; }
@@ -197,6 +203,8 @@ class DDCheckerboard : public DD {
     int nGroups = 0;
     CopyOp* dCopies = nullptr;  // trace copies to run after this step
     int nCopies = 0;
+    bool deferred = false;      // the final full solves of every (d, cell)
+    std::vector<std::vector<Rect>> hNeed;  // solution lines needed per slot (traces)
   };
   std::vector<Step> mSteps;
   const SolvePlan& planFor(const Step& st, int nrhs) const;
@@ -212,6 +220,7 @@ class DDCheckerboard : public DD {
   mutable void* dG = nullptr;
   mutable void* dT = nullptr;
   mutable double2* dY = nullptr;  // 4 accumulators of N entries per right-hand side
+  mutable void* dF = nullptr;     // summed sweep right-hand sides per (d, cell) and right-hand side
   mutable int mBufRhs = 0;
   long long mXs = 0, mGs = 0;
 
@@ -226,7 +235,7 @@ class DDCheckerboard : public DD {
     }
   };
   static constexpr size_t kMaxGraphs = 96;
-  bool mUseGraphs = std::getenv("LSW_NO_GRAPHS") == nullptr;
+  bool mUseGraphs = std::getenv("LSW_GRAPHS") != nullptr;
   mutable std::map<GraphKey, hipGraphExec_t> mGraphs;
   void clearGraphs() const;
 };

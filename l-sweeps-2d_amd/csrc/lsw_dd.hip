@@ -32,12 +32,13 @@ __global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __
                             long long traceStride, long long Nglob, const double2* __restrict__ xin,
                             const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT,
                             int Gy, const double2* __restrict__ coef, const T2* __restrict__ T,
-                            T2* __restrict__ X, long long xs) {
+                            T2* __restrict__ X, long long xs, const T2* __restrict__ F, int nGrp) {
   const int sidx = blockIdx.y;
   const SlotDev& sl = slots[sidx % nSlots1];
   const int rhs = sidx / nSlots1;
   xin += (long long)rhs * Nglob;
   T += (long long)rhs * traceStride;
+  const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
   const CellDev& cd = cells[sl.cell];
   const int nx = cd.nx, ny = cd.ny, N = nx * ny;
   const int slr = sl.d & 1, sud = sl.d >> 1;
@@ -84,7 +85,52 @@ __global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __
         out = CTraits<T2>::make(acc.x, acc.y);
       }
     }
+    if (f) out = cadd(out, f[p]);
     x[p] = out;
+  }
+}
+
+// Sum the right-hand sides of the sweep solves of each (d, cell) of a step
+// into F: by linearity the solutions of all solves of one (d, cell), which
+// the reference adds into its result (`res +=`), are the solution of the sum
+// of their right-hand sides, computed once per application.  A sweep
+// right-hand side lives on its two trace sets (a column and/or a row
+// segment each); every point is added once, the slots of a group one after
+// the other.
+template <typename T2>
+__global__ void k_rhs_accum(const GroupDev* __restrict__ groups, int nGroups1, const SlotDev* __restrict__ slots,
+                            int nSlots1, const T2* __restrict__ X, long long xs, const CellDev* __restrict__ cells,
+                            T2* __restrict__ F, int nGrp) {
+  const GroupDev& g = groups[blockIdx.x % nGroups1];
+  const int rhs = blockIdx.x / nGroups1;
+  for (int k = 0; k < g.n; ++k) {
+    const SlotDev& sl = slots[g.slot[k]];
+    if (sl.kind != 0) {
+      const int ny = cells[sl.cell].ny;
+      const T2* x = X + ((long long)rhs * nSlots1 + g.slot[k]) * xs;
+      T2* f = F + ((long long)rhs * nGrp + sl.grp) * xs;
+      // segments: col 0, row 0, col 1, row 1
+      auto onSeg = [&](int m, int i, int j) -> bool {
+        const int s = m >> 1;
+        if ((m & 1) == 0) return sl.setCol[s] >= 0 && i == sl.setCol[s] && j >= sl.setColLo[s] && j < sl.setColHi[s];
+        return sl.setRow[s] >= 0 && j == sl.setRow[s] && i >= sl.setRowLo[s] && i < sl.setRowHi[s];
+      };
+      for (int m = 0; m < 4; ++m) {
+        const int s = m >> 1;
+        const bool col = (m & 1) == 0;
+        if (col ? sl.setCol[s] < 0 : sl.setRow[s] < 0) continue;
+        const int lo = col ? sl.setColLo[s] : sl.setRowLo[s], hi = col ? sl.setColHi[s] : sl.setRowHi[s];
+        for (int t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+          const int i = col ? sl.setCol[s] : t, j = col ? t : sl.setRow[s];
+          bool dup = false;
+          for (int mm = 0; mm < m; ++mm) dup = dup || onSeg(mm, i, j);
+          if (dup) continue;
+          const long long p = (long long)i * ny + j;
+          f[p] = cadd(f[p], x[p]);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -227,7 +273,7 @@ DDCheckerboard::~DDCheckerboard() {
     if (st.dGroups) (void)hipFree(st.dGroups);
     if (st.dCopies) (void)hipFree(st.dCopies);
   }
-  for (void* p : {dX, dR, dG, dT}) if (p) (void)hipFree(p);
+  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)hipFree(p);
   if (dY) (void)hipFree(dY);
   if (dGlobalCoef) (void)hipFree(dGlobalCoef);
   if (dColTrace) (void)hipFree(dColTrace);
@@ -744,6 +790,21 @@ void DDCheckerboard::buildSchedule() {
     steps.push_back(st);
   }
 
+  // ---- the final solves: one per (d, cell), right-hand side = the windowed
+  // RHS of stage 1 plus the summed sweep right-hand sides of that (d, cell)
+  {
+    HostStep st;
+    for (int d = 0; d < 4; ++d)
+      for (int c = 0; c < nc; ++c) {
+        SlotDev s = baseSlot(d, 0, c);
+        s.deferred = 1;
+        st.slots.push_back(s);
+      }
+    steps.push_back(st);
+  }
+  for (auto& hst : steps)
+    for (auto& sl : hst.slots) sl.grp = sl.d * nc + sl.cell;
+
   // slot buffers, tasks (same cell -> one task of up to kMaxRhs), groups
   mMaxSlots = 0;
   mSolvesPerPrecond = 0;
@@ -751,11 +812,27 @@ void DDCheckerboard::buildSchedule() {
   const double S = (mPrecision == LSW_FP64) ? 16.0 : 8.0;
   for (auto& st : steps) mMaxSlots = std::max(mMaxSlots, (int)st.slots.size());
   mSteps.clear();
-  for (auto& hst : steps) {
+  for (size_t si = 0; si < steps.size(); ++si) {
+    auto& hst = steps[si];
     Step st;
+    st.deferred = si + 1 == steps.size();
     for (size_t k = 0; k < hst.slots.size(); ++k) hst.slots[k].slot = (int)k;
     st.hSlots = hst.slots;
-    for (const auto& sl : hst.slots) st.hSupport.push_back(slotSupport(sl));
+    for (const auto& sl : hst.slots) {
+      // a final solve's right-hand side also holds the summed sweep sources,
+      // which live on trace lines outside the window: no forward pruning
+      st.hSupport.push_back(sl.deferred ? std::vector<Rect>{} : slotSupport(sl));
+      // a sweep solve is only read on its output trace lines
+      std::vector<Rect> need;
+      const CellGeom& cg = mCells[sl.cell];
+      for (int o = 0; o < sl.nOut; ++o) {
+        if (sl.outIsRow[o])
+          need.push_back({0, cg.nx, sl.outIdx[o], sl.outIdx[o] + 1});
+        else
+          need.push_back({sl.outIdx[o], sl.outIdx[o] + 1, 0, cg.ny});
+      }
+      st.hNeed.push_back(need);
+    }
     (void)S;
     mSolvesPerPrecond += (long long)hst.slots.size();
     // groups: (d, cell)
@@ -817,24 +894,31 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
     return std::make_tuple(a.cell, a.d, a.kind, s / n1, s % n1);
   };
   std::sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  // one right-hand side: the four DDs' solves of a cell share a task (4);
+  // blocks: up to 8 per task where the LDS staging allows
+  const int perTask = nrhs == 1 ? kTaskRhs4 : mFactors->maxTaskRhs();
   std::vector<SolveTask> tasks;
-  std::vector<std::vector<Rect>> supports;
+  std::vector<std::vector<Rect>> supports, needs;
   for (size_t k = 0; k < order.size();) {
     SolveTask tk;
     std::memset(&tk, 0, sizeof(tk));
     tk.cell = st.hSlots[order[k] % n1].cell;
-    std::vector<Rect> sup;
-    while (k < order.size() && st.hSlots[order[k] % n1].cell == tk.cell && tk.nrhs < kMaxRhs) {
+    std::vector<Rect> sup, need;
+    while (k < order.size() && st.hSlots[order[k] % n1].cell == tk.cell && tk.nrhs < perTask) {
       const int s = order[k++];
       tk.slot[tk.nrhs++] = s;
       const auto& r = st.hSupport[s % n1];
       sup.insert(sup.end(), r.begin(), r.end());
+      const auto& nd = st.hNeed[s % n1];
+      need.insert(need.end(), nd.begin(), nd.end());
     }
     tasks.push_back(tk);
     supports.push_back(sup);
+    needs.push_back(need);
   }
   auto plan = std::make_unique<SolvePlan>();
-  mFactors->buildPlan(tasks, supports, *plan);
+  // the final solves need the whole solution; the sweep solves only their traces
+  mFactors->buildPlan(tasks, supports, *plan, st.deferred ? nullptr : &needs);
   const SolvePlan& ref = *plan;
   st.plans[nrhs] = std::move(plan);
   return ref;
@@ -843,12 +927,13 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
 void DDCheckerboard::ensureBuffers(int nrhs) const {
   if (nrhs <= mBufRhs) return;
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
-  for (void* p : {dX, dR, dG, dT}) if (p) (void)hipFree(p);
+  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)hipFree(p);
   if (dY) (void)hipFree(dY);
-  dX = dR = dG = dT = nullptr;
+  dX = dR = dG = dT = dF = nullptr;
   dY = nullptr;
   mBufRhs = 0;
   const size_t slots = (size_t)mMaxSlots * nrhs;
+  LSW_HIP(hipMalloc(&dF, (size_t)4 * mCells.size() * nrhs * mXs * es));
   LSW_HIP(hipMalloc(&dX, slots * mXs * es));
   LSW_HIP(hipMalloc(&dR, slots * mXs * es));
   LSW_HIP(hipMalloc(&dG, slots * mGs * es));
@@ -1000,49 +1085,51 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   ensureBuffers(nrhs);
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
   const long long n = N();
-  {
-    const long long wy = 4LL * nrhs * n * 2, wt = (long long)(nrhs * mTraceEntries * es / 8);
-    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>((wy + 255) / 256, 8192)), dim3(256), 0, s,
-                       (unsigned long long*)dY, wy);
-    if (wt > 0)
-      hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>((wt + 255) / 256, 8192)), dim3(256), 0,
-                         s, (unsigned long long*)dT, wt);
-  }
+  const int nGrp = 4 * (int)mCells.size();
+  auto zero = [&](void* p, long long words) {
+    if (words > 0)
+      hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>((words + 255) / 256, 8192)), dim3(256), 0,
+                         s, (unsigned long long*)p, words);
+  };
+  zero(dY, 4LL * nrhs * n * 2);
+  zero(dT, (long long)(nrhs * mTraceEntries * es / 8));
+  zero(dF, (long long)((size_t)nGrp * nrhs * mXs * es / 8));
   const CellDev* dCells = mFactors->deviceCells();
   const double2* dCoef = mFactors->deviceCoef();
   const int tb = 256;
   const unsigned gxN = (unsigned)std::min<long long>((mFactors->maxDofs() + tb - 1) / tb, 1024);
   const unsigned gxT = (unsigned)((mTmax + 63) / 64);
-  for (const auto& st : mSteps) {
-    const unsigned ns = (unsigned)st.nSlots * nrhs, ng = (unsigned)st.nGroups * nrhs;
-    const unsigned nc = (unsigned)st.nCopies * nrhs;
-    if (mPrecision == LSW_FP64)
-      hipLaunchKernelGGL(k_build_rhs<double2>, dim3(gxN, ns), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
-                         mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const double2*)dT, (double2*)dX,
-                         mXs);
-    else
-      hipLaunchKernelGGL(k_build_rhs<float2>, dim3(gxN, ns), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
-                         mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const float2*)dT, (float2*)dX,
-                         mXs);
-    mFactors->solve(planFor(st, nrhs), dX, dR, dG, mXs, mGs, s, prof);
-    if (mPrecision == LSW_FP64) {
-      hipLaunchKernelGGL(k_post<double2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots,
-                         mTraceEntries, (const double2*)dX, mXs, (double2*)dT);
-      hipLaunchKernelGGL(k_accumulate<double2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
-                         st.nSlots, (const double2*)dX, mXs, dY, n, mGy);
-      if (st.nCopies)
-        hipLaunchKernelGGL(k_copy_traces<double2>, dim3(gxT, nc), dim3(64), 0, s, st.dCopies, st.nCopies,
-                           mTraceEntries, (double2*)dT, mTmax);
-    } else {
-      hipLaunchKernelGGL(k_post<float2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots,
-                         mTraceEntries, (const float2*)dX, mXs, (float2*)dT);
-      hipLaunchKernelGGL(k_accumulate<float2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
-                         st.nSlots, (const float2*)dX, mXs, dY, n, mGy);
-      if (st.nCopies)
-        hipLaunchKernelGGL(k_copy_traces<float2>, dim3(gxT, nc), dim3(64), 0, s, st.dCopies, st.nCopies,
-                           mTraceEntries, (float2*)dT, mTmax);
+  // per sweep step: right-hand sides, their sum per (d, cell), the solves
+  // (backward only where traces are read), trace extraction and transfer;
+  // the final step solves each (d, cell) once in full and accumulates it
+  auto run = [&](auto tag) {
+    using T2 = decltype(tag);
+    for (const auto& st : mSteps) {
+      const unsigned ns = (unsigned)st.nSlots * nrhs, ng = (unsigned)st.nGroups * nrhs;
+      const unsigned nc = (unsigned)st.nCopies * nrhs;
+      hipLaunchKernelGGL(k_build_rhs<T2>, dim3(gxN, ns), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
+                         mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const T2*)dT, (T2*)dX, mXs,
+                         (const T2*)dF, nGrp);
+      if (!st.deferred)
+        hipLaunchKernelGGL(k_rhs_accum<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, st.dSlots, st.nSlots,
+                           (const T2*)dX, mXs, dCells, (T2*)dF, nGrp);
+      mFactors->solve(planFor(st, nrhs), dX, dR, dG, mXs, mGs, s, prof);
+      if (st.deferred) {
+        hipLaunchKernelGGL(k_accumulate<T2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
+                           st.nSlots, (const T2*)dX, mXs, dY, n, mGy);
+      } else {
+        hipLaunchKernelGGL(k_post<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots, mTraceEntries,
+                           (const T2*)dX, mXs, (T2*)dT);
+        if (st.nCopies)
+          hipLaunchKernelGGL(k_copy_traces<T2>, dim3(gxT, nc), dim3(64), 0, s, st.dCopies, st.nCopies,
+                             mTraceEntries, (T2*)dT, mTmax);
+      }
     }
-  }
+  };
+  if (mPrecision == LSW_FP64)
+    run(double2{});
+  else
+    run(float2{});
   hipLaunchKernelGGL(k_combine, dim3((unsigned)std::min<long long>((n * nrhs + 255) / 256, 8192)), dim3(256), 0,
                      s, dY, y, n, nrhs);
   LSW_HIP(hipGetLastError());

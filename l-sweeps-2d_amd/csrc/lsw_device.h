@@ -51,7 +51,10 @@ struct ShapeDev {
 };
 
 // A batched solve task: up to kMaxRhs right-hand sides of one cell.
-constexpr int kMaxRhs = 4;
+constexpr int kMaxRhs = 8;
+// tasks of up to 4 right-hand sides always fit the solve kernels' LDS staging;
+// 8 when the widest front allows it (FactorSet::maxTaskRhs)
+constexpr int kTaskRhs4 = 4;
 struct SolveTask {
   int cell;
   int nrhs;

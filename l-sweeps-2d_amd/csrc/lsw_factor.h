@@ -36,21 +36,72 @@ struct Rect {
   int i0, i1, j0, j1;
 };
 
+// Node of any shape of a FactorSet as the solve kernels see it: one load
+// gives everything a work item needs (index pools of all shapes concatenated).
+struct SolveNode {
+  long long fwdOff, bwdOff;  // W_p and K_p offsets within the cell's factor blob
+  int e, b;
+  int E, B, map;             // offsets into the pooled E, B and child-map arrays
+  int gOff;                  // offset of g_p in a right-hand side's update buffer
+  int g1, g2;                // children's update offsets (-1: no child)
+  int pad[2];
+};
+
+// A task of a plan: one cell, up to kMaxRhs right-hand sides.
+struct SolveTaskDev {
+  long long facBase;  // the cell's factor offset
+  int nrhs;
+  int slot[kMaxRhs];
+  int pad;
+};
+
+// One work item: rows [r0, r0+rows) of one node of one task.
+// flags: bit 0/1 = child 1/2 visited by the forward pass, bit 2 = node visited.
+struct SolveEntry {
+  int task, node, r0rows, flags;  // r0rows = r0 | rows << 16
+};
+
+// The bottom levels of the tree are solved subtree by subtree: one workgroup
+// walks the levels of one (task, subtree) with a barrier between levels.
+constexpr int kSubLevels = 8;  // most levels fused per subtree
+struct SubDesc {
+  int ent0;                  // first entry of the subtree
+  int lvl[kSubLevels + 1];   // entries of height h: [ent0 + lvl[h], ent0 + lvl[h+1])
+  int pad[2];
+};
+
 // Launch plan of one batch of solves (device work lists per tree height).
 struct SolvePlan {
   struct Launch {
     long long off = 0;  // first entry in the plan's entry array
     int n = 0;          // entries
-    int ipb = 1;        // entries per workgroup
-    int waves = 1;      // wavefronts per workgroup (they split one entry's columns)
+    int mode = 0;       // 0 packed: P = 64/RP entries side by side per wavefront,
+                        // `wpb` independent wavefronts per workgroup, `rounds`
+                        // entries per lane group; 1 wide: `waves` wavefronts
+                        // split one entry's columns
+    int rp = 64;        // lanes per entry (rows per pass) in packed mode
+    int rounds = 1;
+    int wpb = 1;
+    int waves = 1;
     int cols = 0;       // largest column count (sizes the LDS)
     double bytes = 0;   // algorithmic bytes
   };
-  int ntasks = 0, maxNrhs = 1, activeWords = 0;
+  // fused bottom levels (heights 0 .. subLevels-1), one workgroup per subtree
+  struct Sub {
+    int n = 0;               // subtrees
+    int levels = 0;
+    int rpLog[kSubLevels] = {};
+    int cols[kSubLevels] = {};
+    size_t lds = 0;          // bytes per workgroup
+    double bytes = 0;
+    long long subOff = 0;    // first SubDesc of this launch in dSubs
+  };
+  Sub fwdSub, bwdSub;
+  SubDesc* dSubs = nullptr;
+  int ntasks = 0, maxNrhs = 1;
   std::vector<SolveTask> hTasks;
-  SolveTask* dTasks = nullptr;
-  int4* dEntries = nullptr;        // {task, node, r0, rows}
-  unsigned* dActive = nullptr;     // forward-visited nodes, activeWords per task
+  SolveTaskDev* dTasks = nullptr;
+  SolveEntry* dEntries = nullptr;
   std::vector<Launch> fwd, bwd;    // in launch order
   double fwdBytes = 0, bwdBytes = 0;
   SolvePlan() = default;
@@ -91,8 +142,12 @@ class FactorSet {
   // of task t is zero (empty list = anywhere).  The forward solve visits only
   // tree nodes whose region meets the support; the backward solve skips the
   // F11^{-1} r_E product of the nodes it did not visit.
+  // bwdSupports (optional, one per task): rectangles where the solution is
+  // needed; the backward solve then visits only nodes whose region meets
+  // them (ancestors included: their regions contain their descendants').  An
+  // empty list for a task means no solution values are needed at all.
   void buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
-                 SolvePlan& plan) const;
+                 SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports = nullptr) const;
   // Run a plan.  X holds the right-hand sides on entry and the solutions on
   // exit (slot-major, stride xStride complex entries); R and G are scratch
   // with the same slot layout (G stride gStride).  Element type follows
@@ -112,6 +167,9 @@ class FactorSet {
   const NdTree& tree(int shape) const { return *mTrees[shape]; }
   int numShapes() const { return (int)mTrees.size(); }
   int maxHeight() const { return mMaxHeight; }
+  // right-hand sides one solve task may carry: 8 when the widest front's
+  // staging (8 x front x element bytes) leaves room in the LDS, else 4
+  int maxTaskRhs() const;
   int maxFront() const { return mMaxFront; }
   // pivot/breakdown status of the last factorisation (0 = ok)
   int lastInfo() const { return mInfo; }
@@ -124,6 +182,12 @@ class FactorSet {
   std::map<std::pair<int, int>, int> mShapeOf;
   CellDev* dCells = nullptr;
   ShapeDev* dShapes = nullptr;
+  // flattened solve descriptors
+  std::vector<int> mShapeNodeBase;
+  SolveNode* dSolveNodes = nullptr;
+  int* dPoolE = nullptr;
+  int* dPoolB = nullptr;
+  int* dPoolMap = nullptr;
   std::vector<void*> mShapeAllocs;
   double2* dCoef = nullptr;
   double* dM = nullptr;

@@ -286,6 +286,39 @@ void FactorSet::setup(hipStream_t s) {
     d.maxHeight = t.maxHeight;
     d.nodeCount = (int)t.nodes.size();
   }
+  // flattened solve descriptors over all shapes
+  {
+    std::vector<SolveNode> flat;
+    std::vector<int> pe, pb, pm;
+    mShapeNodeBase.assign(mTrees.size(), 0);
+    for (size_t si = 0; si < mTrees.size(); ++si) {
+      const NdTree& t = *mTrees[si];
+      mShapeNodeBase[si] = (int)flat.size();
+      const int eb = (int)pe.size(), bb = (int)pb.size(), mb = (int)pm.size();
+      pe.insert(pe.end(), t.E.begin(), t.E.end());
+      pb.insert(pb.end(), t.B.begin(), t.B.end());
+      pm.insert(pm.end(), t.cmap.begin(), t.cmap.end());
+      for (const NdNode& nd : t.nodes) {
+        SolveNode sn;
+        std::memset(&sn, 0, sizeof(sn));
+        sn.fwdOff = nd.fwdOff;
+        sn.bwdOff = nd.bwdOff;
+        sn.e = nd.e;
+        sn.b = nd.b;
+        sn.E = eb + nd.eOff;
+        sn.B = bb + nd.bOff;
+        sn.map = mb + nd.mapOff;
+        sn.gOff = nd.gOff;
+        sn.g1 = nd.c1 >= 0 ? t.nodes[nd.c1].gOff : -1;
+        sn.g2 = nd.c2 >= 0 ? t.nodes[nd.c2].gOff : -1;
+        flat.push_back(sn);
+      }
+    }
+    dSolveNodes = devUpload(flat, mShapeAllocs);
+    dPoolE = devUpload(pe, mShapeAllocs);
+    dPoolB = devUpload(pb, mShapeAllocs);
+    dPoolMap = devUpload(pm, mShapeAllocs);
+  }
   LSW_HIP(hipMalloc(&dShapes, shapes.size() * sizeof(ShapeDev)));
   LSW_HIP(hipMemcpy(dShapes, shapes.data(), shapes.size() * sizeof(ShapeDev), hipMemcpyHostToDevice));
   LSW_HIP(hipMalloc(&dCells, mCells.size() * sizeof(CellDev)));

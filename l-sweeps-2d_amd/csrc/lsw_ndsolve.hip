@@ -8,19 +8,25 @@
 //   * the forward list only holds nodes whose subtree region meets the
 //     right-hand side support (sweep right-hand sides live on 2-4 grid lines),
 //   * the backward list holds every node, and skips the F11^{-1} r_E columns of
-//     nodes the forward pass did not visit (their r_E is zero),
-//   * a workgroup walks `ipb` consecutive entries so that tiny leaf fronts do
-//     not pay a workgroup launch each;
-//   * heights with few, large entries (the separators near the root, where a
-//     front step holds only a few dozen of them) give each entry W = 2..8
-//     wavefronts that split its columns, so the launch still keeps enough
-//     loads in flight to approach the HBM bandwidth.
-// Factors are column-major, so the lanes of a wavefront read consecutive rows
-// of one column (coalesced); right-hand-side values are broadcast from LDS.
+//     nodes the forward pass did not visit (their r_E is zero).
+// Everything an entry needs is one SolveEntry, one SolveTaskDev and one
+// SolveNode (loaded side by side), so an entry's chain of dependent memory
+// accesses is: entry -> task/node -> index pools -> vector values -> factors.
+// Two launch shapes:
+//   * packed: a wavefront holds P = 64/RP entries side by side (RP lanes =
+//     rows of one entry), `wpb` wavefronts per workgroup run independently,
+//     each lane group walks `rounds` entries; used wherever a height has
+//     enough entries (the many small fronts near the leaves);
+//   * wide: W = 2..8 wavefronts split the columns of one entry and reduce in
+//     LDS; used for heights with few, large entries (the separators near the
+//     root, where a front step holds only a few dozen of them).
+// Factors are column-major, so the lanes of an entry read consecutive rows of
+// one column (coalesced); right-hand-side values are broadcast from LDS.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 
 #include "lsw_cplx.h"
 #include "lsw_factor.h"
@@ -28,322 +34,609 @@
 namespace lsw {
 
 namespace {
-__device__ __forceinline__ bool bitSet(const unsigned* __restrict__ a, int node) {
-  return (a[node >> 5] >> (node & 31)) & 1u;
+constexpr double kMaxLds = 65536.0;  // bytes of dynamic LDS per workgroup
+
+// LDS accesses of one wavefront are performed in order; this only stops the
+// compiler from moving accesses across the point.
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// y[row] = sum_k M[k*ld + row] * v[k] over k in [k0, K) for rows [0, rows).
-// A workgroup of W wavefronts works on one product: a wavefront splits into
-// RP rows x CG column groups, the W*CG column groups stride over the columns
-// (8 independent loads in flight per lane), partial sums are reduced across
-// the column groups of a wavefront by lane shuffles and across wavefronts in
-// a fixed order through LDS (`red`: W*64*NR entries).  On return the owner
-// lanes (wavefront 0, first column group, r < rows) hold the result.
-template <typename T2, int NR, int W>
-__device__ __forceinline__ bool gemvCols(const T2* __restrict__ M, long long ld, int rows, int k0, int K,
-                                         const T2* v, int vstride, T2* red, T2 (&acc)[NR], int& r) {
+// acc[q] += sum_{k = k0, k0+kstep, ... < K} M[k*ld] * v[q*vs + k]; 8 loads in flight.
+template <typename T2, int NR>
+__device__ __forceinline__ void rowDot(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
+                                       const T2* v, int vs, T2 (&acc)[NR]) {
+  int k = k0;
+  for (; k + 7 * kstep < K; k += 8 * kstep) {
+    T2 m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = M[(long long)(k + u * kstep) * ld];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vs + k + u * kstep], acc[q]);
+  }
+  for (; k < K; k += kstep) {
+    const T2 m0 = M[(long long)k * ld];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = cfma(m0, v[q * vs + k], acc[q]);
+  }
+}
+
+struct Item {
+  SolveTaskDev tk;
+  SolveNode nd;
+  int r0, rows, flags;
+};
+
+__device__ __forceinline__ Item loadItem(const SolveEntry* __restrict__ ent, const SolveTaskDev* __restrict__ tasks,
+                                         const SolveNode* __restrict__ nodes, long long idx) {
+  Item it;
+  const SolveEntry en = ent[idx];
+  it.tk = tasks[en.task];
+  it.nd = nodes[en.node];
+  it.r0 = en.r0rows & 0xffff;
+  it.rows = en.r0rows >> 16;
+  it.flags = en.flags;
+  return it;
+}
+
+// forward gather: v[q*e + k] = X[E[k]] - g_c1[m1[k]] - g_c2[m2[k]]; the first
+// row chunk also keeps r_E for the backward pass
+template <typename T2, int NR>
+__device__ __forceinline__ void fwdGather(const Item& it, int k0, int kstep, const int* __restrict__ poolE,
+                                          const int* __restrict__ poolMap, const T2* __restrict__ X,
+                                          T2* __restrict__ Rb, const T2* __restrict__ G, long long xs, long long gs,
+                                          T2* v) {
+  const int e = it.nd.e, F = e + it.nd.b;
+  const bool c1 = (it.flags & 1) != 0, c2 = (it.flags & 2) != 0;
+  const int* E = poolE + it.nd.E;
+  const int* m1 = poolMap + it.nd.map;
+  const int* m2 = m1 + F;
+  for (int k = k0; k < e; k += kstep) {
+    const int dof = E[k];
+    const int a1 = c1 ? m1[k] : -1, a2 = c2 ? m2[k] : -1;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      if (q >= it.tk.nrhs) break;
+      const long long sl = it.tk.slot[q];
+      T2 val = X[sl * xs + dof];
+      if (a1 >= 0) val = csub(val, G[sl * gs + it.nd.g1 + a1]);
+      if (a2 >= 0) val = csub(val, G[sl * gs + it.nd.g2 + a2]);
+      v[q * e + k] = val;
+      if (it.r0 == 0) Rb[sl * xs + dof] = val;
+    }
+  }
+}
+
+// forward scatter of row r: g_p[r0+r] = acc + children's updates on that row
+template <typename T2, int NR>
+__device__ __forceinline__ void fwdStore(const Item& it, int r, const T2 (&acc)[NR], const int* __restrict__ poolMap,
+                                         T2* __restrict__ G, long long gs) {
+  const int e = it.nd.e, F = e + it.nd.b;
+  const int f = e + it.r0 + r;
+  const int* m1 = poolMap + it.nd.map;
+  const int a1 = (it.flags & 1) ? m1[f] : -1, a2 = (it.flags & 2) ? m1[F + f] : -1;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    if (q >= it.tk.nrhs) break;
+    const long long sl = it.tk.slot[q];
+    T2 g = acc[q];
+    if (a1 >= 0) g = cadd(g, G[sl * gs + it.nd.g1 + a1]);
+    if (a2 >= 0) g = cadd(g, G[sl * gs + it.nd.g2 + a2]);
+    G[sl * gs + it.nd.gOff + it.r0 + r] = g;
+  }
+}
+
+// backward gather: v[q*F + k] = r_E[k] (k < e) or x_B[k-e]
+template <typename T2, int NR>
+__device__ __forceinline__ void bwdGather(const Item& it, int k0, int kstep, const int* __restrict__ poolE,
+                                          const int* __restrict__ poolB, const T2* __restrict__ X,
+                                          const T2* __restrict__ Rb, long long xs, T2* v) {
+  const int e = it.nd.e, F = e + it.nd.b;
+  const int* E = poolE + it.nd.E;
+  const int* Bp = poolB + it.nd.B;
+  for (int k = k0; k < F; k += kstep) {
+    const bool inE = k < e;
+    const int dof = inE ? E[k] : Bp[k - e];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      if (q >= it.tk.nrhs) break;
+      const long long sl = it.tk.slot[q];
+      v[q * F + k] = inE ? Rb[sl * xs + dof] : X[sl * xs + dof];
+    }
+  }
+}
+
+template <typename T2, int NR>
+__device__ __forceinline__ void bwdStore(const Item& it, int r, const T2 (&acc)[NR], const int* __restrict__ poolE,
+                                         T2* __restrict__ X, long long xs) {
+  const int dof = poolE[it.nd.E + it.r0 + r];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    if (q >= it.tk.nrhs) break;
+    X[(long long)it.tk.slot[q] * xs + dof] = acc[q];
+  }
+}
+
+// ---------------------------------------------------------------- packed
+// LDS: per wavefront P regions of NR x cols values.
+template <typename T2, int NR, bool FWD>
+__global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restrict__ tasks,
+                                                   const SolveNode* __restrict__ nodes,
+                                                   const SolveEntry* __restrict__ ent, int nent, int rounds,
+                                                   int rpLog, int cols, const int* __restrict__ poolE,
+                                                   const int* __restrict__ poolB, const int* __restrict__ poolMap,
+                                                   const T2* __restrict__ factor, T2* __restrict__ X,
+                                                   T2* __restrict__ Rb, T2* __restrict__ G, long long xs,
+                                                   long long gs) {
+  extern __shared__ char smem_raw[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int RP = 1 << rpLog, P = 64 >> rpLog;
+  const int grp = lane >> rpLog, r = lane & (RP - 1);
+  T2* v = reinterpret_cast<T2*>(smem_raw) + ((size_t)wave * P + grp) * NR * cols;
+  const long long waveId = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  for (int round = 0; round < rounds; ++round) {
+    const long long idx = (waveId * rounds + round) * P + grp;
+    const bool valid = idx < nent;
+    Item it;
+    if (valid) {
+      it = loadItem(ent, tasks, nodes, idx);
+      if constexpr (FWD)
+        fwdGather<T2, NR>(it, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
+      else
+        bwdGather<T2, NR>(it, (it.flags & 4) ? r : it.nd.e + r, RP, poolE, poolB, X, Rb, xs, v);
+    }
+    waveSync();
+    if (valid && r < it.rows) {
+      T2 acc[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
+      if constexpr (FWD) {
+        rowDot<T2, NR>(factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r, it.nd.b, 0, it.nd.e, 1, v, it.nd.e,
+                       acc);
+        fwdStore<T2, NR>(it, r, acc, poolMap, G, gs);
+      } else {
+        const int F = it.nd.e + it.nd.b;
+        rowDot<T2, NR>(factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r, it.nd.e, (it.flags & 4) ? 0 : it.nd.e,
+                       F, 1, v, F, acc);
+        bwdStore<T2, NR>(it, r, acc, poolE, X, xs);
+      }
+    }
+    waveSync();
+  }
+}
+
+// ------------------------------------------------------------------ wide
+// W wavefronts per entry: 64 rows x W*(64/RP) column groups; partial sums
+// reduced by shuffles inside a wavefront and in a fixed order across
+// wavefronts through LDS.  LDS: NR x cols values, then W x 64 x NR partials.
+template <typename T2, int NR, int W, bool FWD>
+__global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restrict__ tasks,
+                                                    const SolveNode* __restrict__ nodes,
+                                                    const SolveEntry* __restrict__ ent, int nent, int cols,
+                                                    const int* __restrict__ poolE, const int* __restrict__ poolB,
+                                                    const int* __restrict__ poolMap, const T2* __restrict__ factor,
+                                                    T2* __restrict__ X, T2* __restrict__ Rb, T2* __restrict__ G,
+                                                    long long xs, long long gs) {
+  extern __shared__ char smem_raw[];
+  T2* v = reinterpret_cast<T2*>(smem_raw);
+  T2* red = v + (size_t)NR * cols;
+  const long long idx = blockIdx.x;
+  if (idx >= nent) return;
+  const Item it = loadItem(ent, tasks, nodes, idx);
+  const int e = it.nd.e, F = e + it.nd.b;
+  const int k0 = FWD ? 0 : ((it.flags & 4) ? 0 : e);
+  if constexpr (FWD)
+    fwdGather<T2, NR>(it, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
+  else
+    bwdGather<T2, NR>(it, k0 + threadIdx.x, 64 * W, poolE, poolB, X, Rb, xs, v);
+  __syncthreads();
+  const int rows = it.rows;
+  if (rows == 0) return;  // uniform: a root's forward entry only keeps r_E
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int RP = 1;
   while (RP < rows) RP <<= 1;
   const int CG = 64 / RP;
-  r = lane & (RP - 1);
-  const int cg = lane / RP;
-  const int CGA = W * CG;
+  const int r = lane & (RP - 1), cg = lane / RP;
+  T2 acc[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
   if (r < rows) {
-    const T2* p = M + r;
-    int k = k0 + w * CG + cg;
-    for (; k + 7 * CGA < K; k += 8 * CGA) {
-      T2 m[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) m[u] = p[(long long)(k + u * CGA) * ld];
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vstride + k + u * CGA], acc[q]);
-    }
-    for (; k < K; k += CGA) {
-      const T2 m0 = p[(long long)k * ld];
-#pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = cfma(m0, v[q * vstride + k], acc[q]);
-    }
+    if constexpr (FWD)
+      rowDot<T2, NR>(factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r, it.nd.b, w * CG + cg, e, W * CG, v, e,
+                     acc);
+    else
+      rowDot<T2, NR>(factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r, e, k0 + w * CG + cg, F, W * CG, v, F, acc);
   }
   for (int off = RP; off < 64; off <<= 1) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
   }
-  if constexpr (W == 1) {
-    return cg == 0 && r < rows;
-  } else {
-    if (cg == 0 && r < rows) {
+  if (cg == 0 && r < rows) {
 #pragma unroll
-      for (int q = 0; q < NR; ++q) red[(w * 64 + r) * NR + q] = acc[q];
+    for (int q = 0; q < NR; ++q) red[(w * 64 + r) * NR + q] = acc[q];
+  }
+  __syncthreads();
+  if (w == 0 && cg == 0 && r < rows) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      T2 s = red[r * NR + q];
+      for (int ww = 1; ww < W; ++ww) s = cadd(s, red[(ww * 64 + r) * NR + q]);
+      acc[q] = s;
     }
-    __syncthreads();
-    const bool own = w == 0 && cg == 0 && r < rows;
-    if (own) {
+    if constexpr (FWD)
+      fwdStore<T2, NR>(it, r, acc, poolMap, G, gs);
+    else
+      bwdStore<T2, NR>(it, r, acc, poolE, X, xs);
+  }
+}
+// ------------------------------------------------------------- subtrees
+// One workgroup (4 wavefronts) per (task, subtree): the levels of the
+// subtree in order (forward: leaves first; backward: the subtree root
+// first), every level's entries spread over the workgroup's lane groups,
+// and a workgroup barrier between levels.  The update vectors and solution
+// values a level writes are read by the next level from global memory; all
+// wavefronts of a workgroup share one CU and its L1, so the barrier is
+// enough to make them visible.
+struct SubLevels {
+  int n;
+  int rpLog[kSubLevels];
+  int cols[kSubLevels];
+};
+
+template <typename T2, int NR, bool FWD>
+__global__ __launch_bounds__(256) void k_nd_sub(const SolveTaskDev* __restrict__ tasks,
+                                                const SolveNode* __restrict__ nodes,
+                                                const SolveEntry* __restrict__ ent, const SubDesc* __restrict__ subs,
+                                                SubLevels lv, const int* __restrict__ poolE,
+                                                const int* __restrict__ poolB, const int* __restrict__ poolMap,
+                                                const T2* __restrict__ factor, T2* __restrict__ X,
+                                                T2* __restrict__ Rb, T2* __restrict__ G, long long xs, long long gs) {
+  extern __shared__ char smem_raw[];
+  const SubDesc sd = subs[blockIdx.x];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int li = 0; li < lv.n; ++li) {
+    const int h = FWD ? li : lv.n - 1 - li;
+    const int b0 = sd.ent0 + sd.lvl[h], n = sd.lvl[h + 1] - sd.lvl[h];
+    if (n > 0) {
+      const int rpLog = lv.rpLog[h], RP = 1 << rpLog, P = 64 >> rpLog;
+      const int grp = lane >> rpLog, r = lane & (RP - 1);
+      const int gid = wave * P + grp, ng = nw * P;
+      T2* v = reinterpret_cast<T2*>(smem_raw) + (size_t)gid * NR * lv.cols[h];
+      const int iters = (n + ng - 1) / ng;
+      for (int pass = 0; pass < iters; ++pass) {
+        const int i = pass * ng + gid;
+        const bool valid = i < n;
+        Item it;
+        if (valid) {
+          it = loadItem(ent, tasks, nodes, b0 + i);
+          if constexpr (FWD)
+            fwdGather<T2, NR>(it, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
+          else
+            bwdGather<T2, NR>(it, (it.flags & 4) ? r : it.nd.e + r, RP, poolE, poolB, X, Rb, xs, v);
+        }
+        waveSync();
+        if (valid && r < it.rows) {
+          T2 acc[NR];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        T2 s = red[r * NR + q];
-        for (int ww = 1; ww < W; ++ww) s = cadd(s, red[(ww * 64 + r) * NR + q]);
-        acc[q] = s;
+          for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
+          if constexpr (FWD) {
+            rowDot<T2, NR>(factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r, it.nd.b, 0, it.nd.e, 1, v, it.nd.e,
+                           acc);
+            fwdStore<T2, NR>(it, r, acc, poolMap, G, gs);
+          } else {
+            const int F = it.nd.e + it.nd.b;
+            rowDot<T2, NR>(factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r, it.nd.e,
+                           (it.flags & 4) ? 0 : it.nd.e, F, 1, v, F, acc);
+            bwdStore<T2, NR>(it, r, acc, poolE, X, xs);
+          }
+        }
+        waveSync();
       }
     }
-    return own;
+    __syncthreads();
   }
 }
 }  // namespace
-
-// LDS layout of both kernels: v (NR x cols right-hand-side values), then the
-// cross-wavefront reduction buffer (W x 64 x NR) when W > 1.
-template <typename T2, int NR, int W>
-__global__ __launch_bounds__(64 * W) void k_nd_fwd(const CellDev* __restrict__ cells,
-                                                   const ShapeDev* __restrict__ shapes,
-                                                   const SolveTask* __restrict__ tasks,
-                                                   const int4* __restrict__ ent, int nent, int ipb, int cols,
-                                                   const unsigned* __restrict__ active, int aw,
-                                                   const T2* __restrict__ factor, T2* __restrict__ X,
-                                                   T2* __restrict__ Rb, T2* __restrict__ G, long long xs,
-                                                   long long gs) {
-  extern __shared__ char smem_raw[];
-  T2* v = reinterpret_cast<T2*>(smem_raw);
-  T2* red = v + (size_t)NR * cols;
-  const int tid = threadIdx.x;
-  for (int q = 0; q < ipb; ++q) {
-    const int idx = blockIdx.x * ipb + q;
-    if (idx >= nent) break;
-    const int4 en = ent[idx];
-    const SolveTask& tk = tasks[en.x];
-    const CellDev& cd = cells[tk.cell];
-    const ShapeDev& sh = shapes[cd.shape];
-    const NdNodeDev nd = sh.nodes[en.y];
-    const int e = nd.e, b = nd.b, F = e + b;
-    const unsigned* act = active + (long long)en.x * aw;
-    const int* E = sh.E + nd.eOff;
-    const int* m1 = sh.cmap + nd.mapOff;
-    const int* m2 = m1 + F;
-    const bool a1ok = nd.c1 >= 0 && bitSet(act, nd.c1);
-    const bool a2ok = nd.c2 >= 0 && bitSet(act, nd.c2);
-    const int g1 = a1ok ? sh.nodes[nd.c1].gOff : 0;
-    const int g2 = a2ok ? sh.nodes[nd.c2].gOff : 0;
-    const int nr = tk.nrhs;
-    for (int k = tid; k < e; k += 64 * W) {
-      const int a1 = a1ok ? m1[k] : -1, a2 = a2ok ? m2[k] : -1, dof = E[k];
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        if (r >= nr) break;
-        const long long sl = tk.slot[r];
-        T2 val = X[sl * xs + dof];
-        if (a1 >= 0) val = csub(val, G[sl * gs + g1 + a1]);
-        if (a2 >= 0) val = csub(val, G[sl * gs + g2 + a2]);
-        v[r * e + k] = val;
-        if (en.z == 0) Rb[sl * xs + dof] = val;
-      }
-    }
-    __syncthreads();
-    const int rows = en.w;
-    if (rows > 0) {
-      T2 acc[NR];
-      int r;
-      const bool own = gemvCols<T2, NR, W>(factor + cd.factorOff + nd.fwdOff + en.z, b, rows, 0, e, v, e, red,
-                                           acc, r);
-      if (own) {
-        const int f = e + en.z + r;
-        const int a1 = a1ok ? m1[f] : -1, a2 = a2ok ? m2[f] : -1;
-#pragma unroll
-        for (int qq = 0; qq < NR; ++qq) {
-          if (qq >= nr) break;
-          const long long sl = tk.slot[qq];
-          T2 g = acc[qq];
-          if (a1 >= 0) g = cadd(g, G[sl * gs + g1 + a1]);
-          if (a2 >= 0) g = cadd(g, G[sl * gs + g2 + a2]);
-          G[sl * gs + nd.gOff + en.z + r] = g;
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T2, int NR, int W>
-__global__ __launch_bounds__(64 * W) void k_nd_bwd(const CellDev* __restrict__ cells,
-                                                   const ShapeDev* __restrict__ shapes,
-                                                   const SolveTask* __restrict__ tasks,
-                                                   const int4* __restrict__ ent, int nent, int ipb, int cols,
-                                                   const unsigned* __restrict__ active, int aw,
-                                                   const T2* __restrict__ factor, T2* __restrict__ X,
-                                                   const T2* __restrict__ Rb, long long xs) {
-  extern __shared__ char smem_raw[];
-  T2* v = reinterpret_cast<T2*>(smem_raw);
-  T2* red = v + (size_t)NR * cols;
-  const int tid = threadIdx.x;
-  for (int q = 0; q < ipb; ++q) {
-    const int idx = blockIdx.x * ipb + q;
-    if (idx >= nent) break;
-    const int4 en = ent[idx];
-    const SolveTask& tk = tasks[en.x];
-    const CellDev& cd = cells[tk.cell];
-    const ShapeDev& sh = shapes[cd.shape];
-    const NdNodeDev nd = sh.nodes[en.y];
-    const int e = nd.e, b = nd.b, F = e + b;
-    const bool visited = bitSet(active + (long long)en.x * aw, en.y);
-    const int k0 = visited ? 0 : e;
-    const int* E = sh.E + nd.eOff;
-    const int* Bp = sh.B + nd.bOff;
-    const int nr = tk.nrhs;
-    for (int k = k0 + tid; k < F; k += 64 * W) {
-#pragma unroll
-      for (int qq = 0; qq < NR; ++qq) {
-        if (qq >= nr) break;
-        const long long sl = tk.slot[qq];
-        v[qq * F + k] = (k < e) ? Rb[sl * xs + E[k]] : X[sl * xs + Bp[k - e]];
-      }
-    }
-    __syncthreads();
-    T2 acc[NR];
-    int r;
-    const int rows = en.w;
-    const bool own = gemvCols<T2, NR, W>(factor + cd.factorOff + nd.bwdOff + en.z, e, rows, k0, F, v, F, red,
-                                         acc, r);
-    if (own) {
-      const int dof = E[en.z + r];
-#pragma unroll
-      for (int qq = 0; qq < NR; ++qq) {
-        if (qq >= nr) break;
-        X[(long long)tk.slot[qq] * xs + dof] = acc[qq];
-      }
-    }
-    __syncthreads();
-  }
-}
 
 // ------------------------------------------------------------------ plans
 SolvePlan::~SolvePlan() {
   if (dTasks) (void)hipFree(dTasks);
   if (dEntries) (void)hipFree(dEntries);
-  if (dActive) (void)hipFree(dActive);
+  if (dSubs) (void)hipFree(dSubs);
 }
 
 namespace {
-constexpr double kMaxLds = 65536.0;  // bytes of dynamic LDS per workgroup
-
 bool meets(const NdNode& nd, const std::vector<Rect>& sup) {
   if (sup.empty()) return true;
   for (const Rect& r : sup)
     if (r.i0 < nd.i1 && nd.i0 < r.i1 && r.j0 < nd.j1 && nd.j0 < r.j1) return true;
   return false;
 }
+
+int ceilLog2(int x) {
+  int l = 0;
+  while ((1 << l) < x) ++l;
+  return l;
+}
+
+// Heights 0..subHeight() are solved subtree by subtree (-1: never).  The
+// LSW_SUB_HEIGHT environment variable overrides the default for tuning.
+int subHeight() {
+  static const int h = [] {
+    const char* e = std::getenv("LSW_SUB_HEIGHT");
+    const int v = e ? std::atoi(e) : -1;
+    return std::max(-1, std::min(kSubLevels - 1, v));
+  }();
+  return h;
+}
+
+// subtree sizes of a preorder-numbered tree
+std::vector<int> subtreeSizes(const NdTree& t) {
+  std::vector<int> sz(t.nodes.size(), 1);
+  for (int p = (int)t.nodes.size() - 1; p >= 0; --p) {
+    const NdNode& nd = t.nodes[p];
+    if (nd.c1 >= 0) sz[p] += sz[nd.c1];
+    if (nd.c2 >= 0) sz[p] += sz[nd.c2];
+  }
+  return sz;
+}
 }  // namespace
 
 void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
-                          SolvePlan& plan) const {
+                          SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports) const {
   LSW_REQUIRE(!mTrees.empty(), LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(supports.empty() || supports.size() == tasks.size(), LSW_ERR_ARG, "one support per task");
+  LSW_REQUIRE(!bwdSupports || bwdSupports->size() == tasks.size(), LSW_ERR_ARG, "one backward support per task");
   const int nt = (int)tasks.size();
   LSW_REQUIRE(nt > 0, LSW_ERR_ARG, "empty plan");
-  int maxNodes = 0;
-  for (auto& t : mTrees) maxNodes = std::max(maxNodes, (int)t->nodes.size());
-  plan.activeWords = (maxNodes + 31) / 32;
   plan.ntasks = nt;
   plan.hTasks = tasks;
   plan.maxNrhs = 1;
-  std::vector<unsigned> act((size_t)nt * plan.activeWords, 0u);
   const double S = mPrecision == LSW_FP32 ? 8.0 : 16.0;
+  std::vector<SolveTaskDev> tdev(nt);
   // entries per height
-  std::vector<std::vector<int4>> fw(mMaxHeight + 1), bw(mMaxHeight + 1);
+  std::vector<std::vector<SolveEntry>> fw(mMaxHeight + 1), bw(mMaxHeight + 1);
   std::vector<int> fcols(mMaxHeight + 1, 1), bcols(mMaxHeight + 1, 1);
+  std::vector<int> frows(mMaxHeight + 1, 0), brows(mMaxHeight + 1, 0);
   std::vector<double> fbytes(mMaxHeight + 1, 0.0), bbytes(mMaxHeight + 1, 0.0);
   std::vector<double> fwork(mMaxHeight + 1, 0.0), bwork(mMaxHeight + 1, 0.0);
+  std::vector<char> on, need;
+  // fused bottom levels
+  const int Hc = std::min(subHeight(), mMaxHeight);
+  const int nlev = Hc + 1;
+  std::vector<SubDesc> subF, subB;
+  std::vector<SolveEntry> entF, entB;  // subtree entries, grouped per subtree and height
+  std::vector<int> sfRows(kSubLevels, 0), sfCols(kSubLevels, 1), sbRows(kSubLevels, 0), sbCols(kSubLevels, 1);
+  double sfBytes = 0, sbBytes = 0;
+  std::map<int, std::vector<int>> sizes;
+  std::vector<std::vector<SolveEntry>> lf(kSubLevels), lb(kSubLevels);
   for (int t = 0; t < nt; ++t) {
     const SolveTask& tk = tasks[t];
     LSW_REQUIRE(tk.nrhs >= 1 && tk.nrhs <= kMaxRhs, LSW_ERR_ARG, "bad task right-hand-side count");
     LSW_REQUIRE(tk.cell >= 0 && tk.cell < (int)mCells.size(), LSW_ERR_ARG, "bad task cell");
     plan.maxNrhs = std::max(plan.maxNrhs, tk.nrhs);
-    const NdTree& tr = *mTrees[mCells[tk.cell].shape];
+    SolveTaskDev& td = tdev[t];
+    std::memset(&td, 0, sizeof(td));
+    td.facBase = mCells[tk.cell].factorOff;
+    td.nrhs = tk.nrhs;
+    for (int q = 0; q < kMaxRhs; ++q) td.slot[q] = q < tk.nrhs ? tk.slot[q] : 0;
+    const int shape = mCells[tk.cell].shape;
+    const NdTree& tr = *mTrees[shape];
+    const int base = mShapeNodeBase[shape];
     const std::vector<Rect> none;
     const std::vector<Rect>& sup = supports.empty() ? none : supports[t];
     const double r = tk.nrhs;
+    on.assign(tr.nodes.size(), 0);
+    for (int p = 0; p < (int)tr.nodes.size(); ++p) on[p] = meets(tr.nodes[p], sup) ? 1 : 0;
+    // nodes whose solution values are needed
+    need.assign(tr.nodes.size(), 1);
+    if (bwdSupports) {
+      const std::vector<Rect>& bs = (*bwdSupports)[t];
+      for (int p = 0; p < (int)tr.nodes.size(); ++p) need[p] = !bs.empty() && meets(tr.nodes[p], bs) ? 1 : 0;
+    }
+    auto& sz = sizes[shape];
+    if (sz.empty()) sz = subtreeSizes(tr);
+    // subtrees of the fused bottom levels: roots of height <= Hc under a parent
+    // above Hc; their nodes form a preorder range
+    for (int p = 0; Hc >= 0 && p < (int)tr.nodes.size(); ++p) {
+      const NdNode& sr = tr.nodes[p];
+      if (sr.height > Hc || (sr.parent >= 0 && tr.nodes[sr.parent].height <= Hc)) continue;
+      for (auto& v : lf) v.clear();
+      for (auto& v : lb) v.clear();
+      for (int q = p; q < p + sz[p]; ++q) {
+        const NdNode& nd = tr.nodes[q];
+        const int h = nd.height;
+        const double e = nd.e, b = nd.b;
+        int flags = 0;
+        if (nd.c1 >= 0 && on[nd.c1]) flags |= 1;
+        if (nd.c2 >= 0 && on[nd.c2]) flags |= 2;
+        if (on[q]) flags |= 4;
+        if (on[q]) {
+          if (nd.b == 0) {
+            lf[h].push_back({t, base + q, 0, flags});
+          } else {
+            for (int r0 = 0; r0 < nd.b; r0 += kSolveRows) {
+              const int rows = std::min(kSolveRows, nd.b - r0);
+              lf[h].push_back({t, base + q, r0 | (rows << 16), flags});
+              sfRows[h] = std::max(sfRows[h], rows);
+            }
+          }
+          sfCols[h] = std::max(sfCols[h], nd.e);
+          sfBytes += S * (b * e + r * (2 * e + 2 * b));
+        }
+        if (!need[q]) continue;
+        for (int r0 = 0; r0 < nd.e; r0 += kSolveRows) {
+          const int rows = std::min(kSolveRows, nd.e - r0);
+          lb[h].push_back({t, base + q, r0 | (rows << 16), flags});
+          sbRows[h] = std::max(sbRows[h], rows);
+        }
+        sbCols[h] = std::max(sbCols[h], nd.e + nd.b);
+        const double kcols = on[q] ? e + b : b;
+        sbBytes += S * (e * kcols + r * (kcols + e));
+      }
+      auto emit = [&](std::vector<std::vector<SolveEntry>>& lv, std::vector<SolveEntry>& dst,
+                      std::vector<SubDesc>& subs) {
+        size_t cnt = 0;
+        for (int h = 0; h < nlev; ++h) cnt += lv[h].size();
+        if (cnt == 0) return;
+        SubDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.ent0 = (int)dst.size();
+        for (int h = 0; h < nlev; ++h) {
+          d.lvl[h] = (int)(dst.size() - d.ent0);
+          dst.insert(dst.end(), lv[h].begin(), lv[h].end());
+        }
+        for (int h = nlev; h <= kSubLevels; ++h) d.lvl[h] = (int)(dst.size() - d.ent0);
+        subs.push_back(d);
+      };
+      emit(lf, entF, subF);
+      emit(lb, entB, subB);
+    }
     for (int p = 0; p < (int)tr.nodes.size(); ++p) {
       const NdNode& nd = tr.nodes[p];
-      const bool on = meets(nd, sup);
-      if (on) act[(size_t)t * plan.activeWords + (p >> 5)] |= 1u << (p & 31);
       const int h = nd.height;
+      if (h <= Hc) continue;  // in a fused subtree
       const double e = nd.e, b = nd.b;
-      if (on) {
+      int flags = 0;
+      if (nd.c1 >= 0 && on[nd.c1]) flags |= 1;
+      if (nd.c2 >= 0 && on[nd.c2]) flags |= 2;
+      if (on[p]) flags |= 4;
+      if (on[p]) {
         if (nd.b == 0) {
-          fw[h].push_back(make_int4(t, p, 0, 0));
+          fw[h].push_back({t, base + p, 0, flags});
         } else {
-          for (int r0 = 0; r0 < nd.b; r0 += kSolveRows)
-            fw[h].push_back(make_int4(t, p, r0, std::min(kSolveRows, nd.b - r0)));
+          for (int r0 = 0; r0 < nd.b; r0 += kSolveRows) {
+            const int rows = std::min(kSolveRows, nd.b - r0);
+            fw[h].push_back({t, base + p, r0 | (rows << 16), flags});
+            frows[h] = std::max(frows[h], rows);
+          }
         }
         fcols[h] = std::max(fcols[h], nd.e);
         fbytes[h] += S * (b * e + r * (2 * e + 2 * b));
         fwork[h] += b * e + e;
       }
-      for (int r0 = 0; r0 < nd.e; r0 += kSolveRows)
-        bw[h].push_back(make_int4(t, p, r0, std::min(kSolveRows, nd.e - r0)));
+      if (!need[p]) continue;
+      for (int r0 = 0; r0 < nd.e; r0 += kSolveRows) {
+        const int rows = std::min(kSolveRows, nd.e - r0);
+        bw[h].push_back({t, base + p, r0 | (rows << 16), flags});
+        brows[h] = std::max(brows[h], rows);
+      }
       bcols[h] = std::max(bcols[h], nd.e + nd.b);
-      const double kcols = on ? e + b : b;
+      const double kcols = on[p] ? e + b : b;
       bbytes[h] += S * (e * kcols + r * (kcols + e));
       bwork[h] += e * kcols;
     }
   }
-  // flatten: forward heights ascending, backward descending
-  std::vector<int4> all;
-  auto addLaunch = [&](std::vector<SolvePlan::Launch>& out, const std::vector<int4>& v, int cols, double bytes,
-                       double work) {
+  const double nrk = plan.maxNrhs == 1 ? 1.0 : plan.maxNrhs <= kTaskRhs4 ? (double)kTaskRhs4 : (double)kMaxRhs;
+  std::vector<SolveEntry> all;
+  auto addLaunch = [&](std::vector<SolvePlan::Launch>& out, const std::vector<SolveEntry>& v, int cols, int maxRows,
+                       double bytes, double work) {
     if (v.empty()) return;
     SolvePlan::Launch L;
     L.off = (long long)all.size();
     L.n = (int)v.size();
     L.cols = cols;
     L.bytes = bytes;
-    // several entries per workgroup when entries are small, while keeping
-    // enough workgroups to fill the device
     const double per = work / v.size();
-    int ipb = (int)std::max(1.0, std::min(16.0, 2048.0 / std::max(per, 1.0)));
-    while (ipb > 1 && (v.size() + ipb - 1) / ipb < 4096) ipb /= 2;
-    L.ipb = ipb;
-    // few, large entries: several wavefronts per entry split the columns
+    // few large entries: split their columns over W wavefronts
     int W = 1;
     while (W < 8 && (double)v.size() * W * 2 <= 4096.0 && per >= 1024.0 * W * 2) W *= 2;
-    // LDS: NR x cols right-hand-side values + W x 64 x NR reduction entries
-    const double nrk = plan.maxNrhs == 1 ? 1.0 : (double)kMaxRhs;
-    auto lds = [&](int w) { return S * nrk * ((double)cols + (w > 1 ? 64.0 * w : 0.0)); };
-    while (W > 1 && lds(W) > kMaxLds) W /= 2;
-    LSW_REQUIRE(lds(1) <= kMaxLds, LSW_ERR_CAPACITY, "front too wide for the solve kernel's LDS staging");
-    L.waves = W;
-    if (W > 1) L.ipb = 1;
+    auto ldsWide = [&](int w) { return S * nrk * ((double)cols + 64.0 * w); };
+    while (W > 1 && ldsWide(W) > kMaxLds) W /= 2;
+    if (W > 1) {
+      L.mode = 1;
+      L.waves = W;
+    } else {
+      // packed: RP lanes per entry (>= 8 so that the LDS stays bounded)
+      L.mode = 0;
+      // (a launch of roots only, maxRows == 0, just gathers r_E: all 64 lanes)
+      int rp = maxRows == 0 ? 64 : 1 << std::max(3, ceilLog2(std::max(1, maxRows)));
+      auto ldsWave = [&](int rpv) { return S * nrk * cols * (64.0 / rpv); };
+      while (rp < 64 && ldsWave(rp) > kMaxLds) rp *= 2;
+      LSW_REQUIRE(ldsWave(rp) <= kMaxLds, LSW_ERR_CAPACITY, "front too wide for the solve kernel's LDS staging");
+      L.rp = rp;
+      int wpb = 4;
+      while (wpb > 1 && wpb * ldsWave(rp) > kMaxLds) wpb /= 2;
+      L.wpb = wpb;
+      // entries per lane group: keep at least ~4096 wavefronts in flight
+      const long long groups = (long long)v.size() * rp / 64 + 1;
+      L.rounds = (int)std::min<long long>(16, std::max<long long>(1, groups / 4096));
+    }
     all.insert(all.end(), v.begin(), v.end());
     out.push_back(L);
   };
+  // fused subtree launches: entries first in the entry array, SubDesc offsets
+  // shifted by where each list lands
+  auto makeSub = [&](SolvePlan::Sub& L, std::vector<SubDesc>& subs, std::vector<SolveEntry>& ents,
+                     const std::vector<int>& rowsH, const std::vector<int>& colsH, double bytes,
+                     std::vector<SubDesc>& allSubs) {
+    L = SolvePlan::Sub();
+    if (subs.empty()) return;
+    L.n = (int)subs.size();
+    L.levels = nlev;
+    L.bytes = bytes;
+    L.subOff = (long long)allSubs.size();
+    const int shift = (int)all.size();
+    for (auto& d : subs) d.ent0 += shift;
+    all.insert(all.end(), ents.begin(), ents.end());
+    allSubs.insert(allSubs.end(), subs.begin(), subs.end());
+    double lds = 0;
+    for (int h = 0; h < nlev; ++h) {
+      int rp = 1 << std::max(3, ceilLog2(std::max(1, rowsH[h])));
+      auto need = [&](int rpv) { return S * nrk * colsH[h] * 4.0 * (64.0 / rpv); };  // 4 wavefronts
+      while (rp < 64 && need(rp) > kMaxLds) rp *= 2;
+      LSW_REQUIRE(need(rp) <= kMaxLds, LSW_ERR_CAPACITY, "subtree level too wide for the LDS staging");
+      L.rpLog[h] = ceilLog2(rp);
+      L.cols[h] = colsH[h];
+      lds = std::max(lds, need(rp));
+    }
+    L.lds = (size_t)lds;
+  };
+  std::vector<SubDesc> allSubs;
+  makeSub(plan.fwdSub, subF, entF, sfRows, sfCols, sfBytes, allSubs);
+  makeSub(plan.bwdSub, subB, entB, sbRows, sbCols, sbBytes, allSubs);
   plan.fwd.clear();
   plan.bwd.clear();
-  plan.fwdBytes = plan.bwdBytes = 0;
-  for (int h = 0; h <= mMaxHeight; ++h) {
-    addLaunch(plan.fwd, fw[h], fcols[h], fbytes[h], fwork[h]);
+  plan.fwdBytes = plan.fwdSub.bytes;
+  plan.bwdBytes = plan.bwdSub.bytes;
+  for (int h = Hc + 1; h <= mMaxHeight; ++h) {
+    addLaunch(plan.fwd, fw[h], fcols[h], frows[h], fbytes[h], fwork[h]);
     plan.fwdBytes += fbytes[h];
   }
-  for (int h = mMaxHeight; h >= 0; --h) {
-    addLaunch(plan.bwd, bw[h], bcols[h], bbytes[h], bwork[h]);
+  for (int h = mMaxHeight; h > Hc; --h) {
+    addLaunch(plan.bwd, bw[h], bcols[h], brows[h], bbytes[h], bwork[h]);
     plan.bwdBytes += bbytes[h];
   }
   if (std::getenv("LSW_PLAN_LOG")) {
+    for (int pass = 0; pass < 2; ++pass) {
+      const SolvePlan::Sub& Ls = pass ? plan.bwdSub : plan.fwdSub;
+      std::fprintf(stderr, "plan %s tasks %d nrhs %d: subtrees %d levels %d lds %zu bytes %.0f\n",
+                   pass ? "bws" : "fws", nt, plan.maxNrhs, Ls.n, Ls.levels, Ls.lds, Ls.bytes);
+    }
     for (int pass = 0; pass < 2; ++pass)
       for (const auto& L : pass ? plan.bwd : plan.fwd)
-        std::fprintf(stderr, "plan %s tasks %d nrhs %d: entries %d ipb %d waves %d cols %d bytes %.0f\n",
-                     pass ? "bwd" : "fwd", nt, plan.maxNrhs, L.n, L.ipb, L.waves, L.cols, L.bytes);
+        std::fprintf(stderr, "plan %s tasks %d nrhs %d: entries %d mode %d rp %d rounds %d wpb %d waves %d cols %d "
+                     "bytes %.0f\n", pass ? "bwd" : "fwd", nt, plan.maxNrhs, L.n, L.mode, L.rp, L.rounds, L.wpb,
+                     L.waves, L.cols, L.bytes);
   }
   if (plan.dTasks) (void)hipFree(plan.dTasks);
   if (plan.dEntries) (void)hipFree(plan.dEntries);
-  if (plan.dActive) (void)hipFree(plan.dActive);
-  LSW_HIP(hipMalloc(&plan.dTasks, tasks.size() * sizeof(SolveTask)));
-  LSW_HIP(hipMemcpy(plan.dTasks, tasks.data(), tasks.size() * sizeof(SolveTask), hipMemcpyHostToDevice));
-  LSW_HIP(hipMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(int4)));
-  LSW_HIP(hipMemcpy(plan.dEntries, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
-  LSW_HIP(hipMalloc(&plan.dActive, act.size() * sizeof(unsigned)));
-  LSW_HIP(hipMemcpy(plan.dActive, act.data(), act.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+  if (plan.dSubs) (void)hipFree(plan.dSubs);
+  plan.dTasks = nullptr;
+  plan.dEntries = nullptr;
+  plan.dSubs = nullptr;
+  if (!allSubs.empty()) {
+    LSW_HIP(hipMalloc(&plan.dSubs, allSubs.size() * sizeof(SubDesc)));
+    LSW_HIP(hipMemcpy(plan.dSubs, allSubs.data(), allSubs.size() * sizeof(SubDesc), hipMemcpyHostToDevice));
+  }
+  LSW_HIP(hipMalloc(&plan.dTasks, tdev.size() * sizeof(SolveTaskDev)));
+  LSW_HIP(hipMemcpy(plan.dTasks, tdev.data(), tdev.size() * sizeof(SolveTaskDev), hipMemcpyHostToDevice));
+  LSW_HIP(hipMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(SolveEntry)));
+  LSW_HIP(hipMemcpy(plan.dEntries, all.data(), all.size() * sizeof(SolveEntry), hipMemcpyHostToDevice));
 }
 
 namespace {
@@ -389,48 +682,82 @@ struct LaunchTimer {
   }
 };
 
-template <typename T2, int NR, int W>
-void launchFwd(const SolvePlan& plan, const SolvePlan::Launch& L, const CellDev* dCells, const ShapeDev* dShapes,
-               const T2* factor, T2* X, T2* R, T2* G, long long xs, long long gs, hipStream_t s) {
-  const unsigned grid = (unsigned)((L.n + L.ipb - 1) / L.ipb);
-  const size_t shm = ((size_t)NR * L.cols + (W > 1 ? (size_t)W * 64 * NR : 0)) * sizeof(T2);
-  hipLaunchKernelGGL((k_nd_fwd<T2, NR, W>), dim3(grid), dim3(64 * W), shm, s, dCells, dShapes, plan.dTasks,
-                     plan.dEntries + L.off, L.n, L.ipb, L.cols, plan.dActive, plan.activeWords, factor, X, R, G, xs,
-                     gs);
+struct SolveArgs {
+  const SolveTaskDev* tasks;
+  const SolveNode* nodes;
+  const SolveEntry* ent;
+  const int *poolE, *poolB, *poolMap;
+  long long xs, gs;
+};
+
+template <typename T2, int NR, bool FWD>
+void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, T2* X, T2* R, T2* G, hipStream_t s) {
+  const SolveEntry* ent = a.ent + L.off;
+  if (L.mode == 0) {
+    const int P = 64 / L.rp;
+    const long long perBlock = (long long)P * L.rounds * L.wpb;
+    const unsigned grid = (unsigned)((L.n + perBlock - 1) / perBlock);
+    const size_t shm = (size_t)L.wpb * P * NR * L.cols * sizeof(T2);
+    const int rpLog = ceilLog2(L.rp);
+    hipLaunchKernelGGL((k_nd_packed<T2, NR, FWD>), dim3(grid), dim3(64 * L.wpb), shm, s, a.tasks, a.nodes, ent, L.n,
+                       L.rounds, rpLog, L.cols, a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+    return;
+  }
+  const size_t shm = ((size_t)NR * L.cols + (size_t)L.waves * 64 * NR) * sizeof(T2);
+  const dim3 grid((unsigned)L.n);
+  switch (L.waves) {
+    case 2:
+      hipLaunchKernelGGL((k_nd_wide<T2, NR, 2, FWD>), grid, dim3(128), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_nd_wide<T2, NR, 4, FWD>), grid, dim3(256), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+      break;
+    default:
+      hipLaunchKernelGGL((k_nd_wide<T2, NR, 8, FWD>), grid, dim3(512), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+      break;
+  }
 }
 
-template <typename T2, int NR, int W>
-void launchBwd(const SolvePlan& plan, const SolvePlan::Launch& L, const CellDev* dCells, const ShapeDev* dShapes,
-               const T2* factor, T2* X, T2* R, long long xs, hipStream_t s) {
-  const unsigned grid = (unsigned)((L.n + L.ipb - 1) / L.ipb);
-  const size_t shm = ((size_t)NR * L.cols + (W > 1 ? (size_t)W * 64 * NR : 0)) * sizeof(T2);
-  hipLaunchKernelGGL((k_nd_bwd<T2, NR, W>), dim3(grid), dim3(64 * W), shm, s, dCells, dShapes, plan.dTasks,
-                     plan.dEntries + L.off, L.n, L.ipb, L.cols, plan.dActive, plan.activeWords, factor, X, R, xs);
+template <typename T2, int NR, bool FWD>
+void launchSub(const SolvePlan& plan, const SolvePlan::Sub& L, const SolveArgs& a, const T2* factor, T2* X, T2* R,
+               T2* G, hipStream_t s) {
+  SubLevels lv;
+  std::memset(&lv, 0, sizeof(lv));
+  lv.n = L.levels;
+  for (int h = 0; h < L.levels; ++h) {
+    lv.rpLog[h] = L.rpLog[h];
+    lv.cols[h] = L.cols[h];
+  }
+  hipLaunchKernelGGL((k_nd_sub<T2, NR, FWD>), dim3((unsigned)L.n), dim3(256), L.lds, s, a.tasks, a.nodes, a.ent,
+                     plan.dSubs + L.subOff, lv, a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
 }
 
 template <typename T2, int NR>
-void runPlan(const SolvePlan& plan, const CellDev* dCells, const ShapeDev* dShapes, const T2* factor, T2* X, T2* R,
-             T2* G, long long xs, long long gs, hipStream_t s, SolveProfile* prof) {
+void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, T2* X, T2* R, T2* G, hipStream_t s,
+             SolveProfile* prof) {
   LaunchTimer tm{prof, s, {}, {}, {}};
+  if (plan.fwdSub.n > 0) {
+    tm.begin();
+    launchSub<T2, NR, true>(plan, plan.fwdSub, a, factor, X, R, G, s);
+    tm.end(0, plan.fwdSub.bytes);
+  }
   for (const auto& L : plan.fwd) {
     tm.begin();
-    switch (L.waves) {
-      case 1: launchFwd<T2, NR, 1>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
-      case 2: launchFwd<T2, NR, 2>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
-      case 4: launchFwd<T2, NR, 4>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
-      default: launchFwd<T2, NR, 8>(plan, L, dCells, dShapes, factor, X, R, G, xs, gs, s); break;
-    }
+    launch<T2, NR, true>(a, L, factor, X, R, G, s);
     tm.end(0, L.bytes);
   }
   for (const auto& L : plan.bwd) {
     tm.begin();
-    switch (L.waves) {
-      case 1: launchBwd<T2, NR, 1>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
-      case 2: launchBwd<T2, NR, 2>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
-      case 4: launchBwd<T2, NR, 4>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
-      default: launchBwd<T2, NR, 8>(plan, L, dCells, dShapes, factor, X, R, xs, s); break;
-    }
+    launch<T2, NR, false>(a, L, factor, X, R, G, s);
     tm.end(1, L.bytes);
+  }
+  if (plan.bwdSub.n > 0) {
+    tm.begin();
+    launchSub<T2, NR, false>(plan, plan.bwdSub, a, factor, X, R, G, s);
+    tm.end(1, plan.bwdSub.bytes);
   }
   LSW_HIP(hipGetLastError());
   tm.finish();
@@ -441,19 +768,28 @@ void FactorSet::solve(const SolvePlan& plan, void* X, void* R, void* G, long lon
                       hipStream_t s, SolveProfile* prof) const {
   LSW_REQUIRE(mFactorized, LSW_ERR_STATE, "solve called before factorize");
   if (plan.ntasks <= 0) return;
+  const SolveArgs a{plan.dTasks, dSolveNodes, plan.dEntries, dPoolE, dPoolB, dPoolMap, xs, gs};
   if (mPrecision == LSW_FP64) {
     if (plan.maxNrhs == 1)
-      runPlan<double2, 1>(plan, dCells, dShapes, dFactor, (double2*)X, (double2*)R, (double2*)G, xs, gs, s, prof);
+      runPlan<double2, 1>(plan, a, dFactor, (double2*)X, (double2*)R, (double2*)G, s, prof);
+    else if (plan.maxNrhs <= kTaskRhs4)
+      runPlan<double2, kTaskRhs4>(plan, a, dFactor, (double2*)X, (double2*)R, (double2*)G, s, prof);
     else
-      runPlan<double2, kMaxRhs>(plan, dCells, dShapes, dFactor, (double2*)X, (double2*)R, (double2*)G, xs, gs, s,
-                                prof);
+      runPlan<double2, kMaxRhs>(plan, a, dFactor, (double2*)X, (double2*)R, (double2*)G, s, prof);
   } else {
     if (plan.maxNrhs == 1)
-      runPlan<float2, 1>(plan, dCells, dShapes, dFactor32, (float2*)X, (float2*)R, (float2*)G, xs, gs, s, prof);
+      runPlan<float2, 1>(plan, a, dFactor32, (float2*)X, (float2*)R, (float2*)G, s, prof);
+    else if (plan.maxNrhs <= kTaskRhs4)
+      runPlan<float2, kTaskRhs4>(plan, a, dFactor32, (float2*)X, (float2*)R, (float2*)G, s, prof);
     else
-      runPlan<float2, kMaxRhs>(plan, dCells, dShapes, dFactor32, (float2*)X, (float2*)R, (float2*)G, xs, gs, s,
-                               prof);
+      runPlan<float2, kMaxRhs>(plan, a, dFactor32, (float2*)X, (float2*)R, (float2*)G, s, prof);
   }
+}
+
+int FactorSet::maxTaskRhs() const {
+  const double S = mPrecision == LSW_FP32 ? 8.0 : 16.0;
+  // packed staging of one 64-row entry: 8 x front; wide: plus 2 wavefronts' partials
+  return S * kMaxRhs * ((double)mMaxFront + 128.0) <= kMaxLds ? kMaxRhs : kTaskRhs4;
 }
 
 }  // namespace lsw

@@ -275,6 +275,7 @@ void Server::workerLoop() {
     try {
       if (load) {
         auto dd = makeCheckerboard(e.params, true);
+        dd->setGraphs(false);  // workers run concurrently: no stream capture
         dd->factorize();
         dd->reserveBlock(mOpt.slots);
         e.slots = std::make_unique<GmresSlots>(dd.get(), mOpt.slots, mOpt.restart, mOpt.tol, mOpt.maxIter);

@@ -77,7 +77,8 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
   const long long N = mA->rows;
   const long long G = std::max<long long>(mFS->maxG(), 1);
   const size_t es = mPrecision == LSW_FP64 ? sizeof(double2) : sizeof(float2);
-  const int ntasks = (nrhs + kMaxRhs - 1) / kMaxRhs;
+  const int per = mFS->maxTaskRhs();
+  const int ntasks = (nrhs + per - 1) / per;
   if (nrhs > mCap) {
     for (void* p : {dX, dR, dG}) if (p) (void)hipFree(p);
     if (dTasks) (void)hipFree(dTasks);
@@ -91,8 +92,8 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
   for (int t = 0; t < ntasks; ++t) {
     std::memset(&tasks[t], 0, sizeof(SolveTask));
     tasks[t].cell = 0;
-    tasks[t].nrhs = std::min(kMaxRhs, nrhs - t * kMaxRhs);
-    for (int r = 0; r < tasks[t].nrhs; ++r) tasks[t].slot[r] = t * kMaxRhs + r;
+    tasks[t].nrhs = std::min(per, nrhs - t * per);
+    for (int r = 0; r < tasks[t].nrhs; ++r) tasks[t].slot[r] = t * per + r;
   }
   if (mPrecision == LSW_FP64) {
     LSW_HIP(hipMemcpyAsync(dX, b, (size_t)nrhs * N * sizeof(double2), hipMemcpyHostToDevice, mStream));

@@ -33,6 +33,7 @@ EXPORTED = [
     "lsw_gmres_solve_block", "lsw_gmres_solve_block_dev", "lsw_dd_assemble_rhs_block_dev",
     "lsw_server_create", "lsw_server_destroy", "lsw_server_add_problem", "lsw_server_submit",
     "lsw_server_wait", "lsw_server_get_stats", "lsw_server_problem_bytes", "lsw_dd_write_vtk",
+    "lsw_dd_set_graphs",
 ]
 
 
@@ -133,6 +134,7 @@ _sig("lsw_dd_apply_a", _P, _D, _D)
 _sig("lsw_dd_apply_a_dev", _P, _P, _P, _P)
 _sig("lsw_dd_stats", _P, C.POINTER(C.c_longlong), _D, C.POINTER(C.c_longlong))
 _sig("lsw_dd_sync", _P)
+_sig("lsw_dd_set_graphs", _P, C.c_int)
 _sig("lsw_dd_profile_precond", _P, _P, _P, _D)
 _sig("lsw_dd_apply_precond_block", _P, C.c_int, _D, _D)
 _sig("lsw_dd_apply_precond_block_dev", _P, C.c_int, _P, _P, _P)
@@ -314,6 +316,9 @@ class DDCheckerboard:
         ys = np.zeros(self.Gy)
         _check(_lib.lsw_dd_global_coords(self._h, _dptr(xs), _dptr(ys)))
         return xs, ys
+
+    def set_graphs(self, on=True):
+        _check(_lib.lsw_dd_set_graphs(self._h, int(bool(on))))
 
     def write_vtk(self, x, path):
         """<path>.vtk (OutputVTK layout) and <path>.lswgrid (full precision)."""

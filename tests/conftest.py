@@ -1,6 +1,13 @@
 import os
 import sys
 
+# torch ships its own HIP runtime: load it before the library so that both
+# use the same one (the library then binds to the runtime already loaded)
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "l-sweeps-2d_amd")
 for p in (ROOT, PKG):

@@ -17,6 +17,7 @@ def test_replays_equal_direct_launches(prec, nrhs):
     import torch
     cb, dd = make_pair((3, 3), (19, 19), 2 * math.pi * 3.3, precision=prec)
     dd.factorize()
+    dd.set_graphs(True)
     rng = np.random.default_rng(21)
     X = rng.standard_normal((nrhs, dd.N)) + 1j * rng.standard_normal((nrhs, dd.N))
     ref = dd.apply_precond_block(X) if nrhs > 1 else dd.apply_precond(X[0])[None, :]  # direct launches
