@@ -19,6 +19,7 @@ on the device.  Factorisation is timed separately, as the reference does
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lens1600|const400]
        python bench.py --workload serve    (request stream through the server)
        python bench.py --workload pack48   (48 uneven problems at once)
+       python bench.py --workload layered  (2400x800 layered model, 128 sources as one block)
 """
 import argparse
 import json
@@ -122,6 +123,16 @@ def roofline(prof):
             "kernel": "k_nd_fwd + k_nd_bwd (batched ND local solves, one M^-1 application)",
             "bytes_per_launch": round(byts / nl, 1), "ms_per_launch": round(ms / nl, 5),
             "launches_per_precond": nl, "fwd": prof}
+
+
+def load_traffic(workload):
+    """HBM bytes per solve launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (profiles/traffic_<workload>.json), if present."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def cpu_baseline(cfg, iterations, seconds=20.0):
@@ -255,29 +266,37 @@ def serve_bench(args):
     plain = dict(solves_per_s=pn / pt, s_per_request=plain_req_s, requests=len(sample), solves=pn,
                  worst_true_residual=pres)
     log("plain", plain)
+    # one server for both phases: the offered load of 0.8x the plain capacity
+    # (cold cache: problems load on first use), then the whole stream at once
     out = {}
-    for label, scale in (("load0.8", plain_req_s / 0.8), ("saturated", 0.0)):
-        srv = L.Server(budget_bytes=args.serve_budget, slots=args.serve_slots, restart=args.serve_restart,
-                       tol=1e-6, max_iter=200, precision=L.LSW_PREC_FP32, workers=args.serve_workers)
-        pids = {key: srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)) for key, p in probs.items()}
-        st = stream if label == "saturated" else stream[:args.serve_load_requests]
+    t0 = time.perf_counter()
+    srv = L.Server(budget_bytes=args.serve_budget, slots=args.serve_slots, restart=args.serve_restart,
+                   tol=1e-6, max_iter=200, precision=L.LSW_PREC_FP32, workers=args.serve_workers)
+    pids = {key: srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)) for key, p in probs.items()}
+    log(f"serve: {len(pids)} problems registered in {time.perf_counter() - t0:.1f} s")
+    prev = srv.stats()
+    for label, scale, st in (("load0.8", plain_req_s / 0.8, stream[:args.serve_load_requests]),
+                             ("saturated", 0.0, stream[args.serve_load_requests:])):
         res, wall = run_stream(L, srv, pids, st, t_scale=scale)
         stats = srv.stats()
-        srv.close()
         lat = [r["latency_s"] for r in res]
         bad = [r for r in res if r["status"] != 0]
         nsol = sum(r["nsrc"] for r in res)
         out[label] = dict(requests=len(res), solves=nsol, wall_s=wall, solves_per_s=nsol / wall,
                           p50_s=pct(lat, 50), p99_s=pct(lat, 99), failed=len(bad),
                           worst_true_residual=max(r["max_true_residual"] for r in res),
-                          mean_interarrival_s=scale, **{k: stats[k] for k in ("loads", "evictions", "steps")},
-                          peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
+                          mean_interarrival_s=scale,
+                          **{k: stats[k] - prev[k] for k in ("loads", "evictions", "steps", "merged_steps")},
+                          max_active=stats["max_active"], peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
+        prev = stats
         log(label, out[label])
-        torch.cuda.empty_cache()
+    srv.close()
+    torch.cuda.empty_cache()
     line = {
         "metric": METRIC, "value": round(out["saturated"]["solves_per_s"], 4), "unit": "solves/s",
-        "n_gpus": 1, "steps": len(stream), "warmup": 0,
-        "ms_per_step": round(1e3 * out["saturated"]["wall_s"] / len(stream), 3), "higher_is_better": True,
+        "n_gpus": 1, "steps": out["saturated"]["requests"], "warmup": 0,
+        "ms_per_step": round(1e3 * out["saturated"]["wall_s"] / max(1, out["saturated"]["requests"]), 3),
+        "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "c64 factors/sweeps + c128 Krylov", "data": "synthetic",
         "config": {"workload": f"serve_{M}models_x{F}freq_{len(stream)}req", "budget_gb": args.serve_budget / 1e9,
                    "slots": args.serve_slots, "restart": args.serve_restart, "workers": args.serve_workers},
@@ -343,6 +362,74 @@ def pack_bench(args):
     print(json.dumps(line))
 
 
+def layered_bench(args):
+    """The layered 3x1 workload (2400 x 800 nodes, 12 x 4 cells, general
+    geometry, seed 0) with 128 sources 10 cells below the top: the sources as
+    one block through the slot engine (FP32 factors, FP64 GMRES, `slots`
+    right-hand sides in flight) against the plain FP64 path one source at a
+    time (a sample of sources, timed per solve).  Factorisation is reported
+    separately for both."""
+    import numpy as np
+    import torch
+    import lsweeps_amd as L
+    from lsweeps_amd.workloads import config_layered_3x1
+    cfg = config_layered_3x1(seed=0, nsrc=args.layered_sources)
+    log(f"layered: grid {cfg['grid']}, board {cfg['ncells']}, omega {cfg['omega']:.2f}, {len(cfg['sources'])} sources")
+    # plain FP64: one DD, one source at a time
+    t0 = time.perf_counter()
+    dd = L.DDCheckerboard(_problem_params(L, cfg, L.LSW_PREC_FP64))
+    dd.factorize()
+    dd.sync()
+    t_fact64 = time.perf_counter() - t0
+    b = torch.zeros(dd.N, dtype=torch.complex128, device="cuda:0")
+    x = torch.zeros_like(b)
+    sample = cfg["sources"][:: max(1, len(cfg["sources"]) // args.plain_sample)][: args.plain_sample]
+    ts, worst, its = [], 0.0, []
+    for k, src in enumerate(sample):
+        dd.assemble_rhs_dev([src], b.data_ptr())
+        dd.sync()
+        t1 = time.perf_counter()
+        info = dd.gmres_dev(b.data_ptr(), x.data_ptr(), mode=1, tol=1e-6, max_iter=200, restart=30)
+        dd.sync()
+        if k > 0:  # the first solve builds the plans
+            ts.append(time.perf_counter() - t1)
+        worst = max(worst, info["true_residual"])
+        its.append(info["iterations"])
+    del dd, b, x
+    torch.cuda.empty_cache()
+    plain = dict(solves_per_s=len(ts) / sum(ts), s_per_solve=float(np.mean(ts)), factorize_s=t_fact64,
+                 sample=len(sample), iterations=its, worst_true_residual=worst)
+    log("plain", plain)
+    srv = L.Server(budget_bytes=args.serve_budget, slots=args.serve_slots, restart=args.serve_restart, tol=1e-6,
+                   max_iter=300, precision=L.LSW_PREC_FP32, workers=1)
+    pid = srv.add_problem(_problem_params(L, cfg, L.LSW_PREC_FP32))
+    t0 = time.perf_counter()
+    w = srv.wait(srv.submit(pid, cfg["sources"][:1], want_solution=False))  # load + factorise
+    t_load = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = srv.wait(srv.submit(pid, cfg["sources"], want_solution=False))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    mixed = dict(solves_per_s=len(cfg["sources"]) / wall, wall_s=wall, load_s=t_load, status=r["status"],
+                 max_iterations=r["max_iterations"], worst_true_residual=r["max_true_residual"],
+                 steps=st["steps"], max_active=st["max_active"], warmup_status=w["status"])
+    log("mixed block", mixed)
+    n_dofs = int(np.prod(cfg["grid"]))
+    line = {
+        "metric": METRIC, "value": round(mixed["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(1e3 * wall, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 factors/sweeps + c128 Krylov", "data": "synthetic",
+        "config": {"workload": cfg["name"], "grid": list(cfg["grid"]), "board": list(cfg["ncells"]),
+                   "sources": len(cfg["sources"]), "slots": args.serve_slots, "restart": args.serve_restart,
+                   "geometry": "general"},
+        "plain": plain, "mixed_block": mixed,
+        "speedup_vs_plain": round(mixed["solves_per_s"] / plain["solves_per_s"], 2),
+        "dofs": n_dofs,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,14 +439,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
     ap.add_argument("--serve-models", type=int, default=8)
-    ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--serve-load-requests", type=int, default=12)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-load-requests", type=int, default=6)
     ap.add_argument("--serve-budget", type=float, default=250e9)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-restart", type=int, default=8)
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--plain-sample", type=int, default=4)
     ap.add_argument("--pack-count", type=int, default=48)
+    ap.add_argument("--layered-sources", type=int, default=128)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with one 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -370,6 +460,9 @@ def main():
         return
     if args.workload == "pack48":
         pack_bench(args)
+        return
+    if args.workload == "layered":
+        layered_bench(args)
         return
     import numpy as np
     import torch
@@ -442,7 +535,7 @@ def main():
         "cpu_baseline": cpu,
     }
     for k, v in out.items():
-        line[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+        line[k] = {kk: (float(f"{vv:.6g}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
     if "fp64" in prof:
         line["roofline_fp64"] = roofline(prof["fp64"])
     if "plain" in out:
@@ -458,7 +551,17 @@ def main():
                                                   np.linalg.norm(xs["plain"]))
     line["dofs"] = n_dofs
     line["dofs_per_s"] = round(n_dofs * head["solves_per_s"], 1)
+    traffic = load_traffic(cfg["name"])
+    if traffic:
+        line["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        line["roofline"]["traffic_source"] = traffic["source"]
     print(json.dumps(line))
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":
