@@ -290,6 +290,8 @@ def serve_bench(args):
                           max_active=stats["max_active"], peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
         prev = stats
         log(label, out[label])
+        for r in bad[:5]:
+            log(f"  failed request (problem {r['problem']}): status {r['status']} {r['message']}")
     srv.close()
     torch.cuda.empty_cache()
     line = {
@@ -348,6 +350,9 @@ def pack_bench(args):
                   **{k: stats[k] for k in ("loads", "evictions", "steps")},
                   peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
     log("packed", packed)
+    for r in res:
+        if r["status"] != 0:
+            log(f"  failed request (problem {r['problem']}): status {r['status']} {r['message']}")
     line = {
         "metric": METRIC, "value": round(packed["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
         "steps": len(probs), "warmup": 0, "ms_per_step": round(1e3 * wall / len(probs), 3),
@@ -441,7 +446,7 @@ def main():
     ap.add_argument("--serve-models", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-load-requests", type=int, default=6)
-    ap.add_argument("--serve-budget", type=float, default=250e9)
+    ap.add_argument("--serve-budget", type=float, default=240e9)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-restart", type=int, default=8)
     ap.add_argument("--serve-workers", type=int, default=4)

@@ -194,15 +194,15 @@ int lsw_dd_assemble_rhs(lsw_dd* dd, int nsrc, const double* src_xy, double* b) {
     for (int k = 0; k < nsrc; ++k) s[k] = {src_xy[2 * k], src_xy[2 * k + 1]};
     const long long n = dd->dd->N();
     double2* d = nullptr;
-    LSW_HIP(hipMalloc(&d, n * sizeof(double2)));
+    LSW_HIP(devMalloc(&d, n * sizeof(double2)));
     try {
       dd->dd->assembleRHSDev(s, d, dd->dd->stream());
       LSW_HIP(hipMemcpy(b, d, n * sizeof(double2), hipMemcpyDeviceToHost));
     } catch (...) {
-      (void)hipFree(d);
+      (void)devFree(d);
       throw;
     }
-    (void)hipFree(d);
+    (void)devFree(d);
   });
 }
 
@@ -219,8 +219,8 @@ namespace {
 void hostApply(lsw_dd* dd, const double* x, double* y, bool precond) {
   const long long n = dd->dd->N();
   double2 *dx = nullptr, *dy = nullptr;
-  LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
-  LSW_HIP(hipMalloc(&dy, n * sizeof(double2)));
+  LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
+  LSW_HIP(devMalloc(&dy, n * sizeof(double2)));
   try {
     LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
     if (precond)
@@ -230,12 +230,12 @@ void hostApply(lsw_dd* dd, const double* x, double* y, bool precond) {
     LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
     LSW_HIP(hipMemcpy(y, dy, n * sizeof(double2), hipMemcpyDeviceToHost));
   } catch (...) {
-    (void)hipFree(dx);
-    (void)hipFree(dy);
+    (void)devFree(dx);
+    (void)devFree(dy);
     throw;
   }
-  (void)hipFree(dx);
-  (void)hipFree(dy);
+  (void)devFree(dx);
+  (void)devFree(dy);
 }
 }  // namespace
 
@@ -258,20 +258,20 @@ int lsw_dd_apply_precond_block(lsw_dd* dd, int nrhs, const double* x, double* y)
     LSW_REQUIRE(dd && x && y && nrhs > 0, LSW_ERR_ARG, "null pointer or bad nrhs");
     const long long n = dd->dd->N() * nrhs;
     double2 *dx = nullptr, *dy = nullptr;
-    LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dy, n * sizeof(double2)));
+    LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
+    LSW_HIP(devMalloc(&dy, n * sizeof(double2)));
     try {
       LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
       dd->dd->applyPrecondDev(dx, dy, dd->dd->stream(), nrhs, false);
       LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
       LSW_HIP(hipMemcpy(y, dy, n * sizeof(double2), hipMemcpyDeviceToHost));
     } catch (...) {
-      (void)hipFree(dx);
-      (void)hipFree(dy);
+      (void)devFree(dx);
+      (void)devFree(dy);
       throw;
     }
-    (void)hipFree(dx);
-    (void)hipFree(dy);
+    (void)devFree(dx);
+    (void)devFree(dy);
   });
 }
 
@@ -362,8 +362,8 @@ int lsw_gmres_solve(lsw_dd* dd, const lsw_gmres_params* p, const double* b, doub
     configure(dd, p);
     const long long n = dd->dd->N();
     double2 *db = nullptr, *dx = nullptr;
-    LSW_HIP(hipMalloc(&db, n * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+    LSW_HIP(devMalloc(&db, n * sizeof(double2)));
+    LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
     GmresInfo info;
     try {
       LSW_HIP(hipMemcpy(db, b, n * sizeof(double2), hipMemcpyHostToDevice));
@@ -371,12 +371,12 @@ int lsw_gmres_solve(lsw_dd* dd, const lsw_gmres_params* p, const double* b, doub
       LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
       LSW_HIP(hipMemcpy(x, dx, n * sizeof(double2), hipMemcpyDeviceToHost));
     } catch (...) {
-      (void)hipFree(db);
-      (void)hipFree(dx);
+      (void)devFree(db);
+      (void)devFree(dx);
       throw;
     }
-    (void)hipFree(db);
-    (void)hipFree(dx);
+    (void)devFree(db);
+    (void)devFree(dx);
     fill(info, res);
     if (log)
       for (int k = 0; k < log_cap && k < (int)info.log.size(); ++k) log[k] = info.log[k];
@@ -413,8 +413,8 @@ int lsw_gmres_solve_block(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const
     configure(dd, p);
     const long long n = dd->dd->N() * nrhs;
     double2 *db = nullptr, *dx = nullptr;
-    LSW_HIP(hipMalloc(&db, n * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+    LSW_HIP(devMalloc(&db, n * sizeof(double2)));
+    LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
     std::vector<GmresInfo> infos;
     try {
       LSW_HIP(hipMemcpy(db, B, n * sizeof(double2), hipMemcpyHostToDevice));
@@ -422,12 +422,12 @@ int lsw_gmres_solve_block(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const
       LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
       LSW_HIP(hipMemcpy(X, dx, n * sizeof(double2), hipMemcpyDeviceToHost));
     } catch (...) {
-      (void)hipFree(db);
-      (void)hipFree(dx);
+      (void)devFree(db);
+      (void)devFree(dx);
       throw;
     }
-    (void)hipFree(db);
-    (void)hipFree(dx);
+    (void)devFree(db);
+    (void)devFree(dx);
     if (res)
       for (int r = 0; r < nrhs; ++r) fill(infos[r], res + r);
   });

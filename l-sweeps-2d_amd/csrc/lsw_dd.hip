@@ -269,15 +269,15 @@ DDCheckerboard::~DDCheckerboard() {
   for (auto& kv : mGraphs) (void)hipGraphExecDestroy(kv.second);
   mGraphs.clear();
   for (auto& st : mSteps) {
-    if (st.dSlots) (void)hipFree(st.dSlots);
-    if (st.dGroups) (void)hipFree(st.dGroups);
-    if (st.dCopies) (void)hipFree(st.dCopies);
+    if (st.dSlots) (void)devFree(st.dSlots);
+    if (st.dGroups) (void)devFree(st.dGroups);
+    if (st.dCopies) (void)devFree(st.dCopies);
   }
-  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)hipFree(p);
-  if (dY) (void)hipFree(dY);
-  if (dGlobalCoef) (void)hipFree(dGlobalCoef);
-  if (dColTrace) (void)hipFree(dColTrace);
-  if (dRowTrace) (void)hipFree(dRowTrace);
+  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)devFree(p);
+  if (dY) (void)devFree(dY);
+  if (dGlobalCoef) (void)devFree(dGlobalCoef);
+  if (dColTrace) (void)devFree(dColTrace);
+  if (dRowTrace) (void)devFree(dRowTrace);
   mFactors.reset();
   if (mStream) (void)hipStreamDestroy(mStream);
 }
@@ -512,8 +512,8 @@ void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalL
 
 void DDCheckerboard::initDevice() {
   if (!mStream) LSW_HIP(hipStreamCreate(&mStream));
-  LSW_HIP(hipMalloc(&dColTrace, mGx));
-  LSW_HIP(hipMalloc(&dRowTrace, mGy));
+  LSW_HIP(devMalloc(&dColTrace, mGx));
+  LSW_HIP(devMalloc(&dRowTrace, mGy));
   LSW_HIP(hipMemcpy(dColTrace, mColT.data(), mGx, hipMemcpyHostToDevice));
   LSW_HIP(hipMemcpy(dRowTrace, mRowT.data(), mGy, hipMemcpyHostToDevice));
 
@@ -627,10 +627,20 @@ void DDCheckerboard::buildSchedule() {
       }
     steps.push_back(st);
   }
-  // ---- stage 2
+  // Global steps: stage 1 at step 0, stage-2 step t at step t, and stage-3
+  // anti-diagonal t at step t + 2.  An L-sweep on cell (I, J) reads the
+  // stage-2 traces of that cell from the sweeps that pass it at stage-2 steps
+  // 1+a and 1+b (a + b = t), so they are final after step t + 1; the traces
+  // are copied into the L-sweep's context at the end of step t + 1 (the
+  // reference copies them after the whole of stage 2; they no longer change
+  // after their step, so the values are the same).  This runs the two stages
+  // side by side: 2n-1 steps instead of 3n-3 on an n x n board.
   const int nStage2 = std::max(mNR, mNC) - 1;
+  const int nStage3 = (mNR >= 2 && mNC >= 2) ? (mNR - 2) + (mNC - 2) + 1 : 0;
+  steps.resize((size_t)1 + std::max(nStage2, nStage3 > 0 ? nStage3 + 1 : 0));
+  // ---- stage 2
   for (int t = 1; t <= nStage2; ++t) {
-    HostStep st;
+    HostStep& st = steps[t];
     for (int d = 0; d < 4; ++d) {
       const int slr = d & 1, sud = d >> 1;
       // right sweep: column t (src/DDCheckerboard.cpp:1168-1350)
@@ -696,31 +706,14 @@ void DDCheckerboard::buildSchedule() {
           st.slots.push_back(s);
         }
     }
-    if (t == nStage2) {
-      // copies for stage 3 (the L-sweeps take their traces by value,
-      // include/DDCheckerboard.h:209-239)
-      for (int d = 0; d < 4; ++d)
-        for (int c = 0; c < nc; ++c)
-          for (int k = 0; k < 4; ++k) {
-            st.copies.push_back({tOff(d, c, kBL, 4 + k), tOff(d, c, kR, 4 + k)});
-            st.copies.push_back({tOff(d, c, kTL, 4 + k), tOff(d, c, kR, 4 + k)});
-            st.copies.push_back({tOff(d, c, kTR, 4 + k), tOff(d, c, kL, 4 + k)});
-            st.copies.push_back({tOff(d, c, kBR, 4 + k), tOff(d, c, kL, 4 + k)});
-            st.copies.push_back({tOff(d, c, kBL, k), tOff(d, c, kU, k)});
-            st.copies.push_back({tOff(d, c, kBR, k), tOff(d, c, kU, k)});
-            st.copies.push_back({tOff(d, c, kTR, k), tOff(d, c, kD, k)});
-            st.copies.push_back({tOff(d, c, kTL, k), tOff(d, c, kD, k)});
-          }
-    }
-    steps.push_back(st);
   }
   if (nStage2 == 0) {
     // 1 x 1 board: no sweeps at all
   }
   // ---- stage 3
-  const int nStage3 = (mNR >= 2 && mNC >= 2) ? (mNR - 2) + (mNC - 2) + 1 : 0;
   for (int t = 0; t < nStage3; ++t) {
-    HostStep st;
+    HostStep& st = steps[t + 2];
+    auto& pre = steps[t + 1].copies;
     for (int d = 0; d < 4; ++d) {
       const int slr = d & 1, sud = d >> 1;
       for (int kind = 5; kind <= 8; ++kind) {
@@ -784,10 +777,18 @@ void DDCheckerboard::buildSchedule() {
             addOut(s, true, vu[2], vu[2], vu[3], tOff(d, c, ctx, 4 + 1), true);
           }
           st.slots.push_back(s);
+          // the L-sweep takes its traces by value (include/DDCheckerboard.h:209-239):
+          // UD traces from the horizontal sweep that reaches it (R from the
+          // left, L from the right), LR traces from the vertical one (U from
+          // below, D from above)
+          const int horiz = fromLeft ? kR : kL, vert = fromBottom ? kU : kD;
+          for (int k = 0; k < 4; ++k) {
+            pre.push_back({tOff(d, c, ctx, 4 + k), tOff(d, c, horiz, 4 + k)});
+            pre.push_back({tOff(d, c, ctx, k), tOff(d, c, vert, k)});
+          }
         }
       }
     }
-    steps.push_back(st);
   }
 
   // ---- the final solves: one per (d, cell), right-hand side = the windowed
@@ -860,7 +861,7 @@ void DDCheckerboard::buildSchedule() {
     st.nGroups = (int)groups.size();
     st.nCopies = (int)hst.copies.size();
     LSW_REQUIRE(st.nSlots <= 65535 && st.nCopies <= 65535 * 8, LSW_ERR_CAPACITY, "board too large for one launch");
-    LSW_HIP(hipMalloc(&st.dSlots, std::max<size_t>(1, hst.slots.size()) * sizeof(SlotDev)));
+    LSW_HIP(devMalloc(&st.dSlots, std::max<size_t>(1, hst.slots.size()) * sizeof(SlotDev)));
     LSW_HIP(hipMemcpy(st.dSlots, hst.slots.data(), hst.slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice));
     {
       const SolvePlan& pl = planFor(st, 1);
@@ -868,10 +869,10 @@ void DDCheckerboard::buildSchedule() {
       st.nTasks = pl.ntasks;
       st.maxNrhs = pl.maxNrhs;
     }
-    LSW_HIP(hipMalloc(&st.dGroups, std::max<size_t>(1, groups.size()) * sizeof(GroupDev)));
+    LSW_HIP(devMalloc(&st.dGroups, std::max<size_t>(1, groups.size()) * sizeof(GroupDev)));
     LSW_HIP(hipMemcpy(st.dGroups, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice));
     if (!hst.copies.empty()) {
-      LSW_HIP(hipMalloc(&st.dCopies, hst.copies.size() * sizeof(CopyOp)));
+      LSW_HIP(devMalloc(&st.dCopies, hst.copies.size() * sizeof(CopyOp)));
       LSW_HIP(hipMemcpy(st.dCopies, hst.copies.data(), hst.copies.size() * sizeof(CopyOp), hipMemcpyHostToDevice));
     }
     mSteps.push_back(std::move(st));
@@ -927,18 +928,18 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
 void DDCheckerboard::ensureBuffers(int nrhs) const {
   if (nrhs <= mBufRhs) return;
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
-  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)hipFree(p);
-  if (dY) (void)hipFree(dY);
+  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)devFree(p);
+  if (dY) (void)devFree(dY);
   dX = dR = dG = dT = dF = nullptr;
   dY = nullptr;
   mBufRhs = 0;
   const size_t slots = (size_t)mMaxSlots * nrhs;
-  LSW_HIP(hipMalloc(&dF, (size_t)4 * mCells.size() * nrhs * mXs * es));
-  LSW_HIP(hipMalloc(&dX, slots * mXs * es));
-  LSW_HIP(hipMalloc(&dR, slots * mXs * es));
-  LSW_HIP(hipMalloc(&dG, slots * mGs * es));
-  LSW_HIP(hipMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
-  LSW_HIP(hipMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
+  LSW_HIP(devMalloc(&dF, (size_t)4 * mCells.size() * nrhs * mXs * es));
+  LSW_HIP(devMalloc(&dX, slots * mXs * es));
+  LSW_HIP(devMalloc(&dR, slots * mXs * es));
+  LSW_HIP(devMalloc(&dG, slots * mGs * es));
+  LSW_HIP(devMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
+  LSW_HIP(devMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
   mBufRhs = nrhs;
 }
 
@@ -1002,15 +1003,27 @@ void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady
   const double es = precision == LSW_FP32 ? 8.0 : 16.0;
   const double n = (double)N();
   const double r = std::max(1, nrhs);
-  // stencils (local + global), squared slowness, sweep buffers (X, R, G and
-  // traces per block column, 4 accumulators of global vectors)
+  const double nc = (double)mCells.size();
+  double maxDofs = 0, maxG = 0, tmax = 0;
+  for (const auto& c : mCells) {
+    maxDofs = std::max(maxDofs, (double)c.nx * c.ny);
+    maxG = std::max(maxG, (double)trees[{c.nx, c.ny}]->gSize);
+    tmax = std::max(tmax, (double)std::max(c.nx, c.ny));
+  }
+  // stencils (local + global), squared slowness
   const double fixed = 80.0 * local + 8.0 * local + 80.0 * n;
-  const double sweep = r * (3.0 * es * local + 64.0 * n);
-  const double s = entries * es + fixed + sweep;
+  // sweep buffers per right-hand side: X and R (4 x cells slots), G, the
+  // summed sources F, traces, and 4 FP64 accumulators of global vectors
+  const double slots = 4.0 * nc;
+  const double perRhs = es * (2.0 * slots * maxDofs + slots * maxG + 4.0 * nc * maxDofs + 4.0 * nc * 72.0 * tmax) +
+                        64.0 * n;
+  // solve plans: about 32 bytes per local DOF per block width, a dozen widths
+  const double plans = 12.0 * 32.0 * local;
+  const double s = entries * es + fixed + r * perRhs + plans;
   double ws = 0;
   for (double w : wsPerHeight) ws = std::max(ws, w);
   ws = std::min(ws, (double)(512LL << 20)) * 16.0;
-  const double f = fixed + entries * 16.0 + schur * 16.0 + ws + (precision == LSW_FP32 ? entries * 8.0 : 0.0);
+  const double f = fixed + perRhs + entries * 16.0 + schur * 16.0 + ws + (precision == LSW_FP32 ? entries * 8.0 : 0.0);
   if (steady) *steady = s;
   if (peak) *peak = std::max(s, f);
 }
@@ -1158,7 +1171,7 @@ void DDCheckerboard::assembleRHSDev(const std::vector<DoubleVector>& srcs, doubl
     xy.push_back(p[1]);
   }
   double* dsrc = nullptr;
-  LSW_HIP(hipMalloc(&dsrc, std::max<size_t>(1, xy.size()) * sizeof(double)));
+  LSW_HIP(devMalloc(&dsrc, std::max<size_t>(1, xy.size()) * sizeof(double)));
   LSW_HIP(hipMemcpyAsync(dsrc, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, s));
   const long long n = N();
   hipLaunchKernelGGL(k_point_sources, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0,
@@ -1166,7 +1179,7 @@ void DDCheckerboard::assembleRHSDev(const std::vector<DoubleVector>& srcs, doubl
                      mGlobal.asmb.hy(), mGx, mGy, b);
   LSW_HIP(hipGetLastError());
   LSW_HIP(hipStreamSynchronize(s));
-  (void)hipFree(dsrc);
+  (void)devFree(dsrc);
 }
 
 void DDCheckerboard::ownedGlobalIndices(int c, std::vector<long long>& idx) const {
@@ -1201,9 +1214,9 @@ void DDCheckerboard::scatterOwned(const std::vector<ComplexVector>& x, std::vect
 namespace {
 struct DevBuf {
   double2* p = nullptr;
-  explicit DevBuf(long long n) { LSW_HIP(hipMalloc(&p, std::max<long long>(n, 1) * sizeof(double2))); }
+  explicit DevBuf(long long n) { LSW_HIP(devMalloc(&p, std::max<long long>(n, 1) * sizeof(double2))); }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) (void)devFree(p);
   }
 };
 }  // namespace

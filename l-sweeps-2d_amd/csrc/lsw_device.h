@@ -17,6 +17,19 @@ namespace lsw {
       throw ::lsw::Error(LSW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+// Device memory of the library.  Every allocation is recorded with its size,
+// and each thread keeps the balance of what it allocated minus what it freed:
+// the server charges what a worker allocates while it serves a problem to
+// that problem.
+hipError_t devMallocBytes(void** p, size_t bytes);
+hipError_t devFree(void* p);
+long long threadDeviceBytes();
+long long totalDeviceBytes();
+template <typename T>
+inline hipError_t devMalloc(T** p, size_t bytes) {
+  return devMallocBytes(reinterpret_cast<void**>(p), bytes);
+}
+
 constexpr int kMaxHeight = 40;
 
 // One local problem (a checkerboard cell, or the single matrix of a

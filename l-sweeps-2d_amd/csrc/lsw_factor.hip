@@ -180,7 +180,7 @@ template <typename T>
 T* devUpload(const std::vector<T>& h, std::vector<void*>& allocs) {
   if (h.empty()) return nullptr;
   T* d = nullptr;
-  LSW_HIP(hipMalloc(&d, h.size() * sizeof(T)));
+  LSW_HIP(devMalloc(&d, h.size() * sizeof(T)));
   LSW_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   allocs.push_back(d);
   return d;
@@ -188,13 +188,13 @@ T* devUpload(const std::vector<T>& h, std::vector<void*>& allocs) {
 }  // namespace
 
 FactorSet::~FactorSet() {
-  for (void* p : mShapeAllocs) (void)hipFree(p);
-  if (dCells) (void)hipFree(dCells);
-  if (dShapes) (void)hipFree(dShapes);
-  if (dCoef) (void)hipFree(dCoef);
-  if (dM) (void)hipFree(dM);
-  if (dFactor) (void)hipFree(dFactor);
-  if (dFactor32) (void)hipFree(dFactor32);
+  for (void* p : mShapeAllocs) (void)devFree(p);
+  if (dCells) (void)devFree(dCells);
+  if (dShapes) (void)devFree(dShapes);
+  if (dCoef) (void)devFree(dCoef);
+  if (dM) (void)devFree(dM);
+  if (dFactor) (void)devFree(dFactor);
+  if (dFactor32) (void)devFree(dFactor32);
 }
 
 int FactorSet::addCell(const CellDev& c, const std::vector<double>& m) {
@@ -319,24 +319,24 @@ void FactorSet::setup(hipStream_t s) {
     dPoolB = devUpload(pb, mShapeAllocs);
     dPoolMap = devUpload(pm, mShapeAllocs);
   }
-  LSW_HIP(hipMalloc(&dShapes, shapes.size() * sizeof(ShapeDev)));
+  LSW_HIP(devMalloc(&dShapes, shapes.size() * sizeof(ShapeDev)));
   LSW_HIP(hipMemcpy(dShapes, shapes.data(), shapes.size() * sizeof(ShapeDev), hipMemcpyHostToDevice));
-  LSW_HIP(hipMalloc(&dCells, mCells.size() * sizeof(CellDev)));
+  LSW_HIP(devMalloc(&dCells, mCells.size() * sizeof(CellDev)));
   LSW_HIP(hipMemcpy(dCells, mCells.data(), mCells.size() * sizeof(CellDev), hipMemcpyHostToDevice));
   // stencils
-  LSW_HIP(hipMalloc(&dCoef, co * sizeof(double2)));
+  LSW_HIP(devMalloc(&dCoef, co * sizeof(double2)));
   for (size_t i = 0; i < mCells.size(); ++i)
     if (!mCoefHost[i].empty())
       LSW_HIP(hipMemcpy(dCoef + mCells[i].coefOff, mCoefHost[i].data(),
                         mCoefHost[i].size() * sizeof(double2), hipMemcpyHostToDevice));
   if (mo > 0) {
-    LSW_HIP(hipMalloc(&dM, mo * sizeof(double)));
+    LSW_HIP(devMalloc(&dM, mo * sizeof(double)));
     for (size_t i = 0; i < mCells.size(); ++i)
       if (!mM[i].empty())
         LSW_HIP(hipMemcpy(dM + mCells[i].mOff, mM[i].data(), mM[i].size() * sizeof(double),
                           hipMemcpyHostToDevice));
     int* dBad = nullptr;
-    LSW_HIP(hipMalloc(&dBad, sizeof(int)));
+    LSW_HIP(devMalloc(&dBad, sizeof(int)));
     LSW_HIP(hipMemsetAsync(dBad, 0, sizeof(int), s));
     dim3 grid((unsigned)std::min<long long>((mMaxDofs + 255) / 256, 4096), (unsigned)mCells.size());
     hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, s, dCells, dM, dCoef, dBad);
@@ -344,7 +344,7 @@ void FactorSet::setup(hipStream_t s) {
     int bad = 0;
     LSW_HIP(hipMemcpyAsync(&bad, dBad, sizeof(int), hipMemcpyDeviceToHost, s));
     LSW_HIP(hipStreamSynchronize(s));
-    (void)hipFree(dBad);
+    (void)devFree(dBad);
     LSW_REQUIRE(bad == 0, LSW_ERR_ARG, "PML: Evaluation point out of range!");
   }
 }
@@ -352,7 +352,7 @@ void FactorSet::setup(hipStream_t s) {
 void FactorSet::factorize(int precision, hipStream_t s) {
   LSW_REQUIRE(dCells != nullptr, LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(precision == LSW_FP64 || precision == LSW_FP32, LSW_ERR_ARG, "unknown precision");
-  if (!dFactor) LSW_HIP(hipMalloc(&dFactor, std::max<long long>(mFactorEntries, 1) * sizeof(double2)));
+  if (!dFactor) LSW_HIP(devMalloc(&dFactor, std::max<long long>(mFactorEntries, 1) * sizeof(double2)));
   // Schur workspace for all cells
   std::vector<long long> schurOff(mCells.size());
   long long so = 0;
@@ -363,9 +363,9 @@ void FactorSet::factorize(int precision, hipStream_t s) {
   }
   LSW_HIP(hipMemcpy(dCells, mCells.data(), mCells.size() * sizeof(CellDev), hipMemcpyHostToDevice));
   double2* dSchur = nullptr;
-  LSW_HIP(hipMalloc(&dSchur, std::max<long long>(so, 1) * sizeof(double2)));
+  LSW_HIP(devMalloc(&dSchur, std::max<long long>(so, 1) * sizeof(double2)));
   int* dInfo = nullptr;
-  LSW_HIP(hipMalloc(&dInfo, sizeof(int)));
+  LSW_HIP(devMalloc(&dInfo, sizeof(int)));
   LSW_HIP(hipMemsetAsync(dInfo, 0, sizeof(int), s));
   // front batches per height, bounded by a workspace budget
   const long long budget = 512LL << 20;  // complex entries (8 GiB)
@@ -401,7 +401,7 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     batchWs.push_back(0);
   }
   double2* dWs = nullptr;
-  LSW_HIP(hipMalloc(&dWs, std::max<long long>(maxWs, 1) * sizeof(double2)));
+  LSW_HIP(devMalloc(&dWs, std::max<long long>(maxWs, 1) * sizeof(double2)));
   FrontTask* dFt = nullptr;
   size_t ftCap = 0;
   for (size_t bi = 0; bi < batches.size(); ++bi) {
@@ -410,10 +410,10 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     if (bt.size() > ftCap) {
       if (dFt) {
         LSW_HIP(hipStreamSynchronize(s));
-        (void)hipFree(dFt);
+        (void)devFree(dFt);
       }
       ftCap = bt.size();
-      LSW_HIP(hipMalloc(&dFt, ftCap * sizeof(FrontTask)));
+      LSW_HIP(devMalloc(&dFt, ftCap * sizeof(FrontTask)));
     }
     LSW_HIP(hipMemcpyAsync(dFt, bt.data(), bt.size() * sizeof(FrontTask), hipMemcpyHostToDevice, s));
     const size_t shm = (size_t)mMaxFront * sizeof(double2);
@@ -424,18 +424,18 @@ void FactorSet::factorize(int precision, hipStream_t s) {
   }
   LSW_HIP(hipMemcpyAsync(&mInfo, dInfo, sizeof(int), hipMemcpyDeviceToHost, s));
   LSW_HIP(hipStreamSynchronize(s));
-  if (dFt) (void)hipFree(dFt);
-  (void)hipFree(dWs);
-  (void)hipFree(dSchur);
-  (void)hipFree(dInfo);
+  if (dFt) (void)devFree(dFt);
+  (void)devFree(dWs);
+  (void)devFree(dSchur);
+  (void)devFree(dInfo);
   LSW_REQUIRE(mInfo == 0, LSW_ERR_NUMERIC, "factorization failed: zero or non-finite pivot");
   mPrecision = precision;
   if (precision == LSW_FP32) {
-    if (!dFactor32) LSW_HIP(hipMalloc(&dFactor32, std::max<long long>(mFactorEntries, 1) * sizeof(float2)));
+    if (!dFactor32) LSW_HIP(devMalloc(&dFactor32, std::max<long long>(mFactorEntries, 1) * sizeof(float2)));
     hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, dFactor, dFactor32, mFactorEntries);
     LSW_HIP(hipGetLastError());
     LSW_HIP(hipStreamSynchronize(s));
-    (void)hipFree(dFactor);
+    (void)devFree(dFactor);
     dFactor = nullptr;
   }
   mFactorized = true;
@@ -455,10 +455,10 @@ double2* assembleStencilDev(const CellDev& c, const std::vector<double>& m, hipS
   double* dM = nullptr;
   double2* dCoef = nullptr;
   int* dBad = nullptr;
-  LSW_HIP(hipMalloc(&dC, sizeof(CellDev)));
-  LSW_HIP(hipMalloc(&dM, N * sizeof(double)));
-  LSW_HIP(hipMalloc(&dCoef, 5 * N * sizeof(double2)));
-  LSW_HIP(hipMalloc(&dBad, sizeof(int)));
+  LSW_HIP(devMalloc(&dC, sizeof(CellDev)));
+  LSW_HIP(devMalloc(&dM, N * sizeof(double)));
+  LSW_HIP(devMalloc(&dCoef, 5 * N * sizeof(double2)));
+  LSW_HIP(devMalloc(&dBad, sizeof(int)));
   LSW_HIP(hipMemcpy(dC, &cd, sizeof(CellDev), hipMemcpyHostToDevice));
   LSW_HIP(hipMemcpy(dM, m.data(), N * sizeof(double), hipMemcpyHostToDevice));
   LSW_HIP(hipMemsetAsync(dBad, 0, sizeof(int), s));
@@ -468,11 +468,11 @@ double2* assembleStencilDev(const CellDev& c, const std::vector<double>& m, hipS
   int bad = 0;
   LSW_HIP(hipMemcpyAsync(&bad, dBad, sizeof(int), hipMemcpyDeviceToHost, s));
   LSW_HIP(hipStreamSynchronize(s));
-  (void)hipFree(dC);
-  (void)hipFree(dM);
-  (void)hipFree(dBad);
+  (void)devFree(dC);
+  (void)devFree(dM);
+  (void)devFree(dBad);
   if (bad) {
-    (void)hipFree(dCoef);
+    (void)devFree(dCoef);
     throw Error(LSW_ERR_ARG, "PML: Evaluation point out of range!");
   }
   return dCoef;

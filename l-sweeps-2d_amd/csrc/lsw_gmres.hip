@@ -110,7 +110,7 @@ void norm2sq(const double2* a, long long n, double2* part, double2* out, hipStre
 }  // namespace blas
 
 GlobalIterativeSolverGMRESHIP::~GlobalIterativeSolverGMRESHIP() {
-  for (double2* p : {dV, dZ, dT, dPart, dScal, dBV, dBZ, dBT, dBPart, dBH}) if (p) (void)hipFree(p);
+  for (double2* p : {dV, dZ, dT, dPart, dScal, dBV, dBZ, dBT, dBPart, dBH}) if (p) (void)devFree(p);
 }
 
 GlobalIterativeSolver* GlobalIterativeSolverGMRESHIP::copy() const {
@@ -127,13 +127,13 @@ GlobalIterativeSolver* GlobalIterativeSolverGMRESHIP::copy() const {
 void GlobalIterativeSolverGMRESHIP::ensure(long long n, int vecs) const {
   LSW_REQUIRE(vecs + 1 <= 128, LSW_ERR_ARG, "GMRES: MaxIter/restart must be at most 126");
   if (n == mN && vecs <= mVecs && (mMode != kFlexible || dZ != nullptr)) return;
-  for (double2* p : {dV, dZ, dT}) if (p) (void)hipFree(p);
+  for (double2* p : {dV, dZ, dT}) if (p) (void)devFree(p);
   dV = dZ = dT = nullptr;
-  LSW_HIP(hipMalloc(&dV, (size_t)vecs * n * sizeof(double2)));
-  if (mMode == kFlexible) LSW_HIP(hipMalloc(&dZ, (size_t)vecs * n * sizeof(double2)));
-  LSW_HIP(hipMalloc(&dT, (size_t)3 * n * sizeof(double2)));
-  if (!dPart) LSW_HIP(hipMalloc(&dPart, kRedBlocks * sizeof(double2)));
-  if (!dScal) LSW_HIP(hipMalloc(&dScal, 128 * sizeof(double2)));
+  LSW_HIP(devMalloc(&dV, (size_t)vecs * n * sizeof(double2)));
+  if (mMode == kFlexible) LSW_HIP(devMalloc(&dZ, (size_t)vecs * n * sizeof(double2)));
+  LSW_HIP(devMalloc(&dT, (size_t)3 * n * sizeof(double2)));
+  if (!dPart) LSW_HIP(devMalloc(&dPart, kRedBlocks * sizeof(double2)));
+  if (!dScal) LSW_HIP(devMalloc(&dScal, 128 * sizeof(double2)));
   mN = n;
   mVecs = vecs;
 }
@@ -370,20 +370,20 @@ void GlobalIterativeSolverGMRESHIP::solve(const std::vector<ComplexVector>& rhs,
   dd->scatterOwned(rhs, g);
   const long long n = dd->N();
   double2 *db = nullptr, *dx = nullptr;
-  LSW_HIP(hipMalloc(&db, n * sizeof(double2)));
-  LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+  LSW_HIP(devMalloc(&db, n * sizeof(double2)));
+  LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
   LSW_HIP(hipMemcpy(db, g.data(), n * sizeof(double2), hipMemcpyHostToDevice));
   try {
     solveDev(db, dx, nullptr, dd->stream());
     LSW_HIP(hipStreamSynchronize(dd->stream()));
     LSW_HIP(hipMemcpy(g.data(), dx, n * sizeof(double2), hipMemcpyDeviceToHost));
   } catch (...) {
-    (void)hipFree(db);
-    (void)hipFree(dx);
+    (void)devFree(db);
+    (void)devFree(dx);
     throw;
   }
-  (void)hipFree(db);
-  (void)hipFree(dx);
+  (void)devFree(db);
+  (void)devFree(dx);
   dd->gatherOwned(g, res);
 }
 

@@ -115,13 +115,13 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
   const long long n = dd->N();
   const int m = std::max(1, mRestart);
   if (n != mBN || nrhs > mBRhs || m > mBM) {
-    for (double2* p : {dBV, dBZ, dBT, dBPart, dBH}) if (p) (void)hipFree(p);
+    for (double2* p : {dBV, dBZ, dBT, dBPart, dBH}) if (p) (void)devFree(p);
     dBV = dBZ = dBT = dBPart = dBH = nullptr;
-    LSW_HIP(hipMalloc(&dBV, (size_t)(m + 1) * nrhs * n * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dBZ, (size_t)m * nrhs * n * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dBT, (size_t)2 * nrhs * n * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dBPart, (size_t)(m + 1) * nrhs * kBlocks * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dBH, (size_t)2 * (m + 2) * nrhs * sizeof(double2)));
+    LSW_HIP(devMalloc(&dBV, (size_t)(m + 1) * nrhs * n * sizeof(double2)));
+    LSW_HIP(devMalloc(&dBZ, (size_t)m * nrhs * n * sizeof(double2)));
+    LSW_HIP(devMalloc(&dBT, (size_t)2 * nrhs * n * sizeof(double2)));
+    LSW_HIP(devMalloc(&dBPart, (size_t)(m + 1) * nrhs * kBlocks * sizeof(double2)));
+    LSW_HIP(devMalloc(&dBH, (size_t)2 * (m + 2) * nrhs * sizeof(double2)));
     mBN = n;
     mBRhs = nrhs;
     mBM = m;

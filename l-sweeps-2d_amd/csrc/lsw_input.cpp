@@ -424,8 +424,8 @@ ProblemResult Problem::solve(bool verbose) {
   if (verbose) std::printf("Assemble RHS...\n");
   const long long n = mDD->N();
   double2 *db = nullptr, *dx = nullptr;
-  LSW_HIP(hipMalloc(&db, n * sizeof(double2)));
-  LSW_HIP(hipMalloc(&dx, n * sizeof(double2)));
+  LSW_HIP(devMalloc(&db, n * sizeof(double2)));
+  LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
   try {
     mDD->assembleRHSDev(mSpec.sources, db, mDD->stream());
     GlobalIterativeSolverGMRESHIP g;
@@ -448,12 +448,12 @@ ProblemResult Problem::solve(bool verbose) {
     mSolution.resize((size_t)n);
     LSW_HIP(hipMemcpy(mSolution.data(), dx, n * sizeof(double2), hipMemcpyDeviceToHost));
   } catch (...) {
-    (void)hipFree(db);
-    (void)hipFree(dx);
+    (void)devFree(db);
+    (void)devFree(dx);
     throw;
   }
-  (void)hipFree(db);
-  (void)hipFree(dx);
+  (void)devFree(db);
+  (void)devFree(dx);
   if (verbose) {
     std::printf("Time to factorize system: %g\n", r.factorizeSeconds);
     std::printf("Time to run the GMRES: %g\n", r.gmresSeconds);

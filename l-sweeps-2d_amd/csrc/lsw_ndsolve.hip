@@ -344,9 +344,9 @@ __global__ __launch_bounds__(256) void k_nd_sub(const SolveTaskDev* __restrict__
 
 // ------------------------------------------------------------------ plans
 SolvePlan::~SolvePlan() {
-  if (dTasks) (void)hipFree(dTasks);
-  if (dEntries) (void)hipFree(dEntries);
-  if (dSubs) (void)hipFree(dSubs);
+  if (dTasks) (void)devFree(dTasks);
+  if (dEntries) (void)devFree(dEntries);
+  if (dSubs) (void)devFree(dSubs);
 }
 
 namespace {
@@ -623,19 +623,19 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
                      "bytes %.0f\n", pass ? "bwd" : "fwd", nt, plan.maxNrhs, L.n, L.mode, L.rp, L.rounds, L.wpb,
                      L.waves, L.cols, L.bytes);
   }
-  if (plan.dTasks) (void)hipFree(plan.dTasks);
-  if (plan.dEntries) (void)hipFree(plan.dEntries);
-  if (plan.dSubs) (void)hipFree(plan.dSubs);
+  if (plan.dTasks) (void)devFree(plan.dTasks);
+  if (plan.dEntries) (void)devFree(plan.dEntries);
+  if (plan.dSubs) (void)devFree(plan.dSubs);
   plan.dTasks = nullptr;
   plan.dEntries = nullptr;
   plan.dSubs = nullptr;
   if (!allSubs.empty()) {
-    LSW_HIP(hipMalloc(&plan.dSubs, allSubs.size() * sizeof(SubDesc)));
+    LSW_HIP(devMalloc(&plan.dSubs, allSubs.size() * sizeof(SubDesc)));
     LSW_HIP(hipMemcpy(plan.dSubs, allSubs.data(), allSubs.size() * sizeof(SubDesc), hipMemcpyHostToDevice));
   }
-  LSW_HIP(hipMalloc(&plan.dTasks, tdev.size() * sizeof(SolveTaskDev)));
+  LSW_HIP(devMalloc(&plan.dTasks, tdev.size() * sizeof(SolveTaskDev)));
   LSW_HIP(hipMemcpy(plan.dTasks, tdev.data(), tdev.size() * sizeof(SolveTaskDev), hipMemcpyHostToDevice));
-  LSW_HIP(hipMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(SolveEntry)));
+  LSW_HIP(devMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(SolveEntry)));
   LSW_HIP(hipMemcpy(plan.dEntries, all.data(), all.size() * sizeof(SolveEntry), hipMemcpyHostToDevice));
 }
 

@@ -96,7 +96,7 @@ int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
   auto geo = makeCheckerboard(e->params, false);
   e->Gx = geo->Gx();
   e->Gy = geo->Gy();
-  geo->deviceBytesEstimate(mOpt.precision, mOpt.slots, &e->steadyBytes, &e->peakBytes);
+  geo->deviceBytesEstimate(mOpt.precision, planWidth(mOpt.slots), &e->steadyBytes, &e->peakBytes);
   e->steadyBytes += (double)GmresSlots::bytesFor(geo->N(), mOpt.slots, mOpt.restart);
   e->peakBytes = std::max(e->peakBytes, e->steadyBytes);
   LSW_REQUIRE(e->peakBytes <= mOpt.budgetBytes, LSW_ERR_CAPACITY,
@@ -170,6 +170,7 @@ void Server::evict(Entry& e) {
   e.dd.reset();
   mStats.reservedBytes -= e.reserved;
   e.reserved = 0;
+  e.actual = 0;
   mStats.evictions++;
 }
 
@@ -240,6 +241,7 @@ void Server::failEntry(Entry& e, int code, const std::string& msg) {
   e.dd.reset();
   mStats.reservedBytes -= e.reserved;
   e.reserved = 0;
+  e.actual = 0;
   mDoneCv.notify_all();
 }
 
@@ -268,6 +270,7 @@ void Server::workerLoop() {
       srcs.push_back({r.xy[2 * t.second.col], r.xy[2 * t.second.col + 1]});
     }
     lk.unlock();
+    const long long bytes0 = threadDeviceBytes();
     int err = LSW_OK;
     std::string msg;
     std::vector<std::pair<Column, GmresSlots::Done>> finished;
@@ -277,7 +280,7 @@ void Server::workerLoop() {
         auto dd = makeCheckerboard(e.params, true);
         dd->setGraphs(false);  // workers run concurrently: no stream capture
         dd->factorize();
-        dd->reserveBlock(mOpt.slots);
+        dd->reserveBlock(planWidth(mOpt.slots));
         e.slots = std::make_unique<GmresSlots>(dd.get(), mOpt.slots, mOpt.restart, mOpt.tol, mOpt.maxIter);
         e.dd = std::move(dd);
       }
@@ -323,14 +326,24 @@ void Server::workerLoop() {
       err = LSW_ERR_INTERNAL;
       msg = ex.what();
     }
+    const long long grown = threadDeviceBytes() - bytes0;
     lk.lock();
     e.busy = false;
     e.lastUse = ++mClock;
+    e.actual += (double)grown;
     if (load && err == LSW_OK) {
       mStats.loads++;
-      mStats.reservedBytes += e.steadyBytes - e.reserved;  // factorisation workspace is gone
-      e.reserved = e.steadyBytes;
+      // the factorisation workspace is gone: keep the steady estimate or
+      // what is really held, whichever is larger
+      const double keep = std::max(e.steadyBytes, e.actual);
+      mStats.reservedBytes += keep - e.reserved;
+      e.reserved = keep;
+    } else if (err == LSW_OK && e.actual > e.reserved) {
+      // plans for new block widths were built during the step
+      mStats.reservedBytes += e.actual - e.reserved;
+      e.reserved = e.actual;
     }
+    mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
     if (err != LSW_OK) {
       for (auto& t : take) e.pending.push_front(t.second);  // failed with the entry below
       failEntry(e, err, msg);

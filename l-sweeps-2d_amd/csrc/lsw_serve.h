@@ -113,6 +113,7 @@ class Server {
     std::vector<float> velocity;
     int Gx = 0, Gy = 0;
     double steadyBytes = 0, peakBytes = 0, reserved = 0;
+    double actual = 0;  // device bytes its loads and steps allocated (recorded)
     std::unique_ptr<DDCheckerboard> dd;
     std::unique_ptr<GmresSlots> slots;
     std::deque<Column> pending;

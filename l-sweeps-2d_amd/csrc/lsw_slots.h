@@ -27,6 +27,11 @@ namespace lsw {
 
 constexpr int kMaxSlots = 64;
 
+// Block widths the preconditioner is applied with: the active slots of a step
+// are padded with zero vectors up to the next width, so that a problem holds
+// solve plans for at most a dozen widths (a third more work at most).
+int planWidth(int nactive);
+
 class GmresSlots {
  public:
   struct Done {

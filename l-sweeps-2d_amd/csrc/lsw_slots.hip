@@ -151,8 +151,15 @@ __global__ void k_sscale(double2* __restrict__ dst, const double2* __restrict__ 
 }
 }  // namespace
 
+int planWidth(int na) {
+  static const int w[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
+  for (int x : w)
+    if (x >= na) return x;
+  return na;
+}
+
 size_t GmresSlots::bytesFor(long long n, int cap, int m) {
-  const size_t vecs = (size_t)cap * (m + 1) + (size_t)cap * m + 5 * (size_t)cap + 1;
+  const size_t vecs = (size_t)cap * (m + 1) + (size_t)cap * m + 3 * (size_t)cap + 2 * (size_t)planWidth(cap) + 1;
   return vecs * (size_t)n * sizeof(double2) + (size_t)cap * (m + 1) * kBlocks * sizeof(double2);
 }
 
@@ -166,25 +173,25 @@ GmresSlots::GmresSlots(const DDCheckerboard* dd, int capacity, int restart, doub
   mSlots.resize(mCap);
   const size_t vb = (size_t)mN * sizeof(double2);
   try {
-    LSW_HIP(hipMalloc(&dV, (size_t)mCap * (mM + 1) * vb));
-    LSW_HIP(hipMalloc(&dZ, (size_t)mCap * mM * vb));
-    LSW_HIP(hipMalloc(&dB, (size_t)mCap * vb));
-    LSW_HIP(hipMalloc(&dX, (size_t)mCap * vb));
-    LSW_HIP(hipMalloc(&dIn, (size_t)mCap * vb));
-    LSW_HIP(hipMalloc(&dPZ, (size_t)mCap * vb));
-    LSW_HIP(hipMalloc(&dW, (size_t)mCap * vb));
-    LSW_HIP(hipMalloc(&dStage, vb));
-    LSW_HIP(hipMalloc(&dPart, (size_t)mCap * (mM + 1) * kBlocks * sizeof(double2)));
-    LSW_HIP(hipMalloc(&dH, (size_t)(2 * mCap * (mM + 1) + mCap) * sizeof(double2)));
+    LSW_HIP(devMalloc(&dV, (size_t)mCap * (mM + 1) * vb));
+    LSW_HIP(devMalloc(&dZ, (size_t)mCap * mM * vb));
+    LSW_HIP(devMalloc(&dB, (size_t)mCap * vb));
+    LSW_HIP(devMalloc(&dX, (size_t)mCap * vb));
+    LSW_HIP(devMalloc(&dIn, (size_t)planWidth(mCap) * vb));
+    LSW_HIP(devMalloc(&dPZ, (size_t)planWidth(mCap) * vb));
+    LSW_HIP(devMalloc(&dW, (size_t)mCap * vb));
+    LSW_HIP(devMalloc(&dStage, vb));
+    LSW_HIP(devMalloc(&dPart, (size_t)mCap * (mM + 1) * kBlocks * sizeof(double2)));
+    LSW_HIP(devMalloc(&dH, (size_t)(2 * mCap * (mM + 1) + mCap) * sizeof(double2)));
   } catch (...) {
-    for (double2* p : {dV, dZ, dB, dX, dIn, dPZ, dW, dStage, dPart, dH}) if (p) (void)hipFree(p);
+    for (double2* p : {dV, dZ, dB, dX, dIn, dPZ, dW, dStage, dPart, dH}) if (p) (void)devFree(p);
     throw;
   }
 }
 
 GmresSlots::~GmresSlots() {
   (void)hipStreamSynchronize(mDD->stream());
-  for (double2* p : {dV, dZ, dB, dX, dIn, dPZ, dW, dStage, dPart, dH}) if (p) (void)hipFree(p);
+  for (double2* p : {dV, dZ, dB, dX, dIn, dPZ, dW, dStage, dPart, dH}) if (p) (void)devFree(p);
 }
 
 int GmresSlots::active() const {
@@ -303,7 +310,10 @@ void GmresSlots::step(std::vector<Done>& done) {
   const int na = L.n;
   const unsigned gx = gridN(mN);
   hipLaunchKernelGGL(k_sgather, dim3(gx, na), dim3(256), 0, s, dIn, dV, L, mM, mN);
-  mDD->applyPrecondDev(dIn, dPZ, s, na);
+  const int nb = planWidth(na);
+  if (nb > na)
+    LSW_HIP(hipMemsetAsync(dIn + (long long)na * mN, 0, (size_t)(nb - na) * mN * sizeof(double2), s));
+  mDD->applyPrecondDev(dIn, dPZ, s, nb);
   hipLaunchKernelGGL(k_sscatter, dim3(gx, na), dim3(256), 0, s, dZ, dPZ, L, mM, mN);
   mDD->applyADevBlock(dPZ, dW, na, s);
   ++mBlockApplies;

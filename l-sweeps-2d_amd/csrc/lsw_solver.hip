@@ -7,8 +7,8 @@
 namespace lsw {
 
 LocalLinearSolverHIP::~LocalLinearSolverHIP() {
-  for (void* p : {dX, dR, dG}) if (p) (void)hipFree(p);
-  if (dTasks) (void)hipFree(dTasks);
+  for (void* p : {dX, dR, dG}) if (p) (void)devFree(p);
+  if (dTasks) (void)devFree(dTasks);
   mFS.reset();
   if (mStream) (void)hipStreamDestroy(mStream);
 }
@@ -80,12 +80,12 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
   const int per = mFS->maxTaskRhs();
   const int ntasks = (nrhs + per - 1) / per;
   if (nrhs > mCap) {
-    for (void* p : {dX, dR, dG}) if (p) (void)hipFree(p);
-    if (dTasks) (void)hipFree(dTasks);
-    LSW_HIP(hipMalloc(&dX, (size_t)nrhs * N * es));
-    LSW_HIP(hipMalloc(&dR, (size_t)nrhs * N * es));
-    LSW_HIP(hipMalloc(&dG, (size_t)nrhs * G * es));
-    LSW_HIP(hipMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
+    for (void* p : {dX, dR, dG}) if (p) (void)devFree(p);
+    if (dTasks) (void)devFree(dTasks);
+    LSW_HIP(devMalloc(&dX, (size_t)nrhs * N * es));
+    LSW_HIP(devMalloc(&dR, (size_t)nrhs * N * es));
+    LSW_HIP(devMalloc(&dG, (size_t)nrhs * G * es));
+    LSW_HIP(devMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
     mCap = nrhs;
   }
   std::vector<SolveTask> tasks(ntasks);

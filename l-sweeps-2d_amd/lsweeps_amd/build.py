@@ -17,6 +17,7 @@ LIB = os.path.join(PKG_DIR, "liblsweeps_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = [
+    "lsw_mem.cpp",
     "lsw_nd.cpp",
     "lsw_model.cpp",
     "lsw_factor.hip",
