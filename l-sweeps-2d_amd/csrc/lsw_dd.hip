@@ -713,7 +713,6 @@ void DDCheckerboard::buildSchedule() {
   // ---- stage 3
   for (int t = 0; t < nStage3; ++t) {
     HostStep& st = steps[t + 2];
-    auto& pre = steps[t + 1].copies;
     for (int d = 0; d < 4; ++d) {
       const int slr = d & 1, sud = d >> 1;
       for (int kind = 5; kind <= 8; ++kind) {
@@ -777,18 +776,36 @@ void DDCheckerboard::buildSchedule() {
             addOut(s, true, vu[2], vu[2], vu[3], tOff(d, c, ctx, 4 + 1), true);
           }
           st.slots.push_back(s);
-          // the L-sweep takes its traces by value (include/DDCheckerboard.h:209-239):
-          // UD traces from the horizontal sweep that reaches it (R from the
-          // left, L from the right), LR traces from the vertical one (U from
-          // below, D from above)
-          const int horiz = fromLeft ? kR : kL, vert = fromBottom ? kU : kD;
-          for (int k = 0; k < 4; ++k) {
-            pre.push_back({tOff(d, c, ctx, 4 + k), tOff(d, c, horiz, 4 + k)});
-            pre.push_back({tOff(d, c, ctx, k), tOff(d, c, vert, k)});
-          }
         }
       }
     }
+  }
+  // The L-sweeps take their traces by value (include/DDCheckerboard.h:209-239):
+  // UD traces from the horizontal sweep that reaches the cell (R from the
+  // left, L from the right), LR traces from the vertical one (U from below, D
+  // from above) -- for every cell, also those the L-sweep does not solve,
+  // whose copies its neighbours read.  A cell's copy is made at the end of
+  // the step after which both sources are final: R holds column J from step
+  // J on (column 0: stage 1), L column J from step nC-1-J (last column:
+  // stage 1), U row I from step I, D row I from step nR-1-I.
+  if (nStage3 > 0) {
+    for (int d = 0; d < 4; ++d)
+      for (int kind = 5; kind <= 8; ++kind) {
+        const bool fromLeft = (kind == 5 || kind == 8);
+        const bool fromBottom = (kind == 5 || kind == 7);
+        const int horiz = fromLeft ? kR : kL, vert = fromBottom ? kU : kD;
+        for (int I = 0; I < mNR; ++I)
+          for (int J = 0; J < mNC; ++J) {
+            const int c = cidx(I, J);
+            const int hs = fromLeft ? J : (J <= mNC - 2 ? mNC - 1 - J : 0);
+            const int vs = fromBottom ? I : (I <= mNR - 2 ? mNR - 1 - I : 0);
+            auto& cp = steps[std::max(hs, vs)].copies;
+            for (int k = 0; k < 4; ++k) {
+              cp.push_back({tOff(d, c, kind, 4 + k), tOff(d, c, horiz, 4 + k)});
+              cp.push_back({tOff(d, c, kind, k), tOff(d, c, vert, k)});
+            }
+          }
+      }
   }
 
   // ---- the final solves: one per (d, cell), right-hand side = the windowed
