@@ -65,6 +65,33 @@ __device__ __forceinline__ void rowDot(const T2* __restrict__ M, long long ld, i
   }
 }
 
+// The first 8 factor values of a lane's row do not depend on the gathered
+// right-hand side: they are loaded before the gather, so that their memory
+// latency overlaps it.
+template <typename T2>
+__device__ __forceinline__ void rowPrefetch(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
+                                            T2 (&m)[8]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = k0 + u * kstep;
+    m[u] = k < K ? M[(long long)k * ld] : czero<T2>();
+  }
+}
+
+template <typename T2, int NR>
+__device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
+                                          const T2* v, int vs, const T2 (&m)[8], T2 (&acc)[NR]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int k = k0 + u * kstep;
+    if (k < K) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vs + k], acc[q]);
+    }
+  }
+  rowDot<T2, NR>(M, ld, k0 + 8 * kstep, K, kstep, v, vs, acc);
+}
+
 struct Item {
   SolveTaskDev tk;
   SolveNode nd;
@@ -182,28 +209,38 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
     const long long idx = (waveId * rounds + round) * P + grp;
     const bool valid = idx < nent;
     Item it;
+    T2 pre[8];
+    const T2* M = nullptr;
+    long long ld = 0;
+    int k0 = 0, K = 0;
     if (valid) {
       it = loadItem(ent, tasks, nodes, idx);
+      if constexpr (FWD) {
+        M = factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r;
+        ld = it.nd.b;
+        K = it.nd.e;
+      } else {
+        M = factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r;
+        ld = it.nd.e;
+        k0 = (it.flags & 4) ? 0 : it.nd.e;
+        K = it.nd.e + it.nd.b;
+      }
+      if (r < it.rows) rowPrefetch<T2>(M, ld, k0, K, 1, pre);
       if constexpr (FWD)
         fwdGather<T2, NR>(it, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
       else
-        bwdGather<T2, NR>(it, (it.flags & 4) ? r : it.nd.e + r, RP, poolE, poolB, X, Rb, xs, v);
+        bwdGather<T2, NR>(it, k0 + r, RP, poolE, poolB, X, Rb, xs, v);
     }
     waveSync();
     if (valid && r < it.rows) {
       T2 acc[NR];
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-      if constexpr (FWD) {
-        rowDot<T2, NR>(factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r, it.nd.b, 0, it.nd.e, 1, v, it.nd.e,
-                       acc);
+      rowDotPre<T2, NR>(M, ld, k0, K, 1, v, FWD ? it.nd.e : it.nd.e + it.nd.b, pre, acc);
+      if constexpr (FWD)
         fwdStore<T2, NR>(it, r, acc, poolMap, G, gs);
-      } else {
-        const int F = it.nd.e + it.nd.b;
-        rowDot<T2, NR>(factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r, it.nd.e, (it.flags & 4) ? 0 : it.nd.e,
-                       F, 1, v, F, acc);
+      else
         bwdStore<T2, NR>(it, r, acc, poolE, X, xs);
-      }
     }
     waveSync();
   }
@@ -229,28 +266,27 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   const Item it = loadItem(ent, tasks, nodes, idx);
   const int e = it.nd.e, F = e + it.nd.b;
   const int k0 = FWD ? 0 : ((it.flags & 4) ? 0 : e);
-  if constexpr (FWD)
-    fwdGather<T2, NR>(it, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
-  else
-    bwdGather<T2, NR>(it, k0 + threadIdx.x, 64 * W, poolE, poolB, X, Rb, xs, v);
-  __syncthreads();
   const int rows = it.rows;
-  if (rows == 0) return;  // uniform: a root's forward entry only keeps r_E
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int RP = 1;
   while (RP < rows) RP <<= 1;
   const int CG = 64 / RP;
   const int r = lane & (RP - 1), cg = lane / RP;
+  const T2* M = factor + it.tk.facBase + (FWD ? it.nd.fwdOff : it.nd.bwdOff) + it.r0 + r;
+  const long long ld = FWD ? it.nd.b : e;
+  const int K = FWD ? e : F, kb = k0 + w * CG + cg;
+  T2 pre[8];
+  if (rows > 0 && r < rows) rowPrefetch<T2>(M, ld, kb, K, W * CG, pre);
+  if constexpr (FWD)
+    fwdGather<T2, NR>(it, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
+  else
+    bwdGather<T2, NR>(it, k0 + threadIdx.x, 64 * W, poolE, poolB, X, Rb, xs, v);
+  __syncthreads();
+  if (rows == 0) return;  // uniform: a root's forward entry only keeps r_E
   T2 acc[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-  if (r < rows) {
-    if constexpr (FWD)
-      rowDot<T2, NR>(factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r, it.nd.b, w * CG + cg, e, W * CG, v, e,
-                     acc);
-    else
-      rowDot<T2, NR>(factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r, e, k0 + w * CG + cg, F, W * CG, v, F, acc);
-  }
+  if (r < rows) rowDotPre<T2, NR>(M, ld, kb, K, W * CG, v, FWD ? e : F, pre, acc);
   for (int off = RP; off < 64; off <<= 1) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
