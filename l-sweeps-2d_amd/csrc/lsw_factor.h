@@ -56,18 +56,13 @@ struct SolveTaskDev {
 };
 
 // One work item: rows [r0, r0+rows) of one node of one task.
-// flags: bit 0/1 = child 1/2 visited by the forward pass, bit 2 = node visited.
+// flags: kEnt* bits below.
+constexpr int kEntChild1 = 1;   // child 1 visited by the forward pass
+constexpr int kEntChild2 = 2;   // child 2 visited by the forward pass
+constexpr int kEntVisited = 4;  // the node itself visited (its r_E may be nonzero)
+constexpr int kEntRootRhs = 8;  // a one-chunk root: the backward entry forms r_E from X and the children's updates
 struct SolveEntry {
   int task, node, r0rows, flags;  // r0rows = r0 | rows << 16
-};
-
-// The bottom levels of the tree are solved subtree by subtree: one workgroup
-// walks the levels of one (task, subtree) with a barrier between levels.
-constexpr int kSubLevels = 8;  // most levels fused per subtree
-struct SubDesc {
-  int ent0;                  // first entry of the subtree
-  int lvl[kSubLevels + 1];   // entries of height h: [ent0 + lvl[h], ent0 + lvl[h+1])
-  int pad[2];
 };
 
 // Launch plan of one batch of solves (device work lists per tree height).
@@ -86,18 +81,6 @@ struct SolvePlan {
     int cols = 0;       // largest column count (sizes the LDS)
     double bytes = 0;   // algorithmic bytes
   };
-  // fused bottom levels (heights 0 .. subLevels-1), one workgroup per subtree
-  struct Sub {
-    int n = 0;               // subtrees
-    int levels = 0;
-    int rpLog[kSubLevels] = {};
-    int cols[kSubLevels] = {};
-    size_t lds = 0;          // bytes per workgroup
-    double bytes = 0;
-    long long subOff = 0;    // first SubDesc of this launch in dSubs
-  };
-  Sub fwdSub, bwdSub;
-  SubDesc* dSubs = nullptr;
   int ntasks = 0, maxNrhs = 1;
   std::vector<SolveTask> hTasks;
   SolveTaskDev* dTasks = nullptr;

@@ -2,10 +2,20 @@
 #include "lsw_nd.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <stdexcept>
 #include <unordered_map>
 
 namespace lsw {
+
+int leafDofs() {
+  static const int v = [] {
+    const char* e = std::getenv("LSW_LEAF_DOFS");
+    const int x = e ? std::atoi(e) : 64;
+    return std::max(4, std::min(256, x));
+  }();
+  return v;
+}
 
 namespace {
 struct Region {
@@ -21,7 +31,7 @@ int NdTree::rec(int i0, int i1, int j0, int j1, int parent) {
   nodes[id].parent = parent;
   std::vector<int> Eloc;
   int c1 = -1, c2 = -1;
-  if (w * h <= kLeafDofs) {
+  if (w * h <= leafDofs()) {
     for (int i = i0; i < i1; ++i)
       for (int j = j0; j < j1; ++j) Eloc.push_back(i * ny + j);
   } else if (w >= h) {

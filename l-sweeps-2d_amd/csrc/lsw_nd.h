@@ -3,7 +3,7 @@
 // AssemblerHelmholtz2DFD::getDOFPts, src/AssemblerHelmholtz2DFD.cpp:147-181).
 //
 // The tree is geometric: a region is split by one grid line across its longer
-// side until it holds at most kLeafDofs DOFs.  Node p eliminates E_p (its
+// side until it holds at most leafDofs() DOFs.  Node p eliminates E_p (its
 // separator line, or the whole region for a leaf); its boundary B_p is the
 // set of grid points adjacent to the region, which all lie on ancestor
 // separators.  The front of p is F_p = [E_p, B_p].
@@ -18,7 +18,12 @@
 
 namespace lsw {
 
-constexpr int kLeafDofs = 16;
+// Largest leaf (DOFs).  Default 64 (an 8 x 8 block: one wavefront of rows);
+// LSW_LEAF_DOFS overrides it (4..256) for tuning.  Larger leaves mean fewer
+// tree heights, so fewer solve launches per sweep step, and denser leaf
+// factors: on a 1600^2 grid of 8 x 8 cells, leaves of 16/32/64/128 DOFs gave
+// 57.7/53.2/52.0/53.7 ms per FP32-factor solve.
+int leafDofs();
 constexpr int kSolveRows = 64;  // rows per solve work item (one wavefront)
 
 struct NdNode {

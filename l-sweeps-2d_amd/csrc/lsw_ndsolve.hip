@@ -6,12 +6,20 @@
 // every task of a sweep step.  Work lists are flat {task, node, row chunk}
 // entries prepared on the host once per step:
 //   * the forward list only holds nodes whose subtree region meets the
-//     right-hand side support (sweep right-hand sides live on 2-4 grid lines),
-//   * the backward list holds every node, and skips the F11^{-1} r_E columns of
-//     nodes the forward pass did not visit (their r_E is zero).
+//     right-hand side support (sweep right-hand sides live on 2-4 grid lines);
+//     a root has no boundary rows, so its forward entry only stores r_E;
+//   * the backward list holds the nodes whose solution values are read (all
+//     of them for a full solve), and skips the F11^{-1} r_E columns of nodes
+//     the forward pass did not visit (their r_E is zero); a root of a single
+//     row chunk forms its r_E from X and the children's updates while it
+//     gathers, and has no forward entry.
 // Everything an entry needs is one SolveEntry, one SolveTaskDev and one
-// SolveNode (loaded side by side), so an entry's chain of dependent memory
-// accesses is: entry -> task/node -> index pools -> vector values -> factors.
+// SolveNode (loaded side by side).  Sweep steps are latency-bound (a few
+// microseconds of work per launch), so an entry's chain of dependent memory
+// accesses is kept short: entry -> task/node -> index pools -> vector values
+// -> products; the factor values, the children's updates a forward row adds
+// and the destination index a backward row stores to are all loaded in the
+// same rounds as the gather, not after the product.
 // Two launch shapes:
 //   * packed: a wavefront holds P = 64/RP entries side by side (RP lanes =
 //     rows of one entry), `wpb` wavefronts per workgroup run independently,
@@ -44,17 +52,20 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// acc[q] += sum_{k = k0, k0+kstep, ... < K} M[k*ld] * v[q*vs + k]; 8 loads in flight.
-template <typename T2, int NR>
+// Factor loads a lane keeps in flight in its product.
+constexpr int kDotDepth = 8;
+
+// acc[q] += sum_{k = k0, k0+kstep, ... < K} M[k*ld] * v[q*vs + k], D loads in flight.
+template <typename T2, int NR, int D>
 __device__ __forceinline__ void rowDot(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
                                        const T2* v, int vs, T2 (&acc)[NR]) {
   int k = k0;
-  for (; k + 7 * kstep < K; k += 8 * kstep) {
-    T2 m[8];
+  for (; k + (D - 1) * kstep < K; k += D * kstep) {
+    T2 m[D];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) m[u] = M[(long long)(k + u * kstep) * ld];
+    for (int u = 0; u < D; ++u) m[u] = M[(long long)(k + u * kstep) * ld];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < D; ++u)
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vs + k + u * kstep], acc[q]);
   }
@@ -65,31 +76,31 @@ __device__ __forceinline__ void rowDot(const T2* __restrict__ M, long long ld, i
   }
 }
 
-// The first 8 factor values of a lane's row do not depend on the gathered
+// The first D factor values of a lane's row do not depend on the gathered
 // right-hand side: they are loaded before the gather, so that their memory
 // latency overlaps it.
-template <typename T2>
+template <typename T2, int D>
 __device__ __forceinline__ void rowPrefetch(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
-                                            T2 (&m)[8]) {
+                                            T2 (&m)[D]) {
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < D; ++u) {
     const int k = k0 + u * kstep;
     m[u] = k < K ? M[(long long)k * ld] : czero<T2>();
   }
 }
 
-template <typename T2, int NR>
+template <typename T2, int NR, int D>
 __device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
-                                          const T2* v, int vs, const T2 (&m)[8], T2 (&acc)[NR]) {
+                                          const T2* v, int vs, const T2 (&m)[D], T2 (&acc)[NR]) {
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < D; ++u) {
     const int k = k0 + u * kstep;
     if (k < K) {
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vs + k], acc[q]);
     }
   }
-  rowDot<T2, NR>(M, ld, k0 + 8 * kstep, K, kstep, v, vs, acc);
+  rowDot<T2, NR, D>(M, ld, k0 + D * kstep, K, kstep, v, vs, acc);
 }
 
 struct Item {
@@ -110,21 +121,19 @@ __device__ __forceinline__ Item loadItem(const SolveEntry* __restrict__ ent, con
   return it;
 }
 
-// forward gather: v[q*e + k] = X[E[k]] - g_c1[m1[k]] - g_c2[m2[k]]; the first
-// row chunk also keeps r_E for the backward pass
+// r_E of a front: v[q*vs + k] = X[E[k]] - g_c1[m1[k]] - g_c2[m2[k]] for the
+// lane's k in [k0, K) (step kstep); with Rb != nullptr also kept in Rb for
+// the backward pass.
 template <typename T2, int NR>
-__device__ __forceinline__ void fwdGather(const Item& it, int k0, int kstep, const int* __restrict__ poolE,
-                                          const int* __restrict__ poolMap, const T2* __restrict__ X,
-                                          T2* __restrict__ Rb, const T2* __restrict__ G, long long xs, long long gs,
-                                          T2* v) {
-  const int e = it.nd.e, F = e + it.nd.b;
-  const bool c1 = (it.flags & 1) != 0, c2 = (it.flags & 2) != 0;
-  const int* E = poolE + it.nd.E;
-  const int* m1 = poolMap + it.nd.map;
-  const int* m2 = m1 + F;
-  for (int k = k0; k < e; k += kstep) {
+__device__ __forceinline__ void gatherFrontRhs(const Item& it, int k0, int K, int kstep, const int* __restrict__ E,
+                                               const int* __restrict__ m1, const T2* __restrict__ X,
+                                               T2* __restrict__ Rb, const T2* __restrict__ G, long long xs,
+                                               long long gs, T2* v, int vs) {
+  const int F = it.nd.e + it.nd.b;
+  const bool c1 = (it.flags & kEntChild1) != 0, c2 = (it.flags & kEntChild2) != 0;
+  for (int k = k0; k < K; k += kstep) {
     const int dof = E[k];
-    const int a1 = c1 ? m1[k] : -1, a2 = c2 ? m2[k] : -1;
+    const int a1 = c1 ? m1[k] : -1, a2 = c2 ? m1[F + k] : -1;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       if (q >= it.tk.nrhs) break;
@@ -132,37 +141,69 @@ __device__ __forceinline__ void fwdGather(const Item& it, int k0, int kstep, con
       T2 val = X[sl * xs + dof];
       if (a1 >= 0) val = csub(val, G[sl * gs + it.nd.g1 + a1]);
       if (a2 >= 0) val = csub(val, G[sl * gs + it.nd.g2 + a2]);
-      v[q * e + k] = val;
-      if (it.r0 == 0) Rb[sl * xs + dof] = val;
+      v[q * vs + k] = val;
+      if (Rb) Rb[sl * xs + dof] = val;
     }
   }
 }
 
-// forward scatter of row r: g_p[r0+r] = acc + children's updates on that row
+// forward gather: v[q*e + k] = r_E[k]; the first row chunk also keeps r_E
+// for the backward pass
 template <typename T2, int NR>
-__device__ __forceinline__ void fwdStore(const Item& it, int r, const T2 (&acc)[NR], const int* __restrict__ poolMap,
-                                         T2* __restrict__ G, long long gs) {
+__device__ __forceinline__ void fwdGather(const Item& it, int k0, int kstep, const int* __restrict__ poolE,
+                                          const int* __restrict__ poolMap, const T2* __restrict__ X,
+                                          T2* __restrict__ Rb, const T2* __restrict__ G, long long xs, long long gs,
+                                          T2* v) {
+  gatherFrontRhs<T2, NR>(it, k0, it.nd.e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, it.r0 == 0 ? Rb : nullptr,
+                         G, xs, gs, v, it.nd.e);
+}
+
+// The values a forward row adds to its product (the children's updates on
+// that row of the update vector) do not depend on the product: they are
+// loaded together with the gather.
+template <typename T2, int NR>
+__device__ __forceinline__ void fwdRowAdd(const Item& it, int r, const int* __restrict__ poolMap,
+                                          const T2* __restrict__ G, long long gs, T2 (&add)[NR]) {
   const int e = it.nd.e, F = e + it.nd.b;
   const int f = e + it.r0 + r;
   const int* m1 = poolMap + it.nd.map;
-  const int a1 = (it.flags & 1) ? m1[f] : -1, a2 = (it.flags & 2) ? m1[F + f] : -1;
+  const int a1 = (it.flags & kEntChild1) ? m1[f] : -1, a2 = (it.flags & kEntChild2) ? m1[F + f] : -1;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    add[q] = czero<T2>();
+    if (q >= it.tk.nrhs) continue;
+    const long long sl = it.tk.slot[q];
+    if (a1 >= 0) add[q] = cadd(add[q], G[sl * gs + it.nd.g1 + a1]);
+    if (a2 >= 0) add[q] = cadd(add[q], G[sl * gs + it.nd.g2 + a2]);
+  }
+}
+
+// forward scatter of row r: g_p[r0+r] = product + children's updates
+template <typename T2, int NR>
+__device__ __forceinline__ void fwdStore(const Item& it, int r, const T2 (&acc)[NR], const T2 (&add)[NR],
+                                         T2* __restrict__ G, long long gs) {
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     if (q >= it.tk.nrhs) break;
     const long long sl = it.tk.slot[q];
-    T2 g = acc[q];
-    if (a1 >= 0) g = cadd(g, G[sl * gs + it.nd.g1 + a1]);
-    if (a2 >= 0) g = cadd(g, G[sl * gs + it.nd.g2 + a2]);
-    G[sl * gs + it.nd.gOff + it.r0 + r] = g;
+    G[sl * gs + it.nd.gOff + it.r0 + r] = cadd(acc[q], add[q]);
   }
 }
 
-// backward gather: v[q*F + k] = r_E[k] (k < e) or x_B[k-e]
+// backward gather: v[q*F + k] = r_E[k] (k < e) or x_B[k-e].  The r_E of a
+// root of one row chunk is formed here from X and its children's updates
+// (its single entry reads all of X[E] before it stores any of it); a root
+// has no boundary, so its columns are all r_E.
 template <typename T2, int NR>
 __device__ __forceinline__ void bwdGather(const Item& it, int k0, int kstep, const int* __restrict__ poolE,
-                                          const int* __restrict__ poolB, const T2* __restrict__ X,
-                                          const T2* __restrict__ Rb, long long xs, T2* v) {
+                                          const int* __restrict__ poolB, const int* __restrict__ poolMap,
+                                          const T2* __restrict__ X, const T2* __restrict__ Rb,
+                                          const T2* __restrict__ G, long long xs, long long gs, T2* v) {
   const int e = it.nd.e, F = e + it.nd.b;
+  if (it.flags & kEntRootRhs) {
+    gatherFrontRhs<T2, NR>(it, k0, e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, nullptr, G, xs, gs, v, F);
+    return;
+  }
   const int* E = poolE + it.nd.E;
   const int* Bp = poolB + it.nd.B;
   for (int k = k0; k < F; k += kstep) {
@@ -178,9 +219,8 @@ __device__ __forceinline__ void bwdGather(const Item& it, int k0, int kstep, con
 }
 
 template <typename T2, int NR>
-__device__ __forceinline__ void bwdStore(const Item& it, int r, const T2 (&acc)[NR], const int* __restrict__ poolE,
-                                         T2* __restrict__ X, long long xs) {
-  const int dof = poolE[it.nd.E + it.r0 + r];
+__device__ __forceinline__ void bwdStore(const Item& it, int dof, const T2 (&acc)[NR], T2* __restrict__ X,
+                                         long long xs) {
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     if (q >= it.tk.nrhs) break;
@@ -208,8 +248,11 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
   for (int round = 0; round < rounds; ++round) {
     const long long idx = (waveId * rounds + round) * P + grp;
     const bool valid = idx < nent;
+    constexpr int D = kDotDepth;
     Item it;
-    T2 pre[8];
+    T2 pre[D];
+    T2 add[NR];
+    int dof = 0;
     const T2* M = nullptr;
     long long ld = 0;
     int k0 = 0, K = 0;
@@ -222,25 +265,31 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
       } else {
         M = factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r;
         ld = it.nd.e;
-        k0 = (it.flags & 4) ? 0 : it.nd.e;
+        k0 = (it.flags & kEntVisited) ? 0 : it.nd.e;
         K = it.nd.e + it.nd.b;
       }
-      if (r < it.rows) rowPrefetch<T2>(M, ld, k0, K, 1, pre);
+      if (r < it.rows) {
+        rowPrefetch<T2, D>(M, ld, k0, K, 1, pre);
+        if constexpr (FWD)
+          fwdRowAdd<T2, NR>(it, r, poolMap, G, gs, add);
+        else
+          dof = poolE[it.nd.E + it.r0 + r];
+      }
       if constexpr (FWD)
         fwdGather<T2, NR>(it, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
       else
-        bwdGather<T2, NR>(it, k0 + r, RP, poolE, poolB, X, Rb, xs, v);
+        bwdGather<T2, NR>(it, k0 + r, RP, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
     }
     waveSync();
     if (valid && r < it.rows) {
       T2 acc[NR];
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-      rowDotPre<T2, NR>(M, ld, k0, K, 1, v, FWD ? it.nd.e : it.nd.e + it.nd.b, pre, acc);
+      rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, FWD ? it.nd.e : it.nd.e + it.nd.b, pre, acc);
       if constexpr (FWD)
-        fwdStore<T2, NR>(it, r, acc, poolMap, G, gs);
+        fwdStore<T2, NR>(it, r, acc, add, G, gs);
       else
-        bwdStore<T2, NR>(it, r, acc, poolE, X, xs);
+        bwdStore<T2, NR>(it, dof, acc, X, xs);
     }
     waveSync();
   }
@@ -265,7 +314,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   if (idx >= nent) return;
   const Item it = loadItem(ent, tasks, nodes, idx);
   const int e = it.nd.e, F = e + it.nd.b;
-  const int k0 = FWD ? 0 : ((it.flags & 4) ? 0 : e);
+  const int k0 = FWD ? 0 : ((it.flags & kEntVisited) ? 0 : e);
   const int rows = it.rows;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int RP = 1;
@@ -275,18 +324,27 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   const T2* M = factor + it.tk.facBase + (FWD ? it.nd.fwdOff : it.nd.bwdOff) + it.r0 + r;
   const long long ld = FWD ? it.nd.b : e;
   const int K = FWD ? e : F, kb = k0 + w * CG + cg;
-  T2 pre[8];
-  if (rows > 0 && r < rows) rowPrefetch<T2>(M, ld, kb, K, W * CG, pre);
+  const bool storer = w == 0 && cg == 0 && r < rows;
+  constexpr int D = kDotDepth;
+  T2 pre[D];
+  T2 add[NR];
+  int dof = 0;
+  if (r < rows) rowPrefetch<T2, D>(M, ld, kb, K, W * CG, pre);
+  if (storer) {
+    if constexpr (FWD)
+      fwdRowAdd<T2, NR>(it, r, poolMap, G, gs, add);
+    else
+      dof = poolE[it.nd.E + it.r0 + r];
+  }
   if constexpr (FWD)
     fwdGather<T2, NR>(it, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
   else
-    bwdGather<T2, NR>(it, k0 + threadIdx.x, 64 * W, poolE, poolB, X, Rb, xs, v);
+    bwdGather<T2, NR>(it, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
   __syncthreads();
-  if (rows == 0) return;  // uniform: a root's forward entry only keeps r_E
   T2 acc[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-  if (r < rows) rowDotPre<T2, NR>(M, ld, kb, K, W * CG, v, FWD ? e : F, pre, acc);
+  if (r < rows) rowDotPre<T2, NR, D>(M, ld, kb, K, W * CG, v, FWD ? e : F, pre, acc);
   for (int off = RP; off < 64; off <<= 1) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
@@ -296,7 +354,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
     for (int q = 0; q < NR; ++q) red[(w * 64 + r) * NR + q] = acc[q];
   }
   __syncthreads();
-  if (w == 0 && cg == 0 && r < rows) {
+  if (storer) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       T2 s = red[r * NR + q];
@@ -304,76 +362,9 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
       acc[q] = s;
     }
     if constexpr (FWD)
-      fwdStore<T2, NR>(it, r, acc, poolMap, G, gs);
+      fwdStore<T2, NR>(it, r, acc, add, G, gs);
     else
-      bwdStore<T2, NR>(it, r, acc, poolE, X, xs);
-  }
-}
-// ------------------------------------------------------------- subtrees
-// One workgroup (4 wavefronts) per (task, subtree): the levels of the
-// subtree in order (forward: leaves first; backward: the subtree root
-// first), every level's entries spread over the workgroup's lane groups,
-// and a workgroup barrier between levels.  The update vectors and solution
-// values a level writes are read by the next level from global memory; all
-// wavefronts of a workgroup share one CU and its L1, so the barrier is
-// enough to make them visible.
-struct SubLevels {
-  int n;
-  int rpLog[kSubLevels];
-  int cols[kSubLevels];
-};
-
-template <typename T2, int NR, bool FWD>
-__global__ __launch_bounds__(256) void k_nd_sub(const SolveTaskDev* __restrict__ tasks,
-                                                const SolveNode* __restrict__ nodes,
-                                                const SolveEntry* __restrict__ ent, const SubDesc* __restrict__ subs,
-                                                SubLevels lv, const int* __restrict__ poolE,
-                                                const int* __restrict__ poolB, const int* __restrict__ poolMap,
-                                                const T2* __restrict__ factor, T2* __restrict__ X,
-                                                T2* __restrict__ Rb, T2* __restrict__ G, long long xs, long long gs) {
-  extern __shared__ char smem_raw[];
-  const SubDesc sd = subs[blockIdx.x];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int li = 0; li < lv.n; ++li) {
-    const int h = FWD ? li : lv.n - 1 - li;
-    const int b0 = sd.ent0 + sd.lvl[h], n = sd.lvl[h + 1] - sd.lvl[h];
-    if (n > 0) {
-      const int rpLog = lv.rpLog[h], RP = 1 << rpLog, P = 64 >> rpLog;
-      const int grp = lane >> rpLog, r = lane & (RP - 1);
-      const int gid = wave * P + grp, ng = nw * P;
-      T2* v = reinterpret_cast<T2*>(smem_raw) + (size_t)gid * NR * lv.cols[h];
-      const int iters = (n + ng - 1) / ng;
-      for (int pass = 0; pass < iters; ++pass) {
-        const int i = pass * ng + gid;
-        const bool valid = i < n;
-        Item it;
-        if (valid) {
-          it = loadItem(ent, tasks, nodes, b0 + i);
-          if constexpr (FWD)
-            fwdGather<T2, NR>(it, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
-          else
-            bwdGather<T2, NR>(it, (it.flags & 4) ? r : it.nd.e + r, RP, poolE, poolB, X, Rb, xs, v);
-        }
-        waveSync();
-        if (valid && r < it.rows) {
-          T2 acc[NR];
-#pragma unroll
-          for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-          if constexpr (FWD) {
-            rowDot<T2, NR>(factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r, it.nd.b, 0, it.nd.e, 1, v, it.nd.e,
-                           acc);
-            fwdStore<T2, NR>(it, r, acc, poolMap, G, gs);
-          } else {
-            const int F = it.nd.e + it.nd.b;
-            rowDot<T2, NR>(factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r, it.nd.e,
-                           (it.flags & 4) ? 0 : it.nd.e, F, 1, v, F, acc);
-            bwdStore<T2, NR>(it, r, acc, poolE, X, xs);
-          }
-        }
-        waveSync();
-      }
-    }
-    __syncthreads();
+      bwdStore<T2, NR>(it, dof, acc, X, xs);
   }
 }
 }  // namespace
@@ -382,7 +373,6 @@ __global__ __launch_bounds__(256) void k_nd_sub(const SolveTaskDev* __restrict__
 SolvePlan::~SolvePlan() {
   if (dTasks) (void)devFree(dTasks);
   if (dEntries) (void)devFree(dEntries);
-  if (dSubs) (void)devFree(dSubs);
 }
 
 namespace {
@@ -399,27 +389,6 @@ int ceilLog2(int x) {
   return l;
 }
 
-// Heights 0..subHeight() are solved subtree by subtree (-1: never).  The
-// LSW_SUB_HEIGHT environment variable overrides the default for tuning.
-int subHeight() {
-  static const int h = [] {
-    const char* e = std::getenv("LSW_SUB_HEIGHT");
-    const int v = e ? std::atoi(e) : -1;
-    return std::max(-1, std::min(kSubLevels - 1, v));
-  }();
-  return h;
-}
-
-// subtree sizes of a preorder-numbered tree
-std::vector<int> subtreeSizes(const NdTree& t) {
-  std::vector<int> sz(t.nodes.size(), 1);
-  for (int p = (int)t.nodes.size() - 1; p >= 0; --p) {
-    const NdNode& nd = t.nodes[p];
-    if (nd.c1 >= 0) sz[p] += sz[nd.c1];
-    if (nd.c2 >= 0) sz[p] += sz[nd.c2];
-  }
-  return sz;
-}
 }  // namespace
 
 void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
@@ -441,15 +410,6 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   std::vector<double> fbytes(mMaxHeight + 1, 0.0), bbytes(mMaxHeight + 1, 0.0);
   std::vector<double> fwork(mMaxHeight + 1, 0.0), bwork(mMaxHeight + 1, 0.0);
   std::vector<char> on, need;
-  // fused bottom levels
-  const int Hc = std::min(subHeight(), mMaxHeight);
-  const int nlev = Hc + 1;
-  std::vector<SubDesc> subF, subB;
-  std::vector<SolveEntry> entF, entB;  // subtree entries, grouped per subtree and height
-  std::vector<int> sfRows(kSubLevels, 0), sfCols(kSubLevels, 1), sbRows(kSubLevels, 0), sbCols(kSubLevels, 1);
-  double sfBytes = 0, sbBytes = 0;
-  std::map<int, std::vector<int>> sizes;
-  std::vector<std::vector<SolveEntry>> lf(kSubLevels), lb(kSubLevels);
   for (int t = 0; t < nt; ++t) {
     const SolveTask& tk = tasks[t];
     LSW_REQUIRE(tk.nrhs >= 1 && tk.nrhs <= kMaxRhs, LSW_ERR_ARG, "bad task right-hand-side count");
@@ -474,74 +434,20 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       const std::vector<Rect>& bs = (*bwdSupports)[t];
       for (int p = 0; p < (int)tr.nodes.size(); ++p) need[p] = !bs.empty() && meets(tr.nodes[p], bs) ? 1 : 0;
     }
-    auto& sz = sizes[shape];
-    if (sz.empty()) sz = subtreeSizes(tr);
-    // subtrees of the fused bottom levels: roots of height <= Hc under a parent
-    // above Hc; their nodes form a preorder range
-    for (int p = 0; Hc >= 0 && p < (int)tr.nodes.size(); ++p) {
-      const NdNode& sr = tr.nodes[p];
-      if (sr.height > Hc || (sr.parent >= 0 && tr.nodes[sr.parent].height <= Hc)) continue;
-      for (auto& v : lf) v.clear();
-      for (auto& v : lb) v.clear();
-      for (int q = p; q < p + sz[p]; ++q) {
-        const NdNode& nd = tr.nodes[q];
-        const int h = nd.height;
-        const double e = nd.e, b = nd.b;
-        int flags = 0;
-        if (nd.c1 >= 0 && on[nd.c1]) flags |= 1;
-        if (nd.c2 >= 0 && on[nd.c2]) flags |= 2;
-        if (on[q]) flags |= 4;
-        if (on[q]) {
-          if (nd.b == 0) {
-            lf[h].push_back({t, base + q, 0, flags});
-          } else {
-            for (int r0 = 0; r0 < nd.b; r0 += kSolveRows) {
-              const int rows = std::min(kSolveRows, nd.b - r0);
-              lf[h].push_back({t, base + q, r0 | (rows << 16), flags});
-              sfRows[h] = std::max(sfRows[h], rows);
-            }
-          }
-          sfCols[h] = std::max(sfCols[h], nd.e);
-          sfBytes += S * (b * e + r * (2 * e + 2 * b));
-        }
-        if (!need[q]) continue;
-        for (int r0 = 0; r0 < nd.e; r0 += kSolveRows) {
-          const int rows = std::min(kSolveRows, nd.e - r0);
-          lb[h].push_back({t, base + q, r0 | (rows << 16), flags});
-          sbRows[h] = std::max(sbRows[h], rows);
-        }
-        sbCols[h] = std::max(sbCols[h], nd.e + nd.b);
-        const double kcols = on[q] ? e + b : b;
-        sbBytes += S * (e * kcols + r * (kcols + e));
-      }
-      auto emit = [&](std::vector<std::vector<SolveEntry>>& lv, std::vector<SolveEntry>& dst,
-                      std::vector<SubDesc>& subs) {
-        size_t cnt = 0;
-        for (int h = 0; h < nlev; ++h) cnt += lv[h].size();
-        if (cnt == 0) return;
-        SubDesc d;
-        std::memset(&d, 0, sizeof(d));
-        d.ent0 = (int)dst.size();
-        for (int h = 0; h < nlev; ++h) {
-          d.lvl[h] = (int)(dst.size() - d.ent0);
-          dst.insert(dst.end(), lv[h].begin(), lv[h].end());
-        }
-        for (int h = nlev; h <= kSubLevels; ++h) d.lvl[h] = (int)(dst.size() - d.ent0);
-        subs.push_back(d);
-      };
-      emit(lf, entF, subF);
-      emit(lb, entB, subB);
-    }
     for (int p = 0; p < (int)tr.nodes.size(); ++p) {
       const NdNode& nd = tr.nodes[p];
       const int h = nd.height;
-      if (h <= Hc) continue;  // in a fused subtree
       const double e = nd.e, b = nd.b;
       int flags = 0;
-      if (nd.c1 >= 0 && on[nd.c1]) flags |= 1;
-      if (nd.c2 >= 0 && on[nd.c2]) flags |= 2;
-      if (on[p]) flags |= 4;
-      if (on[p]) {
+      if (nd.c1 >= 0 && on[nd.c1]) flags |= kEntChild1;
+      if (nd.c2 >= 0 && on[nd.c2]) flags |= kEntChild2;
+      if (on[p]) flags |= kEntVisited;
+      // A root (no boundary rows) has no forward product, only r_E to form.
+      // A root of one row chunk forms it in its backward entry; a larger
+      // root keeps a forward entry that stores r_E, because the chunks of
+      // its backward entries overwrite X[E] while the others read it.
+      if (nd.b == 0 && nd.e <= kSolveRows) flags |= kEntRootRhs;
+      if (on[p] && !(flags & kEntRootRhs)) {
         if (nd.b == 0) {
           fw[h].push_back({t, base + p, 0, flags});
         } else {
@@ -605,54 +511,19 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     all.insert(all.end(), v.begin(), v.end());
     out.push_back(L);
   };
-  // fused subtree launches: entries first in the entry array, SubDesc offsets
-  // shifted by where each list lands
-  auto makeSub = [&](SolvePlan::Sub& L, std::vector<SubDesc>& subs, std::vector<SolveEntry>& ents,
-                     const std::vector<int>& rowsH, const std::vector<int>& colsH, double bytes,
-                     std::vector<SubDesc>& allSubs) {
-    L = SolvePlan::Sub();
-    if (subs.empty()) return;
-    L.n = (int)subs.size();
-    L.levels = nlev;
-    L.bytes = bytes;
-    L.subOff = (long long)allSubs.size();
-    const int shift = (int)all.size();
-    for (auto& d : subs) d.ent0 += shift;
-    all.insert(all.end(), ents.begin(), ents.end());
-    allSubs.insert(allSubs.end(), subs.begin(), subs.end());
-    double lds = 0;
-    for (int h = 0; h < nlev; ++h) {
-      int rp = 1 << std::max(3, ceilLog2(std::max(1, rowsH[h])));
-      auto need = [&](int rpv) { return S * nrk * colsH[h] * 4.0 * (64.0 / rpv); };  // 4 wavefronts
-      while (rp < 64 && need(rp) > kMaxLds) rp *= 2;
-      LSW_REQUIRE(need(rp) <= kMaxLds, LSW_ERR_CAPACITY, "subtree level too wide for the LDS staging");
-      L.rpLog[h] = ceilLog2(rp);
-      L.cols[h] = colsH[h];
-      lds = std::max(lds, need(rp));
-    }
-    L.lds = (size_t)lds;
-  };
-  std::vector<SubDesc> allSubs;
-  makeSub(plan.fwdSub, subF, entF, sfRows, sfCols, sfBytes, allSubs);
-  makeSub(plan.bwdSub, subB, entB, sbRows, sbCols, sbBytes, allSubs);
   plan.fwd.clear();
   plan.bwd.clear();
-  plan.fwdBytes = plan.fwdSub.bytes;
-  plan.bwdBytes = plan.bwdSub.bytes;
-  for (int h = Hc + 1; h <= mMaxHeight; ++h) {
+  plan.fwdBytes = 0;
+  plan.bwdBytes = 0;
+  for (int h = 0; h <= mMaxHeight; ++h) {
     addLaunch(plan.fwd, fw[h], fcols[h], frows[h], fbytes[h], fwork[h]);
     plan.fwdBytes += fbytes[h];
   }
-  for (int h = mMaxHeight; h > Hc; --h) {
+  for (int h = mMaxHeight; h >= 0; --h) {
     addLaunch(plan.bwd, bw[h], bcols[h], brows[h], bbytes[h], bwork[h]);
     plan.bwdBytes += bbytes[h];
   }
   if (std::getenv("LSW_PLAN_LOG")) {
-    for (int pass = 0; pass < 2; ++pass) {
-      const SolvePlan::Sub& Ls = pass ? plan.bwdSub : plan.fwdSub;
-      std::fprintf(stderr, "plan %s tasks %d nrhs %d: subtrees %d levels %d lds %zu bytes %.0f\n",
-                   pass ? "bws" : "fws", nt, plan.maxNrhs, Ls.n, Ls.levels, Ls.lds, Ls.bytes);
-    }
     for (int pass = 0; pass < 2; ++pass)
       for (const auto& L : pass ? plan.bwd : plan.fwd)
         std::fprintf(stderr, "plan %s tasks %d nrhs %d: entries %d mode %d rp %d rounds %d wpb %d waves %d cols %d "
@@ -661,14 +532,8 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   }
   if (plan.dTasks) (void)devFree(plan.dTasks);
   if (plan.dEntries) (void)devFree(plan.dEntries);
-  if (plan.dSubs) (void)devFree(plan.dSubs);
   plan.dTasks = nullptr;
   plan.dEntries = nullptr;
-  plan.dSubs = nullptr;
-  if (!allSubs.empty()) {
-    LSW_HIP(devMalloc(&plan.dSubs, allSubs.size() * sizeof(SubDesc)));
-    LSW_HIP(hipMemcpy(plan.dSubs, allSubs.data(), allSubs.size() * sizeof(SubDesc), hipMemcpyHostToDevice));
-  }
   LSW_HIP(devMalloc(&plan.dTasks, tdev.size() * sizeof(SolveTaskDev)));
   LSW_HIP(hipMemcpy(plan.dTasks, tdev.data(), tdev.size() * sizeof(SolveTaskDev), hipMemcpyHostToDevice));
   LSW_HIP(devMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(SolveEntry)));
@@ -757,29 +622,10 @@ void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, T2
   }
 }
 
-template <typename T2, int NR, bool FWD>
-void launchSub(const SolvePlan& plan, const SolvePlan::Sub& L, const SolveArgs& a, const T2* factor, T2* X, T2* R,
-               T2* G, hipStream_t s) {
-  SubLevels lv;
-  std::memset(&lv, 0, sizeof(lv));
-  lv.n = L.levels;
-  for (int h = 0; h < L.levels; ++h) {
-    lv.rpLog[h] = L.rpLog[h];
-    lv.cols[h] = L.cols[h];
-  }
-  hipLaunchKernelGGL((k_nd_sub<T2, NR, FWD>), dim3((unsigned)L.n), dim3(256), L.lds, s, a.tasks, a.nodes, a.ent,
-                     plan.dSubs + L.subOff, lv, a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
-}
-
 template <typename T2, int NR>
 void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, T2* X, T2* R, T2* G, hipStream_t s,
              SolveProfile* prof) {
   LaunchTimer tm{prof, s, {}, {}, {}};
-  if (plan.fwdSub.n > 0) {
-    tm.begin();
-    launchSub<T2, NR, true>(plan, plan.fwdSub, a, factor, X, R, G, s);
-    tm.end(0, plan.fwdSub.bytes);
-  }
   for (const auto& L : plan.fwd) {
     tm.begin();
     launch<T2, NR, true>(a, L, factor, X, R, G, s);
@@ -789,11 +635,6 @@ void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, T2* X,
     tm.begin();
     launch<T2, NR, false>(a, L, factor, X, R, G, s);
     tm.end(1, L.bytes);
-  }
-  if (plan.bwdSub.n > 0) {
-    tm.begin();
-    launchSub<T2, NR, false>(plan, plan.bwdSub, a, factor, X, R, G, s);
-    tm.end(1, plan.bwdSub.bytes);
   }
   LSW_HIP(hipGetLastError());
   tm.finish();
