@@ -957,6 +957,11 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   LSW_HIP(devMalloc(&dG, slots * mGs * es));
   LSW_HIP(devMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
   LSW_HIP(devMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
+  // the solve kernels read (and multiply by zero) entries of X, R and G they
+  // do not use: they must hold finite values from the start
+  LSW_HIP(hipMemset(dX, 0, slots * mXs * es));
+  LSW_HIP(hipMemset(dR, 0, slots * mXs * es));
+  LSW_HIP(hipMemset(dG, 0, slots * mGs * es));
   mBufRhs = nrhs;
 }
 

@@ -52,55 +52,91 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Factor loads a lane keeps in flight in its product.
-constexpr int kDotDepth = 8;
+// Every global load below is issued unconditionally, from an address that
+// is valid for every lane (indices are clamped, unused right-hand sides read
+// slot 0, a missing child reads offset 0 of the update buffer), and every
+// loaded value feeds arithmetic on every path: values a lane must ignore are
+// multiplied by zero (cmask), LDS staging is written at clamped positions
+// (rewriting a value already there), and a store a lane must not make goes
+// to a trash word.  A load whose value is only used under a divergent
+// branch is moved into that branch by the compiler and waited for before
+// the branches join, which turns a chain of independent loads into one
+// memory round trip each; a sweep step's launches move only a few MB, so
+// their duration is that chain.  Branches that skip loads test wave-uniform
+// conditions (waveAny).  The update buffers are zeroed when allocated, so a
+// masked read never meets a NaN.
+constexpr int kDotDepth = 8;  // factor loads per batch of a product
 
-// acc[q] += sum_{k = k0, k0+kstep, ... < K} M[k*ld] * v[q*vs + k], D loads in flight.
-template <typename T2, int NR, int D>
-__device__ __forceinline__ void rowDot(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
-                                       const T2* v, int vs, T2 (&acc)[NR]) {
-  int k = k0;
-  for (; k + (D - 1) * kstep < K; k += D * kstep) {
-    T2 m[D];
-#pragma unroll
-    for (int u = 0; u < D; ++u) m[u] = M[(long long)(k + u * kstep) * ld];
-#pragma unroll
-    for (int u = 0; u < D; ++u)
-#pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vs + k + u * kstep], acc[q]);
-  }
-  for (; k < K; k += kstep) {
-    const T2 m0 = M[(long long)k * ld];
-#pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = cfma(m0, v[q * vs + k], acc[q]);
-  }
+// true in every lane when pred holds in any lane of the wavefront
+__device__ __forceinline__ bool waveAny(bool pred) { return __ballot(pred) != 0; }
+
+// values a lane gathers per batch: all index loads of a batch, then all
+// value loads
+template <int NR>
+constexpr int gatherBatch() {
+  return NR >= 8 ? 2 : NR >= 4 ? 4 : 8;
 }
 
-// The first D factor values of a lane's row do not depend on the gathered
-// right-hand side: they are loaded before the gather, so that their memory
-// latency overlaps it.
+// x if keep else 0, by arithmetic (x finite)
+template <typename T2>
+__device__ __forceinline__ T2 cmask(T2 x, bool keep) {
+  const auto s = keep ? decltype(x.x)(1) : decltype(x.x)(0);
+  T2 y;
+  y.x = x.x * s;
+  y.y = x.y * s;
+  return y;
+}
+
+// store target of a lane that must not store
+__device__ double2 gSolveTrash[512];
+
+template <typename T2>
+__device__ __forceinline__ T2* trashSlot() {
+  return reinterpret_cast<T2*>(gSolveTrash) + (threadIdx.x & 511);
+}
+
+// m[u] = M[(k + u*kstep)*ld] for k + u*kstep < K, else 0 (M[0] must be valid)
 template <typename T2, int D>
-__device__ __forceinline__ void rowPrefetch(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
-                                            T2 (&m)[D]) {
+__device__ __forceinline__ void rowLoad(const T2* __restrict__ M, long long ld, int k, int K, int kstep, T2 (&m)[D]) {
 #pragma unroll
   for (int u = 0; u < D; ++u) {
-    const int k = k0 + u * kstep;
-    m[u] = k < K ? M[(long long)k * ld] : czero<T2>();
+    const int kk = k + u * kstep;
+    const bool in = kk < K;
+    m[u] = cmask(M[(long long)(in ? kk : 0) * ld], in);
   }
 }
 
+// acc[q] += sum_u m[u] * v[q*vs + k + u*kstep] over the terms below K (m is
+// zero past K; the staged value is replaced by zero there, as it may be
+// unwritten)
 template <typename T2, int NR, int D>
-__device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
-                                          const T2* v, int vs, const T2 (&m)[D], T2 (&acc)[NR]) {
+__device__ __forceinline__ void rowAcc(const T2 (&m)[D], int k, int K, int kstep, const T2* v, int vs,
+                                       T2 (&acc)[NR]) {
 #pragma unroll
   for (int u = 0; u < D; ++u) {
-    const int k = k0 + u * kstep;
-    if (k < K) {
+    const int kk = k + u * kstep;
+    const bool in = kk < K;
+    const int kc = in ? kk : 0;
 #pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], v[q * vs + k], acc[q]);
+    for (int q = 0; q < NR; ++q) {
+      const T2 vv = v[q * vs + kc];
+      acc[q] = cfma(m[u], in ? vv : czero<T2>(), acc[q]);
     }
   }
-  rowDot<T2, NR, D>(M, ld, k0 + D * kstep, K, kstep, v, vs, acc);
+}
+
+// acc[q] += sum_{k = k0 + j*kstep < K} M[k*ld] * v[q*vs + k]: the first batch
+// comes from pre (loaded before the gather, so that its latency overlaps
+// it), the rest in batches of D.
+template <typename T2, int NR, int D>
+__device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
+                                          const T2* v, int vs, const T2 (&pre)[D], T2 (&acc)[NR]) {
+  rowAcc<T2, NR, D>(pre, k0, K, kstep, v, vs, acc);
+  for (int k = k0 + D * kstep; k < K; k += D * kstep) {
+    T2 m[D];
+    rowLoad<T2, D>(M, ld, k, K, kstep, m);
+    rowAcc<T2, NR, D>(m, k, K, kstep, v, vs, acc);
+  }
 }
 
 struct Item {
@@ -122,27 +158,62 @@ __device__ __forceinline__ Item loadItem(const SolveEntry* __restrict__ ent, con
 }
 
 // r_E of a front: v[q*vs + k] = X[E[k]] - g_c1[m1[k]] - g_c2[m2[k]] for the
-// lane's k in [k0, K) (step kstep); with Rb != nullptr also kept in Rb for
-// the backward pass.
+// lane's k in [k0, K) (step kstep) when `act`; with keepR also kept in Rb
+// for the backward pass.
 template <typename T2, int NR>
-__device__ __forceinline__ void gatherFrontRhs(const Item& it, int k0, int K, int kstep, const int* __restrict__ E,
-                                               const int* __restrict__ m1, const T2* __restrict__ X,
-                                               T2* __restrict__ Rb, const T2* __restrict__ G, long long xs,
-                                               long long gs, T2* v, int vs) {
+__device__ __forceinline__ void gatherFrontRhs(const Item& it, bool act, int k0, int K, int kstep,
+                                               const int* __restrict__ E, const int* __restrict__ m1,
+                                               const T2* __restrict__ X, T2* __restrict__ Rb, bool keepR,
+                                               const T2* __restrict__ G, long long xs, long long gs, T2* v, int vs) {
+  constexpr int B = gatherBatch<NR>();
   const int F = it.nd.e + it.nd.b;
   const bool c1 = (it.flags & kEntChild1) != 0, c2 = (it.flags & kEntChild2) != 0;
-  for (int k = k0; k < K; k += kstep) {
-    const int dof = E[k];
-    const int a1 = c1 ? m1[k] : -1, a2 = c2 ? m1[F + k] : -1;
+  const bool kids = waveAny(act && (c1 || c2));
+  if (!act) K = k0;
+  for (int kb = k0; kb < K; kb += B * kstep) {
+    int dof[B], i1[B], i2[B];
+    bool h1[B], h2[B];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      if (q >= it.tk.nrhs) break;
-      const long long sl = it.tk.slot[q];
-      T2 val = X[sl * xs + dof];
-      if (a1 >= 0) val = csub(val, G[sl * gs + it.nd.g1 + a1]);
-      if (a2 >= 0) val = csub(val, G[sl * gs + it.nd.g2 + a2]);
-      v[q * vs + k] = val;
-      if (Rb) Rb[sl * xs + dof] = val;
+    for (int u = 0; u < B; ++u) {
+      const int kk = kb + u * kstep;
+      const int kc = kk < K ? kk : kb;
+      dof[u] = E[kc];
+      int a1 = -1, a2 = -1;
+      if (kids) {
+        a1 = m1[kc];
+        a2 = m1[F + kc];
+      }
+      h1[u] = c1 && a1 >= 0;
+      h2[u] = c2 && a2 >= 0;
+      i1[u] = h1[u] ? it.nd.g1 + a1 : 0;
+      i2[u] = h2[u] ? it.nd.g2 + a2 : 0;
+    }
+    // all loads of the batch first: the stores below may alias them as far
+    // as the compiler knows
+    T2 val[B][NR];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) val[u][q] = X[(long long)it.tk.slot[q] * xs + dof[u]];
+    if (kids) {
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const long long sl = it.tk.slot[q];
+          const T2 g1v = G[sl * gs + i1[u]], g2v = G[sl * gs + i2[u]];
+          val[u][q] = csub(csub(val[u][q], cmask(g1v, h1[u])), cmask(g2v, h2[u]));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int kk = kb + u * kstep;
+      const int kc = kk < K ? kk : kb;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        v[q * vs + kc] = val[u][q];  // past K: the value of kb again
+        if (keepR && kk < K && q < it.tk.nrhs) Rb[(long long)it.tk.slot[q] * xs + dof[u]] = val[u][q];
+      }
     }
   }
 }
@@ -150,43 +221,45 @@ __device__ __forceinline__ void gatherFrontRhs(const Item& it, int k0, int K, in
 // forward gather: v[q*e + k] = r_E[k]; the first row chunk also keeps r_E
 // for the backward pass
 template <typename T2, int NR>
-__device__ __forceinline__ void fwdGather(const Item& it, int k0, int kstep, const int* __restrict__ poolE,
+__device__ __forceinline__ void fwdGather(const Item& it, bool act, int k0, int kstep, const int* __restrict__ poolE,
                                           const int* __restrict__ poolMap, const T2* __restrict__ X,
                                           T2* __restrict__ Rb, const T2* __restrict__ G, long long xs, long long gs,
                                           T2* v) {
-  gatherFrontRhs<T2, NR>(it, k0, it.nd.e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, it.r0 == 0 ? Rb : nullptr,
-                         G, xs, gs, v, it.nd.e);
+  gatherFrontRhs<T2, NR>(it, act, k0, it.nd.e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, Rb, it.r0 == 0, G,
+                         xs, gs, v, it.nd.e);
 }
 
-// The values a forward row adds to its product (the children's updates on
-// that row of the update vector) do not depend on the product: they are
-// loaded together with the gather.
+// The children's updates a forward row adds to its product do not depend
+// on the product: they are loaded with the gather and start the
+// accumulators.
 template <typename T2, int NR>
-__device__ __forceinline__ void fwdRowAdd(const Item& it, int r, const int* __restrict__ poolMap,
-                                          const T2* __restrict__ G, long long gs, T2 (&add)[NR]) {
+__device__ __forceinline__ void fwdRowAdd(const Item& it, bool ok, int r, const int* __restrict__ poolMap,
+                                          const T2* __restrict__ G, long long gs, T2 (&acc)[NR]) {
   const int e = it.nd.e, F = e + it.nd.b;
-  const int f = e + it.r0 + r;
+  const int f = ok ? e + it.r0 + r : 0;
   const int* m1 = poolMap + it.nd.map;
-  const int a1 = (it.flags & kEntChild1) ? m1[f] : -1, a2 = (it.flags & kEntChild2) ? m1[F + f] : -1;
+  const bool c1 = (it.flags & kEntChild1) != 0, c2 = (it.flags & kEntChild2) != 0;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
+  if (!waveAny(ok && (c1 || c2))) return;
+  const int a1 = m1[f], a2 = m1[F + f];
+  const bool h1 = ok && c1 && a1 >= 0, h2 = ok && c2 && a2 >= 0;
+  const int i1 = h1 ? it.nd.g1 + a1 : 0, i2 = h2 ? it.nd.g2 + a2 : 0;
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    add[q] = czero<T2>();
-    if (q >= it.tk.nrhs) continue;
     const long long sl = it.tk.slot[q];
-    if (a1 >= 0) add[q] = cadd(add[q], G[sl * gs + it.nd.g1 + a1]);
-    if (a2 >= 0) add[q] = cadd(add[q], G[sl * gs + it.nd.g2 + a2]);
+    acc[q] = cadd(cmask(G[sl * gs + i1], h1), cmask(G[sl * gs + i2], h2));
   }
 }
 
 // forward scatter of row r: g_p[r0+r] = product + children's updates
 template <typename T2, int NR>
-__device__ __forceinline__ void fwdStore(const Item& it, int r, const T2 (&acc)[NR], const T2 (&add)[NR],
-                                         T2* __restrict__ G, long long gs) {
+__device__ __forceinline__ void fwdStore(const Item& it, bool ok, int r, const T2 (&acc)[NR], T2* __restrict__ G,
+                                         long long gs) {
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    if (q >= it.tk.nrhs) break;
     const long long sl = it.tk.slot[q];
-    G[sl * gs + it.nd.gOff + it.r0 + r] = cadd(acc[q], add[q]);
+    if (ok && q < it.tk.nrhs) G[sl * gs + it.nd.gOff + it.r0 + r] = acc[q];
   }
 }
 
@@ -195,41 +268,55 @@ __device__ __forceinline__ void fwdStore(const Item& it, int r, const T2 (&acc)[
 // (its single entry reads all of X[E] before it stores any of it); a root
 // has no boundary, so its columns are all r_E.
 template <typename T2, int NR>
-__device__ __forceinline__ void bwdGather(const Item& it, int k0, int kstep, const int* __restrict__ poolE,
+__device__ __forceinline__ void bwdGather(const Item& it, bool act, int k0, int kstep, const int* __restrict__ poolE,
                                           const int* __restrict__ poolB, const int* __restrict__ poolMap,
                                           const T2* __restrict__ X, const T2* __restrict__ Rb,
                                           const T2* __restrict__ G, long long xs, long long gs, T2* v) {
+  constexpr int B = gatherBatch<NR>();
   const int e = it.nd.e, F = e + it.nd.b;
-  if (it.flags & kEntRootRhs) {
-    gatherFrontRhs<T2, NR>(it, k0, e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, nullptr, G, xs, gs, v, F);
-    return;
-  }
+  const bool root = (it.flags & kEntRootRhs) != 0;
+  if (waveAny(act && root))
+    gatherFrontRhs<T2, NR>(it, act && root, k0, e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, nullptr, false,
+                           G, xs, gs, v, F);
+  if (!waveAny(act && !root)) return;
   const int* E = poolE + it.nd.E;
   const int* Bp = poolB + it.nd.B;
-  for (int k = k0; k < F; k += kstep) {
-    const bool inE = k < e;
-    const int dof = inE ? E[k] : Bp[k - e];
+  const int K = (act && !root) ? F : k0;
+  for (int kb = k0; kb < K; kb += B * kstep) {
+    int dof[B];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      if (q >= it.tk.nrhs) break;
-      const long long sl = it.tk.slot[q];
-      v[q * F + k] = inE ? Rb[sl * xs + dof] : X[sl * xs + dof];
+    for (int u = 0; u < B; ++u) {
+      const int kk = kb + u * kstep;
+      const int kc = kk < K ? kk : kb;
+      dof[u] = *(kc < e ? E + kc : Bp + (kc - e));
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int kk = kb + u * kstep;
+      const int kc = kk < K ? kk : kb;
+      const T2* src = kc < e ? Rb : X;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const long long sl = it.tk.slot[q];
+        v[q * F + kc] = src[sl * xs + dof[u]];  // past K: the value of kb again
+      }
     }
   }
 }
 
 template <typename T2, int NR>
-__device__ __forceinline__ void bwdStore(const Item& it, int dof, const T2 (&acc)[NR], T2* __restrict__ X,
+__device__ __forceinline__ void bwdStore(const Item& it, bool ok, int dof, const T2 (&acc)[NR], T2* __restrict__ X,
                                          long long xs) {
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    if (q >= it.tk.nrhs) break;
-    X[(long long)it.tk.slot[q] * xs + dof] = acc[q];
+    T2* dst = (ok && q < it.tk.nrhs) ? X + (long long)it.tk.slot[q] * xs + dof : trashSlot<T2>();
+    *dst = acc[q];
   }
 }
 
 // ---------------------------------------------------------------- packed
-// LDS: per wavefront P regions of NR x cols values.
+// LDS: per wavefront P regions of NR x cols values.  Lanes past the last
+// entry work on the last entry and store nothing.
 template <typename T2, int NR, bool FWD>
 __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restrict__ tasks,
                                                    const SolveNode* __restrict__ nodes,
@@ -240,6 +327,7 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
                                                    T2* __restrict__ Rb, T2* __restrict__ G, long long xs,
                                                    long long gs) {
   extern __shared__ char smem_raw[];
+  constexpr int D = kDotDepth;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int RP = 1 << rpLog, P = 64 >> rpLog;
   const int grp = lane >> rpLog, r = lane & (RP - 1);
@@ -248,49 +336,35 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
   for (int round = 0; round < rounds; ++round) {
     const long long idx = (waveId * rounds + round) * P + grp;
     const bool valid = idx < nent;
-    constexpr int D = kDotDepth;
-    Item it;
+    const Item it = loadItem(ent, tasks, nodes, valid ? idx : nent - 1);
+    const bool ok = valid && r < it.rows;
+    const int e = it.nd.e, F = e + it.nd.b;
+    const long long ld = FWD ? it.nd.b : e;
+    const int k0 = FWD ? 0 : ((it.flags & kEntVisited) ? 0 : e);
+    const int K = ok ? (FWD ? e : F) : k0;
+    // a lane without columns reads the factor base (valid) instead of its row
+    const T2* M = K > k0 ? factor + it.tk.facBase + (FWD ? it.nd.fwdOff : it.nd.bwdOff) + it.r0 + r : factor;
     T2 pre[D];
-    T2 add[NR];
+    rowLoad<T2, D>(M, ld, k0, K, 1, pre);
+    T2 acc[NR];
     int dof = 0;
-    const T2* M = nullptr;
-    long long ld = 0;
-    int k0 = 0, K = 0;
-    if (valid) {
-      it = loadItem(ent, tasks, nodes, idx);
-      if constexpr (FWD) {
-        M = factor + it.tk.facBase + it.nd.fwdOff + it.r0 + r;
-        ld = it.nd.b;
-        K = it.nd.e;
-      } else {
-        M = factor + it.tk.facBase + it.nd.bwdOff + it.r0 + r;
-        ld = it.nd.e;
-        k0 = (it.flags & kEntVisited) ? 0 : it.nd.e;
-        K = it.nd.e + it.nd.b;
-      }
-      if (r < it.rows) {
-        rowPrefetch<T2, D>(M, ld, k0, K, 1, pre);
-        if constexpr (FWD)
-          fwdRowAdd<T2, NR>(it, r, poolMap, G, gs, add);
-        else
-          dof = poolE[it.nd.E + it.r0 + r];
-      }
-      if constexpr (FWD)
-        fwdGather<T2, NR>(it, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
-      else
-        bwdGather<T2, NR>(it, k0 + r, RP, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
-    }
-    waveSync();
-    if (valid && r < it.rows) {
-      T2 acc[NR];
+    if constexpr (FWD) {
+      fwdRowAdd<T2, NR>(it, ok, r, poolMap, G, gs, acc);
+    } else {
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-      rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, FWD ? it.nd.e : it.nd.e + it.nd.b, pre, acc);
-      if constexpr (FWD)
-        fwdStore<T2, NR>(it, r, acc, add, G, gs);
-      else
-        bwdStore<T2, NR>(it, dof, acc, X, xs);
+      dof = poolE[it.nd.E + (ok ? it.r0 + r : 0)];
     }
+    if constexpr (FWD)
+      fwdGather<T2, NR>(it, valid, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
+    else
+      bwdGather<T2, NR>(it, valid, k0 + r, RP, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
+    waveSync();
+    rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, FWD ? e : F, pre, acc);
+    if constexpr (FWD)
+      fwdStore<T2, NR>(it, ok, r, acc, G, gs);
+    else
+      bwdStore<T2, NR>(it, ok, dof, acc, X, xs);
     waveSync();
   }
 }
@@ -308,6 +382,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
                                                     T2* __restrict__ X, T2* __restrict__ Rb, T2* __restrict__ G,
                                                     long long xs, long long gs) {
   extern __shared__ char smem_raw[];
+  constexpr int D = kDotDepth;
   T2* v = reinterpret_cast<T2*>(smem_raw);
   T2* red = v + (size_t)NR * cols;
   const long long idx = blockIdx.x;
@@ -321,35 +396,36 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   while (RP < rows) RP <<= 1;
   const int CG = 64 / RP;
   const int r = lane & (RP - 1), cg = lane / RP;
-  const T2* M = factor + it.tk.facBase + (FWD ? it.nd.fwdOff : it.nd.bwdOff) + it.r0 + r;
+  const bool ok = r < rows;
   const long long ld = FWD ? it.nd.b : e;
-  const int K = FWD ? e : F, kb = k0 + w * CG + cg;
-  const bool storer = w == 0 && cg == 0 && r < rows;
-  constexpr int D = kDotDepth;
+  const int kb = k0 + w * CG + cg;
+  const int K = ok ? (FWD ? e : F) : kb;
+  // a lane without columns reads the factor base (valid) instead of its row
+  const T2* M = K > kb ? factor + it.tk.facBase + (FWD ? it.nd.fwdOff : it.nd.bwdOff) + it.r0 + r : factor;
+  const bool storer = w == 0 && cg == 0 && ok;
   T2 pre[D];
+  rowLoad<T2, D>(M, ld, kb, K, W * CG, pre);
+  // the children's updates of a forward row are added by the storing lanes
   T2 add[NR];
   int dof = 0;
-  if (r < rows) rowPrefetch<T2, D>(M, ld, kb, K, W * CG, pre);
-  if (storer) {
-    if constexpr (FWD)
-      fwdRowAdd<T2, NR>(it, r, poolMap, G, gs, add);
-    else
-      dof = poolE[it.nd.E + it.r0 + r];
-  }
   if constexpr (FWD)
-    fwdGather<T2, NR>(it, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
+    fwdRowAdd<T2, NR>(it, storer, r, poolMap, G, gs, add);
   else
-    bwdGather<T2, NR>(it, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
+    dof = poolE[it.nd.E + (ok ? it.r0 + r : 0)];
+  if constexpr (FWD)
+    fwdGather<T2, NR>(it, true, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
+  else
+    bwdGather<T2, NR>(it, true, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
   __syncthreads();
   T2 acc[NR];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) acc[q] = czero<T2>();
-  if (r < rows) rowDotPre<T2, NR, D>(M, ld, kb, K, W * CG, v, FWD ? e : F, pre, acc);
+  for (int q = 0; q < NR; ++q) acc[q] = FWD ? add[q] : czero<T2>();
+  rowDotPre<T2, NR, D>(M, ld, kb, K, W * CG, v, FWD ? e : F, pre, acc);
   for (int off = RP; off < 64; off <<= 1) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
   }
-  if (cg == 0 && r < rows) {
+  if (cg == 0 && ok) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) red[(w * 64 + r) * NR + q] = acc[q];
   }
@@ -361,11 +437,11 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
       for (int ww = 1; ww < W; ++ww) s = cadd(s, red[(ww * 64 + r) * NR + q]);
       acc[q] = s;
     }
-    if constexpr (FWD)
-      fwdStore<T2, NR>(it, r, acc, add, G, gs);
-    else
-      bwdStore<T2, NR>(it, dof, acc, X, xs);
   }
+  if constexpr (FWD)
+    fwdStore<T2, NR>(it, storer, r, acc, G, gs);
+  else
+    bwdStore<T2, NR>(it, storer, dof, acc, X, xs);
 }
 }  // namespace
 

@@ -85,6 +85,11 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
     LSW_HIP(devMalloc(&dX, (size_t)nrhs * N * es));
     LSW_HIP(devMalloc(&dR, (size_t)nrhs * N * es));
     LSW_HIP(devMalloc(&dG, (size_t)nrhs * G * es));
+    // the solve kernels read (and multiply by zero) entries of X, R and G
+    // they do not use: they must hold finite values from the start
+    LSW_HIP(hipMemset(dX, 0, (size_t)nrhs * N * es));
+    LSW_HIP(hipMemset(dR, 0, (size_t)nrhs * N * es));
+    LSW_HIP(hipMemset(dG, 0, (size_t)nrhs * G * es));
     LSW_HIP(devMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
     mCap = nrhs;
   }
