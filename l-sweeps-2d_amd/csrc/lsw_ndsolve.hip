@@ -25,9 +25,11 @@
 //     rows of one entry), `wpb` wavefronts per workgroup run independently,
 //     each lane group walks `rounds` entries; used wherever a height has
 //     enough entries (the many small fronts near the leaves);
-//   * wide: W = 2..8 wavefronts split the columns of one entry and reduce in
-//     LDS; used for heights with few, large entries (the separators near the
-//     root, where a front step holds only a few dozen of them).
+//   * wide: W = 1..8 wavefronts split the columns of one entry (64/RP column
+//     groups per wavefront) and reduce in LDS; used for heights with few,
+//     large entries, and for every launch with few wavefronts (the launches
+//     of a sweep step), where a packed lane walking its whole row would make
+//     the product a long chain of memory round trips.
 // Factors are column-major, so the lanes of an entry read consecutive rows of
 // one column (coalesced); right-hand-side values are broadcast from LDS.
 #include <algorithm>
@@ -66,6 +68,16 @@ __device__ __forceinline__ void waveSync() {
 // conditions (waveAny).  The update buffers are zeroed when allocated, so a
 // masked read never meets a NaN.
 constexpr int kDotDepth = 8;  // factor loads per batch of a product
+// Packed launches of fewer wavefronts are latency-bound and take the
+// column-split shape (LSW_LATENCY_WAVES overrides the threshold for tuning;
+// 0 turns the rule off).
+long long latencyWaves() {
+  static const long long v = [] {
+    const char* e = std::getenv("LSW_LATENCY_WAVES");
+    return e ? std::max(0LL, std::atoll(e)) : 4096LL;
+  }();
+  return v;
+}
 
 // true in every lane when pred holds in any lane of the wavefront
 __device__ __forceinline__ bool waveAny(bool pred) { return __ballot(pred) != 0; }
@@ -560,21 +572,39 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     L.cols = cols;
     L.bytes = bytes;
     const double per = work / v.size();
-    // few large entries: split their columns over W wavefronts
-    int W = 1;
-    while (W < 8 && (double)v.size() * W * 2 <= 4096.0 && per >= 1024.0 * W * 2) W *= 2;
     auto ldsWide = [&](int w) { return S * nrk * ((double)cols + 64.0 * w); };
+    // packed shape: RP lanes per entry (>= 8 so that the LDS stays bounded)
+    // (a launch of roots only, maxRows == 0, just gathers r_E: all 64 lanes)
+    int rp = maxRows == 0 ? 64 : 1 << std::max(3, ceilLog2(std::max(1, maxRows)));
+    auto ldsWave = [&](int rpv) { return S * nrk * cols * (64.0 / rpv); };
+    while (rp < 64 && ldsWave(rp) > kMaxLds) rp *= 2;
+    const long long packedWaves = ((long long)v.size() * rp + 63) / 64;
+    int W = 0;
+    if (packedWaves < latencyWaves() && maxRows > 0 && maxRows <= 16) {
+      // Few wavefronts of short rows: the launch is a chain of memory round
+      // trips, and a packed lane walks its whole row while most lanes of an
+      // entry's wavefront share its few rows.  Split the columns instead (W
+      // wavefronts x 64/RP column groups per entry) until a lane holds at
+      // most one prefetched batch of them.  (Measured on 8- and 16-row
+      // heights of a sweep step: 14-20 us -> 8-9 us; entries of 32-64 rows
+      // gained nothing or lost, as the split only added wavefronts.)
+      const int RPw = 1 << ceilLog2(maxRows);
+      const int CG = 64 / RPw;
+      W = 1;
+      const int depth = kDotDepth;
+      while (W < 8 && (cols + CG * W - 1) / (CG * W) > depth && (long long)v.size() * W * 2 <= 8192) W *= 2;
+    } else {
+      // few large entries: split their columns over W wavefronts
+      int w = 1;
+      while (w < 8 && (double)v.size() * w * 2 <= 4096.0 && per >= 1024.0 * w * 2) w *= 2;
+      if (w > 1) W = w;
+    }
     while (W > 1 && ldsWide(W) > kMaxLds) W /= 2;
-    if (W > 1) {
+    if (W > 0 && ldsWide(W) <= kMaxLds) {
       L.mode = 1;
       L.waves = W;
     } else {
-      // packed: RP lanes per entry (>= 8 so that the LDS stays bounded)
       L.mode = 0;
-      // (a launch of roots only, maxRows == 0, just gathers r_E: all 64 lanes)
-      int rp = maxRows == 0 ? 64 : 1 << std::max(3, ceilLog2(std::max(1, maxRows)));
-      auto ldsWave = [&](int rpv) { return S * nrk * cols * (64.0 / rpv); };
-      while (rp < 64 && ldsWave(rp) > kMaxLds) rp *= 2;
       LSW_REQUIRE(ldsWave(rp) <= kMaxLds, LSW_ERR_CAPACITY, "front too wide for the solve kernel's LDS staging");
       L.rp = rp;
       int wpb = 4;
@@ -683,6 +713,10 @@ void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, T2
   const size_t shm = ((size_t)NR * L.cols + (size_t)L.waves * 64 * NR) * sizeof(T2);
   const dim3 grid((unsigned)L.n);
   switch (L.waves) {
+    case 1:
+      hipLaunchKernelGGL((k_nd_wide<T2, NR, 1, FWD>), grid, dim3(64), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+      break;
     case 2:
       hipLaunchKernelGGL((k_nd_wide<T2, NR, 2, FWD>), grid, dim3(128), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
                          a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);

@@ -46,8 +46,11 @@ def test_lu_multi_rhs_and_fp32():
     # conditioning level
     assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) < 1e-9
     assert relerr(X, ref) < 1e-8
+    # one right-hand side gives the column of the block up to the order of
+    # the sums: the launch shapes (how a row's columns are split over lanes)
+    # depend on the block width through the LDS budget and prefetch depth
     for k in range(B.shape[1]):
-        assert relerr(s.solve(B[:, k]), X[:, k]) < 1e-14  # one RHS == column of the block
+        assert relerr(s.solve(B[:, k]), X[:, k]) < 1e-12
     # FP32 factors on this grid are conditioning-limited (about 7e-2 forward
     # error measured); the FP32 accuracy check is on an interior cell below
     s32 = L.LocalLinearSolverHIP(L.LSW_PREC_FP32)
