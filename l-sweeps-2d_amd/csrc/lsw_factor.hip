@@ -3,6 +3,7 @@
 #include "lsw_factor.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "lsw_cplx.h"
@@ -37,41 +38,58 @@ __global__ void k_assemble(const CellDev* __restrict__ cells, const double* __re
   }
 }
 
-// One workgroup per front: assemble F = [F11 F12; F21 F22] (original entries
-// plus the children's Schur complements), then Gauss-Jordan with partial
-// pivoting restricted to the e fully-summed rows on [F11 F12 I; F21 F22 0]:
+// The dense front of node p, in the workspace, row-major R x Cc with
+// R = e + b and Cc = 2e + b: [F11 F12 I; F21 F22 0] (original entries plus
+// the children's Schur complements).  Gauss-Jordan with partial pivoting
+// restricted to the e fully-summed rows turns it into
 //   top rows    -> [I | F11^{-1} F12 | F11^{-1}]
 //   bottom rows -> [0 | S            | -F21 F11^{-1}]
-// Writes W = F21 F11^{-1}, K = [F11^{-1} | -F11^{-1} F12] and S.
-__global__ __launch_bounds__(256) void k_front_factor(
-    const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes,
-    const FrontTask* __restrict__ fts, const double2* __restrict__ coef,
-    double2* __restrict__ factor, double2* __restrict__ schur, double2* __restrict__ ws,
-    int* __restrict__ info) {
-  extern __shared__ double2 colk[];
-  __shared__ double redv[256];
-  __shared__ int redi[256];
-  const FrontTask ft = fts[blockIdx.x];
+// from which W = F21 F11^{-1}, K = [F11^{-1} | -F11^{-1} F12] and S are
+// written.  Two schedules: one workgroup per front doing all of it (kept
+// for tuning), and the blocked one used by default: panels of kPanel pivots
+// (one workgroup per front) each followed by a batched rank-kPanel update of
+// the rest of every front of the height, tiled over many workgroups (the
+// bulk of the arithmetic, and no longer one CU per front).
+struct FrontView {
+  NdNodeDev nd;
+  int e, b, R, Cc, N;
+  double2* M;
+};
+
+__device__ __forceinline__ FrontView frontView(const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes,
+                                               const FrontTask& ft, double2* __restrict__ ws) {
+  FrontView f;
   const CellDev& cd = cells[ft.cell];
   const ShapeDev& sh = shapes[cd.shape];
-  const NdNodeDev nd = sh.nodes[ft.node];
-  const int e = nd.e, b = nd.b, R = e + b, Cc = 2 * e + b;
-  double2* M = ws + ft.wsOff;
-  const int N = cd.nx * cd.ny;
+  f.nd = sh.nodes[ft.node];
+  f.e = f.nd.e;
+  f.b = f.nd.b;
+  f.R = f.e + f.b;
+  f.Cc = 2 * f.e + f.b;
+  f.N = cd.nx * cd.ny;
+  f.M = ws + ft.wsOff;
+  return f;
+}
+
+// zero the front, put the identity block, add the original entries and the
+// children's Schur complements (one workgroup)
+__device__ void frontAssemble(const CellDev& cd, const ShapeDev& sh, const FrontView& f,
+                              const double2* __restrict__ coef, const double2* __restrict__ schur) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const double2 zero = make_double2(0.0, 0.0);
-
-  for (long long idx = tid; idx < (long long)R * Cc; idx += nt) M[idx] = zero;
+  double2* M = f.M;
+  const int e = f.e, Cc = f.Cc, b = f.b;
+  for (long long idx = tid; idx < (long long)f.R * Cc; idx += nt) M[idx] = zero;
   __syncthreads();
   for (int k = tid; k < e; k += nt) M[(long long)k * Cc + e + b + k] = make_double2(1.0, 0.0);
-  for (int a = tid; a < nd.asmCnt; a += nt) {
-    const AsmEntry en = sh.asmList[nd.asmOff + a];
-    const double2 v = coef[cd.coefOff + (long long)en.dir * N + en.dof];
+  for (int a = tid; a < f.nd.asmCnt; a += nt) {
+    const AsmEntry en = sh.asmList[f.nd.asmOff + a];
+    const double2 v = coef[cd.coefOff + (long long)en.dir * f.N + en.dof];
     double2* t = &M[(long long)en.fr * Cc + en.fc];
     *t = cadd(*t, v);
   }
   __syncthreads();
-  const int cs[2] = {nd.c1, nd.c2};
+  const int cs[2] = {f.nd.c1, f.nd.c2};
   for (int ci = 0; ci < 2; ++ci) {
     const int c = cs[ci];
     if (c < 0) continue;
@@ -86,36 +104,89 @@ __global__ __launch_bounds__(256) void k_front_factor(
     }
     __syncthreads();
   }
+}
 
+// write W, K and S from the reduced front (one workgroup)
+__device__ void frontStore(const CellDev& cd, const FrontView& f, double2* __restrict__ factor,
+                           double2* __restrict__ schur) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const double2* M = f.M;
+  const int e = f.e, b = f.b, R = f.R, Cc = f.Cc;
+  double2* W = factor + cd.factorOff + f.nd.fwdOff;
+  double2* K = factor + cd.factorOff + f.nd.bwdOff;
+  for (long long idx = tid; idx < (long long)b * e; idx += nt) {
+    const int k = (int)(idx / b), t = (int)(idx - (long long)k * b);
+    W[idx] = cneg(M[(long long)(e + t) * Cc + e + b + k]);
+  }
+  for (long long idx = tid; idx < (long long)e * R; idx += nt) {
+    const int k = (int)(idx / e), r = (int)(idx - (long long)k * e);
+    K[idx] = (k < e) ? M[(long long)r * Cc + e + b + k] : cneg(M[(long long)r * Cc + k]);
+  }
+  if (b > 0) {
+    double2* S = schur + cd.schurOff + f.nd.sOff;
+    for (long long idx = tid; idx < (long long)b * b; idx += nt) {
+      const int t1 = (int)(idx / b), t2 = (int)(idx - (long long)t1 * b);
+      S[idx] = M[(long long)(e + t1) * Cc + e + t2];
+    }
+  }
+}
+
+// Pivot of column k: the largest modulus among rows k..e-1 (lowest index on
+// ties), reduced over the workgroup (256 threads).
+__device__ __forceinline__ int pivotRow(const double2* __restrict__ M, int Cc, int k, int e, double* redv, int* redi,
+                                        double& pmag) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double best = -1.0;
+  int bi = e;
+  for (int i = k + tid; i < e; i += nt) {
+    const double2 v = M[(long long)i * Cc + k];
+    const double a = v.x * v.x + v.y * v.y;
+    if (a > best) {
+      best = a;
+      bi = i;
+    }
+  }
+  redv[tid] = best;
+  redi[tid] = bi;
+  __syncthreads();
+  for (int s = nt / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double o = redv[tid + s];
+      const int oi = redi[tid + s];
+      if (o > redv[tid] || (o == redv[tid] && oi < redi[tid])) {
+        redv[tid] = o;
+        redi[tid] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const int p = redi[0];
+  pmag = redv[0];
+  __syncthreads();
+  return p;
+}
+
+// One workgroup per front: assemble, full Gauss-Jordan, store.
+__global__ __launch_bounds__(256) void k_front_factor(
+    const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes,
+    const FrontTask* __restrict__ fts, const double2* __restrict__ coef,
+    double2* __restrict__ factor, double2* __restrict__ schur, double2* __restrict__ ws,
+    int* __restrict__ info) {
+  extern __shared__ double2 colk[];
+  __shared__ double redv[256];
+  __shared__ int redi[256];
+  const FrontTask ft = fts[blockIdx.x];
+  const CellDev& cd = cells[ft.cell];
+  const ShapeDev& sh = shapes[cd.shape];
+  const FrontView f = frontView(cells, shapes, ft, ws);
+  frontAssemble(cd, sh, f, coef, schur);
+  double2* M = f.M;
+  const int e = f.e, R = f.R, Cc = f.Cc;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const double2 zero = make_double2(0.0, 0.0);
   for (int k = 0; k < e; ++k) {
-    // pivot: largest modulus in column k among rows k..e-1 (lowest index on ties)
-    double best = -1.0;
-    int bi = e;
-    for (int i = k + tid; i < e; i += nt) {
-      const double2 v = M[(long long)i * Cc + k];
-      const double a = v.x * v.x + v.y * v.y;
-      if (a > best) {
-        best = a;
-        bi = i;
-      }
-    }
-    redv[tid] = best;
-    redi[tid] = bi;
-    __syncthreads();
-    for (int s = nt / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        const double o = redv[tid + s];
-        const int oi = redi[tid + s];
-        if (o > redv[tid] || (o == redv[tid] && oi < redi[tid])) {
-          redv[tid] = o;
-          redi[tid] = oi;
-        }
-      }
-      __syncthreads();
-    }
-    const int p = redi[0];
-    const double pmag = redv[0];
-    __syncthreads();
+    double pmag;
+    const int p = pivotRow(M, Cc, k, e, redv, redi, pmag);
     if (p != k) {
       for (int j = tid; j < Cc; j += nt) {
         const double2 t = M[(long long)k * Cc + j];
@@ -148,22 +219,185 @@ __global__ __launch_bounds__(256) void k_front_factor(
     for (int i = tid; i < R; i += nt) M[(long long)i * Cc + k] = (i == k) ? make_double2(1.0, 0.0) : zero;
     __syncthreads();
   }
+  frontStore(cd, f, factor, schur);
+}
 
-  double2* W = factor + cd.factorOff + nd.fwdOff;
-  double2* K = factor + cd.factorOff + nd.bwdOff;
-  for (long long idx = tid; idx < (long long)b * e; idx += nt) {
-    const int k = (int)(idx / b), t = (int)(idx - (long long)k * b);
-    W[idx] = cneg(M[(long long)(e + t) * Cc + e + b + k]);
+// ---- blocked schedule for the large fronts
+// Per front, after the front itself (R x Cc) the workspace holds T (R x
+// kPanel, row-major) and B (kPanel x Cc, row-major).
+constexpr int kPanel = 16;
+
+__global__ __launch_bounds__(256) void k_front_assemble(const CellDev* __restrict__ cells,
+                                                        const ShapeDev* __restrict__ shapes,
+                                                        const FrontTask* __restrict__ fts,
+                                                        const double2* __restrict__ coef,
+                                                        const double2* __restrict__ schur, double2* __restrict__ ws) {
+  const FrontTask ft = fts[blockIdx.x];
+  const CellDev& cd = cells[ft.cell];
+  frontAssemble(cd, shapes[cd.shape], frontView(cells, shapes, ft, ws), coef, schur);
+}
+
+__global__ __launch_bounds__(256) void k_front_store(const CellDev* __restrict__ cells,
+                                                     const ShapeDev* __restrict__ shapes,
+                                                     const FrontTask* __restrict__ fts, double2* __restrict__ factor,
+                                                     double2* __restrict__ schur, double2* __restrict__ ws) {
+  const FrontTask ft = fts[blockIdx.x];
+  frontStore(cells[ft.cell], frontView(cells, shapes, ft, ws), factor, schur);
+}
+
+// Pivots k0 .. k0+nbk-1 of one front.  Gauss-Jordan with partial pivoting
+// on the panel columns only, with the row swaps applied to whole rows (from
+// column k0 on) and the eliminations recorded in T: T column c starts as the
+// unit vector of pivot row k0+c when that pivot is reached and takes every
+// later scaling, swap and elimination, so that at the end, in the swapped
+// row order, T = [A11^{-1}; -A21 A11^{-1}] (A11 = the pivot rows' panel
+// block, A21 = the other rows').  The rest of the front is then
+//   new = old + (T - E) B,  B = old pivot rows (columns after the panel),
+// E = 1 at (pivot row c, c): the batched update below.  B is copied here,
+// before that update changes the pivot rows.
+__global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__ cells,
+                                                     const ShapeDev* __restrict__ shapes,
+                                                     const FrontTask* __restrict__ fts, double2* __restrict__ ws,
+                                                     int k0, int* __restrict__ info) {
+  extern __shared__ double2 colk[];
+  __shared__ double redv[256];
+  __shared__ int redi[256];
+  const FrontTask ft = fts[blockIdx.x];
+  const FrontView f = frontView(cells, shapes, ft, ws);
+  const int e = f.e, R = f.R, Cc = f.Cc;
+  const int nbk = min(kPanel, e - k0);
+  if (nbk <= 0) return;
+  double2* M = f.M;
+  double2* T = M + (long long)R * Cc;
+  double2* B = T + (long long)R * kPanel;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const double2 zero = make_double2(0.0, 0.0);
+  const int pend = k0 + nbk;  // panel columns [k0, pend)
+  for (int idx = tid; idx < R * kPanel; idx += nt) T[idx] = zero;
+  __syncthreads();
+  for (int kk = 0; kk < nbk; ++kk) {
+    const int k = k0 + kk;
+    double pmag;
+    const int p = pivotRow(M, Cc, k, e, redv, redi, pmag);
+    if (p != k) {
+      for (int j = k0 + tid; j < Cc; j += nt) {
+        const double2 t = M[(long long)k * Cc + j];
+        M[(long long)k * Cc + j] = M[(long long)p * Cc + j];
+        M[(long long)p * Cc + j] = t;
+      }
+      for (int c = tid; c < kk; c += nt) {
+        const double2 t = T[k * kPanel + c];
+        T[k * kPanel + c] = T[p * kPanel + c];
+        T[p * kPanel + c] = t;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) T[k * kPanel + kk] = make_double2(1.0, 0.0);
+    double2 piv = M[(long long)k * Cc + k];
+    if (!(pmag > 0.0) || !isfinite(pmag)) {
+      if (tid == 0) atomicMax(info, 1);
+      piv = make_double2(1.0, 0.0);
+    }
+    const double2 inv = cinv(piv);
+    __syncthreads();
+    // scale the pivot row (panel columns after k, and T)
+    for (int j = tid; j < (pend - k - 1) + (kk + 1); j += nt) {
+      double2* t = j < pend - k - 1 ? &M[(long long)k * Cc + k + 1 + j] : &T[k * kPanel + (j - (pend - k - 1))];
+      *t = cmul(*t, inv);
+    }
+    for (int i = tid; i < R; i += nt) colk[i] = (i == k) ? zero : M[(long long)i * Cc + k];
+    __syncthreads();
+    // eliminate column k from every other row (panel columns after k, and T)
+    const int w = (pend - k - 1) + (kk + 1);
+    for (long long idx = tid; idx < (long long)R * w; idx += nt) {
+      const int i = (int)(idx / w), j = (int)(idx - (long long)i * w);
+      if (i == k) continue;
+      const double2 fi = colk[i];
+      if (fi.x == 0.0 && fi.y == 0.0) continue;
+      if (j < pend - k - 1) {
+        double2* t = &M[(long long)i * Cc + k + 1 + j];
+        *t = cfms(fi, M[(long long)k * Cc + k + 1 + j], *t);
+      } else {
+        const int c = j - (pend - k - 1);
+        double2* t = &T[i * kPanel + c];
+        *t = cfms(fi, T[k * kPanel + c], *t);
+      }
+    }
+    for (int i = tid; i < R; i += nt) M[(long long)i * Cc + k] = (i == k) ? make_double2(1.0, 0.0) : zero;
+    __syncthreads();
   }
-  for (long long idx = tid; idx < (long long)e * R; idx += nt) {
-    const int k = (int)(idx / e), r = (int)(idx - (long long)k * e);
-    K[idx] = (k < e) ? M[(long long)r * Cc + e + b + k] : cneg(M[(long long)r * Cc + k]);
+  // B = the pivot rows from the first column after the panel; T -= E
+  const int cw = Cc - pend;
+  for (long long idx = tid; idx < (long long)nbk * cw; idx += nt) {
+    const int c = (int)(idx / cw), j = (int)(idx - (long long)c * cw);
+    B[(long long)c * Cc + j] = M[(long long)(k0 + c) * Cc + pend + j];
   }
-  if (b > 0) {
-    double2* S = schur + cd.schurOff + nd.sOff;
-    for (long long idx = tid; idx < (long long)b * b; idx += nt) {
-      const int t1 = (int)(idx / b), t2 = (int)(idx - (long long)t1 * b);
-      S[idx] = M[(long long)(e + t1) * Cc + e + t2];
+  for (int c = tid; c < nbk; c += nt) {
+    double2* t = &T[(k0 + c) * kPanel + c];
+    t->x -= 1.0;
+  }
+}
+
+// new = old + (T - E) B over rows [0, R) and columns [pend, Cc) of every
+// front: 64 x 64 output tiles, 4 x 4 per thread, the panel's T and B slices
+// staged in LDS.  blockIdx.y = front, blockIdx.x = tile.
+__global__ __launch_bounds__(256) void k_front_trail(const CellDev* __restrict__ cells,
+                                                     const ShapeDev* __restrict__ shapes,
+                                                     const FrontTask* __restrict__ fts, double2* __restrict__ ws,
+                                                     int k0) {
+  __shared__ double2 sT[64][kPanel + 1];
+  __shared__ double2 sB[kPanel][64 + 1];
+  const FrontTask ft = fts[blockIdx.y];
+  const FrontView f = frontView(cells, shapes, ft, ws);
+  const int e = f.e, R = f.R, Cc = f.Cc;
+  const int nbk = min(kPanel, e - k0);
+  if (nbk <= 0) return;
+  const int pend = k0 + nbk, cw = Cc - pend;
+  const int tilesC = (cw + 63) / 64, tilesR = (R + 63) / 64;
+  const int tile = blockIdx.x;
+  if (tile >= tilesC * tilesR) return;
+  const int r0 = (tile / tilesC) * 64, c0 = (tile % tilesC) * 64;
+  double2* M = f.M;
+  const double2* T = M + (long long)R * Cc;
+  const double2* B = T + (long long)R * kPanel;
+  const int tid = threadIdx.x;
+  const double2 zero = make_double2(0.0, 0.0);
+  for (int idx = tid; idx < 64 * kPanel; idx += 256) {
+    const int i = idx / kPanel, c = idx - i * kPanel;
+    sT[i][c] = (r0 + i < R && c < nbk) ? T[(r0 + i) * kPanel + c] : zero;
+  }
+  for (int idx = tid; idx < kPanel * 64; idx += 256) {
+    const int c = idx / 64, j = idx - c * 64;
+    sB[c][j] = (c < nbk && c0 + j < cw) ? B[(long long)c * Cc + c0 + j] : zero;
+  }
+  __syncthreads();
+  const int tx = tid & 15, ty = tid >> 4;
+  double2 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[a][q] = zero;
+  for (int c = 0; c < nbk; ++c) {
+    double2 tv[4], bv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) tv[a] = sT[ty * 4 + a][c];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bv[q] = sB[c][tx * 4 + q];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[a][q] = cfma(tv[a], bv[q], acc[a][q]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = r0 + ty * 4 + a;
+    if (i >= R) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = c0 + tx * 4 + q;
+      if (j >= cw) continue;
+      double2* t = &M[(long long)i * Cc + pend + j];
+      *t = cadd(*t, acc[a][q]);
     }
   }
 }
@@ -349,6 +583,21 @@ void FactorSet::setup(hipStream_t s) {
   }
 }
 
+namespace {
+// Heights whose largest front has at least this many rows take the blocked
+// schedule.  Default 1 (every height): on the 1600^2 lens workload the
+// factorisation took 0.39 s with one workgroup per front everywhere, 0.20 s
+// with the blocked schedule from 192 rows up and 0.149 s with it on every
+// height.  LSW_FACTOR_BLOCKED_ROWS overrides it for tuning.
+int blockedMinRows() {
+  static const int v = [] {
+    const char* e = std::getenv("LSW_FACTOR_BLOCKED_ROWS");
+    return e ? std::max(1, std::atoi(e)) : 1;
+  }();
+  return v;
+}
+}  // namespace
+
 void FactorSet::factorize(int precision, hipStream_t s) {
   LSW_REQUIRE(dCells != nullptr, LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(precision == LSW_FP64 || precision == LSW_FP32, LSW_ERR_ARG, "unknown precision");
@@ -367,12 +616,22 @@ void FactorSet::factorize(int precision, hipStream_t s) {
   int* dInfo = nullptr;
   LSW_HIP(devMalloc(&dInfo, sizeof(int)));
   LSW_HIP(hipMemsetAsync(dInfo, 0, sizeof(int), s));
-  // front batches per height, bounded by a workspace budget
+  // front batches per height, bounded by a workspace budget; a height whose
+  // largest front has at least blockedMinRows() rows takes the blocked
+  // schedule (panels + batched updates), which also needs T and B per front
   const long long budget = 512LL << 20;  // complex entries (8 GiB)
   std::vector<std::vector<FrontTask>> batches;
   std::vector<long long> batchWs;
+  std::vector<char> batchBlocked;
   long long maxWs = 0;
   for (int h = 0; h <= mMaxHeight; ++h) {
+    int maxR = 0;
+    for (size_t ci = 0; ci < mCells.size(); ++ci) {
+      const NdTree& t = *mTrees[mCells[ci].shape];
+      if (h > t.maxHeight) continue;
+      for (int p : t.byHeight[h]) maxR = std::max(maxR, t.nodes[p].e + t.nodes[p].b);
+    }
+    const bool blocked = maxR >= blockedMinRows();
     std::vector<FrontTask> cur;
     long long ws = 0;
     for (size_t ci = 0; ci < mCells.size(); ++ci) {
@@ -380,10 +639,12 @@ void FactorSet::factorize(int precision, hipStream_t s) {
       if (h > t.maxHeight) continue;
       for (int p : t.byHeight[h]) {
         const NdNode& nd = t.nodes[p];
-        const long long need = (long long)(nd.e + nd.b) * (2 * nd.e + nd.b);
+        const long long R = nd.e + nd.b, Cc = 2 * nd.e + nd.b;
+        const long long need = R * Cc + (blocked ? R * kPanel + kPanel * Cc : 0);
         if (!cur.empty() && ws + need > budget) {
           batches.push_back(cur);
           batchWs.push_back(ws);
+          batchBlocked.push_back(blocked);
           maxWs = std::max(maxWs, ws);
           cur.clear();
           ws = 0;
@@ -395,11 +656,14 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     if (!cur.empty()) {
       batches.push_back(cur);
       batchWs.push_back(ws);
+      batchBlocked.push_back(blocked);
       maxWs = std::max(maxWs, ws);
     }
     batches.push_back({});  // height separator (keeps order explicit)
     batchWs.push_back(0);
+    batchBlocked.push_back(0);
   }
+  auto nodeOf = [&](const FrontTask& ft) -> const NdNode& { return mTrees[mCells[ft.cell].shape]->nodes[ft.node]; };
   double2* dWs = nullptr;
   LSW_HIP(devMalloc(&dWs, std::max<long long>(maxWs, 1) * sizeof(double2)));
   FrontTask* dFt = nullptr;
@@ -417,8 +681,29 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     }
     LSW_HIP(hipMemcpyAsync(dFt, bt.data(), bt.size() * sizeof(FrontTask), hipMemcpyHostToDevice, s));
     const size_t shm = (size_t)mMaxFront * sizeof(double2);
-    hipLaunchKernelGGL(k_front_factor, dim3((unsigned)bt.size()), dim3(256), shm, s, dCells, dShapes,
-                       dFt, dCoef, dFactor, dSchur, dWs, dInfo);
+    const unsigned nf = (unsigned)bt.size();
+    if (!batchBlocked[bi]) {
+      hipLaunchKernelGGL(k_front_factor, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dCoef, dFactor, dSchur,
+                         dWs, dInfo);
+    } else {
+      hipLaunchKernelGGL(k_front_assemble, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dCoef, dSchur, dWs);
+      int maxE = 0;
+      for (const auto& ft : bt) maxE = std::max(maxE, nodeOf(ft).e);
+      for (int k0 = 0; k0 < maxE; k0 += kPanel) {
+        hipLaunchKernelGGL(k_front_panel, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dWs, k0, dInfo);
+        long long tiles = 0;
+        for (const auto& ft : bt) {
+          const NdNode& nd = nodeOf(ft);
+          if (nd.e <= k0) continue;
+          const long long R = nd.e + nd.b, cw = 2LL * nd.e + nd.b - std::min(k0 + kPanel, nd.e);
+          tiles = std::max(tiles, ((R + 63) / 64) * ((cw + 63) / 64));
+        }
+        if (tiles > 0)
+          hipLaunchKernelGGL(k_front_trail, dim3((unsigned)tiles, nf), dim3(256), 0, s, dCells, dShapes, dFt, dWs,
+                             k0);
+      }
+      hipLaunchKernelGGL(k_front_store, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dFactor, dSchur, dWs);
+    }
     LSW_HIP(hipGetLastError());
     LSW_HIP(hipStreamSynchronize(s));  // the host task buffer is reused
   }
