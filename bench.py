@@ -287,7 +287,9 @@ def serve_bench(args):
                           worst_true_residual=max(r["max_true_residual"] for r in res),
                           mean_interarrival_s=scale,
                           **{k: stats[k] - prev[k] for k in ("loads", "evictions", "steps", "merged_steps")},
-                          max_active=stats["max_active"], peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
+                          max_active=stats["max_active"], peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9,
+                          worker_s={k: round(stats[k] - prev[k], 3) for k in ("setup_s", "factor_s", "block_s",
+                                                                              "step_s")})
         prev = stats
         log(label, out[label])
         for r in bad[:5]:
@@ -348,7 +350,8 @@ def pack_bench(args):
                   worst_true_residual=max(r["max_true_residual"] for r in res),
                   dofs=int(sum(p["grid"][0] * p["grid"][1] for p in probs)),
                   **{k: stats[k] for k in ("loads", "evictions", "steps")},
-                  peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9)
+                  peak_reserved_gb=stats["peak_reserved_bytes"] / 1e9,
+                  worker_s={k: round(stats[k], 3) for k in ("setup_s", "factor_s", "block_s", "step_s")})
     log("packed", packed)
     for r in res:
         if r["status"] != 0:

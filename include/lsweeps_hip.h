@@ -241,6 +241,9 @@ typedef struct lsw_server_stats {
   int resident;
   long long merged_steps; /* steps whose right-hand sides came from >= 2 requests */
   int max_active;         /* most right-hand sides in flight in one step */
+  /* worker seconds summed over the workers: problem set-up (trees, schedule,
+   * plans, assembly), factorisation, block plans + Krylov state, solve steps */
+  double setup_s, factor_s, block_s, step_s;
 } lsw_server_stats;
 
 int lsw_server_create(const lsw_server_options* opt, lsw_server** out);

@@ -222,7 +222,7 @@ void hostApply(lsw_dd* dd, const double* x, double* y, bool precond) {
   LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
   LSW_HIP(devMalloc(&dy, n * sizeof(double2)));
   try {
-    LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
+    LSW_HIP(uploadSync(dx, x, n * sizeof(double2)));
     if (precond)
       dd->dd->applyPrecondDev(dx, dy, dd->dd->stream(), 1, false);
     else
@@ -261,7 +261,7 @@ int lsw_dd_apply_precond_block(lsw_dd* dd, int nrhs, const double* x, double* y)
     LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
     LSW_HIP(devMalloc(&dy, n * sizeof(double2)));
     try {
-      LSW_HIP(hipMemcpy(dx, x, n * sizeof(double2), hipMemcpyHostToDevice));
+      LSW_HIP(uploadSync(dx, x, n * sizeof(double2)));
       dd->dd->applyPrecondDev(dx, dy, dd->dd->stream(), nrhs, false);
       LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
       LSW_HIP(hipMemcpy(y, dy, n * sizeof(double2), hipMemcpyDeviceToHost));
@@ -366,7 +366,7 @@ int lsw_gmres_solve(lsw_dd* dd, const lsw_gmres_params* p, const double* b, doub
     LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
     GmresInfo info;
     try {
-      LSW_HIP(hipMemcpy(db, b, n * sizeof(double2), hipMemcpyHostToDevice));
+      LSW_HIP(uploadSync(db, b, n * sizeof(double2)));
       dd->gmres.solveDev(db, dx, &info, dd->dd->stream());
       LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
       LSW_HIP(hipMemcpy(x, dx, n * sizeof(double2), hipMemcpyDeviceToHost));
@@ -417,7 +417,7 @@ int lsw_gmres_solve_block(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const
     LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
     std::vector<GmresInfo> infos;
     try {
-      LSW_HIP(hipMemcpy(db, B, n * sizeof(double2), hipMemcpyHostToDevice));
+      LSW_HIP(uploadSync(db, B, n * sizeof(double2)));
       dd->gmres.solveBlockDev(db, dx, nrhs, &infos, dd->dd->stream());
       LSW_HIP(hipStreamSynchronize(dd->dd->stream()));
       LSW_HIP(hipMemcpy(X, dx, n * sizeof(double2), hipMemcpyDeviceToHost));
@@ -607,6 +607,10 @@ int lsw_server_get_stats(lsw_server* s, lsw_server_stats* st) {
     st->resident = v.resident;
     st->merged_steps = v.mergedSteps;
     st->max_active = v.maxActive;
+    st->setup_s = v.setupS;
+    st->factor_s = v.factorS;
+    st->block_s = v.blockS;
+    st->step_s = v.stepS;
   });
 }
 

@@ -22,6 +22,7 @@
 #include <map>
 #include <tuple>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "lsw_factor.h"
@@ -221,6 +222,7 @@ class DDCheckerboard : public DD {
   mutable void* dT = nullptr;
   mutable double2* dY = nullptr;  // 4 accumulators of N entries per right-hand side
   mutable void* dF = nullptr;     // summed sweep right-hand sides per (d, cell) and right-hand side
+  std::vector<std::pair<std::string, double>> mSetupLog;  // set-up phase times (LSW_SETUP_LOG prints them)
   mutable int mBufRhs = 0;
   long long mXs = 0, mGs = 0;
 

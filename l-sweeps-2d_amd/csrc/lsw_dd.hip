@@ -2,6 +2,8 @@
 // L-sweeps preconditioner (:687-2608) and the global operator (:2611-2635).
 #include "lsw_dd.h"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -319,8 +321,13 @@ void sevenView(const int s[12], int n, int v[2][8]) {
 void DDCheckerboard::setUp(const Assembler* inputAssembler, const LocalLinearSolver* solver,
                            std::vector<DoubleVector> domain, std::vector<UInt> nCells,
                            std::vector<UInt> nElsInCell) {
+  const auto t0 = std::chrono::steady_clock::now();
   setUpGeometry(inputAssembler, solver, domain, nCells, nElsInCell);
+  mSetupLog.emplace_back("geometry", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   initDevice();
+  if (std::getenv("LSW_SETUP_LOG")) {
+    for (const auto& kv : mSetupLog) std::fprintf(stderr, "setup %-10s %.4f s\n", kv.first.c_str(), kv.second);
+  }
 }
 
 void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalLinearSolver* solver,
@@ -511,11 +518,14 @@ void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalL
 }
 
 void DDCheckerboard::initDevice() {
-  if (!mStream) LSW_HIP(hipStreamCreate(&mStream));
+  // non-blocking: the synchronous copies of other problems' set-up (on the
+  // legacy stream, from other server workers) must not wait for this
+  // problem's solves
+  if (!mStream) LSW_HIP(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking));
   LSW_HIP(devMalloc(&dColTrace, mGx));
   LSW_HIP(devMalloc(&dRowTrace, mGy));
-  LSW_HIP(hipMemcpy(dColTrace, mColT.data(), mGx, hipMemcpyHostToDevice));
-  LSW_HIP(hipMemcpy(dRowTrace, mRowT.data(), mGy, hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(dColTrace, mColT.data(), mGx));
+  LSW_HIP(uploadSync(dRowTrace, mRowT.data(), mGy));
 
   auto toDev = [&](const CellGeom& c) {
     CellDev d;
@@ -539,6 +549,12 @@ void DDCheckerboard::initDevice() {
     d.omega = c.asmb.getOmega();
     return d;
   };
+  auto lap = [&](const char* what, std::chrono::steady_clock::time_point& t) {
+    const auto n = std::chrono::steady_clock::now();
+    mSetupLog.emplace_back(what, std::chrono::duration<double>(n - t).count());
+    t = n;
+  };
+  auto t = std::chrono::steady_clock::now();
   mFactors = std::make_unique<FactorSet>();
   mFactors->setSolvePrecision(mPrecision);
   for (auto& c : mCells) {
@@ -546,7 +562,9 @@ void DDCheckerboard::initDevice() {
     c.asmb.squaredSlowness(m);
     mFactors->addCell(toDev(c), m);
   }
+  lap("media", t);
   mFactors->setup(mStream);
+  lap("trees", t);
   {
     std::vector<double> m;
     mGlobal.asmb.squaredSlowness(m);
@@ -555,8 +573,11 @@ void DDCheckerboard::initDevice() {
     mGlobalDev.coefOff = 0;
     dGlobalCoef = assembleStencilDev(mGlobalDev, m, mStream);
   }
+  lap("global", t);
   buildSchedule();
+  lap("schedule", t);
   allocBuffers();
+  lap("buffers", t);
 }
 
 // ------------------------------------------------------------ the schedule
@@ -879,7 +900,7 @@ void DDCheckerboard::buildSchedule() {
     st.nCopies = (int)hst.copies.size();
     LSW_REQUIRE(st.nSlots <= 65535 && st.nCopies <= 65535 * 8, LSW_ERR_CAPACITY, "board too large for one launch");
     LSW_HIP(devMalloc(&st.dSlots, std::max<size_t>(1, hst.slots.size()) * sizeof(SlotDev)));
-    LSW_HIP(hipMemcpy(st.dSlots, hst.slots.data(), hst.slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice));
+    LSW_HIP(uploadSync(st.dSlots, hst.slots.data(), hst.slots.size() * sizeof(SlotDev)));
     {
       const SolvePlan& pl = planFor(st, 1);
       mSolveBytesPerPrecond += pl.fwdBytes + pl.bwdBytes;
@@ -887,10 +908,10 @@ void DDCheckerboard::buildSchedule() {
       st.maxNrhs = pl.maxNrhs;
     }
     LSW_HIP(devMalloc(&st.dGroups, std::max<size_t>(1, groups.size()) * sizeof(GroupDev)));
-    LSW_HIP(hipMemcpy(st.dGroups, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice));
+    LSW_HIP(uploadSync(st.dGroups, groups.data(), groups.size() * sizeof(GroupDev)));
     if (!hst.copies.empty()) {
       LSW_HIP(devMalloc(&st.dCopies, hst.copies.size() * sizeof(CopyOp)));
-      LSW_HIP(hipMemcpy(st.dCopies, hst.copies.data(), hst.copies.size() * sizeof(CopyOp), hipMemcpyHostToDevice));
+      LSW_HIP(uploadSync(st.dCopies, hst.copies.data(), hst.copies.size() * sizeof(CopyOp)));
     }
     mSteps.push_back(std::move(st));
   }
@@ -958,10 +979,11 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   LSW_HIP(devMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
   LSW_HIP(devMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
   // the solve kernels read (and multiply by zero) entries of X, R and G they
-  // do not use: they must hold finite values from the start
-  LSW_HIP(hipMemset(dX, 0, slots * mXs * es));
-  LSW_HIP(hipMemset(dR, 0, slots * mXs * es));
-  LSW_HIP(hipMemset(dG, 0, slots * mGs * es));
+  // do not use: they must hold finite values from the start (cleared on the
+  // problem's stream, ahead of its kernels)
+  LSW_HIP(hipMemsetAsync(dX, 0, slots * mXs * es, mStream));
+  LSW_HIP(hipMemsetAsync(dR, 0, slots * mXs * es, mStream));
+  LSW_HIP(hipMemsetAsync(dG, 0, slots * mGs * es, mStream));
   mBufRhs = nrhs;
 }
 
@@ -1247,7 +1269,7 @@ void DDCheckerboard::applyPrecond(const std::vector<ComplexVector>& x, std::vect
   std::vector<Complex> g;
   scatterOwned(x, g);
   DevBuf dx(N()), dy(N());
-  LSW_HIP(hipMemcpy(dx.p, g.data(), N() * sizeof(double2), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(dx.p, g.data(), N() * sizeof(double2)));
   applyPrecondDev(dx.p, dy.p, mStream, 1, false);
   LSW_HIP(hipStreamSynchronize(mStream));
   LSW_HIP(hipMemcpy(g.data(), dy.p, N() * sizeof(double2), hipMemcpyDeviceToHost));
@@ -1258,7 +1280,7 @@ void DDCheckerboard::applySystemMatrix(const std::vector<ComplexVector>& x, std:
   std::vector<Complex> g;
   scatterOwned(x, g);
   DevBuf dx(N()), dy(N());
-  LSW_HIP(hipMemcpy(dx.p, g.data(), N() * sizeof(double2), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(dx.p, g.data(), N() * sizeof(double2)));
   applyADev(dx.p, dy.p, mStream);
   LSW_HIP(hipStreamSynchronize(mStream));
   LSW_HIP(hipMemcpy(g.data(), dy.p, N() * sizeof(double2), hipMemcpyDeviceToHost));

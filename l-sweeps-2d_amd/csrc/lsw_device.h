@@ -23,6 +23,13 @@ namespace lsw {
 // that problem.
 hipError_t devMallocBytes(void** p, size_t bytes);
 hipError_t devFree(void* p);
+// Copy host memory to the device and return once the data is in device
+// memory (an asynchronous copy on the calling thread's own non-blocking
+// transfer stream, then a wait on that stream): kernels launched afterwards
+// on any stream see it.  A plain hipMemcpy from pageable memory may return
+// as soon as the data is staged, and only kernels on blocking streams are
+// ordered behind it.
+hipError_t uploadSync(void* dst, const void* src, size_t bytes);
 long long threadDeviceBytes();
 long long totalDeviceBytes();
 template <typename T>

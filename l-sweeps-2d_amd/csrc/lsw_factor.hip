@@ -415,7 +415,7 @@ T* devUpload(const std::vector<T>& h, std::vector<void*>& allocs) {
   if (h.empty()) return nullptr;
   T* d = nullptr;
   LSW_HIP(devMalloc(&d, h.size() * sizeof(T)));
-  LSW_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(d, h.data(), h.size() * sizeof(T)));
   allocs.push_back(d);
   return d;
 }
@@ -554,21 +554,20 @@ void FactorSet::setup(hipStream_t s) {
     dPoolMap = devUpload(pm, mShapeAllocs);
   }
   LSW_HIP(devMalloc(&dShapes, shapes.size() * sizeof(ShapeDev)));
-  LSW_HIP(hipMemcpy(dShapes, shapes.data(), shapes.size() * sizeof(ShapeDev), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(dShapes, shapes.data(), shapes.size() * sizeof(ShapeDev)));
   LSW_HIP(devMalloc(&dCells, mCells.size() * sizeof(CellDev)));
-  LSW_HIP(hipMemcpy(dCells, mCells.data(), mCells.size() * sizeof(CellDev), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(dCells, mCells.data(), mCells.size() * sizeof(CellDev)));
   // stencils
   LSW_HIP(devMalloc(&dCoef, co * sizeof(double2)));
   for (size_t i = 0; i < mCells.size(); ++i)
     if (!mCoefHost[i].empty())
-      LSW_HIP(hipMemcpy(dCoef + mCells[i].coefOff, mCoefHost[i].data(),
-                        mCoefHost[i].size() * sizeof(double2), hipMemcpyHostToDevice));
+      LSW_HIP(uploadSync(dCoef + mCells[i].coefOff, mCoefHost[i].data(),
+                        mCoefHost[i].size() * sizeof(double2)));
   if (mo > 0) {
     LSW_HIP(devMalloc(&dM, mo * sizeof(double)));
     for (size_t i = 0; i < mCells.size(); ++i)
       if (!mM[i].empty())
-        LSW_HIP(hipMemcpy(dM + mCells[i].mOff, mM[i].data(), mM[i].size() * sizeof(double),
-                          hipMemcpyHostToDevice));
+        LSW_HIP(uploadSync(dM + mCells[i].mOff, mM[i].data(), mM[i].size() * sizeof(double)));
     int* dBad = nullptr;
     LSW_HIP(devMalloc(&dBad, sizeof(int)));
     LSW_HIP(hipMemsetAsync(dBad, 0, sizeof(int), s));
@@ -610,7 +609,7 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     so += mTrees[mCells[i].shape]->schurSize;
     mCells[i].schurOff = so - mTrees[mCells[i].shape]->schurSize;
   }
-  LSW_HIP(hipMemcpy(dCells, mCells.data(), mCells.size() * sizeof(CellDev), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(dCells, mCells.data(), mCells.size() * sizeof(CellDev)));
   double2* dSchur = nullptr;
   LSW_HIP(devMalloc(&dSchur, std::max<long long>(so, 1) * sizeof(double2)));
   int* dInfo = nullptr;
@@ -744,8 +743,8 @@ double2* assembleStencilDev(const CellDev& c, const std::vector<double>& m, hipS
   LSW_HIP(devMalloc(&dM, N * sizeof(double)));
   LSW_HIP(devMalloc(&dCoef, 5 * N * sizeof(double2)));
   LSW_HIP(devMalloc(&dBad, sizeof(int)));
-  LSW_HIP(hipMemcpy(dC, &cd, sizeof(CellDev), hipMemcpyHostToDevice));
-  LSW_HIP(hipMemcpy(dM, m.data(), N * sizeof(double), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(dC, &cd, sizeof(CellDev)));
+  LSW_HIP(uploadSync(dM, m.data(), N * sizeof(double)));
   LSW_HIP(hipMemsetAsync(dBad, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_assemble, dim3((unsigned)std::min<long long>((N + 255) / 256, 8192), 1), dim3(256), 0, s,
                      dC, dM, dCoef, dBad);

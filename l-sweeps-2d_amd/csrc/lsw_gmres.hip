@@ -372,7 +372,7 @@ void GlobalIterativeSolverGMRESHIP::solve(const std::vector<ComplexVector>& rhs,
   double2 *db = nullptr, *dx = nullptr;
   LSW_HIP(devMalloc(&db, n * sizeof(double2)));
   LSW_HIP(devMalloc(&dx, n * sizeof(double2)));
-  LSW_HIP(hipMemcpy(db, g.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(db, g.data(), n * sizeof(double2)));
   try {
     solveDev(db, dx, nullptr, dd->stream());
     LSW_HIP(hipStreamSynchronize(dd->stream()));

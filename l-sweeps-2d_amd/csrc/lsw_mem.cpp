@@ -39,6 +39,18 @@ hipError_t devFree(void* p) {
   return hipFree(p);
 }
 
+hipError_t uploadSync(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  thread_local hipStream_t ts = nullptr;  // one per thread, kept for the thread's life
+  if (!ts) {
+    const hipError_t e = hipStreamCreateWithFlags(&ts, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+  }
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ts);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ts);
+}
+
 long long threadDeviceBytes() { return tBalance; }
 long long totalDeviceBytes() { return gTotal.load(); }
 

@@ -641,9 +641,9 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   plan.dTasks = nullptr;
   plan.dEntries = nullptr;
   LSW_HIP(devMalloc(&plan.dTasks, tdev.size() * sizeof(SolveTaskDev)));
-  LSW_HIP(hipMemcpy(plan.dTasks, tdev.data(), tdev.size() * sizeof(SolveTaskDev), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(plan.dTasks, tdev.data(), tdev.size() * sizeof(SolveTaskDev)));
   LSW_HIP(devMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(SolveEntry)));
-  LSW_HIP(hipMemcpy(plan.dEntries, all.data(), all.size() * sizeof(SolveEntry), hipMemcpyHostToDevice));
+  LSW_HIP(uploadSync(plan.dEntries, all.data(), all.size() * sizeof(SolveEntry)));
 }
 
 namespace {

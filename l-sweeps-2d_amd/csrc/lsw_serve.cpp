@@ -271,19 +271,27 @@ void Server::workerLoop() {
     }
     lk.unlock();
     const long long bytes0 = threadDeviceBytes();
+    double tSetup = 0, tFactor = 0, tBlock = 0, tStep = 0;
     int err = LSW_OK;
     std::string msg;
     std::vector<std::pair<Column, GmresSlots::Done>> finished;
     std::vector<std::pair<long long, Column>> admitted;
     try {
       if (load) {
+        const double t0 = now();
         auto dd = makeCheckerboard(e.params, true);
         dd->setGraphs(false);  // workers run concurrently: no stream capture
+        const double t1 = now();
         dd->factorize();
+        const double t2 = now();
         dd->reserveBlock(planWidth(mOpt.slots));
         e.slots = std::make_unique<GmresSlots>(dd.get(), mOpt.slots, mOpt.restart, mOpt.tol, mOpt.maxIter);
         e.dd = std::move(dd);
+        tSetup = t1 - t0;
+        tFactor = t2 - t1;
+        tBlock = now() - t2;
       }
+      const double ts = now();
       DDCheckerboard& dd = *e.dd;
       GmresSlots& gs = *e.slots;
       for (size_t k = 0; k < take.size(); ++k) {
@@ -293,6 +301,7 @@ void Server::workerLoop() {
       }
       done.clear();
       gs.step(done);
+      tStep = now() - ts;
       for (const auto& d : done) {
         Column c{-1, -1};
         {
@@ -328,6 +337,10 @@ void Server::workerLoop() {
     }
     const long long grown = threadDeviceBytes() - bytes0;
     lk.lock();
+    mStats.setupS += tSetup;
+    mStats.factorS += tFactor;
+    mStats.blockS += tBlock;
+    mStats.stepS += tStep;
     e.busy = false;
     e.lastUse = ++mClock;
     e.actual += (double)grown;

@@ -74,6 +74,10 @@ struct ServeStats {
   int resident = 0;
   long long mergedSteps = 0;  // steps whose right-hand sides came from >= 2 requests
   int maxActive = 0;          // most right-hand sides in one step
+  // worker seconds, summed over the workers: building a problem's
+  // checkerboard (trees, schedule, plans, assembly), factorising it, making
+  // its block plans and Krylov state, and running solve steps
+  double setupS = 0, factorS = 0, blockS = 0, stepS = 0;
 };
 
 class Server {

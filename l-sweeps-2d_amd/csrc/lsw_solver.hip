@@ -50,7 +50,7 @@ void LocalLinearSolverHIP::setMatrix(ComplexSparseMatrix* A) {
 void LocalLinearSolverHIP::factorize() {
   LSW_REQUIRE(mA != nullptr, LSW_ERR_STATE, "factorize called before setMatrix");
   if (mFS && mFS->factorized()) return;  // idempotent (src/LocalLinearSolverEigenSparseLU.cpp:42-45)
-  if (!mStream) LSW_HIP(hipStreamCreate(&mStream));
+  if (!mStream) LSW_HIP(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking));
   const int N = mA->rows, ny = mNy;
   std::vector<double2> coef((size_t)5 * N, make_double2(0.0, 0.0));
   for (int d = 0; d < N; ++d)
@@ -87,9 +87,9 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
     LSW_HIP(devMalloc(&dG, (size_t)nrhs * G * es));
     // the solve kernels read (and multiply by zero) entries of X, R and G
     // they do not use: they must hold finite values from the start
-    LSW_HIP(hipMemset(dX, 0, (size_t)nrhs * N * es));
-    LSW_HIP(hipMemset(dR, 0, (size_t)nrhs * N * es));
-    LSW_HIP(hipMemset(dG, 0, (size_t)nrhs * G * es));
+    LSW_HIP(hipMemsetAsync(dX, 0, (size_t)nrhs * N * es, mStream));
+    LSW_HIP(hipMemsetAsync(dR, 0, (size_t)nrhs * N * es, mStream));
+    LSW_HIP(hipMemsetAsync(dG, 0, (size_t)nrhs * G * es, mStream));
     LSW_HIP(devMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
     mCap = nrhs;
   }

@@ -96,7 +96,8 @@ class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_longlong), ("completed", C.c_longlong), ("solves", C.c_longlong),
                 ("loads", C.c_longlong), ("evictions", C.c_longlong), ("steps", C.c_longlong),
                 ("reserved_bytes", C.c_double), ("peak_reserved_bytes", C.c_double), ("resident", C.c_int),
-                ("merged_steps", C.c_longlong), ("max_active", C.c_int)]
+                ("merged_steps", C.c_longlong), ("max_active", C.c_int), ("setup_s", C.c_double),
+                ("factor_s", C.c_double), ("block_s", C.c_double), ("step_s", C.c_double)]
 
 
 _lib.lsw_last_error.restype = C.c_char_p
