@@ -216,7 +216,8 @@ class DDCheckerboard : public DD {
   double mSolveBytesPerPrecond = 0;
 
   // buffers (element type follows precision), sized for mBufRhs right-hand sides
-  mutable void* dX = nullptr;
+  mutable void* dX = nullptr;   // right-hand sides of the local solves (slot-major)
+  mutable void* dXs = nullptr;  // their solutions (same layout)
   mutable void* dR = nullptr;
   mutable void* dG = nullptr;
   mutable void* dT = nullptr;

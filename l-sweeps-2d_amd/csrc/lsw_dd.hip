@@ -29,22 +29,93 @@ enum Ctx { kS1 = 0, kR = 1, kL = 2, kU = 3, kD = 4, kBL = 5, kTR = 6, kBR = 7, k
 // Slot s of a block of right-hand sides is (slots[s % nSlots1], rhs = s / nSlots1);
 // its buffer is X + s*xs, its trace storage T + rhs*traceStride and its input
 // vector xin + rhs*Nglob.
+// Right-hand side of sweep slot `sl` at point (i, j) of its cell (0 off the
+// two trace sets): res0 = A01 v1 on set 0, res1 = -A10 v0 on set 1.
+template <typename T2>
+__device__ T2 sweepRhs(const SlotDev& sl, const CellDev& cd, int i, int j, const double2* __restrict__ coef,
+                       const T2* __restrict__ T) {
+  const int nx = cd.nx, ny = cd.ny, N = nx * ny;
+  auto inSet = [&](int s, int ii, int jj) -> bool {
+    return (ii == sl.setCol[s] && jj >= sl.setColLo[s] && jj < sl.setColHi[s]) ||
+           (jj == sl.setRow[s] && ii >= sl.setRowLo[s] && ii < sl.setRowHi[s]);
+  };
+  auto val = [&](int s, int ii, int jj) -> T2 {
+    if (sl.rowVal[s] >= 0 && jj == sl.setRow[s] && ii >= sl.setRowLo[s] && ii < sl.setRowHi[s])
+      return T[sl.rowVal[s] + ii];
+    return T[sl.colVal[s] + jj];
+  };
+  const bool in0 = inSet(0, i, j);
+  const bool in1 = !in0 && inSet(1, i, j);
+  if (!in0 && !in1) return CTraits<T2>::make(0, 0);
+  const int other = in0 ? 1 : 0;
+  const int ni[4] = {i - 1, i + 1, i, i};
+  const int nj[4] = {j, j, j - 1, j + 1};
+  const long long p = (long long)i * ny + j;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int q = 0; q < 4; ++q) {
+    if (ni[q] < 0 || ni[q] >= nx || nj[q] < 0 || nj[q] >= ny) continue;
+    if (!inSet(other, ni[q], nj[q])) continue;
+    const double2 a = coef[cd.coefOff + (long long)(q + 1) * N + p];
+    const T2 vv = val(other, ni[q], nj[q]);
+    acc = cfma(a, make_double2((double)vv.x, (double)vv.y), acc);
+  }
+  if (in1) acc = cneg(acc);
+  return CTraits<T2>::make(acc.x, acc.y);
+}
+
+// Point t of trace-set segment m of a sweep slot (m: column 0, row 0,
+// column 1, row 1); false past the segment's end.
+__device__ __forceinline__ bool segmentPoint(const SlotDev& sl, int m, int t, int& i, int& j) {
+  const int s = m >> 1;
+  if ((m & 1) == 0) {
+    if (sl.setCol[s] < 0 || t >= sl.setColHi[s] - sl.setColLo[s]) return false;
+    i = sl.setCol[s];
+    j = sl.setColLo[s] + t;
+  } else {
+    if (sl.setRow[s] < 0 || t >= sl.setRowHi[s] - sl.setRowLo[s]) return false;
+    i = sl.setRowLo[s] + t;
+    j = sl.setRow[s];
+  }
+  return true;
+}
+
+// Right-hand side of every local solve of one sweep step.
+//   stage 1: the windowed input restricted to the cell's owned set of its DD
+//            (getSeparatedRHS + reduce/expand, src/DDCheckerboard.cpp:3564-3674)
+//   sweeps : equivalent sources res0 = A01 v1 on set 0, res1 = -A10 v0 on set 1
+//            (SubdomainStructured::getEquivalentSources/extractLTraces,
+//            src/SubdomainStructured.cpp:1304-1539)
+// Slot s of a block of right-hand sides is (slots[s % nSlots1], rhs = s / nSlots1);
+// its buffer is X + s*xs, its trace storage T + rhs*traceStride and its input
+// vector xin + rhs*Nglob.  Full mode writes every point of the slot's cell;
+// sparse mode (the sweep steps after the first) writes only the trace-set
+// points of sweep slots: the buffers are zero elsewhere (cleared before the
+// first sweep step, and each sweep step's k_post clears its own sets).
 template <typename T2>
 __global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
                             long long traceStride, long long Nglob, const double2* __restrict__ xin,
                             const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT,
                             int Gy, const double2* __restrict__ coef, const T2* __restrict__ T,
-                            T2* __restrict__ X, long long xs, const T2* __restrict__ F, int nGrp) {
+                            T2* __restrict__ X, long long xs, const T2* __restrict__ F, int nGrp, int sparse) {
   const int sidx = blockIdx.y;
   const SlotDev& sl = slots[sidx % nSlots1];
   const int rhs = sidx / nSlots1;
   xin += (long long)rhs * Nglob;
   T += (long long)rhs * traceStride;
-  const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
   const CellDev& cd = cells[sl.cell];
   const int nx = cd.nx, ny = cd.ny, N = nx * ny;
-  const int slr = sl.d & 1, sud = sl.d >> 1;
   T2* x = X + (long long)sidx * xs;
+  if (sparse && sl.kind != 0) {
+    for (int m = 0; m < 4; ++m)
+      for (int t = blockIdx.x * blockDim.x + threadIdx.x;; t += gridDim.x * blockDim.x) {
+        int i, j;
+        if (!segmentPoint(sl, m, t, i, j)) break;
+        x[(long long)i * ny + j] = sweepRhs<T2>(sl, cd, i, j, coef, T);
+      }
+    return;
+  }
+  const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
+  const int slr = sl.d & 1, sud = sl.d >> 1;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
     const int i = p / ny, j = p - i * ny;
     T2 out = CTraits<T2>::make(0, 0);
@@ -60,32 +131,7 @@ __global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __
         }
       }
     } else {
-      auto inSet = [&](int s, int ii, int jj) -> bool {
-        return (ii == sl.setCol[s] && jj >= sl.setColLo[s] && jj < sl.setColHi[s]) ||
-               (jj == sl.setRow[s] && ii >= sl.setRowLo[s] && ii < sl.setRowHi[s]);
-      };
-      auto val = [&](int s, int ii, int jj) -> T2 {
-        if (sl.rowVal[s] >= 0 && jj == sl.setRow[s] && ii >= sl.setRowLo[s] && ii < sl.setRowHi[s])
-          return T[sl.rowVal[s] + ii];
-        return T[sl.colVal[s] + jj];
-      };
-      const bool in0 = inSet(0, i, j);
-      const bool in1 = !in0 && inSet(1, i, j);
-      if (in0 || in1) {
-        const int other = in0 ? 1 : 0;
-        const int ni[4] = {i - 1, i + 1, i, i};
-        const int nj[4] = {j, j, j - 1, j + 1};
-        double2 acc = make_double2(0.0, 0.0);
-        for (int q = 0; q < 4; ++q) {
-          if (ni[q] < 0 || ni[q] >= nx || nj[q] < 0 || nj[q] >= ny) continue;
-          if (!inSet(other, ni[q], nj[q])) continue;
-          const double2 a = coef[cd.coefOff + (long long)(q + 1) * N + p];
-          const T2 vv = val(other, ni[q], nj[q]);
-          acc = cfma(a, make_double2((double)vv.x, (double)vv.y), acc);
-        }
-        if (in1) acc = cneg(acc);
-        out = CTraits<T2>::make(acc.x, acc.y);
-      }
+      out = sweepRhs<T2>(sl, cd, i, j, coef, T);
     }
     if (f) out = cadd(out, f[p]);
     x[p] = out;
@@ -140,16 +186,26 @@ __global__ void k_rhs_accum(const GroupDev* __restrict__ groups, int nGroups1, c
 // into the slot's own trace context), with the stage-3 overwrite of the
 // incoming outer traces (src/DDCheckerboard.cpp:1861-1864,2085-2088,
 // 2305-2308,2520-2526).
+// Xs holds the solutions; the slot's trace sets in the right-hand-side
+// buffer X are cleared (the next sweep step's sparse build relies on it).
 template <typename T2>
 __global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
-                       long long traceStride, const T2* __restrict__ X, long long xs, T2* __restrict__ T) {
+                       long long traceStride, const T2* __restrict__ Xs, T2* __restrict__ X, long long xs,
+                       T2* __restrict__ T) {
   const int sidx = blockIdx.y;
   const SlotDev& sl = slots[sidx % nSlots1];
   T += (long long)(sidx / nSlots1) * traceStride;
   const CellDev& cd = cells[sl.cell];
   const int nx = cd.nx, ny = cd.ny;
-  const T2* x = X + (long long)sidx * xs;
+  const T2* x = Xs + (long long)sidx * xs;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl.kind != 0) {
+    T2* xr = X + (long long)sidx * xs;
+    for (int m = 0; m < 4; ++m) {
+      int i, j;
+      if (segmentPoint(sl, m, t, i, j)) xr[(long long)i * ny + j] = CTraits<T2>::make(0, 0);
+    }
+  }
   for (int o = 0; o < sl.nOut; ++o) {
     const bool isRow = sl.outIsRow[o] != 0;
     const int len = isRow ? nx : ny;
@@ -275,7 +331,7 @@ DDCheckerboard::~DDCheckerboard() {
     if (st.dGroups) (void)devFree(st.dGroups);
     if (st.dCopies) (void)devFree(st.dCopies);
   }
-  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)devFree(p);
+  for (void* p : {dX, dXs, dR, dG, dT, dF}) if (p) (void)devFree(p);
   if (dY) (void)devFree(dY);
   if (dGlobalCoef) (void)devFree(dGlobalCoef);
   if (dColTrace) (void)devFree(dColTrace);
@@ -957,7 +1013,7 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
   }
   auto plan = std::make_unique<SolvePlan>();
   // the final solves need the whole solution; the sweep solves only their traces
-  mFactors->buildPlan(tasks, supports, *plan, st.deferred ? nullptr : &needs);
+  mFactors->buildPlan(tasks, supports, *plan, st.deferred ? nullptr : &needs, /*outOfPlace=*/true);
   const SolvePlan& ref = *plan;
   st.plans[nrhs] = std::move(plan);
   return ref;
@@ -966,14 +1022,15 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
 void DDCheckerboard::ensureBuffers(int nrhs) const {
   if (nrhs <= mBufRhs) return;
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
-  for (void* p : {dX, dR, dG, dT, dF}) if (p) (void)devFree(p);
+  for (void* p : {dX, dXs, dR, dG, dT, dF}) if (p) (void)devFree(p);
   if (dY) (void)devFree(dY);
-  dX = dR = dG = dT = dF = nullptr;
+  dX = dXs = dR = dG = dT = dF = nullptr;
   dY = nullptr;
   mBufRhs = 0;
   const size_t slots = (size_t)mMaxSlots * nrhs;
   LSW_HIP(devMalloc(&dF, (size_t)4 * mCells.size() * nrhs * mXs * es));
   LSW_HIP(devMalloc(&dX, slots * mXs * es));
+  LSW_HIP(devMalloc(&dXs, slots * mXs * es));
   LSW_HIP(devMalloc(&dR, slots * mXs * es));
   LSW_HIP(devMalloc(&dG, slots * mGs * es));
   LSW_HIP(devMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
@@ -982,6 +1039,7 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   // do not use: they must hold finite values from the start (cleared on the
   // problem's stream, ahead of its kernels)
   LSW_HIP(hipMemsetAsync(dX, 0, slots * mXs * es, mStream));
+  LSW_HIP(hipMemsetAsync(dXs, 0, slots * mXs * es, mStream));
   LSW_HIP(hipMemsetAsync(dR, 0, slots * mXs * es, mStream));
   LSW_HIP(hipMemsetAsync(dG, 0, slots * mGs * es, mStream));
   mBufRhs = nrhs;
@@ -1056,10 +1114,10 @@ void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady
   }
   // stencils (local + global), squared slowness
   const double fixed = 80.0 * local + 8.0 * local + 80.0 * n;
-  // sweep buffers per right-hand side: X and R (4 x cells slots), G, the
+  // sweep buffers per right-hand side: X, Xs and R (4 x cells slots), G, the
   // summed sources F, traces, and 4 FP64 accumulators of global vectors
   const double slots = 4.0 * nc;
-  const double perRhs = es * (2.0 * slots * maxDofs + slots * maxG + 4.0 * nc * maxDofs + 4.0 * nc * 72.0 * tmax) +
+  const double perRhs = es * (3.0 * slots * maxDofs + slots * maxG + 4.0 * nc * maxDofs + 4.0 * nc * 72.0 * tmax) +
                         64.0 * n;
   // solve plans: about 32 bytes per local DOF per block width, a dozen widths
   const double plans = 12.0 * 32.0 * local;
@@ -1161,22 +1219,29 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   // the final step solves each (d, cell) once in full and accumulates it
   auto run = [&](auto tag) {
     using T2 = decltype(tag);
-    for (const auto& st : mSteps) {
+    for (size_t si = 0; si < mSteps.size(); ++si) {
+      const Step& st = mSteps[si];
       const unsigned ns = (unsigned)st.nSlots * nrhs, ng = (unsigned)st.nGroups * nrhs;
       const unsigned nc = (unsigned)st.nCopies * nrhs;
-      hipLaunchKernelGGL(k_build_rhs<T2>, dim3(gxN, ns), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
-                         mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const T2*)dT, (T2*)dX, mXs,
-                         (const T2*)dF, nGrp);
+      // stage 1 and the final solves write whole buffers; the sweep steps
+      // write only their trace sets into buffers that are zero elsewhere:
+      // cleared once here after stage 1, then by each sweep step's k_post
+      const bool sparse = si > 0 && !st.deferred;
+      if (si == 1) zero(dX, (long long)((size_t)mMaxSlots * nrhs * mXs * es / 8));
+      hipLaunchKernelGGL(k_build_rhs<T2>, dim3(sparse ? gxT : gxN, ns), dim3(sparse ? 64 : tb), 0, s, dCells,
+                         st.dSlots, st.nSlots, mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const T2*)dT,
+                         (T2*)dX, mXs, (const T2*)dF, nGrp, sparse ? 1 : 0);
       if (!st.deferred)
         hipLaunchKernelGGL(k_rhs_accum<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, st.dSlots, st.nSlots,
                            (const T2*)dX, mXs, dCells, (T2*)dF, nGrp);
-      mFactors->solve(planFor(st, nrhs), dX, dR, dG, mXs, mGs, s, prof);
+      // solutions go to dXs: dX keeps the right-hand sides
+      mFactors->solve(planFor(st, nrhs), dX, dXs, dR, dG, mXs, mGs, s, prof);
       if (st.deferred) {
         hipLaunchKernelGGL(k_accumulate<T2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
-                           st.nSlots, (const T2*)dX, mXs, dY, n, mGy);
+                           st.nSlots, (const T2*)dXs, mXs, dY, n, mGy);
       } else {
         hipLaunchKernelGGL(k_post<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots, mTraceEntries,
-                           (const T2*)dX, mXs, (T2*)dT);
+                           (const T2*)dXs, (T2*)dX, mXs, (T2*)dT);
         if (st.nCopies)
           hipLaunchKernelGGL(k_copy_traces<T2>, dim3(gxT, nc), dim3(64), 0, s, st.dCopies, st.nCopies,
                              mTraceEntries, (T2*)dT, mTmax);

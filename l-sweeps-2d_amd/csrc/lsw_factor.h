@@ -82,6 +82,7 @@ struct SolvePlan {
     double bytes = 0;   // algorithmic bytes
   };
   int ntasks = 0, maxNrhs = 1;
+  bool outOfPlace = false;  // built for separate right-hand side and solution buffers
   std::vector<SolveTask> hTasks;
   SolveTaskDev* dTasks = nullptr;
   SolveEntry* dEntries = nullptr;
@@ -129,14 +130,22 @@ class FactorSet {
   // needed; the backward solve then visits only nodes whose region meets
   // them (ancestors included: their regions contain their descendants').  An
   // empty list for a task means no solution values are needed at all.
+  // outOfPlace: the plan will run with separate right-hand side and solution
+  // buffers (solve(plan, X, Xo, ...), Xo != X); its roots then form r_E in
+  // their backward entries, and the forward pass has no launch for them.
   void buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
-                 SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports = nullptr) const;
+                 SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports = nullptr,
+                 bool outOfPlace = false) const;
   // Run a plan.  X holds the right-hand sides on entry and the solutions on
   // exit (slot-major, stride xStride complex entries); R and G are scratch
   // with the same slot layout (G stride gStride).  Element type follows
   // precision(): double2 or float2.  With prof, every launch is timed with
   // HIP events (synchronises at the end).
   void solve(const SolvePlan& plan, void* X, void* R, void* G, long long xStride, long long gStride,
+             hipStream_t s, SolveProfile* prof = nullptr) const;
+  // The same with the solutions written to Xo (same layout as X); X is only
+  // read, so it still holds the right-hand sides afterwards.
+  void solve(const SolvePlan& plan, const void* X, void* Xo, void* R, void* G, long long xStride, long long gStride,
              hipStream_t s, SolveProfile* prof = nullptr) const;
 
   int numCells() const { return (int)mCells.size(); }

@@ -275,14 +275,16 @@ __device__ __forceinline__ void fwdStore(const Item& it, bool ok, int r, const T
   }
 }
 
-// backward gather: v[q*F + k] = r_E[k] (k < e) or x_B[k-e].  The r_E of a
-// root of one row chunk is formed here from X and its children's updates
-// (its single entry reads all of X[E] before it stores any of it); a root
-// has no boundary, so its columns are all r_E.
+// backward gather: v[q*F + k] = r_E[k] (k < e) or x_B[k-e] (from the
+// solution buffer Xo).  The r_E of a root is formed here from the right-hand
+// sides X and its children's updates when the plan says so (the solution
+// goes to another buffer, or the root has one row chunk, whose single entry
+// reads all of X[E] before it stores any of it); a root has no boundary, so
+// its columns are all r_E.
 template <typename T2, int NR>
 __device__ __forceinline__ void bwdGather(const Item& it, bool act, int k0, int kstep, const int* __restrict__ poolE,
                                           const int* __restrict__ poolB, const int* __restrict__ poolMap,
-                                          const T2* __restrict__ X, const T2* __restrict__ Rb,
+                                          const T2* X, const T2* Xo, const T2* __restrict__ Rb,
                                           const T2* __restrict__ G, long long xs, long long gs, T2* v) {
   constexpr int B = gatherBatch<NR>();
   const int e = it.nd.e, F = e + it.nd.b;
@@ -306,7 +308,7 @@ __device__ __forceinline__ void bwdGather(const Item& it, bool act, int k0, int 
     for (int u = 0; u < B; ++u) {
       const int kk = kb + u * kstep;
       const int kc = kk < K ? kk : kb;
-      const T2* src = kc < e ? Rb : X;
+      const T2* src = kc < e ? Rb : Xo;
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
         const long long sl = it.tk.slot[q];
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
                                                    const SolveEntry* __restrict__ ent, int nent, int rounds,
                                                    int rpLog, int cols, const int* __restrict__ poolE,
                                                    const int* __restrict__ poolB, const int* __restrict__ poolMap,
-                                                   const T2* __restrict__ factor, T2* __restrict__ X,
+                                                   const T2* __restrict__ factor, const T2* X, T2* Xo,
                                                    T2* __restrict__ Rb, T2* __restrict__ G, long long xs,
                                                    long long gs) {
   extern __shared__ char smem_raw[];
@@ -370,13 +372,13 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
     if constexpr (FWD)
       fwdGather<T2, NR>(it, valid, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
     else
-      bwdGather<T2, NR>(it, valid, k0 + r, RP, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
+      bwdGather<T2, NR>(it, valid, k0 + r, RP, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
     waveSync();
     rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, FWD ? e : F, pre, acc);
     if constexpr (FWD)
       fwdStore<T2, NR>(it, ok, r, acc, G, gs);
     else
-      bwdStore<T2, NR>(it, ok, dof, acc, X, xs);
+      bwdStore<T2, NR>(it, ok, dof, acc, Xo, xs);
     waveSync();
   }
 }
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
                                                     const SolveEntry* __restrict__ ent, int nent, int cols,
                                                     const int* __restrict__ poolE, const int* __restrict__ poolB,
                                                     const int* __restrict__ poolMap, const T2* __restrict__ factor,
-                                                    T2* __restrict__ X, T2* __restrict__ Rb, T2* __restrict__ G,
+                                                    const T2* X, T2* Xo, T2* __restrict__ Rb, T2* __restrict__ G,
                                                     long long xs, long long gs) {
   extern __shared__ char smem_raw[];
   constexpr int D = kDotDepth;
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   if constexpr (FWD)
     fwdGather<T2, NR>(it, true, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
   else
-    bwdGather<T2, NR>(it, true, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Rb, G, xs, gs, v);
+    bwdGather<T2, NR>(it, true, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
   __syncthreads();
   T2 acc[NR];
 #pragma unroll
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   if constexpr (FWD)
     fwdStore<T2, NR>(it, storer, r, acc, G, gs);
   else
-    bwdStore<T2, NR>(it, storer, dof, acc, X, xs);
+    bwdStore<T2, NR>(it, storer, dof, acc, Xo, xs);
 }
 }  // namespace
 
@@ -480,7 +482,7 @@ int ceilLog2(int x) {
 }  // namespace
 
 void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
-                          SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports) const {
+                          SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports, bool outOfPlace) const {
   LSW_REQUIRE(!mTrees.empty(), LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(supports.empty() || supports.size() == tasks.size(), LSW_ERR_ARG, "one support per task");
   LSW_REQUIRE(!bwdSupports || bwdSupports->size() == tasks.size(), LSW_ERR_ARG, "one backward support per task");
@@ -489,6 +491,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   plan.ntasks = nt;
   plan.hTasks = tasks;
   plan.maxNrhs = 1;
+  plan.outOfPlace = outOfPlace;
   const double S = mPrecision == LSW_FP32 ? 8.0 : 16.0;
   std::vector<SolveTaskDev> tdev(nt);
   // entries per height
@@ -531,10 +534,11 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       if (nd.c2 >= 0 && on[nd.c2]) flags |= kEntChild2;
       if (on[p]) flags |= kEntVisited;
       // A root (no boundary rows) has no forward product, only r_E to form.
-      // A root of one row chunk forms it in its backward entry; a larger
-      // root keeps a forward entry that stores r_E, because the chunks of
-      // its backward entries overwrite X[E] while the others read it.
-      if (nd.b == 0 && nd.e <= kSolveRows) flags |= kEntRootRhs;
+      // Its backward entries form it themselves when the solution goes to
+      // another buffer, or when the root is one row chunk; otherwise (in
+      // place, several chunks) a forward entry stores r_E, because the chunks
+      // of its backward entries overwrite X[E] while the others read it.
+      if (nd.b == 0 && (outOfPlace || nd.e <= kSolveRows)) flags |= kEntRootRhs;
       if (on[p] && !(flags & kEntRootRhs)) {
         if (nd.b == 0) {
           fw[h].push_back({t, base + p, 0, flags});
@@ -698,7 +702,8 @@ struct SolveArgs {
 };
 
 template <typename T2, int NR, bool FWD>
-void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, T2* X, T2* R, T2* G, hipStream_t s) {
+void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, const T2* X, T2* Xo, T2* R, T2* G,
+            hipStream_t s) {
   const SolveEntry* ent = a.ent + L.off;
   if (L.mode == 0) {
     const int P = 64 / L.rp;
@@ -707,7 +712,7 @@ void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, T2
     const size_t shm = (size_t)L.wpb * P * NR * L.cols * sizeof(T2);
     const int rpLog = ceilLog2(L.rp);
     hipLaunchKernelGGL((k_nd_packed<T2, NR, FWD>), dim3(grid), dim3(64 * L.wpb), shm, s, a.tasks, a.nodes, ent, L.n,
-                       L.rounds, rpLog, L.cols, a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+                       L.rounds, rpLog, L.cols, a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
     return;
   }
   const size_t shm = ((size_t)NR * L.cols + (size_t)L.waves * 64 * NR) * sizeof(T2);
@@ -715,35 +720,35 @@ void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, T2
   switch (L.waves) {
     case 1:
       hipLaunchKernelGGL((k_nd_wide<T2, NR, 1, FWD>), grid, dim3(64), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
-                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+                         a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
       break;
     case 2:
       hipLaunchKernelGGL((k_nd_wide<T2, NR, 2, FWD>), grid, dim3(128), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
-                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+                         a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
       break;
     case 4:
       hipLaunchKernelGGL((k_nd_wide<T2, NR, 4, FWD>), grid, dim3(256), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
-                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+                         a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
       break;
     default:
       hipLaunchKernelGGL((k_nd_wide<T2, NR, 8, FWD>), grid, dim3(512), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
-                         a.poolE, a.poolB, a.poolMap, factor, X, R, G, a.xs, a.gs);
+                         a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
       break;
   }
 }
 
 template <typename T2, int NR>
-void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, T2* X, T2* R, T2* G, hipStream_t s,
-             SolveProfile* prof) {
+void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, const T2* X, T2* Xo, T2* R, T2* G,
+             hipStream_t s, SolveProfile* prof) {
   LaunchTimer tm{prof, s, {}, {}, {}};
   for (const auto& L : plan.fwd) {
     tm.begin();
-    launch<T2, NR, true>(a, L, factor, X, R, G, s);
+    launch<T2, NR, true>(a, L, factor, X, Xo, R, G, s);
     tm.end(0, L.bytes);
   }
   for (const auto& L : plan.bwd) {
     tm.begin();
-    launch<T2, NR, false>(a, L, factor, X, R, G, s);
+    launch<T2, NR, false>(a, L, factor, X, Xo, R, G, s);
     tm.end(1, L.bytes);
   }
   LSW_HIP(hipGetLastError());
@@ -753,23 +758,33 @@ void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, T2* X,
 
 void FactorSet::solve(const SolvePlan& plan, void* X, void* R, void* G, long long xs, long long gs,
                       hipStream_t s, SolveProfile* prof) const {
+  solve(plan, X, X, R, G, xs, gs, s, prof);
+}
+
+void FactorSet::solve(const SolvePlan& plan, const void* X, void* Xo, void* R, void* G, long long xs, long long gs,
+                      hipStream_t s, SolveProfile* prof) const {
   LSW_REQUIRE(mFactorized, LSW_ERR_STATE, "solve called before factorize");
+  LSW_REQUIRE(Xo != X || !plan.outOfPlace, LSW_ERR_ARG, "plan built for separate right-hand side and solution buffers");
   if (plan.ntasks <= 0) return;
   const SolveArgs a{plan.dTasks, dSolveNodes, plan.dEntries, dPoolE, dPoolB, dPoolMap, xs, gs};
   if (mPrecision == LSW_FP64) {
+    const auto* x = (const double2*)X;
+    auto* xo = (double2*)Xo;
     if (plan.maxNrhs == 1)
-      runPlan<double2, 1>(plan, a, dFactor, (double2*)X, (double2*)R, (double2*)G, s, prof);
+      runPlan<double2, 1>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof);
     else if (plan.maxNrhs <= kTaskRhs4)
-      runPlan<double2, kTaskRhs4>(plan, a, dFactor, (double2*)X, (double2*)R, (double2*)G, s, prof);
+      runPlan<double2, kTaskRhs4>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof);
     else
-      runPlan<double2, kMaxRhs>(plan, a, dFactor, (double2*)X, (double2*)R, (double2*)G, s, prof);
+      runPlan<double2, kMaxRhs>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof);
   } else {
+    const auto* x = (const float2*)X;
+    auto* xo = (float2*)Xo;
     if (plan.maxNrhs == 1)
-      runPlan<float2, 1>(plan, a, dFactor32, (float2*)X, (float2*)R, (float2*)G, s, prof);
+      runPlan<float2, 1>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof);
     else if (plan.maxNrhs <= kTaskRhs4)
-      runPlan<float2, kTaskRhs4>(plan, a, dFactor32, (float2*)X, (float2*)R, (float2*)G, s, prof);
+      runPlan<float2, kTaskRhs4>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof);
     else
-      runPlan<float2, kMaxRhs>(plan, a, dFactor32, (float2*)X, (float2*)R, (float2*)G, s, prof);
+      runPlan<float2, kMaxRhs>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof);
   }
 }
 
