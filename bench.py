@@ -511,7 +511,10 @@ def main():
         st32 = dd.stats()
         del dd
         torch.cuda.empty_cache()
-    head_name = "mixed" if "mixed" in out else ("plain" if "plain" in out else "plain_ref")
+    head_name = next((k for k in ("mixed", "plain", "plain_ref") if k in out), None)
+    if head_name is None:
+        log("no single-source path in --paths: results above, no bench line")
+        return
     head = out[head_name]
     roof = roofline(prof["fp32"] if "fp32" in prof else prof["fp64"])
     cpu = None

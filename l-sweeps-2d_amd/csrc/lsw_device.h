@@ -75,6 +75,8 @@ constexpr int kMaxRhs = 8;
 // tasks of up to 4 right-hand sides always fit the solve kernels' LDS staging;
 // 8 when the widest front allows it (FactorSet::maxTaskRhs)
 constexpr int kTaskRhs4 = 4;
+// right-hand sides per task of the solve kernel that serves tasks of up to n
+constexpr int taskWidth(int n) { return n <= 1 ? 1 : n <= kTaskRhs4 ? kTaskRhs4 : kMaxRhs; }
 struct SolveTask {
   int cell;
   int nrhs;

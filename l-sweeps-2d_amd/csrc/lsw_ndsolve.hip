@@ -79,6 +79,28 @@ long long latencyWaves() {
   return v;
 }
 
+// A packed launch of more than this many wavefronts gives each lane group
+// several entries in turn ("rounds").  By default every lane group takes one
+// entry: on the 1600^2 lens, rounds that kept 4096 wavefronts were 4% slower
+// overall than one entry per lane group (LSW_PACKED_WAVES overrides it).
+long long packedWaveTarget() {
+  static const long long v = [] {
+    const char* e = std::getenv("LSW_PACKED_WAVES");
+    return e ? std::max(1LL, std::atoll(e)) : (1LL << 40);
+  }();
+  return v;
+}
+
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) in turn:
+// workgroup b runs on XCD b % 8.  This gives each XCD a contiguous range of
+// the work, so that neighbouring entries (the tasks of one cell reading the
+// same factor block) meet in one L2.  A bijection of [0, G).
+__device__ __forceinline__ unsigned xcdRemap(unsigned b, unsigned G) {
+  constexpr unsigned kXcd = 8;
+  const unsigned q = G / kXcd, r = G % kXcd, x = b % kXcd;
+  return x * q + (x < r ? x : r) + b / kXcd;
+}
+
 // true in every lane when pred holds in any lane of the wavefront
 __device__ __forceinline__ bool waveAny(bool pred) { return __ballot(pred) != 0; }
 
@@ -118,36 +140,39 @@ __device__ __forceinline__ void rowLoad(const T2* __restrict__ M, long long ld, 
   }
 }
 
-// acc[q] += sum_u m[u] * v[q*vs + k + u*kstep] over the terms below K (m is
-// zero past K; the staged value is replaced by zero there, as it may be
-// unwritten)
+// acc[q] += sum_u m[u] * v[(k + u*kstep)*NR + q] over the terms below K (m
+// is zero past K; the staged value is replaced by zero there, as it may be
+// unwritten).  The staging is column-major over right-hand sides: the NR
+// values of one column are adjacent (wide LDS reads).  With many right-hand
+// sides the values of one column are consumed before the next column's are
+// read, which bounds the registers they hold.
 template <typename T2, int NR, int D>
-__device__ __forceinline__ void rowAcc(const T2 (&m)[D], int k, int K, int kstep, const T2* v, int vs,
-                                       T2 (&acc)[NR]) {
+__device__ __forceinline__ void rowAcc(const T2 (&m)[D], int k, int K, int kstep, const T2* v, T2 (&acc)[NR]) {
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     const int kk = k + u * kstep;
     const bool in = kk < K;
-    const int kc = in ? kk : 0;
+    const T2* vk = v + (in ? kk : 0) * NR;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
-      const T2 vv = v[q * vs + kc];
+      const T2 vv = vk[q];
       acc[q] = cfma(m[u], in ? vv : czero<T2>(), acc[q]);
     }
+    if constexpr (NR >= 8) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// acc[q] += sum_{k = k0 + j*kstep < K} M[k*ld] * v[q*vs + k]: the first batch
+// acc[q] += sum_{k = k0 + j*kstep < K} M[k*ld] * v[k*NR + q]: the first batch
 // comes from pre (loaded before the gather, so that its latency overlaps
 // it), the rest in batches of D.
 template <typename T2, int NR, int D>
 __device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
-                                          const T2* v, int vs, const T2 (&pre)[D], T2 (&acc)[NR]) {
-  rowAcc<T2, NR, D>(pre, k0, K, kstep, v, vs, acc);
+                                          const T2* v, const T2 (&pre)[D], T2 (&acc)[NR]) {
+  rowAcc<T2, NR, D>(pre, k0, K, kstep, v, acc);
   for (int k = k0 + D * kstep; k < K; k += D * kstep) {
     T2 m[D];
     rowLoad<T2, D>(M, ld, k, K, kstep, m);
-    rowAcc<T2, NR, D>(m, k, K, kstep, v, vs, acc);
+    rowAcc<T2, NR, D>(m, k, K, kstep, v, acc);
   }
 }
 
@@ -169,14 +194,14 @@ __device__ __forceinline__ Item loadItem(const SolveEntry* __restrict__ ent, con
   return it;
 }
 
-// r_E of a front: v[q*vs + k] = X[E[k]] - g_c1[m1[k]] - g_c2[m2[k]] for the
+// r_E of a front: v[k*NR + q] = X[E[k]] - g_c1[m1[k]] - g_c2[m2[k]] for the
 // lane's k in [k0, K) (step kstep) when `act`; with keepR also kept in Rb
 // for the backward pass.
 template <typename T2, int NR>
 __device__ __forceinline__ void gatherFrontRhs(const Item& it, bool act, int k0, int K, int kstep,
                                                const int* __restrict__ E, const int* __restrict__ m1,
                                                const T2* __restrict__ X, T2* __restrict__ Rb, bool keepR,
-                                               const T2* __restrict__ G, long long xs, long long gs, T2* v, int vs) {
+                                               const T2* __restrict__ G, long long xs, long long gs, T2* v) {
   constexpr int B = gatherBatch<NR>();
   const int F = it.nd.e + it.nd.b;
   const bool c1 = (it.flags & kEntChild1) != 0, c2 = (it.flags & kEntChild2) != 0;
@@ -223,14 +248,14 @@ __device__ __forceinline__ void gatherFrontRhs(const Item& it, bool act, int k0,
       const int kc = kk < K ? kk : kb;
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
-        v[q * vs + kc] = val[u][q];  // past K: the value of kb again
+        v[kc * NR + q] = val[u][q];  // past K: the value of kb again
         if (keepR && kk < K && q < it.tk.nrhs) Rb[(long long)it.tk.slot[q] * xs + dof[u]] = val[u][q];
       }
     }
   }
 }
 
-// forward gather: v[q*e + k] = r_E[k]; the first row chunk also keeps r_E
+// forward gather: v[k*NR + q] = r_E[k]; the first row chunk also keeps r_E
 // for the backward pass
 template <typename T2, int NR>
 __device__ __forceinline__ void fwdGather(const Item& it, bool act, int k0, int kstep, const int* __restrict__ poolE,
@@ -238,7 +263,7 @@ __device__ __forceinline__ void fwdGather(const Item& it, bool act, int k0, int 
                                           T2* __restrict__ Rb, const T2* __restrict__ G, long long xs, long long gs,
                                           T2* v) {
   gatherFrontRhs<T2, NR>(it, act, k0, it.nd.e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, Rb, it.r0 == 0, G,
-                         xs, gs, v, it.nd.e);
+                         xs, gs, v);
 }
 
 // The children's updates a forward row adds to its product do not depend
@@ -275,7 +300,7 @@ __device__ __forceinline__ void fwdStore(const Item& it, bool ok, int r, const T
   }
 }
 
-// backward gather: v[q*F + k] = r_E[k] (k < e) or x_B[k-e] (from the
+// backward gather: v[k*NR + q] = r_E[k] (k < e) or x_B[k-e] (from the
 // solution buffer Xo).  The r_E of a root is formed here from the right-hand
 // sides X and its children's updates when the plan says so (the solution
 // goes to another buffer, or the root has one row chunk, whose single entry
@@ -291,7 +316,7 @@ __device__ __forceinline__ void bwdGather(const Item& it, bool act, int k0, int 
   const bool root = (it.flags & kEntRootRhs) != 0;
   if (waveAny(act && root))
     gatherFrontRhs<T2, NR>(it, act && root, k0, e, kstep, poolE + it.nd.E, poolMap + it.nd.map, X, nullptr, false,
-                           G, xs, gs, v, F);
+                           G, xs, gs, v);
   if (!waveAny(act && !root)) return;
   const int* E = poolE + it.nd.E;
   const int* Bp = poolB + it.nd.B;
@@ -312,7 +337,7 @@ __device__ __forceinline__ void bwdGather(const Item& it, bool act, int k0, int 
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
         const long long sl = it.tk.slot[q];
-        v[q * F + kc] = src[sl * xs + dof[u]];  // past K: the value of kb again
+        v[kc * NR + q] = src[sl * xs + dof[u]];  // past K: the value of kb again
       }
     }
   }
@@ -346,7 +371,7 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
   const int RP = 1 << rpLog, P = 64 >> rpLog;
   const int grp = lane >> rpLog, r = lane & (RP - 1);
   T2* v = reinterpret_cast<T2*>(smem_raw) + ((size_t)wave * P + grp) * NR * cols;
-  const long long waveId = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  const long long waveId = (long long)xcdRemap(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;
   for (int round = 0; round < rounds; ++round) {
     const long long idx = (waveId * rounds + round) * P + grp;
     const bool valid = idx < nent;
@@ -374,7 +399,7 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
     else
       bwdGather<T2, NR>(it, valid, k0 + r, RP, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
     waveSync();
-    rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, FWD ? e : F, pre, acc);
+    rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, pre, acc);
     if constexpr (FWD)
       fwdStore<T2, NR>(it, ok, r, acc, G, gs);
     else
@@ -399,7 +424,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   constexpr int D = kDotDepth;
   T2* v = reinterpret_cast<T2*>(smem_raw);
   T2* red = v + (size_t)NR * cols;
-  const long long idx = blockIdx.x;
+  const long long idx = xcdRemap(blockIdx.x, gridDim.x);
   if (idx >= nent) return;
   const Item it = loadItem(ent, tasks, nodes, idx);
   const int e = it.nd.e, F = e + it.nd.b;
@@ -434,7 +459,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   T2 acc[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) acc[q] = FWD ? add[q] : czero<T2>();
-  rowDotPre<T2, NR, D>(M, ld, kb, K, W * CG, v, FWD ? e : F, pre, acc);
+  rowDotPre<T2, NR, D>(M, ld, kb, K, W * CG, v, pre, acc);
   for (int off = RP; off < 64; off <<= 1) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
@@ -565,7 +590,19 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       bwork[h] += e * kcols;
     }
   }
-  const double nrk = plan.maxNrhs == 1 ? 1.0 : plan.maxNrhs <= kTaskRhs4 ? (double)kTaskRhs4 : (double)kMaxRhs;
+  const double nrk = taskWidth(plan.maxNrhs);
+  // Within a height, the entries of one node (and row chunk) for all tasks of
+  // a cell are adjacent: they read the same factor block, and adjacent
+  // entries run together on one XCD (xcdRemap), so the block comes from HBM
+  // once and from L2 for the other tasks.
+  auto byNode = [&](const SolveEntry& a, const SolveEntry& b) {
+    const int ca = tasks[a.task].cell, cb = tasks[b.task].cell;
+    if (ca != cb) return ca < cb;
+    if (a.node != b.node) return a.node < b.node;
+    return (a.r0rows & 0xffff) < (b.r0rows & 0xffff);
+  };
+  for (auto& v : fw) std::stable_sort(v.begin(), v.end(), byNode);
+  for (auto& v : bw) std::stable_sort(v.begin(), v.end(), byNode);
   std::vector<SolveEntry> all;
   auto addLaunch = [&](std::vector<SolvePlan::Launch>& out, const std::vector<SolveEntry>& v, int cols, int maxRows,
                        double bytes, double work) {
@@ -614,9 +651,9 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       int wpb = 4;
       while (wpb > 1 && wpb * ldsWave(rp) > kMaxLds) wpb /= 2;
       L.wpb = wpb;
-      // entries per lane group: keep at least ~4096 wavefronts in flight
+      // entries per lane group: keep at least packedWaveTarget() wavefronts
       const long long groups = (long long)v.size() * rp / 64 + 1;
-      L.rounds = (int)std::min<long long>(16, std::max<long long>(1, groups / 4096));
+      L.rounds = (int)std::min<long long>(16, std::max<long long>(1, groups / packedWaveTarget()));
     }
     all.insert(all.end(), v.begin(), v.end());
     out.push_back(L);
@@ -625,6 +662,14 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   plan.bwd.clear();
   plan.fwdBytes = 0;
   plan.bwdBytes = 0;
+  // The forward pass only feeds the backward pass: with no solution value
+  // read (sweep solves whose traces nobody uses), there is nothing to do.
+  bool anyBwd = false;
+  for (const auto& v : bw) anyBwd = anyBwd || !v.empty();
+  if (!anyBwd) {
+    for (auto& v : fw) v.clear();
+    std::fill(fbytes.begin(), fbytes.end(), 0.0);
+  }
   for (int h = 0; h <= mMaxHeight; ++h) {
     addLaunch(plan.fwd, fw[h], fcols[h], frows[h], fbytes[h], fwork[h]);
     plan.fwdBytes += fbytes[h];
@@ -770,21 +815,19 @@ void FactorSet::solve(const SolvePlan& plan, const void* X, void* Xo, void* R, v
   if (mPrecision == LSW_FP64) {
     const auto* x = (const double2*)X;
     auto* xo = (double2*)Xo;
-    if (plan.maxNrhs == 1)
-      runPlan<double2, 1>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof);
-    else if (plan.maxNrhs <= kTaskRhs4)
-      runPlan<double2, kTaskRhs4>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof);
-    else
-      runPlan<double2, kMaxRhs>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof);
+    switch (taskWidth(plan.maxNrhs)) {
+      case 1: runPlan<double2, 1>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof); break;
+      case 4: runPlan<double2, 4>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof); break;
+      default: runPlan<double2, 8>(plan, a, dFactor, x, xo, (double2*)R, (double2*)G, s, prof); break;
+    }
   } else {
     const auto* x = (const float2*)X;
     auto* xo = (float2*)Xo;
-    if (plan.maxNrhs == 1)
-      runPlan<float2, 1>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof);
-    else if (plan.maxNrhs <= kTaskRhs4)
-      runPlan<float2, kTaskRhs4>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof);
-    else
-      runPlan<float2, kMaxRhs>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof);
+    switch (taskWidth(plan.maxNrhs)) {
+      case 1: runPlan<float2, 1>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof); break;
+      case 4: runPlan<float2, 4>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof); break;
+      default: runPlan<float2, 8>(plan, a, dFactor32, x, xo, (float2*)R, (float2*)G, s, prof); break;
+    }
   }
 }
 
