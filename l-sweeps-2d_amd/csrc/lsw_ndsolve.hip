@@ -129,35 +129,33 @@ __device__ __forceinline__ T2* trashSlot() {
   return reinterpret_cast<T2*>(gSolveTrash) + (threadIdx.x & 511);
 }
 
-// m[u] = M[(k + u*kstep)*ld] for k + u*kstep < K, else 0 (M[0] must be valid)
+// m[u] = M[(k + u*kstep)*ld] for k + u*kstep < K, else M[0] (which must be
+// valid; it meets the zero pad column in rowAcc)
 template <typename T2, int D>
 __device__ __forceinline__ void rowLoad(const T2* __restrict__ M, long long ld, int k, int K, int kstep, T2 (&m)[D]) {
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     const int kk = k + u * kstep;
-    const bool in = kk < K;
-    m[u] = cmask(M[(long long)(in ? kk : 0) * ld], in);
+    m[u] = M[(long long)(kk < K ? kk : 0) * ld];
   }
 }
 
-// acc[q] += sum_u m[u] * v[(k + u*kstep)*NR + q] over the terms below K (m
-// is zero past K; the staged value is replaced by zero there, as it may be
-// unwritten).  The staging is column-major over right-hand sides: the NR
-// values of one column are adjacent (wide LDS reads).  With many right-hand
-// sides the values of one column are consumed before the next column's are
-// read, which bounds the registers they hold.
+// acc[q] += sum_u m[u] * v[(k + u*kstep)*NR + q] over the terms below K:
+// past K the product reads the staging's zero pad column instead (m is then
+// finite but meaningless), so no select is needed per term.  The staging is
+// column-major over right-hand sides: the NR values of one column are
+// adjacent (wide LDS reads).  With many right-hand sides the values of one
+// column are consumed before the next column's are read, which bounds the
+// registers they hold.
 template <typename T2, int NR, int D>
-__device__ __forceinline__ void rowAcc(const T2 (&m)[D], int k, int K, int kstep, const T2* v, T2 (&acc)[NR]) {
+__device__ __forceinline__ void rowAcc(const T2 (&m)[D], int k, int K, int kstep, const T2* v, int pad,
+                                       T2 (&acc)[NR]) {
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     const int kk = k + u * kstep;
-    const bool in = kk < K;
-    const T2* vk = v + (in ? kk : 0) * NR;
+    const T2* vk = v + (kk < K ? kk : pad) * NR;
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      const T2 vv = vk[q];
-      acc[q] = cfma(m[u], in ? vv : czero<T2>(), acc[q]);
-    }
+    for (int q = 0; q < NR; ++q) acc[q] = cfma(m[u], vk[q], acc[q]);
     if constexpr (NR >= 8) __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -167,12 +165,12 @@ __device__ __forceinline__ void rowAcc(const T2 (&m)[D], int k, int K, int kstep
 // it), the rest in batches of D.
 template <typename T2, int NR, int D>
 __device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
-                                          const T2* v, const T2 (&pre)[D], T2 (&acc)[NR]) {
-  rowAcc<T2, NR, D>(pre, k0, K, kstep, v, acc);
+                                          const T2* v, int pad, const T2 (&pre)[D], T2 (&acc)[NR]) {
+  rowAcc<T2, NR, D>(pre, k0, K, kstep, v, pad, acc);
   for (int k = k0 + D * kstep; k < K; k += D * kstep) {
     T2 m[D];
     rowLoad<T2, D>(M, ld, k, K, kstep, m);
-    rowAcc<T2, NR, D>(m, k, K, kstep, v, acc);
+    rowAcc<T2, NR, D>(m, k, K, kstep, v, pad, acc);
   }
 }
 
@@ -354,7 +352,8 @@ __device__ __forceinline__ void bwdStore(const Item& it, bool ok, int dof, const
 }
 
 // ---------------------------------------------------------------- packed
-// LDS: per wavefront P regions of NR x cols values.  Lanes past the last
+// LDS: per wavefront P regions of NR x cols values (the last column is the
+// zero pad, cleared here and never overwritten).  Lanes past the last
 // entry work on the last entry and store nothing.
 template <typename T2, int NR, bool FWD>
 __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restrict__ tasks,
@@ -371,6 +370,8 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
   const int RP = 1 << rpLog, P = 64 >> rpLog;
   const int grp = lane >> rpLog, r = lane & (RP - 1);
   T2* v = reinterpret_cast<T2*>(smem_raw) + ((size_t)wave * P + grp) * NR * cols;
+  const int pad = cols - 1;
+  if (r < NR) v[pad * NR + r] = czero<T2>();  // RP >= 8 >= NR
   const long long waveId = (long long)xcdRemap(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;
   for (int round = 0; round < rounds; ++round) {
     const long long idx = (waveId * rounds + round) * P + grp;
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
     else
       bwdGather<T2, NR>(it, valid, k0 + r, RP, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
     waveSync();
-    rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, pre, acc);
+    rowDotPre<T2, NR, D>(M, ld, k0, K, 1, v, pad, pre, acc);
     if constexpr (FWD)
       fwdStore<T2, NR>(it, ok, r, acc, G, gs);
     else
@@ -411,7 +412,8 @@ __global__ __launch_bounds__(256) void k_nd_packed(const SolveTaskDev* __restric
 // ------------------------------------------------------------------ wide
 // W wavefronts per entry: 64 rows x W*(64/RP) column groups; partial sums
 // reduced by shuffles inside a wavefront and in a fixed order across
-// wavefronts through LDS.  LDS: NR x cols values, then W x 64 x NR partials.
+// wavefronts through LDS.  LDS: NR x cols values (the last column is the
+// zero pad), then W x 64 x NR partials.
 template <typename T2, int NR, int W, bool FWD>
 __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restrict__ tasks,
                                                     const SolveNode* __restrict__ nodes,
@@ -424,6 +426,8 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   constexpr int D = kDotDepth;
   T2* v = reinterpret_cast<T2*>(smem_raw);
   T2* red = v + (size_t)NR * cols;
+  const int pad = cols - 1;
+  if (threadIdx.x < NR) v[pad * NR + threadIdx.x] = czero<T2>();  // ordered by the barrier after the gather
   const long long idx = xcdRemap(blockIdx.x, gridDim.x);
   if (idx >= nent) return;
   const Item it = loadItem(ent, tasks, nodes, idx);
@@ -459,7 +463,7 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   T2 acc[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) acc[q] = FWD ? add[q] : czero<T2>();
-  rowDotPre<T2, NR, D>(M, ld, kb, K, W * CG, v, pre, acc);
+  rowDotPre<T2, NR, D>(M, ld, kb, K, W * CG, v, pad, pre, acc);
   for (int off = RP; off < 64; off <<= 1) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = cadd(acc[q], shfl_xor2(acc[q], off));
@@ -610,14 +614,16 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     SolvePlan::Launch L;
     L.off = (long long)all.size();
     L.n = (int)v.size();
-    L.cols = cols;
+    // staging columns: the front's widest r_E / [r_E; x_B] plus a zero pad
+    const int stage = cols + 1;
+    L.cols = stage;
     L.bytes = bytes;
     const double per = work / v.size();
-    auto ldsWide = [&](int w) { return S * nrk * ((double)cols + 64.0 * w); };
+    auto ldsWide = [&](int w) { return S * nrk * ((double)stage + 64.0 * w); };
     // packed shape: RP lanes per entry (>= 8 so that the LDS stays bounded)
     // (a launch of roots only, maxRows == 0, just gathers r_E: all 64 lanes)
     int rp = maxRows == 0 ? 64 : 1 << std::max(3, ceilLog2(std::max(1, maxRows)));
-    auto ldsWave = [&](int rpv) { return S * nrk * cols * (64.0 / rpv); };
+    auto ldsWave = [&](int rpv) { return S * nrk * stage * (64.0 / rpv); };
     while (rp < 64 && ldsWave(rp) > kMaxLds) rp *= 2;
     const long long packedWaves = ((long long)v.size() * rp + 63) / 64;
     int W = 0;
@@ -834,7 +840,7 @@ void FactorSet::solve(const SolvePlan& plan, const void* X, void* Xo, void* R, v
 int FactorSet::maxTaskRhs() const {
   const double S = mPrecision == LSW_FP32 ? 8.0 : 16.0;
   // packed staging of one 64-row entry: 8 x front; wide: plus 2 wavefronts' partials
-  return S * kMaxRhs * ((double)mMaxFront + 128.0) <= kMaxLds ? kMaxRhs : kTaskRhs4;
+  return S * kMaxRhs * ((double)mMaxFront + 129.0) <= kMaxLds ? kMaxRhs : kTaskRhs4;
 }
 
 }  // namespace lsw
