@@ -162,15 +162,26 @@ __device__ __forceinline__ void rowAcc(const T2 (&m)[D], int k, int K, int kstep
 
 // acc[q] += sum_{k = k0 + j*kstep < K} M[k*ld] * v[k*NR + q]: the first batch
 // comes from pre (loaded before the gather, so that its latency overlaps
-// it), the rest in batches of D.
+// it), the rest in batches of D, each loaded while the previous one is
+// multiplied (one memory round trip per row instead of one per batch).  The
+// loop runs while any lane of the wavefront has columns left, so its loads
+// are not under a divergent branch; lanes past their K load M[0] and meet
+// the zero pad.
 template <typename T2, int NR, int D>
 __device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld, int k0, int K, int kstep,
                                           const T2* v, int pad, const T2 (&pre)[D], T2 (&acc)[NR]) {
-  rowAcc<T2, NR, D>(pre, k0, K, kstep, v, pad, acc);
-  for (int k = k0 + D * kstep; k < K; k += D * kstep) {
-    T2 m[D];
-    rowLoad<T2, D>(M, ld, k, K, kstep, m);
-    rowAcc<T2, NR, D>(m, k, K, kstep, v, pad, acc);
+  T2 cur[D];
+#pragma unroll
+  for (int u = 0; u < D; ++u) cur[u] = pre[u];
+  for (int k = k0;; k += D * kstep) {
+    const int kn = k + D * kstep;
+    const bool more = waveAny(kn < K);
+    T2 nxt[D];
+    if (more) rowLoad<T2, D>(M, ld, kn, K, kstep, nxt);
+    rowAcc<T2, NR, D>(cur, k, K, kstep, v, pad, acc);
+    if (!more) break;
+#pragma unroll
+    for (int u = 0; u < D; ++u) cur[u] = nxt[u];
   }
 }
 
