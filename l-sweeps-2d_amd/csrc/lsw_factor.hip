@@ -402,6 +402,88 @@ __global__ __launch_bounds__(256) void k_front_trail(const CellDev* __restrict__
   }
 }
 
+// The same update on the matrix cores: each wavefront owns a 32 x 32
+// quadrant of the 64 x 64 tile as 2 x 2 blocks of 16 x 16, and every complex
+// product-accumulate is four real v_mfma_f64_16x16x4f64 (Cr += Tr Br - Ti Bi,
+// Ci += Tr Bi + Ti Br) over k-steps of 4 panel columns.  Operand layout of
+// the 16x16x4 f64 instruction (checked on gfx950): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; it holds D[l / 16 + 4 r][l % 16]
+// in result register r (the f32 variant holds D[4 (l / 16) + r][l % 16]).
+typedef double lswD4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_front_trail_mfma(const CellDev* __restrict__ cells,
+                                                          const ShapeDev* __restrict__ shapes,
+                                                          const FrontTask* __restrict__ fts,
+                                                          double2* __restrict__ ws, int k0) {
+  __shared__ double2 sT[64][kPanel + 1];
+  __shared__ double2 sB[kPanel][64 + 1];
+  const FrontTask ft = fts[blockIdx.y];
+  const FrontView f = frontView(cells, shapes, ft, ws);
+  const int e = f.e, R = f.R, Cc = f.Cc;
+  const int nbk = min(kPanel, e - k0);
+  if (nbk <= 0) return;
+  const int pend = k0 + nbk, cw = Cc - pend;
+  const int tilesC = (cw + 63) / 64, tilesR = (R + 63) / 64;
+  const int tile = blockIdx.x;
+  if (tile >= tilesC * tilesR) return;
+  const int r0 = (tile / tilesC) * 64, c0 = (tile % tilesC) * 64;
+  double2* M = f.M;
+  const double2* T = M + (long long)R * Cc;
+  const double2* B = T + (long long)R * kPanel;
+  const int tid = threadIdx.x;
+  const double2 zero = make_double2(0.0, 0.0);
+  for (int idx = tid; idx < 64 * kPanel; idx += 256) {
+    const int i = idx / kPanel, c = idx - i * kPanel;
+    sT[i][c] = (r0 + i < R && c < nbk) ? T[(r0 + i) * kPanel + c] : zero;
+  }
+  for (int idx = tid; idx < kPanel * 64; idx += 256) {
+    const int c = idx / 64, j = idx - c * 64;
+    sB[c][j] = (c < nbk && c0 + j < cw) ? B[(long long)c * Cc + c0 + j] : zero;
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const int qr = (wave >> 1) * 32, qc = (wave & 1) * 32;  // the wavefront's quadrant
+  const int li = lane & 15, lk = lane >> 4;
+  lswD4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = lswD4{0.0, 0.0, 0.0, 0.0};
+      ci[a][b] = lswD4{0.0, 0.0, 0.0, 0.0};
+    }
+  for (int kk = 0; kk < nbk; kk += 4) {  // columns past nbk are zero in the staging
+    double2 ta[2], tb[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) ta[a] = sT[qr + a * 16 + li][kk + lk];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) tb[b] = sB[kk + lk][qc + b * 16 + li];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a].x, tb[b].x, cr[a][b], 0, 0, 0);
+        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ta[a].y, tb[b].y, cr[a][b], 0, 0, 0);
+        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a].x, tb[b].y, ci[a][b], 0, 0, 0);
+        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a].y, tb[b].x, ci[a][b], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = c0 + qc + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = r0 + qr + a * 16 + lk + 4 * r;
+        if (i < R && j < cw) {
+          double2* t = &M[(long long)i * Cc + pend + j];
+          *t = cadd(*t, make_double2(cr[a][b][r], ci[a][b][r]));
+        }
+      }
+    }
+}
+
 __global__ void k_to_f32(const double2* __restrict__ a, float2* __restrict__ o, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
@@ -595,6 +677,16 @@ int blockedMinRows() {
   }();
   return v;
 }
+
+// Trailing updates on the matrix cores (k_front_trail_mfma) unless
+// LSW_FACTOR_MFMA=0 selects the vector-unit kernel (kept for comparison).
+bool trailMfma() {
+  static const bool v = [] {
+    const char* e = std::getenv("LSW_FACTOR_MFMA");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return v;
+}
 }  // namespace
 
 void FactorSet::factorize(int precision, hipStream_t s) {
@@ -697,9 +789,14 @@ void FactorSet::factorize(int precision, hipStream_t s) {
           const long long R = nd.e + nd.b, cw = 2LL * nd.e + nd.b - std::min(k0 + kPanel, nd.e);
           tiles = std::max(tiles, ((R + 63) / 64) * ((cw + 63) / 64));
         }
-        if (tiles > 0)
-          hipLaunchKernelGGL(k_front_trail, dim3((unsigned)tiles, nf), dim3(256), 0, s, dCells, dShapes, dFt, dWs,
-                             k0);
+        if (tiles > 0) {
+          if (trailMfma())
+            hipLaunchKernelGGL(k_front_trail_mfma, dim3((unsigned)tiles, nf), dim3(256), 0, s, dCells, dShapes, dFt,
+                               dWs, k0);
+          else
+            hipLaunchKernelGGL(k_front_trail, dim3((unsigned)tiles, nf), dim3(256), 0, s, dCells, dShapes, dFt, dWs,
+                               k0);
+        }
       }
       hipLaunchKernelGGL(k_front_store, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dFactor, dSchur, dWs);
     }

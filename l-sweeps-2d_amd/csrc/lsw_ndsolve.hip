@@ -37,6 +37,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <type_traits>
 
 #include "lsw_cplx.h"
 #include "lsw_factor.h"
@@ -87,6 +88,16 @@ long long packedWaveTarget() {
   static const long long v = [] {
     const char* e = std::getenv("LSW_PACKED_WAVES");
     return e ? std::max(1LL, std::atoll(e)) : (1LL << 40);
+  }();
+  return v;
+}
+
+// Packed launches of FP32 tasks of 8 right-hand sides run on the matrix
+// cores (k_nd_mfma) unless LSW_SOLVE_MFMA=0.
+bool solveMfma() {
+  static const bool v = [] {
+    const char* e = std::getenv("LSW_SOLVE_MFMA");
+    return !(e && std::atoi(e) == 0);
   }();
   return v;
 }
@@ -497,6 +508,156 @@ __global__ __launch_bounds__(64 * W) void k_nd_wide(const SolveTaskDev* __restri
   else
     bwdStore<T2, NR>(it, storer, dof, acc, Xo, xs);
 }
+
+// ------------------------------------------------------------------ MFMA
+// Packed entries of FP32 tasks of 8 right-hand sides on the matrix cores.
+// The gather is the packed kernel's (RP >= 16 lanes per entry, P = 64/RP
+// entries per wavefront); then the wavefront's 64 lanes are four 16-row
+// tiles, each of one entry, and every tile takes two v_mfma_f32_16x16x4f32
+// per k-step of 4 columns: with A = Ar + i Ai the tile's factor rows and V =
+// Vr + i Vi the staged right-hand sides,
+//   C = Ar [Vr Vi] + Ai [-Vi Vr] = [Re(A V) Im(A V)]   (16 x 16 real).
+// Operand layout of the instruction: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; it holds C[4 (l / 16) + r][l % 16] in result register r.
+// The entries' scalars the other lanes of a tile need are staged in LDS, and
+// the products return to one row per lane through LDS for the packed
+// kernel's scatter.
+// Per wavefront after the staging: its entries' MfmaEnt records, then the
+// 64 x 16 product (rows padded to kMfmaRow floats); kMfmaOut float2 in all.
+constexpr int kMfmaRow = 20;
+struct MfmaEnt;
+struct MfmaEnt {
+  long long mOff;  // factor offset of the entry's first row
+  int ld, k0, K, rows;
+};
+typedef float lswF4 __attribute__((ext_vector_type(4)));
+// float2 words per wavefront: 4 MfmaEnt (at most 4 entries) + 64 x kMfmaRow floats
+constexpr int kMfmaOut = (4 * (int)sizeof(MfmaEnt) + 64 * kMfmaRow * 4) / 8;
+
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_nd_mfma(const SolveTaskDev* __restrict__ tasks,
+                                                 const SolveNode* __restrict__ nodes,
+                                                 const SolveEntry* __restrict__ ent, int nent, int rpLog, int cols,
+                                                 const int* __restrict__ poolE, const int* __restrict__ poolB,
+                                                 const int* __restrict__ poolMap, const float2* __restrict__ factor,
+                                                 const float2* X, float2* Xo, float2* __restrict__ Rb,
+                                                 float2* __restrict__ G, long long xs, long long gs) {
+  constexpr int NR = 8;
+  extern __shared__ char smem_raw[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int RP = 1 << rpLog, P = 64 >> rpLog, wpb = blockDim.x >> 6;
+  const int grp = lane >> rpLog, r = lane & (RP - 1);
+  float2* vw = reinterpret_cast<float2*>(smem_raw) + (size_t)wave * P * NR * cols;  // the wavefront's regions
+  float2* v = vw + (size_t)grp * NR * cols;
+  MfmaEnt* info = reinterpret_cast<MfmaEnt*>(reinterpret_cast<float2*>(smem_raw) + (size_t)wpb * P * NR * cols +
+                                              (size_t)wave * kMfmaOut);
+  const int pad = cols - 1;
+  if (r < NR) v[pad * NR + r] = czero<float2>();  // RP >= 16 > NR
+  const long long waveId = (long long)xcdRemap(blockIdx.x, gridDim.x) * wpb + wave;
+  const long long idx = waveId * P + grp;
+  const bool valid = idx < nent;
+  const Item it = loadItem(ent, tasks, nodes, valid ? idx : nent - 1);
+  const int e = it.nd.e, F = e + it.nd.b;
+  const int k0 = FWD ? 0 : ((it.flags & kEntVisited) ? 0 : e);
+  // (a forward root entry of rows = 0 only keeps r_E: no product)
+  const int K = valid && it.rows > 0 ? (FWD ? e : F) : k0;
+  if (r == 0) {
+    MfmaEnt& m = info[grp];
+    // an entry without columns points at the factor base (valid) with ld 0
+    m.mOff = K > k0 ? it.tk.facBase + (FWD ? it.nd.fwdOff : it.nd.bwdOff) + it.r0 : 0;
+    m.ld = K > k0 ? (FWD ? it.nd.b : e) : 0;
+    m.k0 = k0;
+    m.K = K;
+    m.rows = valid ? it.rows : 0;
+  }
+  const bool ok = valid && r < it.rows;
+  float2 add[NR];
+  int dof = 0;
+  if constexpr (FWD)
+    fwdRowAdd<float2, NR>(it, ok, r, poolMap, G, gs, add);
+  else
+    dof = poolE[it.nd.E + (ok ? it.r0 + r : 0)];
+  if constexpr (FWD)
+    fwdGather<float2, NR>(it, valid, r, RP, poolE, poolMap, X, Rb, G, xs, gs, v);
+  else
+    bwdGather<float2, NR>(it, valid, k0 + r, RP, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
+  waveSync();
+  // this lane's part of tile t: row rb_t + (lane & 15) of the tile's entry
+  const int li = lane & 15, lk = lane >> 4;
+  int tg[4], trb[4], tk0[4], tK[4], tld[4];
+  const float2* tM[4];
+  int kmax = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    tg[t] = (16 * t) >> rpLog;
+    trb[t] = (16 * t) & (RP - 1);
+    const MfmaEnt& m = info[tg[t]];
+    tk0[t] = m.k0;
+    tK[t] = m.K;
+    tld[t] = m.ld;
+    const int row = trb[t] + li;
+    tM[t] = factor + m.mOff + (row < m.rows ? row : 0);
+    kmax = max(kmax, m.K - m.k0);
+  }
+  lswF4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = lswF4{0.f, 0.f, 0.f, 0.f};
+  const bool lo = li < 8;
+  // factor operands in batches of KS k-steps, the next batch loaded while
+  // the current one is multiplied (the loop is uniform over the wavefront)
+  constexpr int KS = 4;
+  float2 ca[KS][4], na[KS][4];
+  auto loadA = [&](int kb, float2 (&dst)[KS][4]) {
+#pragma unroll
+    for (int st = 0; st < KS; ++st)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int col = tk0[t] + kb + 4 * st + lk;
+        dst[st][t] = tM[t][(long long)(col < tK[t] ? col : 0) * tld[t]];
+      }
+  };
+  loadA(0, ca);
+  for (int kb = 0; kb < kmax; kb += 4 * KS) {
+    const bool more = kb + 4 * KS < kmax;
+    if (more) loadA(kb + 4 * KS, na);
+#pragma unroll
+    for (int st = 0; st < KS; ++st)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int col = tk0[t] + kb + 4 * st + lk;
+        const float2 w = vw[((size_t)tg[t] * cols + (col < tK[t] ? col : pad)) * NR + (li & 7)];
+        const float b1 = lo ? w.x : w.y;
+        const float b2 = lo ? -w.y : w.x;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[st][t].x, b1, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[st][t].y, b2, acc[t], 0, 0, 0);
+      }
+    if (!more) break;
+#pragma unroll
+    for (int st = 0; st < KS; ++st)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ca[st][t] = na[st][t];
+  }
+  // back to one row per lane (lane l is row l & (RP - 1) of entry l >> rpLog,
+  // which is row l % 16 of tile l / 16) through LDS, then the packed
+  // kernel's scatter: children's updates and destinations were loaded with
+  // the gather
+  float* cb = reinterpret_cast<float*>(info + 4);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) cb[(16 * t + 4 * lk + rr) * kMfmaRow + li] = acc[t][rr];
+  waveSync();
+  float2 res[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    res[q] = make_float2(cb[lane * kMfmaRow + q], cb[lane * kMfmaRow + 8 + q]);
+    if constexpr (FWD) res[q] = cadd(res[q], add[q]);
+  }
+  if constexpr (FWD)
+    fwdStore<float2, NR>(it, ok, r, res, G, gs);
+  else
+    bwdStore<float2, NR>(it, ok, dof, res, Xo, xs);
+}
 }  // namespace
 
 // ------------------------------------------------------------------ plans
@@ -658,7 +819,16 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       if (w > 1) W = w;
     }
     while (W > 1 && ldsWide(W) > kMaxLds) W /= 2;
-    if (W > 0 && ldsWide(W) <= kMaxLds) {
+    // FP32 tasks of 8 right-hand sides: packed entries on the matrix cores
+    const int rpm = std::max(16, rp);
+    auto ldsMfma = [&](int wpb) { return wpb * ((64.0 / rpm) * S * nrk * stage + 8.0 * kMfmaOut); };
+    if (W == 0 && S == 8.0 && nrk == kMaxRhs && solveMfma() && ldsMfma(1) <= kMaxLds) {
+      L.mode = 2;
+      L.rp = rpm;
+      int wpb = 4;
+      while (wpb > 1 && ldsMfma(wpb) > kMaxLds) wpb /= 2;
+      L.wpb = wpb;
+    } else if (W > 0 && ldsWide(W) <= kMaxLds) {
       L.mode = 1;
       L.waves = W;
     } else {
@@ -767,6 +937,19 @@ template <typename T2, int NR, bool FWD>
 void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, const T2* X, T2* Xo, T2* R, T2* G,
             hipStream_t s) {
   const SolveEntry* ent = a.ent + L.off;
+  if (L.mode == 2) {
+    if constexpr (std::is_same<T2, float2>::value && NR == kMaxRhs) {
+      const int P = 64 / L.rp;
+      const long long perBlock = (long long)P * L.wpb;
+      const unsigned grid = (unsigned)((L.n + perBlock - 1) / perBlock);
+      const size_t shm = (size_t)L.wpb * ((size_t)P * NR * L.cols + kMfmaOut) * sizeof(T2);
+      hipLaunchKernelGGL((k_nd_mfma<FWD>), dim3(grid), dim3(64 * L.wpb), shm, s, a.tasks, a.nodes, ent, L.n,
+                         ceilLog2(L.rp), L.cols, a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+    } else {
+      LSW_REQUIRE(false, LSW_ERR_STATE, "matrix-core solve launch for a non-FP32 or narrow task plan");
+    }
+    return;
+  }
   if (L.mode == 0) {
     const int P = 64 / L.rp;
     const long long perBlock = (long long)P * L.rounds * L.wpb;
