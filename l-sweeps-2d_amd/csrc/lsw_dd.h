@@ -169,6 +169,7 @@ class DDCheckerboard : public DD {
   void allocBuffers();
   void applyPrecondImpl(const double2* x, double2* y, hipStream_t s, SolveProfile* prof, int nrhs) const;
   void ensureBuffers(int nrhs) const;
+  long long taskRowsFor(int nrhs) const;
   std::vector<Rect> slotSupport(const SlotDev& s) const;
   void initDevice();
   std::vector<unsigned char> mColT, mRowT;
@@ -225,6 +226,7 @@ class DDCheckerboard : public DD {
   mutable void* dF = nullptr;     // summed sweep right-hand sides per (d, cell) and right-hand side
   std::vector<std::pair<std::string, double>> mSetupLog;  // set-up phase times (LSW_SETUP_LOG prints them)
   mutable int mBufRhs = 0;
+  mutable long long mTaskRows = 0;  // rows of dR / dG (task-major solve scratch)
   long long mXs = 0, mGs = 0;
 
   // captured applyPrecondDev graphs per (x, y, nrhs, stream)

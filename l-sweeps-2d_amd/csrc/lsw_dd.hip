@@ -1019,6 +1019,26 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
   return ref;
 }
 
+// Rows of the task-major solve scratch (R, G) a block of nrhs right-hand
+// sides needs in its largest step: planFor cuts each cell's slots into tasks
+// of perTask, and every task takes taskWidth(largest task) rows.
+long long DDCheckerboard::taskRowsFor(int nrhs) const {
+  const int perTask = nrhs == 1 ? kTaskRhs4 : mFactors->maxTaskRhs();
+  long long best = 0;
+  std::vector<long long> perCell(mCells.size());
+  for (const auto& st : mSteps) {
+    std::fill(perCell.begin(), perCell.end(), 0);
+    for (const auto& sl : st.hSlots) perCell[sl.cell] += nrhs;
+    long long tasks = 0, widest = 0;
+    for (long long c : perCell) {
+      tasks += (c + perTask - 1) / perTask;
+      widest = std::max(widest, std::min<long long>(c, perTask));
+    }
+    best = std::max(best, tasks * taskWidth((int)std::max<long long>(widest, 1)));
+  }
+  return best;
+}
+
 void DDCheckerboard::ensureBuffers(int nrhs) const {
   if (nrhs <= mBufRhs) return;
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
@@ -1028,11 +1048,15 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   dY = nullptr;
   mBufRhs = 0;
   const size_t slots = (size_t)mMaxSlots * nrhs;
+  // R and G serve every block width up to nrhs
+  long long rows = 0;
+  for (int n = 1; n <= nrhs; ++n) rows = std::max(rows, taskRowsFor(n));
+  mTaskRows = rows;
   LSW_HIP(devMalloc(&dF, (size_t)4 * mCells.size() * nrhs * mXs * es));
   LSW_HIP(devMalloc(&dX, slots * mXs * es));
   LSW_HIP(devMalloc(&dXs, slots * mXs * es));
-  LSW_HIP(devMalloc(&dR, slots * mXs * es));
-  LSW_HIP(devMalloc(&dG, slots * mGs * es));
+  LSW_HIP(devMalloc(&dR, (size_t)rows * mXs * es));
+  LSW_HIP(devMalloc(&dG, (size_t)rows * mGs * es));
   LSW_HIP(devMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
   LSW_HIP(devMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
   // the solve kernels read (and multiply by zero) entries of X, R and G they
@@ -1040,8 +1064,8 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   // problem's stream, ahead of its kernels)
   LSW_HIP(hipMemsetAsync(dX, 0, slots * mXs * es, mStream));
   LSW_HIP(hipMemsetAsync(dXs, 0, slots * mXs * es, mStream));
-  LSW_HIP(hipMemsetAsync(dR, 0, slots * mXs * es, mStream));
-  LSW_HIP(hipMemsetAsync(dG, 0, slots * mGs * es, mStream));
+  LSW_HIP(hipMemsetAsync(dR, 0, (size_t)rows * mXs * es, mStream));
+  LSW_HIP(hipMemsetAsync(dG, 0, (size_t)rows * mGs * es, mStream));
   mBufRhs = nrhs;
 }
 
@@ -1235,7 +1259,9 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
         hipLaunchKernelGGL(k_rhs_accum<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, st.dSlots, st.nSlots,
                            (const T2*)dX, mXs, dCells, (T2*)dF, nGrp);
       // solutions go to dXs: dX keeps the right-hand sides
-      mFactors->solve(planFor(st, nrhs), dX, dXs, dR, dG, mXs, mGs, s, prof);
+      const SolvePlan& plan = planFor(st, nrhs);
+      LSW_REQUIRE(plan.taskRows() <= mTaskRows, LSW_ERR_STATE, "solve scratch smaller than the plan needs");
+      mFactors->solve(plan, dX, dXs, dR, dG, mXs, mGs, s, prof);
       if (st.deferred) {
         hipLaunchKernelGGL(k_accumulate<T2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
                            st.nSlots, (const T2*)dXs, mXs, dY, n, mGy);

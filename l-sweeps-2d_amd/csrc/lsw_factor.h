@@ -82,7 +82,10 @@ struct SolvePlan {
     double bytes = 0;   // algorithmic bytes
   };
   int ntasks = 0, maxNrhs = 1;
+  int width = 1;            // right-hand sides per task in the kernels (taskWidth(maxNrhs))
   bool outOfPlace = false;  // built for separate right-hand side and solution buffers
+  // rows of the task-major scratch R and G a solve of this plan needs
+  long long taskRows() const { return (long long)ntasks * width; }
   std::vector<SolveTask> hTasks;
   SolveTaskDev* dTasks = nullptr;
   SolveEntry* dEntries = nullptr;
@@ -137,10 +140,12 @@ class FactorSet {
                  SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports = nullptr,
                  bool outOfPlace = false) const;
   // Run a plan.  X holds the right-hand sides on entry and the solutions on
-  // exit (slot-major, stride xStride complex entries); R and G are scratch
-  // with the same slot layout (G stride gStride).  Element type follows
-  // precision(): double2 or float2.  With prof, every launch is timed with
-  // HIP events (synchronises at the end).
+  // exit (slot-major, stride xStride complex entries).  R and G are scratch
+  // laid out by task: plan.taskRows() rows of xStride (R) and gStride (G)
+  // entries, the plan.width right-hand sides of a task interleaved (value q
+  // of position i of task t at (t*stride + i)*width + q).  Element type
+  // follows precision(): double2 or float2.  With prof, every launch is
+  // timed with HIP events (synchronises at the end).
   void solve(const SolvePlan& plan, void* X, void* R, void* G, long long xStride, long long gStride,
              hipStream_t s, SolveProfile* prof = nullptr) const;
   // The same with the solutions written to Xo (same layout as X); X is only

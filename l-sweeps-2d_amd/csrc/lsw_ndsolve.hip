@@ -140,6 +140,13 @@ __device__ __forceinline__ T2* trashSlot() {
   return reinterpret_cast<T2*>(gSolveTrash) + (threadIdx.x & 511);
 }
 
+// store target of NR <= 8 consecutive values a lane must not store
+template <typename T2, int NR>
+__device__ __forceinline__ T2* trashRow() {
+  static_assert(NR <= 8, "trash rows hold at most 8 values");
+  return reinterpret_cast<T2*>(gSolveTrash) + (threadIdx.x & 63) * NR;
+}
+
 // m[u] = M[(k + u*kstep)*ld] for k + u*kstep < K, else M[0] (which must be
 // valid; it meets the zero pad column in rowAcc)
 template <typename T2, int D>
@@ -199,8 +206,31 @@ __device__ __forceinline__ void rowDotPre(const T2* __restrict__ M, long long ld
 struct Item {
   SolveTaskDev tk;
   SolveNode nd;
-  int r0, rows, flags;
+  int task, r0, rows, flags;
 };
+
+// The update vectors G and the kept r_E (R) of a plan are task-major with the
+// NR right-hand sides of a task interleaved: value q of position i of task t
+// is at ((t * stride + i) * NR + q), so one position of all right-hand sides
+// is one contiguous NR-value load or store.
+template <typename T2, int NR>
+__device__ __forceinline__ long long tix(const Item& it, long long stride, long long i) {
+  return ((long long)it.task * stride + i) * NR;
+}
+
+template <typename T2, int NR>
+__device__ __forceinline__ void loadNR(const T2* __restrict__ p, T2 (&o)[NR]) {
+  const T2* a = reinterpret_cast<const T2*>(__builtin_assume_aligned(p, NR * sizeof(T2) >= 16 ? 16 : sizeof(T2)));
+#pragma unroll
+  for (int q = 0; q < NR; ++q) o[q] = a[q];
+}
+
+template <typename T2, int NR>
+__device__ __forceinline__ void storeNR(T2* __restrict__ p, const T2 (&o)[NR]) {
+  T2* a = reinterpret_cast<T2*>(__builtin_assume_aligned(p, NR * sizeof(T2) >= 16 ? 16 : sizeof(T2)));
+#pragma unroll
+  for (int q = 0; q < NR; ++q) a[q] = o[q];
+}
 
 __device__ __forceinline__ Item loadItem(const SolveEntry* __restrict__ ent, const SolveTaskDev* __restrict__ tasks,
                                          const SolveNode* __restrict__ nodes, long long idx) {
@@ -208,6 +238,7 @@ __device__ __forceinline__ Item loadItem(const SolveEntry* __restrict__ ent, con
   const SolveEntry en = ent[idx];
   it.tk = tasks[en.task];
   it.nd = nodes[en.node];
+  it.task = en.task;
   it.r0 = en.r0rows & 0xffff;
   it.rows = en.r0rows >> 16;
   it.flags = en.flags;
@@ -254,23 +285,21 @@ __device__ __forceinline__ void gatherFrontRhs(const Item& it, bool act, int k0,
       for (int q = 0; q < NR; ++q) val[u][q] = X[(long long)it.tk.slot[q] * xs + dof[u]];
     if (kids) {
 #pragma unroll
-      for (int u = 0; u < B; ++u)
+      for (int u = 0; u < B; ++u) {
+        T2 g1v[NR], g2v[NR];
+        loadNR<T2, NR>(G + tix<T2, NR>(it, gs, i1[u]), g1v);
+        loadNR<T2, NR>(G + tix<T2, NR>(it, gs, i2[u]), g2v);
 #pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          const long long sl = it.tk.slot[q];
-          const T2 g1v = G[sl * gs + i1[u]], g2v = G[sl * gs + i2[u]];
-          val[u][q] = csub(csub(val[u][q], cmask(g1v, h1[u])), cmask(g2v, h2[u]));
-        }
+        for (int q = 0; q < NR; ++q) val[u][q] = csub(csub(val[u][q], cmask(g1v[q], h1[u])), cmask(g2v[q], h2[u]));
+      }
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       const int kk = kb + u * kstep;
       const int kc = kk < K ? kk : kb;
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        v[kc * NR + q] = val[u][q];  // past K: the value of kb again
-        if (keepR && kk < K && q < it.tk.nrhs) Rb[(long long)it.tk.slot[q] * xs + dof[u]] = val[u][q];
-      }
+      for (int q = 0; q < NR; ++q) v[kc * NR + q] = val[u][q];  // past K: the value of kb again
+      if (keepR && kk < K) storeNR<T2, NR>(Rb + tix<T2, NR>(it, xs, dof[u]), val[u]);
     }
   }
 }
@@ -302,22 +331,21 @@ __device__ __forceinline__ void fwdRowAdd(const Item& it, bool ok, int r, const 
   const int a1 = m1[f], a2 = m1[F + f];
   const bool h1 = ok && c1 && a1 >= 0, h2 = ok && c2 && a2 >= 0;
   const int i1 = h1 ? it.nd.g1 + a1 : 0, i2 = h2 ? it.nd.g2 + a2 : 0;
+  T2 g1v[NR], g2v[NR];
+  loadNR<T2, NR>(G + tix<T2, NR>(it, gs, i1), g1v);
+  loadNR<T2, NR>(G + tix<T2, NR>(it, gs, i2), g2v);
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    const long long sl = it.tk.slot[q];
-    acc[q] = cadd(cmask(G[sl * gs + i1], h1), cmask(G[sl * gs + i2], h2));
-  }
+  for (int q = 0; q < NR; ++q) acc[q] = cadd(cmask(g1v[q], h1), cmask(g2v[q], h2));
 }
 
-// forward scatter of row r: g_p[r0+r] = product + children's updates
+// forward scatter of row r: g_p[r0+r] = product + children's updates (all
+// NR right-hand sides of the task: unused ones hold finite values nobody
+// reads into a result)
 template <typename T2, int NR>
 __device__ __forceinline__ void fwdStore(const Item& it, bool ok, int r, const T2 (&acc)[NR], T2* __restrict__ G,
                                          long long gs) {
-#pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    const long long sl = it.tk.slot[q];
-    if (ok && q < it.tk.nrhs) G[sl * gs + it.nd.gOff + it.r0 + r] = acc[q];
-  }
+  T2* dst = ok ? G + tix<T2, NR>(it, gs, it.nd.gOff + it.r0 + r) : trashRow<T2, NR>();
+  storeNR<T2, NR>(dst, acc);
 }
 
 // backward gather: v[k*NR + q] = r_E[k] (k < e) or x_B[k-e] (from the
@@ -353,11 +381,17 @@ __device__ __forceinline__ void bwdGather(const Item& it, bool act, int k0, int 
     for (int u = 0; u < B; ++u) {
       const int kk = kb + u * kstep;
       const int kc = kk < K ? kk : kb;
-      const T2* src = kc < e ? Rb : Xo;
+      // r_E from the task-major R, x_B from the right-hand sides' solutions
+      T2 rv[NR];
+      const bool fromR = kc < e;
+      loadNR<T2, NR>(Rb + tix<T2, NR>(it, xs, fromR ? dof[u] : 0), rv);
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
-        const long long sl = it.tk.slot[q];
-        v[kc * NR + q] = src[sl * xs + dof[u]];  // past K: the value of kb again
+        const T2 xb = Xo[(long long)it.tk.slot[q] * xs + dof[u]];
+        T2 w;  // selected per component (a select of whole values went through scratch)
+        w.x = fromR ? rv[q].x : xb.x;
+        w.y = fromR ? rv[q].y : xb.y;
+        v[kc * NR + q] = w;  // past K: the value of kb again
       }
     }
   }
@@ -766,7 +800,8 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       bwork[h] += e * kcols;
     }
   }
-  const double nrk = taskWidth(plan.maxNrhs);
+  plan.width = taskWidth(plan.maxNrhs);
+  const double nrk = plan.width;
   // Within a height, the entries of one node (and row chunk) for all tasks of
   // a cell are adjacent: they read the same factor block, and adjacent
   // entries run together on one XCD (xcdRemap), so the block comes from HBM

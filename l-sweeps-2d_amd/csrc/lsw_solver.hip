@@ -82,14 +82,16 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
   if (nrhs > mCap) {
     for (void* p : {dX, dR, dG}) if (p) (void)devFree(p);
     if (dTasks) (void)devFree(dTasks);
+    // R and G are task-major: ntasks rows of up to per right-hand sides
+    const size_t rows = (size_t)ntasks * per;
     LSW_HIP(devMalloc(&dX, (size_t)nrhs * N * es));
-    LSW_HIP(devMalloc(&dR, (size_t)nrhs * N * es));
-    LSW_HIP(devMalloc(&dG, (size_t)nrhs * G * es));
+    LSW_HIP(devMalloc(&dR, rows * N * es));
+    LSW_HIP(devMalloc(&dG, rows * G * es));
     // the solve kernels read (and multiply by zero) entries of X, R and G
     // they do not use: they must hold finite values from the start
     LSW_HIP(hipMemsetAsync(dX, 0, (size_t)nrhs * N * es, mStream));
-    LSW_HIP(hipMemsetAsync(dR, 0, (size_t)nrhs * N * es, mStream));
-    LSW_HIP(hipMemsetAsync(dG, 0, (size_t)nrhs * G * es, mStream));
+    LSW_HIP(hipMemsetAsync(dR, 0, rows * N * es, mStream));
+    LSW_HIP(hipMemsetAsync(dG, 0, rows * G * es, mStream));
     LSW_HIP(devMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
     mCap = nrhs;
   }
@@ -110,6 +112,8 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
   }
   SolvePlan plan;
   mFS->buildPlan(tasks, {}, plan);
+  LSW_REQUIRE(plan.taskRows() <= (long long)((mCap + per - 1) / per) * per, LSW_ERR_STATE,
+              "solve scratch smaller than the plan needs");
   mFS->solve(plan, dX, dR, dG, N, G, mStream);
   if (mPrecision == LSW_FP64) {
     LSW_HIP(hipMemcpyAsync(x, dX, (size_t)nrhs * N * sizeof(double2), hipMemcpyDeviceToHost, mStream));
