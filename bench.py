@@ -445,6 +445,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="lens1600")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip timing alternative launch shapes per DD and block width before the timed solves")
     ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
     ap.add_argument("--serve-models", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=40)
@@ -480,10 +482,14 @@ def main():
     cfg = CONFIGS[args.workload]()
     paths = args.paths.split(",")
     log(f"workload {cfg['name']}: grid {cfg['grid']}, board {cfg['ncells']}, omega {cfg['omega']:.3f}")
-    out, xs, prof = {}, {}, {}
+    out, xs, prof, tune = {}, {}, {}, {}
     if "plain_ref" in paths or "plain" in paths:
         dd, ts, tf = make_dd(L, cfg, L.LSW_PREC_FP64)
         log(f"FP64 setup {ts:.2f} s, factorize {tf:.3f} s")
+        if args.autotune:
+            t0 = time.perf_counter()
+            tune["fp64_x1"] = dd.autotune(1)
+            log(f"  autotune {time.perf_counter() - t0:.1f} s", tune)
         if "plain_ref" in paths:
             out["plain_ref"], xs["plain_ref"] = time_single(dd, torch, cfg, 0, cfg["tol"], cfg["max_iter"],
                                                             args.steps, args.warmup)
@@ -500,6 +506,12 @@ def main():
     if "mixed" in paths or "batched" in paths:
         dd, ts, tf = make_dd(L, cfg, L.LSW_PREC_FP32)
         log(f"FP32 setup {ts:.2f} s, factorize {tf:.3f} s")
+        if args.autotune:
+            t0 = time.perf_counter()
+            tune["fp32_x1"] = dd.autotune(1)
+            if "batched" in paths:
+                tune["fp32_x16"] = dd.autotune(16)
+            log(f"  autotune {time.perf_counter() - t0:.1f} s", tune)
         if "mixed" in paths:
             out["mixed"], xs["mixed"] = time_single(dd, torch, cfg, 1, 1e-6, 60, args.steps, args.warmup)
             out["mixed"].update(factorize_s=tf, setup_s=ts, gmres="flexible", tol=1e-6)
@@ -560,6 +572,8 @@ def main():
     if "plain_ref" in xs and "plain" in xs:
         line["plain_ref_vs_plain_rel_l2"] = float(np.linalg.norm(xs["plain_ref"] - xs["plain"]) /
                                                   np.linalg.norm(xs["plain"]))
+    if tune:
+        line["launch_shapes"] = tune
     line["dofs"] = n_dofs
     line["dofs_per_s"] = round(n_dofs * head["solves_per_s"], 1)
     traffic = load_traffic(cfg["name"])

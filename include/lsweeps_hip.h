@@ -113,6 +113,13 @@ int lsw_dd_stats(lsw_dd* dd, long long* solves_per_precond, double* solve_bytes_
  * (off by default).  While a graph is being captured no other thread may
  * issue legacy-stream work.  No counterpart in the reference. */
 int lsw_dd_set_graphs(lsw_dd* dd, int on);
+/* Launch-shape autotuning for blocks of nrhs right-hand sides: times M^-1
+ * under the default shape and under alternatives (column-split thresholds
+ * and caps, matrix cores on/off), one choice at a time, and keeps the
+ * fastest for that block width (synchronous).  out[0..6] = default ms, best
+ * ms per application, shapes tried, chosen latency-wave threshold, wide
+ * budget, wide cap, matrix cores (0/1).  No counterpart in the reference. */
+int lsw_dd_autotune(lsw_dd* dd, int nrhs, double* out);
 /* Wait for all work queued on the DD's own stream. */
 int lsw_dd_sync(lsw_dd* dd);
 /* One applyPrecond on device vectors with every batched local-solve launch

@@ -326,6 +326,20 @@ int lsw_dd_set_graphs(lsw_dd* dd, int on) {
   });
 }
 
+int lsw_dd_autotune(lsw_dd* dd, int nrhs, double* out) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && out, LSW_ERR_ARG, "null pointer");
+    const auto r = dd->dd->autotune(nrhs);
+    out[0] = r.defaultMs;
+    out[1] = r.bestMs;
+    out[2] = (double)r.tried;
+    out[3] = (double)r.shape.latencyWaves;
+    out[4] = (double)r.shape.wideBudget;
+    out[5] = (double)r.shape.wideCap;
+    out[6] = r.shape.mfma ? 1.0 : 0.0;
+  });
+}
+
 int lsw_dd_sync(lsw_dd* dd) {
   return guarded([&] {
     LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");

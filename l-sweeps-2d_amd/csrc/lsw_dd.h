@@ -131,6 +131,15 @@ class DDCheckerboard : public DD {
   // Same, timing every batched-solve launch with HIP events (synchronises
   // after each sweep step; use outside timed regions).
   void applyPrecondProfiled(const double2* x, double2* y, SolveProfile& prof, int nrhs = 1) const;
+  // Times M^{-1} for blocks of nrhs right-hand sides under alternative launch
+  // shapes (PlanShape) and keeps the fastest for that block width.  Uses the
+  // DD's stream and synchronises; not for timed regions.
+  struct TuneResult {
+    double defaultMs = 0, bestMs = 0;  // per application
+    int tried = 0;
+    PlanShape shape;
+  };
+  TuneResult autotune(int nrhs) const;
   // Sizes the sweep buffers for blocks of up to nrhs right-hand sides now
   // (otherwise they grow on first use).
   void reserveBlock(int nrhs) const { ensureBuffers(nrhs); }
@@ -210,6 +219,10 @@ class DDCheckerboard : public DD {
   };
   std::vector<Step> mSteps;
   const SolvePlan& planFor(const Step& st, int nrhs) const;
+  // launch shapes per block width (PlanShape::defaults() until autotuned)
+  mutable std::map<int, PlanShape> mShapes;
+  const PlanShape& shapeFor(int nrhs) const;
+  void dropPlans(int nrhs) const;
   int mMaxSlots = 0;
   int mTmax = 0;
   long long mTraceEntries = 0;

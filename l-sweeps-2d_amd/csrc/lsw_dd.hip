@@ -1013,7 +1013,8 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
   }
   auto plan = std::make_unique<SolvePlan>();
   // the final solves need the whole solution; the sweep solves only their traces
-  mFactors->buildPlan(tasks, supports, *plan, st.deferred ? nullptr : &needs, /*outOfPlace=*/true);
+  mFactors->buildPlan(tasks, supports, *plan, st.deferred ? nullptr : &needs, /*outOfPlace=*/true,
+                      &shapeFor(nrhs));
   const SolvePlan& ref = *plan;
   st.plans[nrhs] = std::move(plan);
   return ref;
@@ -1198,6 +1199,98 @@ void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s
     it = mGraphs.emplace(key, ex).first;
   }
   LSW_HIP(hipGraphLaunch(it->second, s));
+}
+
+const PlanShape& DDCheckerboard::shapeFor(int nrhs) const {
+  auto it = mShapes.find(nrhs);
+  if (it == mShapes.end()) it = mShapes.emplace(nrhs, PlanShape::defaults()).first;
+  return it->second;
+}
+
+void DDCheckerboard::dropPlans(int nrhs) const {
+  clearGraphs();  // captured graphs point at the plans' work lists
+  for (const auto& st : mSteps) st.plans.erase(nrhs);
+}
+
+// Launch-shape autotuning: time M^{-1} for blocks of nrhs right-hand sides
+// (random input, 1 warm-up + 3 timed applications each) under the default
+// PlanShape, then change one choice at a time (column-split threshold of
+// latency-bound launches, column-split budget and cap of the launches of few
+// large entries, matrix cores) and keep a change when it is at least 1%
+// faster.  The plans for nrhs are rebuilt with the winner on next use.
+DDCheckerboard::TuneResult DDCheckerboard::autotune(int nrhs) const {
+  LSW_REQUIRE(mFactors && mFactors->factorized(), LSW_ERR_STATE, "autotune before factorize");
+  LSW_REQUIRE(nrhs >= 1, LSW_ERR_ARG, "autotune: nrhs must be positive");
+  ensureBuffers(nrhs);
+  const long long n = N() * nrhs;
+  std::vector<double2> hx((size_t)n);
+  unsigned long long st = 0x9E3779B97F4A7C15ULL;
+  for (auto& v : hx) {  // fixed pseudo-random input (xorshift)
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    const double a = (double)(st >> 11) / 9007199254740992.0;
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    const double b = (double)(st >> 11) / 9007199254740992.0;
+    v = make_double2(a - 0.5, b - 0.5);
+  }
+  double2 *dx = nullptr, *dy = nullptr;
+  LSW_HIP(devMalloc(&dx, (size_t)n * sizeof(double2)));
+  LSW_HIP(devMalloc(&dy, (size_t)n * sizeof(double2)));
+  LSW_HIP(uploadSync(dx, hx.data(), (size_t)n * sizeof(double2)));
+  hipEvent_t e0, e1;
+  LSW_HIP(hipEventCreate(&e0));
+  LSW_HIP(hipEventCreate(&e1));
+  auto timeShape = [&](const PlanShape& sh) {
+    mShapes[nrhs] = sh;
+    dropPlans(nrhs);
+    applyPrecondImpl(dx, dy, mStream, nullptr, nrhs);  // builds the plans, warms the caches
+    LSW_HIP(hipEventRecord(e0, mStream));
+    for (int k = 0; k < 3; ++k) applyPrecondImpl(dx, dy, mStream, nullptr, nrhs);
+    LSW_HIP(hipEventRecord(e1, mStream));
+    LSW_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LSW_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms / 3.0;
+  };
+  TuneResult r;
+  PlanShape best = PlanShape::defaults();
+  r.defaultMs = r.bestMs = timeShape(best);
+  r.tried = 1;
+  auto consider = [&](PlanShape cand) {
+    const double ms = timeShape(cand);
+    ++r.tried;
+    if (ms < 0.99 * r.bestMs) {
+      r.bestMs = ms;
+      best = cand;
+    }
+  };
+  for (long long lw : {0LL, 1024LL, 16384LL}) {
+    PlanShape c = best;
+    c.latencyWaves = lw;
+    consider(c);
+  }
+  for (long long wb : {0LL, 16384LL}) {
+    PlanShape c = best;
+    c.wideBudget = wb;
+    consider(c);
+  }
+  {
+    PlanShape c = best;
+    c.wideCap = 4;
+    consider(c);
+  }
+  if (mPrecision == LSW_FP32 && nrhs > 1) {
+    PlanShape c = best;
+    c.mfma = !c.mfma;
+    consider(c);
+  }
+  mShapes[nrhs] = best;
+  dropPlans(nrhs);
+  r.shape = best;
+  LSW_HIP(hipEventDestroy(e0));
+  LSW_HIP(hipEventDestroy(e1));
+  (void)devFree(dx);
+  (void)devFree(dy);
+  return r;
 }
 
 void DDCheckerboard::clearGraphs() const {

@@ -65,6 +65,18 @@ struct SolveEntry {
   int task, node, r0rows, flags;  // r0rows = r0 | rows << 16
 };
 
+// Launch-shape choices of a solve plan (FactorSet::buildPlan).  defaults()
+// are the built-in rules (with the LSW_LATENCY_WAVES and LSW_SOLVE_MFMA
+// environment overrides); DDCheckerboard::autotune times alternatives on the
+// problem itself and keeps the fastest per block width.
+struct PlanShape {
+  long long latencyWaves = 4096;  // packed launches of fewer wavefronts and rows <= 16 split columns
+  long long wideBudget = 4096;    // heights of few large entries split columns while entries x W x 2 fit this
+  int wideCap = 8;                // most wavefronts one entry's columns are split over
+  bool mfma = true;               // FP32 8-right-hand-side packed launches on the matrix cores
+  static PlanShape defaults();
+};
+
 // Launch plan of one batch of solves (device work lists per tree height).
 struct SolvePlan {
   struct Launch {
@@ -136,9 +148,10 @@ class FactorSet {
   // outOfPlace: the plan will run with separate right-hand side and solution
   // buffers (solve(plan, X, Xo, ...), Xo != X); its roots then form r_E in
   // their backward entries, and the forward pass has no launch for them.
+  // shape: launch-shape choices (nullptr: PlanShape::defaults()).
   void buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
                  SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports = nullptr,
-                 bool outOfPlace = false) const;
+                 bool outOfPlace = false, const PlanShape* shape = nullptr) const;
   // Run a plan.  X holds the right-hand sides on entry and the solutions on
   // exit (slot-major, stride xStride complex entries).  R and G are scratch
   // laid out by task: plan.taskRows() rows of xStride (R) and gStride (G)

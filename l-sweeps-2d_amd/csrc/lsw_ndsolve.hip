@@ -69,16 +69,6 @@ __device__ __forceinline__ void waveSync() {
 // conditions (waveAny).  The update buffers are zeroed when allocated, so a
 // masked read never meets a NaN.
 constexpr int kDotDepth = 8;  // factor loads per batch of a product
-// Packed launches of fewer wavefronts are latency-bound and take the
-// column-split shape (LSW_LATENCY_WAVES overrides the threshold for tuning;
-// 0 turns the rule off).
-long long latencyWaves() {
-  static const long long v = [] {
-    const char* e = std::getenv("LSW_LATENCY_WAVES");
-    return e ? std::max(0LL, std::atoll(e)) : 4096LL;
-  }();
-  return v;
-}
 
 // A packed launch of more than this many wavefronts gives each lane group
 // several entries in turn ("rounds").  By default every lane group takes one
@@ -92,15 +82,6 @@ long long packedWaveTarget() {
   return v;
 }
 
-// Packed launches of FP32 tasks of 8 right-hand sides run on the matrix
-// cores (k_nd_mfma) unless LSW_SOLVE_MFMA=0.
-bool solveMfma() {
-  static const bool v = [] {
-    const char* e = std::getenv("LSW_SOLVE_MFMA");
-    return !(e && std::atoi(e) == 0);
-  }();
-  return v;
-}
 
 // Workgroups are dispatched to the 8 XCDs (each with its own L2) in turn:
 // workgroup b runs on XCD b % 8.  This gives each XCD a contiguous range of
@@ -716,8 +697,24 @@ int ceilLog2(int x) {
 
 }  // namespace
 
+// Packed launches of fewer wavefronts are latency-bound and take the
+// column-split shape (LSW_LATENCY_WAVES overrides the threshold; 0 turns the
+// rule off); packed launches of FP32 tasks of 8 right-hand sides run on the
+// matrix cores (k_nd_mfma) unless LSW_SOLVE_MFMA=0.
+PlanShape PlanShape::defaults() {
+  static const PlanShape d = [] {
+    PlanShape p;
+    if (const char* e = std::getenv("LSW_LATENCY_WAVES")) p.latencyWaves = std::max(0LL, std::atoll(e));
+    if (const char* e = std::getenv("LSW_SOLVE_MFMA")) p.mfma = std::atoi(e) != 0;
+    return p;
+  }();
+  return d;
+}
+
 void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
-                          SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports, bool outOfPlace) const {
+                          SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports, bool outOfPlace,
+                          const PlanShape* shapeIn) const {
+  const PlanShape shp = shapeIn ? *shapeIn : PlanShape::defaults();
   LSW_REQUIRE(!mTrees.empty(), LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(supports.empty() || supports.size() == tasks.size(), LSW_ERR_ARG, "one support per task");
   LSW_REQUIRE(!bwdSupports || bwdSupports->size() == tasks.size(), LSW_ERR_ARG, "one backward support per task");
@@ -834,7 +831,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     while (rp < 64 && ldsWave(rp) > kMaxLds) rp *= 2;
     const long long packedWaves = ((long long)v.size() * rp + 63) / 64;
     int W = 0;
-    if (packedWaves < latencyWaves() && maxRows > 0 && maxRows <= 16) {
+    if (packedWaves < shp.latencyWaves && maxRows > 0 && maxRows <= 16) {
       // Few wavefronts of short rows: the launch is a chain of memory round
       // trips, and a packed lane walks its whole row while most lanes of an
       // entry's wavefront share its few rows.  Split the columns instead (W
@@ -846,18 +843,19 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       const int CG = 64 / RPw;
       W = 1;
       const int depth = kDotDepth;
-      while (W < 8 && (cols + CG * W - 1) / (CG * W) > depth && (long long)v.size() * W * 2 <= 8192) W *= 2;
+      while (W < shp.wideCap && (cols + CG * W - 1) / (CG * W) > depth && (long long)v.size() * W * 2 <= 8192)
+        W *= 2;
     } else {
       // few large entries: split their columns over W wavefronts
       int w = 1;
-      while (w < 8 && (double)v.size() * w * 2 <= 4096.0 && per >= 1024.0 * w * 2) w *= 2;
+      while (w < shp.wideCap && (double)v.size() * w * 2 <= (double)shp.wideBudget && per >= 1024.0 * w * 2) w *= 2;
       if (w > 1) W = w;
     }
     while (W > 1 && ldsWide(W) > kMaxLds) W /= 2;
     // FP32 tasks of 8 right-hand sides: packed entries on the matrix cores
     const int rpm = std::max(16, rp);
     auto ldsMfma = [&](int wpb) { return wpb * ((64.0 / rpm) * S * nrk * stage + 8.0 * kMfmaOut); };
-    if (W == 0 && S == 8.0 && nrk == kMaxRhs && solveMfma() && ldsMfma(1) <= kMaxLds) {
+    if (W == 0 && S == 8.0 && nrk == kMaxRhs && shp.mfma && ldsMfma(1) <= kMaxLds) {
       L.mode = 2;
       L.rp = rpm;
       int wpb = 4;

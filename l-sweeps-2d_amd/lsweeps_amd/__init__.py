@@ -33,7 +33,7 @@ EXPORTED = [
     "lsw_gmres_solve_block", "lsw_gmres_solve_block_dev", "lsw_dd_assemble_rhs_block_dev",
     "lsw_server_create", "lsw_server_destroy", "lsw_server_add_problem", "lsw_server_submit",
     "lsw_server_wait", "lsw_server_get_stats", "lsw_server_problem_bytes", "lsw_dd_write_vtk",
-    "lsw_dd_set_graphs",
+    "lsw_dd_set_graphs", "lsw_dd_autotune",
 ]
 
 
@@ -136,6 +136,7 @@ _sig("lsw_dd_apply_a_dev", _P, _P, _P, _P)
 _sig("lsw_dd_stats", _P, C.POINTER(C.c_longlong), _D, C.POINTER(C.c_longlong))
 _sig("lsw_dd_sync", _P)
 _sig("lsw_dd_set_graphs", _P, C.c_int)
+_sig("lsw_dd_autotune", _P, C.c_int, _D)
 _sig("lsw_dd_profile_precond", _P, _P, _P, _D)
 _sig("lsw_dd_apply_precond_block", _P, C.c_int, _D, _D)
 _sig("lsw_dd_apply_precond_block_dev", _P, C.c_int, _P, _P, _P)
@@ -320,6 +321,15 @@ class DDCheckerboard:
 
     def set_graphs(self, on=True):
         _check(_lib.lsw_dd_set_graphs(self._h, int(bool(on))))
+
+    def autotune(self, nrhs=1):
+        """Time M^{-1} for blocks of nrhs right-hand sides under alternative
+        launch shapes and keep the fastest for that block width."""
+        out = np.zeros(7)
+        _check(_lib.lsw_dd_autotune(self._h, int(nrhs), _dptr(out)))
+        return {"default_ms": float(out[0]), "best_ms": float(out[1]), "tried": int(out[2]),
+                "latency_waves": int(out[3]), "wide_budget": int(out[4]), "wide_cap": int(out[5]),
+                "mfma": bool(out[6])}
 
     def write_vtk(self, x, path):
         """<path>.vtk (OutputVTK layout) and <path>.lswgrid (full precision)."""
