@@ -1273,9 +1273,10 @@ DDCheckerboard::TuneResult DDCheckerboard::autotune(int nrhs) const {
     c.wideBudget = wb;
     consider(c);
   }
-  {
+  for (int cap : {4, 16}) {  // 16: FP32 tasks of up to 4 right-hand sides (clamped to 8 otherwise)
+    if (cap == 16 && !(mPrecision == LSW_FP32 && nrhs == 1)) continue;
     PlanShape c = best;
-    c.wideCap = 4;
+    c.wideCap = cap;
     consider(c);
   }
   if (mPrecision == LSW_FP32 && nrhs > 1) {

@@ -72,7 +72,8 @@ struct SolveEntry {
 struct PlanShape {
   long long latencyWaves = 4096;  // packed launches of fewer wavefronts and rows <= 16 split columns
   long long wideBudget = 4096;    // heights of few large entries split columns while entries x W x 2 fit this
-  int wideCap = 8;                // most wavefronts one entry's columns are split over
+  int wideCap = 8;                // most wavefronts one entry's columns are split over (2..16; 16 only
+                                  // for FP32 tasks of up to 4 right-hand sides)
   bool mfma = true;               // FP32 8-right-hand-side packed launches on the matrix cores
   static PlanShape defaults();
 };

@@ -714,7 +714,7 @@ PlanShape PlanShape::defaults() {
 void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector<std::vector<Rect>>& supports,
                           SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports, bool outOfPlace,
                           const PlanShape* shapeIn) const {
-  const PlanShape shp = shapeIn ? *shapeIn : PlanShape::defaults();
+  PlanShape shp = shapeIn ? *shapeIn : PlanShape::defaults();
   LSW_REQUIRE(!mTrees.empty(), LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(supports.empty() || supports.size() == tasks.size(), LSW_ERR_ARG, "one support per task");
   LSW_REQUIRE(!bwdSupports || bwdSupports->size() == tasks.size(), LSW_ERR_ARG, "one backward support per task");
@@ -799,6 +799,8 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   }
   plan.width = taskWidth(plan.maxNrhs);
   const double nrk = plan.width;
+  // 16-wavefront splits exist for FP32 tasks of up to 4 right-hand sides
+  shp.wideCap = std::min(shp.wideCap, (mPrecision == LSW_FP32 && plan.width <= kTaskRhs4) ? 16 : 8);
   // Within a height, the entries of one node (and row chunk) for all tasks of
   // a cell are adjacent: they read the same factor block, and adjacent
   // entries run together on one XCD (xcdRemap), so the block comes from HBM
@@ -1008,9 +1010,18 @@ void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, co
       hipLaunchKernelGGL((k_nd_wide<T2, NR, 4, FWD>), grid, dim3(256), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
                          a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
       break;
-    default:
+    case 8:
       hipLaunchKernelGGL((k_nd_wide<T2, NR, 8, FWD>), grid, dim3(512), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
                          a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+      break;
+    default:
+      // 16 wavefronts (1024 threads, at most 128 VGPRs): FP32 tasks of up to 4
+      // right-hand sides only; the others would spill
+      if constexpr (std::is_same<T2, float2>::value && NR <= kTaskRhs4)
+        hipLaunchKernelGGL((k_nd_wide<T2, NR, 16, FWD>), grid, dim3(1024), shm, s, a.tasks, a.nodes, ent, L.n,
+                           L.cols, a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+      else
+        LSW_REQUIRE(false, LSW_ERR_STATE, "16-wavefront solve launch for FP64 or 8-right-hand-side tasks");
       break;
   }
 }
