@@ -337,6 +337,7 @@ int lsw_dd_autotune(lsw_dd* dd, int nrhs, double* out) {
     out[4] = (double)r.shape.wideBudget;
     out[5] = (double)r.shape.wideCap;
     out[6] = r.shape.mfma ? 1.0 : 0.0;
+    out[7] = (double)r.shape.latencyRows;
   });
 }
 

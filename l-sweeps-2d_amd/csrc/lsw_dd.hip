@@ -1214,9 +1214,9 @@ void DDCheckerboard::dropPlans(int nrhs) const {
 
 // Launch-shape autotuning: time M^{-1} for blocks of nrhs right-hand sides
 // (random input, 1 warm-up + 3 timed applications each) under the default
-// PlanShape, then change one choice at a time (column-split threshold of
-// latency-bound launches, column-split budget and cap of the launches of few
-// large entries, matrix cores) and keep a change when it is at least 1%
+// PlanShape, then change one choice at a time (column-split threshold and
+// row limit of latency-bound launches, column-split budget and cap of the
+// launches of few large entries, matrix cores) and keep a change when it is at least 1%
 // faster.  The plans for nrhs are rebuilt with the winner on next use.
 DDCheckerboard::TuneResult DDCheckerboard::autotune(int nrhs) const {
   LSW_REQUIRE(mFactors && mFactors->factorized(), LSW_ERR_STATE, "autotune before factorize");
@@ -1266,6 +1266,11 @@ DDCheckerboard::TuneResult DDCheckerboard::autotune(int nrhs) const {
   for (long long lw : {0LL, 1024LL, 16384LL}) {
     PlanShape c = best;
     c.latencyWaves = lw;
+    consider(c);
+  }
+  for (int lr : {32, 64}) {
+    PlanShape c = best;
+    c.latencyRows = lr;
     consider(c);
   }
   for (long long wb : {0LL, 16384LL}) {

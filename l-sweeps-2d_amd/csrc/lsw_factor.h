@@ -70,7 +70,8 @@ struct SolveEntry {
 // environment overrides); DDCheckerboard::autotune times alternatives on the
 // problem itself and keeps the fastest per block width.
 struct PlanShape {
-  long long latencyWaves = 4096;  // packed launches of fewer wavefronts and rows <= 16 split columns
+  long long latencyWaves = 4096;  // packed launches of fewer wavefronts and rows <= latencyRows split columns
+  int latencyRows = 16;
   long long wideBudget = 4096;    // heights of few large entries split columns while entries x W x 2 fit this
   int wideCap = 8;                // most wavefronts one entry's columns are split over (2..16; 16 only
                                   // for FP32 tasks of up to 4 right-hand sides)

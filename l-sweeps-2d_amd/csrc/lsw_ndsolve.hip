@@ -833,7 +833,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     while (rp < 64 && ldsWave(rp) > kMaxLds) rp *= 2;
     const long long packedWaves = ((long long)v.size() * rp + 63) / 64;
     int W = 0;
-    if (packedWaves < shp.latencyWaves && maxRows > 0 && maxRows <= 16) {
+    if (packedWaves < shp.latencyWaves && maxRows > 0 && maxRows <= shp.latencyRows) {
       // Few wavefronts of short rows: the launch is a chain of memory round
       // trips, and a packed lane walks its whole row while most lanes of an
       // entry's wavefront share its few rows.  Split the columns instead (W
