@@ -20,15 +20,15 @@ enum Ctx { kS1 = 0, kR = 1, kL = 2, kU = 3, kD = 4, kBL = 5, kTR = 6, kBR = 7, k
 
 // ------------------------------------------------------------------ kernels
 
-// Right-hand side of every local solve of one sweep step.
-//   stage 1: the windowed input restricted to the cell's owned set of its DD
-//            (getSeparatedRHS + reduce/expand, src/DDCheckerboard.cpp:3564-3674)
-//   sweeps : equivalent sources res0 = A01 v1 on set 0, res1 = -A10 v0 on set 1
-//            (SubdomainStructured::getEquivalentSources/extractLTraces,
-//            src/SubdomainStructured.cpp:1304-1539)
-// Slot s of a block of right-hand sides is (slots[s % nSlots1], rhs = s / nSlots1);
-// its buffer is X + s*xs, its trace storage T + rhs*traceStride and its input
-// vector xin + rhs*Nglob.
+// The local vectors of a step (right-hand sides X, solutions Xs) are laid
+// out like the solve plan's tasks: slot s of the step's block lives at task
+// t, lane q (tq[s] = t*W + q), and its point p at X[slotBase + p*W].
+__device__ __forceinline__ long long slotBase(const int* __restrict__ tq, int sidx, int W, long long xs) {
+  const int v = tq[sidx];
+  const int t = v / W;
+  return ((long long)t * xs) * W + (v - t * W);
+}
+
 // Right-hand side of sweep slot `sl` at point (i, j) of its cell (0 off the
 // two trace sets): res0 = A01 v1 on set 0, res1 = -A10 v0 on set 1.
 template <typename T2>
@@ -79,63 +79,96 @@ __device__ __forceinline__ bool segmentPoint(const SlotDev& sl, int m, int t, in
   return true;
 }
 
-// Right-hand side of every local solve of one sweep step.
+// Right-hand side of one local solve at point (i, j) = p of its cell.
 //   stage 1: the windowed input restricted to the cell's owned set of its DD
 //            (getSeparatedRHS + reduce/expand, src/DDCheckerboard.cpp:3564-3674)
 //   sweeps : equivalent sources res0 = A01 v1 on set 0, res1 = -A10 v0 on set 1
 //            (SubdomainStructured::getEquivalentSources/extractLTraces,
 //            src/SubdomainStructured.cpp:1304-1539)
 // Slot s of a block of right-hand sides is (slots[s % nSlots1], rhs = s / nSlots1);
-// its buffer is X + s*xs, its trace storage T + rhs*traceStride and its input
-// vector xin + rhs*Nglob.  Full mode writes every point of the slot's cell;
-// sparse mode (the sweep steps after the first) writes only the trace-set
-// points of sweep slots: the buffers are zero elsewhere (cleared before the
-// first sweep step, and each sweep step's k_post clears its own sets).
+// its trace storage is T + rhs*traceStride and its input vector xin + rhs*Nglob;
+// a final solve also adds the summed sweep right-hand sides f.
 template <typename T2>
-__global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
-                            long long traceStride, long long Nglob, const double2* __restrict__ xin,
-                            const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT,
-                            int Gy, const double2* __restrict__ coef, const T2* __restrict__ T,
-                            T2* __restrict__ X, long long xs, const T2* __restrict__ F, int nGrp, int sparse) {
+__device__ T2 slotRhs(const SlotDev& sl, const CellDev& cd, int i, int j, const double2* __restrict__ xin,
+                      const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT, int Gy,
+                      const double2* __restrict__ coef, const T2* __restrict__ T, const T2* __restrict__ f) {
+  T2 out = CTraits<T2>::make(0, 0);
+  if (sl.kind == 0) {
+    const int slr = sl.d & 1, sud = sl.d >> 1;
+    const bool own = i >= cd.vlr[slr][2] && i < cd.vlr[slr][5] && j >= cd.vud[sud][2] && j < cd.vud[sud][5];
+    if (own) {
+      const int gi = i + cd.ox, gj = j + cd.oy;
+      const bool ct = colT[gi] != 0, rt = rowT[gj] != 0;
+      const bool keep = (sl.d == 0) ? (!ct && !rt) : (sl.d == 1) ? (ct && !rt) : (sl.d == 2) ? (!ct && rt) : (ct && rt);
+      if (keep) {
+        const double2 v = xin[(long long)gi * Gy + gj];
+        out = CTraits<T2>::make(v.x, v.y);
+      }
+    }
+  } else {
+    out = sweepRhs<T2>(sl, cd, i, j, coef, T);
+  }
+  if (f) out = cadd(out, f[(long long)i * cd.ny + j]);
+  return out;
+}
+
+// Full right-hand sides (stage 1 and the final solves), one workgroup row per
+// solve task: consecutive threads write consecutive lanes of a point, so the
+// task-major buffer is written contiguously; lanes past the task's
+// right-hand sides get zeros.
+template <typename T2>
+__global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
+                                  const SolveTaskDev* __restrict__ tasks, int W, long long traceStride,
+                                  long long Nglob, const double2* __restrict__ xin,
+                                  const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT,
+                                  int Gy, const double2* __restrict__ coef, const T2* __restrict__ T,
+                                  T2* __restrict__ X, long long xs, const T2* __restrict__ F, int nGrp) {
+  const int t = blockIdx.y;
+  const SolveTaskDev& tk = tasks[t];
+  const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
+  const int ny = cd.ny;
+  const long long total = (long long)cd.nx * ny * W;
+  T2* x = X + (long long)t * xs * W;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(idx / W), q = (int)(idx - (long long)p * W);
+    T2 out = CTraits<T2>::make(0, 0);
+    if (q < tk.nrhs) {
+      const int sidx = tk.slot[q];
+      const SlotDev& sl = slots[sidx % nSlots1];
+      const int rhs = sidx / nSlots1;
+      const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
+      const int i = p / ny, j = p - i * ny;
+      out = slotRhs<T2>(sl, cd, i, j, xin + (long long)rhs * Nglob, colT, rowT, Gy, coef,
+                        T + (long long)rhs * traceStride, f);
+    }
+    x[idx] = out;
+  }
+}
+
+// Sweep right-hand sides of the steps after the first: only the trace-set
+// points of each slot are written (the buffers are zero elsewhere: cleared
+// before the first sweep step, and each sweep step's k_post clears its own
+// sets).
+template <typename T2>
+__global__ void k_build_rhs_sparse(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots,
+                                   int nSlots1, const int* __restrict__ tq, int W, long long traceStride,
+                                   const double2* __restrict__ coef, const T2* __restrict__ T, T2* __restrict__ X,
+                                   long long xs) {
   const int sidx = blockIdx.y;
   const SlotDev& sl = slots[sidx % nSlots1];
+  if (sl.kind == 0) return;  // stage-1 slots are built in full
   const int rhs = sidx / nSlots1;
-  xin += (long long)rhs * Nglob;
   T += (long long)rhs * traceStride;
   const CellDev& cd = cells[sl.cell];
-  const int nx = cd.nx, ny = cd.ny, N = nx * ny;
-  T2* x = X + (long long)sidx * xs;
-  if (sparse && sl.kind != 0) {
-    for (int m = 0; m < 4; ++m)
-      for (int t = blockIdx.x * blockDim.x + threadIdx.x;; t += gridDim.x * blockDim.x) {
-        int i, j;
-        if (!segmentPoint(sl, m, t, i, j)) break;
-        x[(long long)i * ny + j] = sweepRhs<T2>(sl, cd, i, j, coef, T);
-      }
-    return;
-  }
-  const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
-  const int slr = sl.d & 1, sud = sl.d >> 1;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
-    const int i = p / ny, j = p - i * ny;
-    T2 out = CTraits<T2>::make(0, 0);
-    if (sl.kind == 0) {
-      const bool own = i >= cd.vlr[slr][2] && i < cd.vlr[slr][5] && j >= cd.vud[sud][2] && j < cd.vud[sud][5];
-      if (own) {
-        const int gi = i + cd.ox, gj = j + cd.oy;
-        const bool ct = colT[gi] != 0, rt = rowT[gj] != 0;
-        const bool keep = (sl.d == 0) ? (!ct && !rt) : (sl.d == 1) ? (ct && !rt) : (sl.d == 2) ? (!ct && rt) : (ct && rt);
-        if (keep) {
-          const double2 v = xin[(long long)gi * Gy + gj];
-          out = CTraits<T2>::make(v.x, v.y);
-        }
-      }
-    } else {
-      out = sweepRhs<T2>(sl, cd, i, j, coef, T);
+  const int ny = cd.ny;
+  T2* x = X + slotBase(tq, sidx, W, xs);
+  for (int m = 0; m < 4; ++m)
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x;; t += gridDim.x * blockDim.x) {
+      int i, j;
+      if (!segmentPoint(sl, m, t, i, j)) break;
+      x[((long long)i * ny + j) * W] = sweepRhs<T2>(sl, cd, i, j, coef, T);
     }
-    if (f) out = cadd(out, f[p]);
-    x[p] = out;
-  }
 }
 
 // Sum the right-hand sides of the sweep solves of each (d, cell) of a step
@@ -147,15 +180,15 @@ __global__ void k_build_rhs(const CellDev* __restrict__ cells, const SlotDev* __
 // the other.
 template <typename T2>
 __global__ void k_rhs_accum(const GroupDev* __restrict__ groups, int nGroups1, const SlotDev* __restrict__ slots,
-                            int nSlots1, const T2* __restrict__ X, long long xs, const CellDev* __restrict__ cells,
-                            T2* __restrict__ F, int nGrp) {
+                            int nSlots1, const int* __restrict__ tq, int W, const T2* __restrict__ X, long long xs,
+                            const CellDev* __restrict__ cells, T2* __restrict__ F, int nGrp) {
   const GroupDev& g = groups[blockIdx.x % nGroups1];
   const int rhs = blockIdx.x / nGroups1;
   for (int k = 0; k < g.n; ++k) {
     const SlotDev& sl = slots[g.slot[k]];
     if (sl.kind != 0) {
       const int ny = cells[sl.cell].ny;
-      const T2* x = X + ((long long)rhs * nSlots1 + g.slot[k]) * xs;
+      const T2* x = X + slotBase(tq, rhs * nSlots1 + g.slot[k], W, xs);
       T2* f = F + ((long long)rhs * nGrp + sl.grp) * xs;
       // segments: col 0, row 0, col 1, row 1
       auto onSeg = [&](int m, int i, int j) -> bool {
@@ -174,7 +207,7 @@ __global__ void k_rhs_accum(const GroupDev* __restrict__ groups, int nGroups1, c
           for (int mm = 0; mm < m; ++mm) dup = dup || onSeg(mm, i, j);
           if (dup) continue;
           const long long p = (long long)i * ny + j;
-          f[p] = cadd(f[p], x[p]);
+          f[p] = cadd(f[p], x[p * W]);
         }
       }
     }
@@ -190,20 +223,21 @@ __global__ void k_rhs_accum(const GroupDev* __restrict__ groups, int nGroups1, c
 // buffer X are cleared (the next sweep step's sparse build relies on it).
 template <typename T2>
 __global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
-                       long long traceStride, const T2* __restrict__ Xs, T2* __restrict__ X, long long xs,
-                       T2* __restrict__ T) {
+                       const int* __restrict__ tq, int W, long long traceStride, const T2* __restrict__ Xs,
+                       T2* __restrict__ X, long long xs, T2* __restrict__ T) {
   const int sidx = blockIdx.y;
   const SlotDev& sl = slots[sidx % nSlots1];
   T += (long long)(sidx / nSlots1) * traceStride;
   const CellDev& cd = cells[sl.cell];
   const int nx = cd.nx, ny = cd.ny;
-  const T2* x = Xs + (long long)sidx * xs;
+  const long long base = slotBase(tq, sidx, W, xs);
+  const T2* x = Xs + base;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (sl.kind != 0) {
-    T2* xr = X + (long long)sidx * xs;
+    T2* xr = X + base;
     for (int m = 0; m < 4; ++m) {
       int i, j;
-      if (segmentPoint(sl, m, t, i, j)) xr[(long long)i * ny + j] = CTraits<T2>::make(0, 0);
+      if (segmentPoint(sl, m, t, i, j)) xr[((long long)i * ny + j) * W] = CTraits<T2>::make(0, 0);
     }
   }
   for (int o = 0; o < sl.nOut; ++o) {
@@ -218,40 +252,40 @@ __global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restr
     else if (i == sl.owCol)
       v = T[sl.owColVal + j];
     else
-      v = x[(long long)i * ny + j];
+      v = x[((long long)i * ny + j) * W];
     T2* d = T + sl.outDst[o] + t;
     *d = sl.outAdd[o] ? cadd(*d, v) : v;
   }
 }
 
-// Add the owned part of the step's solutions of each (DD, cell) to the DD's
-// global accumulator.
+// Add the owned part of the final solutions to the DDs' global accumulators,
+// one workgroup row per solve task (lanes read contiguously).  Each lane is
+// the final solve of one (DD, cell, right-hand side), so no two lanes or
+// tasks add into the same accumulator entry.
 template <typename T2>
-__global__ void k_accumulate(const CellDev* __restrict__ cells, const GroupDev* __restrict__ groups,
-                             int nGroups1, int nSlots1, const T2* __restrict__ X, long long xs,
-                             double2* __restrict__ Y, long long N, int Gy) {
-  const GroupDev& g = groups[blockIdx.y % nGroups1];
-  const int rhs = blockIdx.y / nGroups1;
-  Y += (long long)rhs * 4 * N;
-  X += (long long)rhs * nSlots1 * xs;
-  const CellDev& cd = cells[g.cell];
-  const int slr = g.d & 1, sud = g.d >> 1;
-  const int i0 = cd.vlr[slr][2], i1 = cd.vlr[slr][5];
-  const int j0 = cd.vud[sud][2], j1 = cd.vud[sud][5];
-  const int w = j1 - j0;
-  const long long cnt = (long long)(i1 - i0) * w;
-  double2* y = Y + (long long)g.d * N;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < cnt;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int i = i0 + (int)(q / w), j = j0 + (int)(q - (q / w) * w);
-    const long long p = (long long)i * cd.ny + j;
-    double2 acc = y[(long long)(i + cd.ox) * Gy + (j + cd.oy)];
-    for (int s = 0; s < g.n; ++s) {
-      const T2 v = X[(long long)g.slot[s] * xs + p];
-      acc.x += (double)v.x;
-      acc.y += (double)v.y;
-    }
-    y[(long long)(i + cd.ox) * Gy + (j + cd.oy)] = acc;
+__global__ void k_accumulate_tasks(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
+                                   const SolveTaskDev* __restrict__ tasks, int W, const T2* __restrict__ Xs,
+                                   long long xs, double2* __restrict__ Y, long long N, int Gy) {
+  const int t = blockIdx.y;
+  const SolveTaskDev& tk = tasks[t];
+  const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
+  const int ny = cd.ny;
+  const long long total = (long long)cd.nx * ny * W;
+  const T2* x = Xs + (long long)t * xs * W;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(idx / W), q = (int)(idx - (long long)p * W);
+    if (q >= tk.nrhs) continue;
+    const int sidx = tk.slot[q];
+    const SlotDev& sl = slots[sidx % nSlots1];
+    const int rhs = sidx / nSlots1;
+    const int slr = sl.d & 1, sud = sl.d >> 1;
+    const int i = p / ny, j = p - i * ny;
+    if (i < cd.vlr[slr][2] || i >= cd.vlr[slr][5] || j < cd.vud[sud][2] || j >= cd.vud[sud][5]) continue;
+    const T2 v = x[idx];
+    double2* y = Y + ((long long)rhs * 4 + sl.d) * N + (long long)(i + cd.ox) * Gy + (j + cd.oy);
+    y->x += (double)v.x;
+    y->y += (double)v.y;
   }
 }
 
@@ -1048,14 +1082,13 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   dX = dXs = dR = dG = dT = dF = nullptr;
   dY = nullptr;
   mBufRhs = 0;
-  const size_t slots = (size_t)mMaxSlots * nrhs;
-  // R and G serve every block width up to nrhs
+  // X, Xs, R and G serve every block width up to nrhs
   long long rows = 0;
   for (int n = 1; n <= nrhs; ++n) rows = std::max(rows, taskRowsFor(n));
   mTaskRows = rows;
   LSW_HIP(devMalloc(&dF, (size_t)4 * mCells.size() * nrhs * mXs * es));
-  LSW_HIP(devMalloc(&dX, slots * mXs * es));
-  LSW_HIP(devMalloc(&dXs, slots * mXs * es));
+  LSW_HIP(devMalloc(&dX, (size_t)rows * mXs * es));
+  LSW_HIP(devMalloc(&dXs, (size_t)rows * mXs * es));
   LSW_HIP(devMalloc(&dR, (size_t)rows * mXs * es));
   LSW_HIP(devMalloc(&dG, (size_t)rows * mGs * es));
   LSW_HIP(devMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
@@ -1063,8 +1096,8 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   // the solve kernels read (and multiply by zero) entries of X, R and G they
   // do not use: they must hold finite values from the start (cleared on the
   // problem's stream, ahead of its kernels)
-  LSW_HIP(hipMemsetAsync(dX, 0, slots * mXs * es, mStream));
-  LSW_HIP(hipMemsetAsync(dXs, 0, slots * mXs * es, mStream));
+  LSW_HIP(hipMemsetAsync(dX, 0, (size_t)rows * mXs * es, mStream));
+  LSW_HIP(hipMemsetAsync(dXs, 0, (size_t)rows * mXs * es, mStream));
   LSW_HIP(hipMemsetAsync(dR, 0, (size_t)rows * mXs * es, mStream));
   LSW_HIP(hipMemsetAsync(dG, 0, (size_t)rows * mGs * es, mStream));
   mBufRhs = nrhs;
@@ -1335,7 +1368,6 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   const CellDev* dCells = mFactors->deviceCells();
   const double2* dCoef = mFactors->deviceCoef();
   const int tb = 256;
-  const unsigned gxN = (unsigned)std::min<long long>((mFactors->maxDofs() + tb - 1) / tb, 1024);
   const unsigned gxT = (unsigned)((mTmax + 63) / 64);
   // per sweep step: right-hand sides, their sum per (d, cell), the solves
   // (backward only where traces are read), trace extraction and transfer;
@@ -1350,23 +1382,32 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       // write only their trace sets into buffers that are zero elsewhere:
       // cleared once here after stage 1, then by each sweep step's k_post
       const bool sparse = si > 0 && !st.deferred;
-      if (si == 1) zero(dX, (long long)((size_t)mMaxSlots * nrhs * mXs * es / 8));
-      hipLaunchKernelGGL(k_build_rhs<T2>, dim3(sparse ? gxT : gxN, ns), dim3(sparse ? 64 : tb), 0, s, dCells,
-                         st.dSlots, st.nSlots, mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const T2*)dT,
-                         (T2*)dX, mXs, (const T2*)dF, nGrp, sparse ? 1 : 0);
-      if (!st.deferred)
-        hipLaunchKernelGGL(k_rhs_accum<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, st.dSlots, st.nSlots,
-                           (const T2*)dX, mXs, dCells, (T2*)dF, nGrp);
-      // solutions go to dXs: dX keeps the right-hand sides
+      // the step's local vectors are laid out by its plan's tasks
       const SolvePlan& plan = planFor(st, nrhs);
       LSW_REQUIRE(plan.taskRows() <= mTaskRows, LSW_ERR_STATE, "solve scratch smaller than the plan needs");
+      LSW_REQUIRE((long long)plan.slotTQ.size() >= (long long)ns, LSW_ERR_STATE, "plan does not cover the step's slots");
+      const int W = plan.width;
+      const unsigned nt = (unsigned)plan.ntasks;
+      const unsigned gxW = (unsigned)std::min<long long>((mFactors->maxDofs() * W + tb - 1) / tb, 2048);
+      if (si == 1) zero(dX, (long long)((size_t)mTaskRows * mXs * es / 8));
+      if (sparse)
+        hipLaunchKernelGGL(k_build_rhs_sparse<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots,
+                           plan.dSlotTQ, W, mTraceEntries, dCoef, (const T2*)dT, (T2*)dX, mXs);
+      else
+        hipLaunchKernelGGL(k_build_rhs_tasks<T2>, dim3(gxW, nt), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
+                           plan.dTasks, W, mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const T2*)dT,
+                           (T2*)dX, mXs, (const T2*)dF, nGrp);
+      if (!st.deferred)
+        hipLaunchKernelGGL(k_rhs_accum<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, st.dSlots, st.nSlots,
+                           plan.dSlotTQ, W, (const T2*)dX, mXs, dCells, (T2*)dF, nGrp);
+      // solutions go to dXs: dX keeps the right-hand sides
       mFactors->solve(plan, dX, dXs, dR, dG, mXs, mGs, s, prof);
       if (st.deferred) {
-        hipLaunchKernelGGL(k_accumulate<T2>, dim3(gxN, ng), dim3(tb), 0, s, dCells, st.dGroups, st.nGroups,
-                           st.nSlots, (const T2*)dXs, mXs, dY, n, mGy);
+        hipLaunchKernelGGL(k_accumulate_tasks<T2>, dim3(gxW, nt), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
+                           plan.dTasks, W, (const T2*)dXs, mXs, dY, n, mGy);
       } else {
-        hipLaunchKernelGGL(k_post<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots, mTraceEntries,
-                           (const T2*)dXs, (T2*)dX, mXs, (T2*)dT);
+        hipLaunchKernelGGL(k_post<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots, plan.dSlotTQ, W,
+                           mTraceEntries, (const T2*)dXs, (T2*)dX, mXs, (T2*)dT);
         if (st.nCopies)
           hipLaunchKernelGGL(k_copy_traces<T2>, dim3(gxT, nc), dim3(64), 0, s, st.dCopies, st.nCopies,
                              mTraceEntries, (T2*)dT, mTmax);

@@ -98,8 +98,12 @@ struct SolvePlan {
   int ntasks = 0, maxNrhs = 1;
   int width = 1;            // right-hand sides per task in the kernels (taskWidth(maxNrhs))
   bool outOfPlace = false;  // built for separate right-hand side and solution buffers
-  // rows of the task-major scratch R and G a solve of this plan needs
+  // rows of the task-major buffers (X, Xo, R, G) a solve of this plan needs
   long long taskRows() const { return (long long)ntasks * width; }
+  // where each slot id of the tasks lives in the task-major buffers: task
+  // t, lane q as t*width + q (-1: not in any task); host copy and device copy
+  std::vector<int> slotTQ;
+  int* dSlotTQ = nullptr;
   std::vector<SolveTask> hTasks;
   SolveTaskDev* dTasks = nullptr;
   SolveEntry* dEntries = nullptr;
@@ -155,12 +159,13 @@ class FactorSet {
                  SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports = nullptr,
                  bool outOfPlace = false, const PlanShape* shape = nullptr) const;
   // Run a plan.  X holds the right-hand sides on entry and the solutions on
-  // exit (slot-major, stride xStride complex entries).  R and G are scratch
-  // laid out by task: plan.taskRows() rows of xStride (R) and gStride (G)
-  // entries, the plan.width right-hand sides of a task interleaved (value q
-  // of position i of task t at (t*stride + i)*width + q).  Element type
-  // follows precision(): double2 or float2.  With prof, every launch is
-  // timed with HIP events (synchronises at the end).
+  // exit; R and G are scratch.  All four are laid out by task:
+  // plan.taskRows() rows of xStride (X, R) or gStride (G) entries, the
+  // plan.width right-hand sides of a task interleaved (value q of position
+  // i of task t at (t*stride + i)*width + q; plan.slotTQ maps the tasks'
+  // slot ids to t*width + q).  Element type follows precision(): double2 or
+  // float2.  With prof, every launch is timed with HIP events (synchronises
+  // at the end).
   void solve(const SolvePlan& plan, void* X, void* R, void* G, long long xStride, long long gStride,
              hipStream_t s, SolveProfile* prof = nullptr) const;
   // The same with the solutions written to Xo (same layout as X); X is only

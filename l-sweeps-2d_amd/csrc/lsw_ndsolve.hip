@@ -261,9 +261,7 @@ __device__ __forceinline__ void gatherFrontRhs(const Item& it, bool act, int k0,
     // as the compiler knows
     T2 val[B][NR];
 #pragma unroll
-    for (int u = 0; u < B; ++u)
-#pragma unroll
-      for (int q = 0; q < NR; ++q) val[u][q] = X[(long long)it.tk.slot[q] * xs + dof[u]];
+    for (int u = 0; u < B; ++u) loadNR<T2, NR>(X + tix<T2, NR>(it, xs, dof[u]), val[u]);
     if (kids) {
 #pragma unroll
       for (int u = 0; u < B; ++u) {
@@ -363,29 +361,27 @@ __device__ __forceinline__ void bwdGather(const Item& it, bool act, int k0, int 
       const int kk = kb + u * kstep;
       const int kc = kk < K ? kk : kb;
       // r_E from the task-major R, x_B from the right-hand sides' solutions
-      T2 rv[NR];
+      T2 rv[NR], xb[NR];
       const bool fromR = kc < e;
       loadNR<T2, NR>(Rb + tix<T2, NR>(it, xs, fromR ? dof[u] : 0), rv);
+      loadNR<T2, NR>(Xo + tix<T2, NR>(it, xs, dof[u]), xb);
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
-        const T2 xb = Xo[(long long)it.tk.slot[q] * xs + dof[u]];
         T2 w;  // selected per component (a select of whole values went through scratch)
-        w.x = fromR ? rv[q].x : xb.x;
-        w.y = fromR ? rv[q].y : xb.y;
+        w.x = fromR ? rv[q].x : xb[q].x;
+        w.y = fromR ? rv[q].y : xb[q].y;
         v[kc * NR + q] = w;  // past K: the value of kb again
       }
     }
   }
 }
 
+// backward store of x_E[row] for all NR right-hand sides of the task (the
+// task's lanes past its right-hand sides are its own: nobody reads them)
 template <typename T2, int NR>
 __device__ __forceinline__ void bwdStore(const Item& it, bool ok, int dof, const T2 (&acc)[NR], T2* __restrict__ X,
                                          long long xs) {
-#pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    T2* dst = (ok && q < it.tk.nrhs) ? X + (long long)it.tk.slot[q] * xs + dof : trashSlot<T2>();
-    *dst = acc[q];
-  }
+  storeNR<T2, NR>(ok ? X + tix<T2, NR>(it, xs, dof) : trashRow<T2, NR>(), acc);
 }
 
 // ---------------------------------------------------------------- packed
@@ -679,6 +675,7 @@ __global__ __launch_bounds__(256) void k_nd_mfma(const SolveTaskDev* __restrict_
 SolvePlan::~SolvePlan() {
   if (dTasks) (void)devFree(dTasks);
   if (dEntries) (void)devFree(dEntries);
+  if (dSlotTQ) (void)devFree(dSlotTQ);
 }
 
 namespace {
@@ -909,8 +906,18 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   }
   if (plan.dTasks) (void)devFree(plan.dTasks);
   if (plan.dEntries) (void)devFree(plan.dEntries);
+  if (plan.dSlotTQ) (void)devFree(plan.dSlotTQ);
   plan.dTasks = nullptr;
   plan.dEntries = nullptr;
+  plan.dSlotTQ = nullptr;
+  int maxSlot = 0;
+  for (const auto& tk : tasks)
+    for (int q = 0; q < tk.nrhs; ++q) maxSlot = std::max(maxSlot, tk.slot[q]);
+  plan.slotTQ.assign((size_t)maxSlot + 1, -1);
+  for (int t = 0; t < nt; ++t)
+    for (int q = 0; q < tasks[t].nrhs; ++q) plan.slotTQ[tasks[t].slot[q]] = t * plan.width + q;
+  LSW_HIP(devMalloc(&plan.dSlotTQ, plan.slotTQ.size() * sizeof(int)));
+  LSW_HIP(uploadSync(plan.dSlotTQ, plan.slotTQ.data(), plan.slotTQ.size() * sizeof(int)));
   LSW_HIP(devMalloc(&plan.dTasks, tdev.size() * sizeof(SolveTaskDev)));
   LSW_HIP(uploadSync(plan.dTasks, tdev.data(), tdev.size() * sizeof(SolveTaskDev)));
   LSW_HIP(devMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(SolveEntry)));

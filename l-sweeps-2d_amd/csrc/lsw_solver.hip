@@ -82,14 +82,14 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
   if (nrhs > mCap) {
     for (void* p : {dX, dR, dG}) if (p) (void)devFree(p);
     if (dTasks) (void)devFree(dTasks);
-    // R and G are task-major: ntasks rows of up to per right-hand sides
+    // X, R and G are task-major: ntasks rows of up to per right-hand sides
     const size_t rows = (size_t)ntasks * per;
-    LSW_HIP(devMalloc(&dX, (size_t)nrhs * N * es));
+    LSW_HIP(devMalloc(&dX, rows * N * es));
     LSW_HIP(devMalloc(&dR, rows * N * es));
     LSW_HIP(devMalloc(&dG, rows * G * es));
     // the solve kernels read (and multiply by zero) entries of X, R and G
     // they do not use: they must hold finite values from the start
-    LSW_HIP(hipMemsetAsync(dX, 0, (size_t)nrhs * N * es, mStream));
+    LSW_HIP(hipMemsetAsync(dX, 0, rows * N * es, mStream));
     LSW_HIP(hipMemsetAsync(dR, 0, rows * N * es, mStream));
     LSW_HIP(hipMemsetAsync(dG, 0, rows * G * es, mStream));
     LSW_HIP(devMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
@@ -102,27 +102,40 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
     tasks[t].nrhs = std::min(per, nrhs - t * per);
     for (int r = 0; r < tasks[t].nrhs; ++r) tasks[t].slot[r] = t * per + r;
   }
-  if (mPrecision == LSW_FP64) {
-    LSW_HIP(hipMemcpyAsync(dX, b, (size_t)nrhs * N * sizeof(double2), hipMemcpyHostToDevice, mStream));
-  } else {
-    std::vector<float2> bf((size_t)nrhs * N);
-    for (size_t k = 0; k < bf.size(); ++k) bf[k] = make_float2((float)b[k].real(), (float)b[k].imag());
-    LSW_HIP(hipMemcpyAsync(dX, bf.data(), bf.size() * sizeof(float2), hipMemcpyHostToDevice, mStream));
-    LSW_HIP(hipStreamSynchronize(mStream));
-  }
   SolvePlan plan;
   mFS->buildPlan(tasks, {}, plan);
   LSW_REQUIRE(plan.taskRows() <= (long long)((mCap + per - 1) / per) * per, LSW_ERR_STATE,
               "solve scratch smaller than the plan needs");
-  mFS->solve(plan, dX, dR, dG, N, G, mStream);
+  // right-hand side r = task r / per, lane r % per of the task-major buffer
+  const int W = plan.width;
+  const size_t cnt = (size_t)plan.taskRows() * N;
+  auto pos = [&](int r, long long p) { return ((size_t)(r / per) * N + p) * W + (r % per); };
   if (mPrecision == LSW_FP64) {
-    LSW_HIP(hipMemcpyAsync(x, dX, (size_t)nrhs * N * sizeof(double2), hipMemcpyDeviceToHost, mStream));
+    std::vector<double2> h(cnt, make_double2(0.0, 0.0));
+    for (int r = 0; r < nrhs; ++r)
+      for (long long p = 0; p < N; ++p) {
+        const Complex v = b[(size_t)r * N + p];
+        h[pos(r, p)] = make_double2(v.real(), v.imag());
+      }
+    LSW_HIP(hipMemcpyAsync(dX, h.data(), cnt * sizeof(double2), hipMemcpyHostToDevice, mStream));
+    mFS->solve(plan, dX, dR, dG, N, G, mStream);
+    LSW_HIP(hipMemcpyAsync(h.data(), dX, cnt * sizeof(double2), hipMemcpyDeviceToHost, mStream));
     LSW_HIP(hipStreamSynchronize(mStream));
+    for (int r = 0; r < nrhs; ++r)
+      for (long long p = 0; p < N; ++p) x[(size_t)r * N + p] = Complex(h[pos(r, p)].x, h[pos(r, p)].y);
   } else {
-    std::vector<float2> xf((size_t)nrhs * N);
-    LSW_HIP(hipMemcpyAsync(xf.data(), dX, xf.size() * sizeof(float2), hipMemcpyDeviceToHost, mStream));
+    std::vector<float2> h(cnt, make_float2(0.f, 0.f));
+    for (int r = 0; r < nrhs; ++r)
+      for (long long p = 0; p < N; ++p) {
+        const Complex v = b[(size_t)r * N + p];
+        h[pos(r, p)] = make_float2((float)v.real(), (float)v.imag());
+      }
+    LSW_HIP(hipMemcpyAsync(dX, h.data(), cnt * sizeof(float2), hipMemcpyHostToDevice, mStream));
+    mFS->solve(plan, dX, dR, dG, N, G, mStream);
+    LSW_HIP(hipMemcpyAsync(h.data(), dX, cnt * sizeof(float2), hipMemcpyDeviceToHost, mStream));
     LSW_HIP(hipStreamSynchronize(mStream));
-    for (size_t k = 0; k < xf.size(); ++k) x[k] = Complex(xf[k].x, xf[k].y);
+    for (int r = 0; r < nrhs; ++r)
+      for (long long p = 0; p < N; ++p) x[(size_t)r * N + p] = Complex(h[pos(r, p)].x, h[pos(r, p)].y);
   }
 }
 
