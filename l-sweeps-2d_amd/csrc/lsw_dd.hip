@@ -3,6 +3,7 @@
 #include "lsw_dd.h"
 
 #include <chrono>
+#include <type_traits>
 
 #include <algorithm>
 #include <cmath>
@@ -113,36 +114,38 @@ __device__ T2 slotRhs(const SlotDev& sl, const CellDev& cd, int i, int j, const 
 }
 
 // Full right-hand sides (stage 1 and the final solves), one workgroup row per
-// solve task: consecutive threads write consecutive lanes of a point, so the
-// task-major buffer is written contiguously; lanes past the task's
-// right-hand sides get zeros.
-template <typename T2>
+// solve task: a thread takes one point and writes its W lanes as one
+// contiguous vector (lanes past the task's right-hand sides get zeros); the
+// lane loop is uniform, so a lane's slot description is one broadcast.
+template <typename T2, int W>
 __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
-                                  const SolveTaskDev* __restrict__ tasks, int W, long long traceStride,
-                                  long long Nglob, const double2* __restrict__ xin,
-                                  const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT,
-                                  int Gy, const double2* __restrict__ coef, const T2* __restrict__ T,
-                                  T2* __restrict__ X, long long xs, const T2* __restrict__ F, int nGrp) {
+                                  const SolveTaskDev* __restrict__ tasks, long long traceStride, long long Nglob,
+                                  const double2* __restrict__ xin, const unsigned char* __restrict__ colT,
+                                  const unsigned char* __restrict__ rowT, int Gy, const double2* __restrict__ coef,
+                                  const T2* __restrict__ T, T2* __restrict__ X, long long xs,
+                                  const T2* __restrict__ F, int nGrp) {
   const int t = blockIdx.y;
   const SolveTaskDev& tk = tasks[t];
   const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
-  const int ny = cd.ny;
-  const long long total = (long long)cd.nx * ny * W;
+  const int ny = cd.ny, N = cd.nx * ny;
   T2* x = X + (long long)t * xs * W;
-  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int p = (int)(idx / W), q = (int)(idx - (long long)p * W);
-    T2 out = CTraits<T2>::make(0, 0);
-    if (q < tk.nrhs) {
-      const int sidx = tk.slot[q];
-      const SlotDev& sl = slots[sidx % nSlots1];
-      const int rhs = sidx / nSlots1;
-      const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
-      const int i = p / ny, j = p - i * ny;
-      out = slotRhs<T2>(sl, cd, i, j, xin + (long long)rhs * Nglob, colT, rowT, Gy, coef,
-                        T + (long long)rhs * traceStride, f);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+    const int i = p / ny, j = p - i * ny;
+    T2 out[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      out[q] = CTraits<T2>::make(0, 0);
+      if (q < tk.nrhs) {
+        const int sidx = tk.slot[q];
+        const SlotDev& sl = slots[sidx % nSlots1];
+        const int rhs = sidx / nSlots1;
+        const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
+        out[q] = slotRhs<T2>(sl, cd, i, j, xin + (long long)rhs * Nglob, colT, rowT, Gy, coef,
+                             T + (long long)rhs * traceStride, f);
+      }
     }
-    x[idx] = out;
+#pragma unroll
+    for (int q = 0; q < W; ++q) x[(long long)p * W + q] = out[q];
   }
 }
 
@@ -259,33 +262,36 @@ __global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restr
 }
 
 // Add the owned part of the final solutions to the DDs' global accumulators,
-// one workgroup row per solve task (lanes read contiguously).  Each lane is
-// the final solve of one (DD, cell, right-hand side), so no two lanes or
-// tasks add into the same accumulator entry.
-template <typename T2>
+// one workgroup row per solve task: a thread takes one point and reads its W
+// lanes as one contiguous vector.  Each lane is the final solve of one (DD,
+// cell, right-hand side), so no two lanes or tasks add into the same
+// accumulator entry.
+template <typename T2, int W>
 __global__ void k_accumulate_tasks(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
-                                   const SolveTaskDev* __restrict__ tasks, int W, const T2* __restrict__ Xs,
-                                   long long xs, double2* __restrict__ Y, long long N, int Gy) {
+                                   const SolveTaskDev* __restrict__ tasks, const T2* __restrict__ Xs, long long xs,
+                                   double2* __restrict__ Y, long long N, int Gy) {
   const int t = blockIdx.y;
   const SolveTaskDev& tk = tasks[t];
   const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
-  const int ny = cd.ny;
-  const long long total = (long long)cd.nx * ny * W;
+  const int ny = cd.ny, Nc = cd.nx * ny;
   const T2* x = Xs + (long long)t * xs * W;
-  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int p = (int)(idx / W), q = (int)(idx - (long long)p * W);
-    if (q >= tk.nrhs) continue;
-    const int sidx = tk.slot[q];
-    const SlotDev& sl = slots[sidx % nSlots1];
-    const int rhs = sidx / nSlots1;
-    const int slr = sl.d & 1, sud = sl.d >> 1;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < Nc; p += gridDim.x * blockDim.x) {
     const int i = p / ny, j = p - i * ny;
-    if (i < cd.vlr[slr][2] || i >= cd.vlr[slr][5] || j < cd.vud[sud][2] || j >= cd.vud[sud][5]) continue;
-    const T2 v = x[idx];
-    double2* y = Y + ((long long)rhs * 4 + sl.d) * N + (long long)(i + cd.ox) * Gy + (j + cd.oy);
-    y->x += (double)v.x;
-    y->y += (double)v.y;
+    T2 v[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) v[q] = x[(long long)p * W + q];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      if (q >= tk.nrhs) continue;
+      const int sidx = tk.slot[q];
+      const SlotDev& sl = slots[sidx % nSlots1];
+      const int rhs = sidx / nSlots1;
+      const int slr = sl.d & 1, sud = sl.d >> 1;
+      if (i < cd.vlr[slr][2] || i >= cd.vlr[slr][5] || j < cd.vud[sud][2] || j >= cd.vud[sud][5]) continue;
+      double2* y = Y + ((long long)rhs * 4 + sl.d) * N + (long long)(i + cd.ox) * Gy + (j + cd.oy);
+      y->x += (double)v[q].x;
+      y->y += (double)v[q].y;
+    }
   }
 }
 
@@ -1388,23 +1394,39 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       LSW_REQUIRE((long long)plan.slotTQ.size() >= (long long)ns, LSW_ERR_STATE, "plan does not cover the step's slots");
       const int W = plan.width;
       const unsigned nt = (unsigned)plan.ntasks;
-      const unsigned gxW = (unsigned)std::min<long long>((mFactors->maxDofs() * W + tb - 1) / tb, 2048);
+      const unsigned gxP = (unsigned)std::min<long long>((mFactors->maxDofs() + tb - 1) / tb, 1024);
+      // kernels instantiated per task width (1, 4, 8)
+      auto withWidth = [&](int w, auto&& f) {
+        if (w == 1)
+          f(std::integral_constant<int, 1>{});
+        else if (w == kTaskRhs4)
+          f(std::integral_constant<int, kTaskRhs4>{});
+        else
+          f(std::integral_constant<int, kMaxRhs>{});
+      };
       if (si == 1) zero(dX, (long long)((size_t)mTaskRows * mXs * es / 8));
-      if (sparse)
+      if (sparse) {
         hipLaunchKernelGGL(k_build_rhs_sparse<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots,
                            plan.dSlotTQ, W, mTraceEntries, dCoef, (const T2*)dT, (T2*)dX, mXs);
-      else
-        hipLaunchKernelGGL(k_build_rhs_tasks<T2>, dim3(gxW, nt), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
-                           plan.dTasks, W, mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef, (const T2*)dT,
-                           (T2*)dX, mXs, (const T2*)dF, nGrp);
+      } else {
+        auto build = [&](auto w) {
+          hipLaunchKernelGGL((k_build_rhs_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
+                             st.dSlots, st.nSlots, plan.dTasks, mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef,
+                             (const T2*)dT, (T2*)dX, mXs, (const T2*)dF, nGrp);
+        };
+        withWidth(W, build);
+      }
       if (!st.deferred)
         hipLaunchKernelGGL(k_rhs_accum<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, st.dSlots, st.nSlots,
                            plan.dSlotTQ, W, (const T2*)dX, mXs, dCells, (T2*)dF, nGrp);
       // solutions go to dXs: dX keeps the right-hand sides
       mFactors->solve(plan, dX, dXs, dR, dG, mXs, mGs, s, prof);
       if (st.deferred) {
-        hipLaunchKernelGGL(k_accumulate_tasks<T2>, dim3(gxW, nt), dim3(tb), 0, s, dCells, st.dSlots, st.nSlots,
-                           plan.dTasks, W, (const T2*)dXs, mXs, dY, n, mGy);
+        auto acc = [&](auto w) {
+          hipLaunchKernelGGL((k_accumulate_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
+                             st.dSlots, st.nSlots, plan.dTasks, (const T2*)dXs, mXs, dY, n, mGy);
+        };
+        withWidth(W, acc);
       } else {
         hipLaunchKernelGGL(k_post<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots, plan.dSlotTQ, W,
                            mTraceEntries, (const T2*)dXs, (T2*)dX, mXs, (T2*)dT);
