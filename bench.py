@@ -454,7 +454,7 @@ def main():
     ap.add_argument("--serve-budget", type=float, default=240e9)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-restart", type=int, default=8)
-    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-workers", type=int, default=6)
     ap.add_argument("--plain-sample", type=int, default=4)
     ap.add_argument("--pack-count", type=int, default=48)
     ap.add_argument("--layered-sources", type=int, default=128)
