@@ -149,51 +149,30 @@ __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotD
   }
 }
 
-// Sweep right-hand sides of the steps after the first: only the trace-set
-// points of each slot are written (the buffers are zero elsewhere: cleared
-// before the first sweep step, and each sweep step's k_post clears its own
-// sets).
+// Sweep right-hand sides of the steps after the first, and their sum per
+// (d, cell) into F, in one pass: one workgroup per (group, right-hand side)
+// writes the trace-set points of each of the group's slots (the buffers are
+// zero elsewhere: cleared before the first sweep step, and each sweep step's
+// k_post clears its own sets) and adds each point once into F, the slots one
+// after the other.  By linearity the solutions of all solves of one (d, cell),
+// which the reference adds into its result (`res +=`), are the solution of
+// the sum of their right-hand sides, computed once per application.
 template <typename T2>
-__global__ void k_build_rhs_sparse(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots,
-                                   int nSlots1, const int* __restrict__ tq, int W, long long traceStride,
+__global__ void k_build_rhs_groups(const GroupDev* __restrict__ groups, int nGroups1,
+                                   const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
+                                   const int* __restrict__ tq, int W, long long traceStride,
                                    const double2* __restrict__ coef, const T2* __restrict__ T, T2* __restrict__ X,
-                                   long long xs) {
-  const int sidx = blockIdx.y;
-  const SlotDev& sl = slots[sidx % nSlots1];
-  if (sl.kind == 0) return;  // stage-1 slots are built in full
-  const int rhs = sidx / nSlots1;
-  T += (long long)rhs * traceStride;
-  const CellDev& cd = cells[sl.cell];
-  const int ny = cd.ny;
-  T2* x = X + slotBase(tq, sidx, W, xs);
-  for (int m = 0; m < 4; ++m)
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x;; t += gridDim.x * blockDim.x) {
-      int i, j;
-      if (!segmentPoint(sl, m, t, i, j)) break;
-      x[((long long)i * ny + j) * W] = sweepRhs<T2>(sl, cd, i, j, coef, T);
-    }
-}
-
-// Sum the right-hand sides of the sweep solves of each (d, cell) of a step
-// into F: by linearity the solutions of all solves of one (d, cell), which
-// the reference adds into its result (`res +=`), are the solution of the sum
-// of their right-hand sides, computed once per application.  A sweep
-// right-hand side lives on its two trace sets (a column and/or a row
-// segment each); every point is added once, the slots of a group one after
-// the other.
-template <typename T2>
-__global__ void k_rhs_accum(const GroupDev* __restrict__ groups, int nGroups1, const SlotDev* __restrict__ slots,
-                            int nSlots1, const int* __restrict__ tq, int W, const T2* __restrict__ X, long long xs,
-                            const CellDev* __restrict__ cells, T2* __restrict__ F, int nGrp) {
+                                   long long xs, T2* __restrict__ F, int nGrp) {
   const GroupDev& g = groups[blockIdx.x % nGroups1];
   const int rhs = blockIdx.x / nGroups1;
+  const T2* Tr = T + (long long)rhs * traceStride;
   for (int k = 0; k < g.n; ++k) {
     const SlotDev& sl = slots[g.slot[k]];
     if (sl.kind != 0) {
-      const int ny = cells[sl.cell].ny;
-      const T2* x = X + slotBase(tq, rhs * nSlots1 + g.slot[k], W, xs);
+      const CellDev& cd = cells[sl.cell];
+      const int ny = cd.ny;
+      T2* x = X + slotBase(tq, rhs * nSlots1 + g.slot[k], W, xs);
       T2* f = F + ((long long)rhs * nGrp + sl.grp) * xs;
-      // segments: col 0, row 0, col 1, row 1
       auto onSeg = [&](int m, int i, int j) -> bool {
         const int s = m >> 1;
         if ((m & 1) == 0) return sl.setCol[s] >= 0 && i == sl.setCol[s] && j >= sl.setColLo[s] && j < sl.setColHi[s];
@@ -206,11 +185,12 @@ __global__ void k_rhs_accum(const GroupDev* __restrict__ groups, int nGroups1, c
         const int lo = col ? sl.setColLo[s] : sl.setRowLo[s], hi = col ? sl.setColHi[s] : sl.setRowHi[s];
         for (int t = lo + threadIdx.x; t < hi; t += blockDim.x) {
           const int i = col ? sl.setCol[s] : t, j = col ? t : sl.setRow[s];
+          const T2 v = sweepRhs<T2>(sl, cd, i, j, coef, Tr);
+          const long long p = (long long)i * ny + j;
+          x[p * W] = v;
           bool dup = false;
           for (int mm = 0; mm < m; ++mm) dup = dup || onSeg(mm, i, j);
-          if (dup) continue;
-          const long long p = (long long)i * ny + j;
-          f[p] = cadd(f[p], x[p * W]);
+          if (!dup) f[p] = cadd(f[p], v);
         }
       }
     }
@@ -1406,8 +1386,9 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       };
       if (si == 1) zero(dX, (long long)((size_t)mTaskRows * mXs * es / 8));
       if (sparse) {
-        hipLaunchKernelGGL(k_build_rhs_sparse<T2>, dim3(gxT, ns), dim3(64), 0, s, dCells, st.dSlots, st.nSlots,
-                           plan.dSlotTQ, W, mTraceEntries, dCoef, (const T2*)dT, (T2*)dX, mXs);
+        hipLaunchKernelGGL(k_build_rhs_groups<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, dCells,
+                           st.dSlots, st.nSlots, plan.dSlotTQ, W, mTraceEntries, dCoef, (const T2*)dT, (T2*)dX, mXs,
+                           (T2*)dF, nGrp);
       } else {
         auto build = [&](auto w) {
           hipLaunchKernelGGL((k_build_rhs_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
@@ -1416,9 +1397,8 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
         };
         withWidth(W, build);
       }
-      if (!st.deferred)
-        hipLaunchKernelGGL(k_rhs_accum<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, st.dSlots, st.nSlots,
-                           plan.dSlotTQ, W, (const T2*)dX, mXs, dCells, (T2*)dF, nGrp);
+      // (stage 1 has no sweep right-hand sides to sum; the sweep steps summed
+      // theirs while building them)
       // solutions go to dXs: dX keeps the right-hand sides
       mFactors->solve(plan, dX, dXs, dR, dG, mXs, mGs, s, prof);
       if (st.deferred) {
