@@ -35,7 +35,7 @@ void FunctionCSV::setFile(const std::string& filename, const std::vector<DoubleV
 
 void FunctionCSV::setData(const float* c, int nrows, int ncols, const std::vector<DoubleVector>& fileDom) {
   LSW_REQUIRE(nrows > 0 && ncols > 0, LSW_ERR_ARG, "FunctionCSV: empty velocity grid");
-  mC.assign(c, c + (size_t)nrows * ncols);
+  mC = std::make_shared<const std::vector<float>>(c, c + (size_t)nrows * ncols);
   mNRows = nrows;
   mNCols = ncols;
   mFileDom = fileDom;
@@ -62,13 +62,14 @@ void FunctionCSV::setDomain(const std::vector<DoubleVector>& dom) const {
   mDomA[1][1] = mFileDom[1][1] - startRow * mH[1];
   mARows = (int)(endRow - startRow + 1);
   mACols = (int)(endCol - startCol + 1);
-  mA.assign((size_t)mARows * mACols, 0.0);
+  auto a = std::make_shared<std::vector<double>>((size_t)mARows * mACols, 0.0);
+  const std::vector<float>& C = *mC;
   bool first = true;
   for (int r = 0; r < mARows; ++r)
     for (int cc = 0; cc < mACols; ++cc) {
-      const double c = (double)mC[(size_t)(startRow + r) * mNCols + startCol + cc];
+      const double c = (double)C[(size_t)(startRow + r) * mNCols + startCol + cc];
       const double v = 1.0 / std::pow(c, 2.0);
-      mA[(size_t)r * mACols + cc] = v;
+      (*a)[(size_t)r * mACols + cc] = v;
       if (first) {
         mMinReal = v;
         first = false;
@@ -76,6 +77,7 @@ void FunctionCSV::setDomain(const std::vector<DoubleVector>& dom) const {
         mMinReal = std::min(mMinReal, v);
       }
     }
+  mA = std::move(a);
   mDomSet = true;
 }
 
@@ -94,7 +96,7 @@ Complex FunctionCSV::eval(const DoubleVector& xin) const {
   const UInt col = (UInt)((x[0] - mDomA[0][0]) / mH[0]);
   const UInt row = (UInt)((mDomA[1][1] - x[1]) / mH[1]);
   LSW_REQUIRE((int)row < mARows && (int)col < mACols, LSW_ERR_ARG, "FunctionCSV: lookup out of range");
-  return mA[(size_t)row * mACols + col];
+  return (*mA)[(size_t)row * mACols + col];
 }
 
 // ---------------------------------------------------------------------- PML

@@ -63,11 +63,14 @@ class FunctionCSV : public Function {
   double getMinReal() const override;
 
  private:
-  std::vector<float> mC;
+  // The velocity grid and the cropped squared slowness are immutable once
+  // made, so copies (one per subdomain, prototype-style) share them; a copy
+  // that is given another domain makes its own cropped values.
+  std::shared_ptr<const std::vector<float>> mC;
   int mNRows = 0, mNCols = 0;
   std::vector<DoubleVector> mFileDom;
   double mH[2] = {0, 0};
-  mutable std::vector<double> mA;  // cropped m values, row-major
+  mutable std::shared_ptr<const std::vector<double>> mA;  // cropped m values, row-major
   mutable int mARows = 0, mACols = 0;
   mutable double mDomA[2][2] = {{0, 0}, {0, 0}};
   mutable double mMinReal = 0;
