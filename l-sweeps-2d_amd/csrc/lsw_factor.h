@@ -105,8 +105,9 @@ struct SolvePlan {
   std::vector<int> slotTQ;
   int* dSlotTQ = nullptr;
   std::vector<SolveTask> hTasks;
-  SolveTaskDev* dTasks = nullptr;
+  SolveTaskDev* dTasks = nullptr;  // these three live in dBlob
   SolveEntry* dEntries = nullptr;
+  void* dBlob = nullptr;
   std::vector<Launch> fwd, bwd;    // in launch order
   double fwdBytes = 0, bwdBytes = 0;
   SolvePlan() = default;

@@ -673,9 +673,7 @@ __global__ __launch_bounds__(256) void k_nd_mfma(const SolveTaskDev* __restrict_
 
 // ------------------------------------------------------------------ plans
 SolvePlan::~SolvePlan() {
-  if (dTasks) (void)devFree(dTasks);
-  if (dEntries) (void)devFree(dEntries);
-  if (dSlotTQ) (void)devFree(dSlotTQ);
+  if (dBlob) (void)devFree(dBlob);
 }
 
 namespace {
@@ -904,9 +902,8 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
                      "bytes %.0f\n", pass ? "bwd" : "fwd", nt, plan.maxNrhs, L.n, L.mode, L.rp, L.rounds, L.wpb,
                      L.waves, L.cols, L.bytes);
   }
-  if (plan.dTasks) (void)devFree(plan.dTasks);
-  if (plan.dEntries) (void)devFree(plan.dEntries);
-  if (plan.dSlotTQ) (void)devFree(plan.dSlotTQ);
+  if (plan.dBlob) (void)devFree(plan.dBlob);
+  plan.dBlob = nullptr;
   plan.dTasks = nullptr;
   plan.dEntries = nullptr;
   plan.dSlotTQ = nullptr;
@@ -916,12 +913,22 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   plan.slotTQ.assign((size_t)maxSlot + 1, -1);
   for (int t = 0; t < nt; ++t)
     for (int q = 0; q < tasks[t].nrhs; ++q) plan.slotTQ[tasks[t].slot[q]] = t * plan.width + q;
-  LSW_HIP(devMalloc(&plan.dSlotTQ, plan.slotTQ.size() * sizeof(int)));
-  LSW_HIP(uploadSync(plan.dSlotTQ, plan.slotTQ.data(), plan.slotTQ.size() * sizeof(int)));
-  LSW_HIP(devMalloc(&plan.dTasks, tdev.size() * sizeof(SolveTaskDev)));
-  LSW_HIP(uploadSync(plan.dTasks, tdev.data(), tdev.size() * sizeof(SolveTaskDev)));
-  LSW_HIP(devMalloc(&plan.dEntries, std::max<size_t>(1, all.size()) * sizeof(SolveEntry)));
-  LSW_HIP(uploadSync(plan.dEntries, all.data(), all.size() * sizeof(SolveEntry)));
+  // tasks, entries and the slot map in one allocation and one upload (a
+  // served problem builds plans for many block widths)
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t bTasks = tdev.size() * sizeof(SolveTaskDev), bEnt = all.size() * sizeof(SolveEntry);
+  const size_t bTQ = plan.slotTQ.size() * sizeof(int);
+  const size_t oEnt = up(bTasks), oTQ = oEnt + up(std::max<size_t>(bEnt, 1)), total = oTQ + up(bTQ);
+  std::vector<unsigned char> blob(total, 0);
+  std::memcpy(blob.data(), tdev.data(), bTasks);
+  if (bEnt) std::memcpy(blob.data() + oEnt, all.data(), bEnt);
+  std::memcpy(blob.data() + oTQ, plan.slotTQ.data(), bTQ);
+  LSW_HIP(devMalloc(&plan.dBlob, total));
+  LSW_HIP(uploadSync(plan.dBlob, blob.data(), total));
+  auto* base = static_cast<unsigned char*>(plan.dBlob);
+  plan.dTasks = reinterpret_cast<SolveTaskDev*>(base);
+  plan.dEntries = reinterpret_cast<SolveEntry*>(base + oEnt);
+  plan.dSlotTQ = reinterpret_cast<int*>(base + oTQ);
 }
 
 namespace {
