@@ -239,6 +239,7 @@ class DDCheckerboard : public DD {
   mutable void* dF = nullptr;     // summed sweep right-hand sides per (d, cell) and right-hand side
   std::vector<std::pair<std::string, double>> mSetupLog;  // set-up phase times (LSW_SETUP_LOG prints them)
   mutable int mBufRhs = 0;
+  bool mOwnedPartition = false;  // owned windows cover every global point once per DD
   mutable long long mTaskRows = 0;  // rows of dR / dG (task-major solve scratch)
   long long mXs = 0, mGs = 0;
 

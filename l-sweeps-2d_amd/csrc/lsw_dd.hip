@@ -245,11 +245,12 @@ __global__ void k_post(const CellDev* __restrict__ cells, const SlotDev* __restr
 // one workgroup row per solve task: a thread takes one point and reads its W
 // lanes as one contiguous vector.  Each lane is the final solve of one (DD,
 // cell, right-hand side), so no two lanes or tasks add into the same
-// accumulator entry.
+// accumulator entry; when the owned windows cover each point once (assign),
+// the accumulators are written rather than added to (and not cleared first).
 template <typename T2, int W>
 __global__ void k_accumulate_tasks(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
                                    const SolveTaskDev* __restrict__ tasks, const T2* __restrict__ Xs, long long xs,
-                                   double2* __restrict__ Y, long long N, int Gy) {
+                                   double2* __restrict__ Y, long long N, int Gy, int assign) {
   const int t = blockIdx.y;
   const SolveTaskDev& tk = tasks[t];
   const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
@@ -269,8 +270,12 @@ __global__ void k_accumulate_tasks(const CellDev* __restrict__ cells, const Slot
       const int slr = sl.d & 1, sud = sl.d >> 1;
       if (i < cd.vlr[slr][2] || i >= cd.vlr[slr][5] || j < cd.vud[sud][2] || j >= cd.vud[sud][5]) continue;
       double2* y = Y + ((long long)rhs * 4 + sl.d) * N + (long long)(i + cd.ox) * Gy + (j + cd.oy);
-      y->x += (double)v[q].x;
-      y->y += (double)v[q].y;
+      if (assign) {
+        *y = make_double2((double)v[q].x, (double)v[q].y);
+      } else {
+        y->x += (double)v[q].x;
+        y->y += (double)v[q].y;
+      }
     }
   }
 }
@@ -578,6 +583,29 @@ void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalL
   }
   for (int gi = 0; gi < mGx; ++gi) LSW_REQUIRE(colOwner[gi] >= 0, LSW_ERR_ARG, "owned columns do not cover the grid");
   for (int gj = 0; gj < mGy; ++gj) LSW_REQUIRE(rowOwner[gj] >= 0, LSW_ERR_ARG, "owned rows do not cover the grid");
+  // For every DD the owned windows of the cells normally cover each global
+  // point exactly once; then the final accumulation assigns instead of adding
+  // into cleared accumulators.
+  {
+    std::vector<unsigned char> cnt((size_t)mGx * mGy);
+    mOwnedPartition = true;
+    for (int d = 0; d < 4 && mOwnedPartition; ++d) {
+      std::fill(cnt.begin(), cnt.end(), 0);
+      const int slr = d & 1, sud = d >> 1;
+      for (const auto& c : mCells)
+        for (int i = c.vlr[slr][2]; i < c.vlr[slr][5]; ++i)
+          for (int j = c.vud[sud][2]; j < c.vud[sud][5]; ++j) {
+            const long long gi = i + c.ox, gj = j + c.oy;
+            if (gi < 0 || gi >= mGx || gj < 0 || gj >= mGy) {
+              mOwnedPartition = false;
+              continue;
+            }
+            unsigned char& k = cnt[(size_t)gi * mGy + gj];
+            k = (unsigned char)std::min(2, k + 1);
+          }
+      for (unsigned char k : cnt) mOwnedPartition = mOwnedPartition && k == 1;
+    }
+  }
   // neighbours share the perpendicular grid
   for (auto& c : mCells) {
     if (c.J + 1 < mNC) {
@@ -1348,7 +1376,7 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>((words + 255) / 256, 8192)), dim3(256), 0,
                          s, (unsigned long long*)p, words);
   };
-  zero(dY, 4LL * nrhs * n * 2);
+  if (!mOwnedPartition) zero(dY, 4LL * nrhs * n * 2);
   zero(dT, (long long)(nrhs * mTraceEntries * es / 8));
   zero(dF, (long long)((size_t)nGrp * nrhs * mXs * es / 8));
   const CellDev* dCells = mFactors->deviceCells();
@@ -1404,7 +1432,8 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       if (st.deferred) {
         auto acc = [&](auto w) {
           hipLaunchKernelGGL((k_accumulate_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
-                             st.dSlots, st.nSlots, plan.dTasks, (const T2*)dXs, mXs, dY, n, mGy);
+                             st.dSlots, st.nSlots, plan.dTasks, (const T2*)dXs, mXs, dY, n, mGy,
+                             mOwnedPartition ? 1 : 0);
         };
         withWidth(W, acc);
       } else {
