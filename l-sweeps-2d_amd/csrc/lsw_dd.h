@@ -240,6 +240,12 @@ class DDCheckerboard : public DD {
   std::vector<std::pair<std::string, double>> mSetupLog;  // set-up phase times (LSW_SETUP_LOG prints them)
   mutable int mBufRhs = 0;
   bool mOwnedPartition = false;  // owned windows cover every global point once per DD
+  // Owner tables of the final gather (valid when mGatherFinal): per DD axis
+  // half, the owning cell column / row and local index of every global
+  // column / row, then the cell index of each (row, column) of the board.
+  std::vector<int> mOwnHost;
+  int* dOwn = nullptr;
+  bool mGatherFinal = false;  // final solutions gathered straight into the output
   mutable long long mTaskRows = 0;  // rows of dR / dG (task-major solve scratch)
   long long mXs = 0, mGs = 0;
 

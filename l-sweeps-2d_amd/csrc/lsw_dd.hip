@@ -309,19 +309,68 @@ __global__ void k_combine(const double2* __restrict__ Y, double2* __restrict__ o
   }
 }
 
-// Global 5-point operator y = A x.
+// The final accumulation and combination in one pass, used when the cells'
+// owned windows partition the grid for every DD: each global point finds,
+// per DD d, the cell that owns it and that cell's final solve in the last
+// step (slot d * nc + cell), converts the four values to FP64 and adds them
+// as res1 + (res2 + res3 + res4) (src/DDCheckerboard.cpp:733-735).  Owner
+// tables: colJ / colLi [2][Gx], rowI / rowLj [2][Gy], cellAt [nR][nC].
+template <typename T2>
+__global__ void k_gather_final(const CellDev* __restrict__ cells, const int* __restrict__ own,
+                               const int* __restrict__ tq, int W, long long xs, int nSlots1, int nc, int nC,
+                               const T2* __restrict__ Xs, double2* __restrict__ out, int Gx, int Gy, int nrhs) {
+  const long long N = (long long)Gx * Gy;
+  const int* colJ = own;
+  const int* colLi = own + 2 * Gx;
+  const int* rowI = own + 4 * Gx;
+  const int* rowLj = own + 4 * Gx + 2 * Gy;
+  const int* cellAt = own + 4 * Gx + 4 * Gy;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < N;
+       g += (long long)gridDim.x * blockDim.x) {
+    const int gi = (int)(g / Gy), gj = (int)(g - (long long)gi * Gy);
+    int sl[4];
+    long long pw[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int slr = d & 1, sud = d >> 1;
+      const int c = cellAt[rowI[sud * Gy + gj] * nC + colJ[slr * Gx + gi]];
+      sl[d] = d * nc + c;
+      pw[d] = ((long long)colLi[slr * Gx + gi] * cells[c].ny + rowLj[sud * Gy + gj]) * W;
+    }
+    for (int r = 0; r < nrhs; ++r) {
+      double2 v[4];
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const T2 a = Xs[slotBase(tq, r * nSlots1 + sl[d], W, xs) + pw[d]];
+        v[d] = make_double2((double)a.x, (double)a.y);
+      }
+      out[r * N + g] = cadd(v[0], cadd(cadd(v[1], v[2]), v[3]));
+    }
+  }
+}
+
+// Global 5-point operator y_r = A x_r for nrhs vectors stored one after the
+// other (stride N): a point's five coefficients are read once and applied to
+// every vector.
 __global__ void k_matvec(const double2* __restrict__ coef, const double2* __restrict__ x,
-                         double2* __restrict__ y, int Gx, int Gy) {
+                         double2* __restrict__ y, int Gx, int Gy, int nrhs) {
   const long long N = (long long)Gx * Gy;
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < N;
        g += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(g / Gy), j = (int)(g - (long long)i * Gy);
-    double2 acc = cmul(coef[g], x[g]);
-    if (i > 0) acc = cfma(coef[N + g], x[g - Gy], acc);
-    if (i < Gx - 1) acc = cfma(coef[2 * N + g], x[g + Gy], acc);
-    if (j > 0) acc = cfma(coef[3 * N + g], x[g - 1], acc);
-    if (j < Gy - 1) acc = cfma(coef[4 * N + g], x[g + 1], acc);
-    y[g] = acc;
+    const double2 c0 = coef[g];
+    const double2 zero = make_double2(0.0, 0.0);
+    const double2 cw = i > 0 ? coef[N + g] : zero, ce = i < Gx - 1 ? coef[2 * N + g] : zero;
+    const double2 cs = j > 0 ? coef[3 * N + g] : zero, cn = j < Gy - 1 ? coef[4 * N + g] : zero;
+    for (int r = 0; r < nrhs; ++r) {
+      const double2* xr = x + r * N;
+      double2 acc = cmul(c0, xr[g]);
+      if (i > 0) acc = cfma(cw, xr[g - Gy], acc);
+      if (i < Gx - 1) acc = cfma(ce, xr[g + Gy], acc);
+      if (j > 0) acc = cfma(cs, xr[g - 1], acc);
+      if (j < Gy - 1) acc = cfma(cn, xr[g + 1], acc);
+      y[r * N + g] = acc;
+    }
   }
 }
 
@@ -358,6 +407,7 @@ DDCheckerboard::~DDCheckerboard() {
   }
   for (void* p : {dX, dXs, dR, dG, dT, dF}) if (p) (void)devFree(p);
   if (dY) (void)devFree(dY);
+  if (dOwn) (void)devFree(dOwn);
   if (dGlobalCoef) (void)devFree(dGlobalCoef);
   if (dColTrace) (void)devFree(dColTrace);
   if (dRowTrace) (void)devFree(dRowTrace);
@@ -606,6 +656,41 @@ void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalL
       for (unsigned char k : cnt) mOwnedPartition = mOwnedPartition && k == 1;
     }
   }
+  // owner tables of the final gather: the partition must be a product of
+  // per-column and per-row ownership (every cell of a board column owns the
+  // same global columns at the same local indices, likewise for rows)
+  // (LSW_FINAL_GATHER=0 keeps the accumulate-and-combine path)
+  const char* gatherEnv = std::getenv("LSW_FINAL_GATHER");
+  mGatherFinal = mOwnedPartition && mNR * mNC == (int)mCells.size() && !(gatherEnv && std::atoi(gatherEnv) == 0);
+  mOwnHost.clear();
+  if (mGatherFinal) {
+    const size_t oLi = 2 * (size_t)mGx, oI = 4 * (size_t)mGx, oLj = oI + 2 * (size_t)mGy, oCell = oI + 4 * (size_t)mGy;
+    mOwnHost.assign(oCell + (size_t)mNR * mNC, -1);
+    auto claim = [&](size_t at, size_t atLocal, int owner, int local) {
+      if (mOwnHost[at] < 0) {
+        mOwnHost[at] = owner;
+        mOwnHost[atLocal] = local;
+      } else if (mOwnHost[at] != owner || mOwnHost[atLocal] != local) {
+        mGatherFinal = false;
+      }
+    };
+    for (size_t ci = 0; ci < mCells.size() && mGatherFinal; ++ci) {
+      const auto& c = mCells[ci];
+      if (c.I < 0 || c.I >= mNR || c.J < 0 || c.J >= mNC || mOwnHost[oCell + (size_t)c.I * mNC + c.J] >= 0) {
+        mGatherFinal = false;
+        break;
+      }
+      mOwnHost[oCell + (size_t)c.I * mNC + c.J] = (int)ci;
+      for (int h = 0; h < 2; ++h) {
+        for (int i = c.vlr[h][2]; i < c.vlr[h][5]; ++i)
+          claim((size_t)h * mGx + (i + c.ox), oLi + (size_t)h * mGx + (i + c.ox), c.J, i);
+        for (int j = c.vud[h][2]; j < c.vud[h][5]; ++j)
+          claim(oI + (size_t)h * mGy + (j + c.oy), oLj + (size_t)h * mGy + (j + c.oy), c.I, j);
+      }
+    }
+    for (int v : mOwnHost) mGatherFinal = mGatherFinal && v >= 0;
+    if (!mGatherFinal) mOwnHost.clear();
+  }
   // neighbours share the perpendicular grid
   for (auto& c : mCells) {
     if (c.J + 1 < mNC) {
@@ -680,6 +765,19 @@ void DDCheckerboard::initDevice() {
   lap("global", t);
   buildSchedule();
   lap("schedule", t);
+  if (mGatherFinal) {
+    // the gather addresses the last step's slot of (DD d, cell c) as d * nc + c
+    const int nc = (int)mCells.size();
+    const Step& fin = mSteps.back();
+    bool ok = fin.deferred && fin.hSlots.size() == (size_t)4 * nc;
+    for (size_t k = 0; ok && k < fin.hSlots.size(); ++k)
+      ok = fin.hSlots[k].d == (int)k / nc && fin.hSlots[k].cell == (int)k % nc;
+    mGatherFinal = ok;
+    if (ok) {
+      LSW_HIP(devMalloc(&dOwn, mOwnHost.size() * sizeof(int)));
+      LSW_HIP(hipMemcpy(dOwn, mOwnHost.data(), mOwnHost.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
   allocBuffers();
   lap("buffers", t);
 }
@@ -1106,7 +1204,8 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   LSW_HIP(devMalloc(&dR, (size_t)rows * mXs * es));
   LSW_HIP(devMalloc(&dG, (size_t)rows * mGs * es));
   LSW_HIP(devMalloc(&dT, (size_t)nrhs * mTraceEntries * es));
-  LSW_HIP(devMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
+  // (the final gather writes the output directly: no DD accumulators)
+  if (!mGatherFinal) LSW_HIP(devMalloc(&dY, (size_t)4 * nrhs * N() * sizeof(double2)));
   // the solve kernels read (and multiply by zero) entries of X, R and G they
   // do not use: they must hold finite values from the start (cleared on the
   // problem's stream, ahead of its kernels)
@@ -1429,7 +1528,15 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       // theirs while building them)
       // solutions go to dXs: dX keeps the right-hand sides
       mFactors->solve(plan, dX, dXs, dR, dG, mXs, mGs, s, prof);
-      if (st.deferred) {
+      if (st.deferred && mGatherFinal) {
+        const size_t need = (size_t)nrhs * st.nSlots;
+        LSW_REQUIRE(plan.slotTQ.size() >= need, LSW_ERR_STATE, "final gather: slot map shorter than the step");
+        for (size_t k = 0; k < need; ++k)
+          LSW_REQUIRE(plan.slotTQ[k] >= 0, LSW_ERR_STATE, "final gather: a final solve has no task");
+        hipLaunchKernelGGL(k_gather_final<T2>, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256),
+                           0, s, dCells, dOwn, plan.dSlotTQ, W, mXs, st.nSlots, (int)mCells.size(), mNC,
+                           (const T2*)dXs, y, mGx, mGy, nrhs);
+      } else if (st.deferred) {
         auto acc = [&](auto w) {
           hipLaunchKernelGGL((k_accumulate_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
                              st.dSlots, st.nSlots, plan.dTasks, (const T2*)dXs, mXs, dY, n, mGy,
@@ -1449,8 +1556,9 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
     run(double2{});
   else
     run(float2{});
-  hipLaunchKernelGGL(k_combine, dim3((unsigned)std::min<long long>((n * nrhs + 255) / 256, 8192)), dim3(256), 0,
-                     s, dY, y, n, nrhs);
+  if (!mGatherFinal)
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)std::min<long long>((n * nrhs + 255) / 256, 8192)), dim3(256), 0,
+                       s, dY, y, n, nrhs);
   LSW_HIP(hipGetLastError());
 }
 
@@ -1462,9 +1570,9 @@ void DDCheckerboard::applyADevBlock(const double2* x, double2* y, int nrhs, hipS
   LSW_REQUIRE(dGlobalCoef != nullptr, LSW_ERR_STATE, "DD has no device state");
   if (!s) s = mStream;
   const long long n = N();
-  for (int r = 0; r < nrhs; ++r)
+  if (nrhs > 0)
     hipLaunchKernelGGL(k_matvec, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s,
-                       dGlobalCoef, x + r * n, y + r * n, mGx, mGy);
+                       dGlobalCoef, x, y, mGx, mGy, nrhs);
   LSW_HIP(hipGetLastError());
 }
 

@@ -37,6 +37,26 @@ def test_operator_and_precond_match_oracle(ncells, nels, omega):
     assert relerr(got, ref) < 1e-9, relerr(got, ref)
 
 
+@pytest.mark.parametrize("ncells,nels,omega", [CASES[2], CASES[4]])
+def test_final_gather_equals_accumulate_and_combine(ncells, nels, omega, monkeypatch):
+    """The one-pass final gather (owned windows partition the grid) gives
+    bit-for-bit the result of the accumulate-then-combine path."""
+    cb, dd = make_pair(ncells, nels, omega)
+    monkeypatch.setenv("LSW_FINAL_GATHER", "0")
+    _, dd_acc = make_pair(ncells, nels, omega)
+    dd_acc.factorize()
+    monkeypatch.delenv("LSW_FINAL_GATHER")
+    cb.factorize()
+    dd.factorize()
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(cb.N) + 1j * rng.standard_normal(cb.N)
+    got = dd.apply_precond(x)
+    acc = dd_acc.apply_precond(x)
+    assert np.array_equal(got, acc)
+    ref = Preconditioner(cb).apply_global(x)
+    assert relerr(acc, ref) < 1e-9, relerr(acc, ref)
+
+
 def test_rhs_matches_oracle():
     cb, dd = make_pair((3, 3), (12, 12), 2 * math.pi * 4.1)
     srcs = [(0.5, 0.5), (0.31, 0.72)]
