@@ -315,9 +315,9 @@ __global__ void k_combine(const double2* __restrict__ Y, double2* __restrict__ o
 // step (slot d * nc + cell), converts the four values to FP64 and adds them
 // as res1 + (res2 + res3 + res4) (src/DDCheckerboard.cpp:733-735).  Owner
 // tables: colJ / colLi [2][Gx], rowI / rowLj [2][Gy], cellAt [nR][nC].
-template <typename T2>
+template <typename T2, int W>
 __global__ void k_gather_final(const CellDev* __restrict__ cells, const int* __restrict__ own,
-                               const int* __restrict__ tq, int W, long long xs, int nSlots1, int nc, int nC,
+                               const int* __restrict__ tq, long long xs, int nSlots1, int nc, int nC,
                                const T2* __restrict__ Xs, double2* __restrict__ out, int Gx, int Gy, int nrhs) {
   const long long N = (long long)Gx * Gy;
   const int* colJ = own;
@@ -337,11 +337,26 @@ __global__ void k_gather_final(const CellDev* __restrict__ cells, const int* __r
       sl[d] = d * nc + c;
       pw[d] = ((long long)colLi[slr * Gx + gi] * cells[c].ny + rowLj[sud * Gy + gj]) * W;
     }
+    // a task's lanes of one point are one contiguous W-vector: load it once
+    // and take every (right-hand side, DD) value it holds from registers
+    T2 lanes[W];
+    long long at = -1;
     for (int r = 0; r < nrhs; ++r) {
       double2 v[4];
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
-        const T2 a = Xs[slotBase(tq, r * nSlots1 + sl[d], W, xs) + pw[d]];
+        const int tqv = tq[r * nSlots1 + sl[d]];
+        const int t = tqv / W, q = tqv - t * W;
+        const long long base = (long long)t * xs * W + pw[d];
+        if (base != at) {
+#pragma unroll
+          for (int k = 0; k < W; ++k) lanes[k] = Xs[base + k];
+          at = base;
+        }
+        T2 a = lanes[0];
+#pragma unroll
+        for (int k = 1; k < W; ++k)
+          if (q == k) a = lanes[k];
         v[d] = make_double2((double)a.x, (double)a.y);
       }
       out[r * N + g] = cadd(v[0], cadd(cadd(v[1], v[2]), v[3]));
@@ -1533,9 +1548,13 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
         LSW_REQUIRE(plan.slotTQ.size() >= need, LSW_ERR_STATE, "final gather: slot map shorter than the step");
         for (size_t k = 0; k < need; ++k)
           LSW_REQUIRE(plan.slotTQ[k] >= 0, LSW_ERR_STATE, "final gather: a final solve has no task");
-        hipLaunchKernelGGL(k_gather_final<T2>, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256),
-                           0, s, dCells, dOwn, plan.dSlotTQ, W, mXs, st.nSlots, (int)mCells.size(), mNC,
-                           (const T2*)dXs, y, mGx, mGy, nrhs);
+        auto gat = [&](auto w) {
+          hipLaunchKernelGGL((k_gather_final<T2, decltype(w)::value>),
+                             dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, dCells,
+                             dOwn, plan.dSlotTQ, mXs, st.nSlots, (int)mCells.size(), mNC, (const T2*)dXs, y, mGx,
+                             mGy, nrhs);
+        };
+        withWidth(W, gat);
       } else if (st.deferred) {
         auto acc = [&](auto w) {
           hipLaunchKernelGGL((k_accumulate_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
