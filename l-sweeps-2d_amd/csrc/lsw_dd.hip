@@ -92,7 +92,7 @@ __device__ __forceinline__ bool segmentPoint(const SlotDev& sl, int m, int t, in
 template <typename T2>
 __device__ T2 slotRhs(const SlotDev& sl, const CellDev& cd, int i, int j, const double2* __restrict__ xin,
                       const unsigned char* __restrict__ colT, const unsigned char* __restrict__ rowT, int Gy,
-                      const double2* __restrict__ coef, const T2* __restrict__ T, const T2* __restrict__ f) {
+                      const double2* __restrict__ coef, const T2* __restrict__ T) {
   T2 out = CTraits<T2>::make(0, 0);
   if (sl.kind == 0) {
     const int slr = sl.d & 1, sud = sl.d >> 1;
@@ -109,21 +109,22 @@ __device__ T2 slotRhs(const SlotDev& sl, const CellDev& cd, int i, int j, const 
   } else {
     out = sweepRhs<T2>(sl, cd, i, j, coef, T);
   }
-  if (f) out = cadd(out, f[(long long)i * cd.ny + j]);
   return out;
 }
 
-// Full right-hand sides (stage 1 and the final solves), one workgroup row per
-// solve task: a thread takes one point and writes its W lanes as one
-// contiguous vector (lanes past the task's right-hand sides get zeros); the
-// lane loop is uniform, so a lane's slot description is one broadcast.
+// Stage-1 right-hand sides, one workgroup row per solve task: a thread takes
+// one point and writes its W lanes as one contiguous vector (lanes past the
+// task's right-hand sides get zeros); the lane loop is uniform, so a lane's
+// slot description is one broadcast.  Each value also starts the final
+// solve's right-hand side of its (d, cell) in F, which is laid out by the
+// final step's plan (slot map tqF, width WF): the sweep steps add into it.
 template <typename T2, int W>
 __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
                                   const SolveTaskDev* __restrict__ tasks, long long traceStride, long long Nglob,
                                   const double2* __restrict__ xin, const unsigned char* __restrict__ colT,
                                   const unsigned char* __restrict__ rowT, int Gy, const double2* __restrict__ coef,
                                   const T2* __restrict__ T, T2* __restrict__ X, long long xs,
-                                  const T2* __restrict__ F, int nGrp) {
+                                  T2* __restrict__ F, const int* __restrict__ tqF, int WF, int nSlotsF) {
   const int t = blockIdx.y;
   const SolveTaskDev& tk = tasks[t];
   const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
@@ -139,9 +140,9 @@ __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotD
         const int sidx = tk.slot[q];
         const SlotDev& sl = slots[sidx % nSlots1];
         const int rhs = sidx / nSlots1;
-        const T2* f = sl.deferred ? F + ((long long)rhs * nGrp + sl.grp) * xs : nullptr;
         out[q] = slotRhs<T2>(sl, cd, i, j, xin + (long long)rhs * Nglob, colT, rowT, Gy, coef,
-                             T + (long long)rhs * traceStride, f);
+                             T + (long long)rhs * traceStride);
+        F[slotBase(tqF, rhs * nSlotsF + sl.grp, WF, xs) + (long long)p * WF] = out[q];
       }
     }
 #pragma unroll
@@ -150,7 +151,9 @@ __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotD
 }
 
 // Sweep right-hand sides of the steps after the first, and their sum per
-// (d, cell) into F, in one pass: one workgroup per (group, right-hand side)
+// (d, cell) into F (the final step's layout: slot map tqF, width WF, the
+// final slot of a group being its index), in one pass: one workgroup per
+// (group, right-hand side)
 // writes the trace-set points of each of the group's slots (the buffers are
 // zero elsewhere: cleared before the first sweep step, and each sweep step's
 // k_post clears its own sets) and adds each point once into F, the slots one
@@ -162,7 +165,8 @@ __global__ void k_build_rhs_groups(const GroupDev* __restrict__ groups, int nGro
                                    const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
                                    const int* __restrict__ tq, int W, long long traceStride,
                                    const double2* __restrict__ coef, const T2* __restrict__ T, T2* __restrict__ X,
-                                   long long xs, T2* __restrict__ F, int nGrp) {
+                                   long long xs, T2* __restrict__ F, const int* __restrict__ tqF, int WF,
+                                   int nSlotsF) {
   const GroupDev& g = groups[blockIdx.x % nGroups1];
   const int rhs = blockIdx.x / nGroups1;
   const T2* Tr = T + (long long)rhs * traceStride;
@@ -172,7 +176,7 @@ __global__ void k_build_rhs_groups(const GroupDev* __restrict__ groups, int nGro
       const CellDev& cd = cells[sl.cell];
       const int ny = cd.ny;
       T2* x = X + slotBase(tq, rhs * nSlots1 + g.slot[k], W, xs);
-      T2* f = F + ((long long)rhs * nGrp + sl.grp) * xs;
+      T2* f = F + slotBase(tqF, rhs * nSlotsF + sl.grp, WF, xs);
       auto onSeg = [&](int m, int i, int j) -> bool {
         const int s = m >> 1;
         if ((m & 1) == 0) return sl.setCol[s] >= 0 && i == sl.setCol[s] && j >= sl.setColLo[s] && j < sl.setColHi[s];
@@ -190,7 +194,7 @@ __global__ void k_build_rhs_groups(const GroupDev* __restrict__ groups, int nGro
           x[p * W] = v;
           bool dup = false;
           for (int mm = 0; mm < m; ++mm) dup = dup || onSeg(mm, i, j);
-          if (!dup) f[p] = cadd(f[p], v);
+          if (!dup) f[p * WF] = cadd(f[p * WF], v);
         }
       }
     }
@@ -1060,6 +1064,13 @@ void DDCheckerboard::buildSchedule() {
   }
   for (auto& hst : steps)
     for (auto& sl : hst.slots) sl.grp = sl.d * nc + sl.cell;
+  // stage 1 starts every (d, cell)'s final right-hand side in F, addressed
+  // through the final step's slot of the same index
+  for (const HostStep* hs : {&steps.front(), &steps.back()}) {
+    LSW_REQUIRE(hs->slots.size() == (size_t)4 * nc, LSW_ERR_STATE, "stage 1 or final step misses a (d, cell)");
+    for (size_t k = 0; k < hs->slots.size(); ++k)
+      LSW_REQUIRE(hs->slots[k].grp == (int)k, LSW_ERR_STATE, "stage 1 or final step out of (d, cell) order");
+  }
 
   // slot buffers, tasks (same cell -> one task of up to kMaxRhs), groups
   mMaxSlots = 0;
@@ -1213,7 +1224,7 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   long long rows = 0;
   for (int n = 1; n <= nrhs; ++n) rows = std::max(rows, taskRowsFor(n));
   mTaskRows = rows;
-  LSW_HIP(devMalloc(&dF, (size_t)4 * mCells.size() * nrhs * mXs * es));
+  LSW_HIP(devMalloc(&dF, (size_t)rows * mXs * es));
   LSW_HIP(devMalloc(&dX, (size_t)rows * mXs * es));
   LSW_HIP(devMalloc(&dXs, (size_t)rows * mXs * es));
   LSW_HIP(devMalloc(&dR, (size_t)rows * mXs * es));
@@ -1228,6 +1239,9 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   LSW_HIP(hipMemsetAsync(dXs, 0, (size_t)rows * mXs * es, mStream));
   LSW_HIP(hipMemsetAsync(dR, 0, (size_t)rows * mXs * es, mStream));
   LSW_HIP(hipMemsetAsync(dG, 0, (size_t)rows * mGs * es, mStream));
+  // F is the final solves' right-hand-side buffer: stage 1 rewrites every
+  // point of every used lane in each application
+  LSW_HIP(hipMemsetAsync(dF, 0, (size_t)rows * mXs * es, mStream));
   mBufRhs = nrhs;
 }
 
@@ -1484,7 +1498,6 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   ensureBuffers(nrhs);
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
   const long long n = N();
-  const int nGrp = 4 * (int)mCells.size();
   auto zero = [&](void* p, long long words) {
     if (words > 0)
       hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>((words + 255) / 256, 8192)), dim3(256), 0,
@@ -1492,23 +1505,30 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   };
   if (!mOwnedPartition) zero(dY, 4LL * nrhs * n * 2);
   zero(dT, (long long)(nrhs * mTraceEntries * es / 8));
-  zero(dF, (long long)((size_t)nGrp * nrhs * mXs * es / 8));
   const CellDev* dCells = mFactors->deviceCells();
   const double2* dCoef = mFactors->deviceCoef();
+  // F, the final solves' right-hand sides, is laid out by the final step's
+  // plan; a (d, cell) group's final slot is the group index
+  const Step& stF = mSteps.back();
+  const SolvePlan& planF = planFor(stF, nrhs);
+  LSW_REQUIRE(stF.deferred && planF.taskRows() <= mTaskRows, LSW_ERR_STATE, "final step plan does not fit F");
+  LSW_REQUIRE(planF.slotTQ.size() >= (size_t)nrhs * stF.nSlots, LSW_ERR_STATE, "final plan misses slots");
+  for (size_t k = 0; k < (size_t)nrhs * stF.nSlots; ++k)
+    LSW_REQUIRE(planF.slotTQ[k] >= 0, LSW_ERR_STATE, "a final solve has no task");
   const int tb = 256;
   const unsigned gxT = (unsigned)((mTmax + 63) / 64);
   // per sweep step: right-hand sides, their sum per (d, cell), the solves
   // (backward only where traces are read), trace extraction and transfer;
-  // the final step solves each (d, cell) once in full and accumulates it
+  // the final step solves each (d, cell) once in full from F and combines
   auto run = [&](auto tag) {
     using T2 = decltype(tag);
     for (size_t si = 0; si < mSteps.size(); ++si) {
       const Step& st = mSteps[si];
       const unsigned ns = (unsigned)st.nSlots * nrhs, ng = (unsigned)st.nGroups * nrhs;
       const unsigned nc = (unsigned)st.nCopies * nrhs;
-      // stage 1 and the final solves write whole buffers; the sweep steps
-      // write only their trace sets into buffers that are zero elsewhere:
-      // cleared once here after stage 1, then by each sweep step's k_post
+      // stage 1 writes whole buffers (X and F); the sweep steps write only
+      // their trace sets into X, which is zero elsewhere (cleared once here
+      // after stage 1, then by each sweep step's k_post), and add them into F
       const bool sparse = si > 0 && !st.deferred;
       // the step's local vectors are laid out by its plan's tasks
       const SolvePlan& plan = planFor(st, nrhs);
@@ -1530,19 +1550,18 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       if (sparse) {
         hipLaunchKernelGGL(k_build_rhs_groups<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, dCells,
                            st.dSlots, st.nSlots, plan.dSlotTQ, W, mTraceEntries, dCoef, (const T2*)dT, (T2*)dX, mXs,
-                           (T2*)dF, nGrp);
-      } else {
+                           (T2*)dF, planF.dSlotTQ, planF.width, stF.nSlots);
+      } else if (!st.deferred) {
         auto build = [&](auto w) {
           hipLaunchKernelGGL((k_build_rhs_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
                              st.dSlots, st.nSlots, plan.dTasks, mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef,
-                             (const T2*)dT, (T2*)dX, mXs, (const T2*)dF, nGrp);
+                             (const T2*)dT, (T2*)dX, mXs, (T2*)dF, planF.dSlotTQ, planF.width, stF.nSlots);
         };
         withWidth(W, build);
       }
-      // (stage 1 has no sweep right-hand sides to sum; the sweep steps summed
-      // theirs while building them)
-      // solutions go to dXs: dX keeps the right-hand sides
-      mFactors->solve(plan, dX, dXs, dR, dG, mXs, mGs, s, prof);
+      // solutions go to dXs: dX keeps the right-hand sides; the final solves
+      // read F (stage 1's right-hand sides plus the sweep sums, nothing to build)
+      mFactors->solve(plan, st.deferred ? dF : dX, dXs, dR, dG, mXs, mGs, s, prof);
       if (st.deferred && mGatherFinal) {
         const size_t need = (size_t)nrhs * st.nSlots;
         LSW_REQUIRE(plan.slotTQ.size() >= need, LSW_ERR_STATE, "final gather: slot map shorter than the step");
