@@ -263,7 +263,8 @@ class LocalLinearSolverHIP:
         return dict(nx=nx.value, ny=ny.value, factor_entries=fe.value, nodes=nodes.value, height=h.value)
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        # (at interpreter exit the module's globals may already be cleared)
+        if getattr(self, "_h", None) and _lib is not None:
             _lib.lsw_lu_destroy(self._h)
             self._h = None
 
@@ -445,7 +446,8 @@ class DDCheckerboard:
                     bwd_ms=out[3], bwd_launches=int(out[4]), bwd_bytes=out[5])
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        # (at interpreter exit the module's globals may already be cleared)
+        if getattr(self, "_h", None) and _lib is not None:
             _lib.lsw_dd_destroy(self._h)
             self._h = None
 
