@@ -118,13 +118,16 @@ __device__ T2 slotRhs(const SlotDev& sl, const CellDev& cd, int i, int j, const 
 // slot description is one broadcast.  Each value also starts the final
 // solve's right-hand side of its (d, cell) in F, which is laid out by the
 // final step's plan (slot map tqF, width WF): the sweep steps add into it.
+// When stage 1's plan has that same layout, F alone is written (writeX = 0)
+// and the stage-1 solves read it.
 template <typename T2, int W>
 __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotDev* __restrict__ slots, int nSlots1,
                                   const SolveTaskDev* __restrict__ tasks, long long traceStride, long long Nglob,
                                   const double2* __restrict__ xin, const unsigned char* __restrict__ colT,
                                   const unsigned char* __restrict__ rowT, int Gy, const double2* __restrict__ coef,
                                   const T2* __restrict__ T, T2* __restrict__ X, long long xs,
-                                  T2* __restrict__ F, const int* __restrict__ tqF, int WF, int nSlotsF) {
+                                  T2* __restrict__ F, const int* __restrict__ tqF, int WF, int nSlotsF,
+                                  int writeX) {
   const int t = blockIdx.y;
   const SolveTaskDev& tk = tasks[t];
   const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
@@ -145,8 +148,10 @@ __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotD
         F[slotBase(tqF, rhs * nSlotsF + sl.grp, WF, xs) + (long long)p * WF] = out[q];
       }
     }
+    if (writeX) {
 #pragma unroll
-    for (int q = 0; q < W; ++q) x[(long long)p * W + q] = out[q];
+      for (int q = 0; q < W; ++q) x[(long long)p * W + q] = out[q];
+    }
   }
 }
 
@@ -1520,6 +1525,7 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   // per sweep step: right-hand sides, their sum per (d, cell), the solves
   // (backward only where traces are read), trace extraction and transfer;
   // the final step solves each (d, cell) once in full from F and combines
+  bool s1InF = false;
   auto run = [&](auto tag) {
     using T2 = decltype(tag);
     for (size_t si = 0; si < mSteps.size(); ++si) {
@@ -1546,7 +1552,11 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
         else
           f(std::integral_constant<int, kMaxRhs>{});
       };
-      if (si == 1) zero(dX, (long long)((size_t)mTaskRows * mXs * es / 8));
+      // stage 1 in F's layout writes and solves from F alone; X then stays
+      // zero off the sweep steps' trace sets and needs no clearing
+      const bool inF = si == 0 && plan.width == planF.width && plan.slotTQ == planF.slotTQ;
+      if (si == 0) s1InF = inF;
+      if (si == 1 && !s1InF) zero(dX, (long long)((size_t)mTaskRows * mXs * es / 8));
       if (sparse) {
         hipLaunchKernelGGL(k_build_rhs_groups<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, dCells,
                            st.dSlots, st.nSlots, plan.dSlotTQ, W, mTraceEntries, dCoef, (const T2*)dT, (T2*)dX, mXs,
@@ -1555,13 +1565,14 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
         auto build = [&](auto w) {
           hipLaunchKernelGGL((k_build_rhs_tasks<T2, decltype(w)::value>), dim3(gxP, nt), dim3(tb), 0, s, dCells,
                              st.dSlots, st.nSlots, plan.dTasks, mTraceEntries, n, x, dColTrace, dRowTrace, mGy, dCoef,
-                             (const T2*)dT, (T2*)dX, mXs, (T2*)dF, planF.dSlotTQ, planF.width, stF.nSlots);
+                             (const T2*)dT, (T2*)dX, mXs, (T2*)dF, planF.dSlotTQ, planF.width, stF.nSlots,
+                             inF ? 0 : 1);
         };
         withWidth(W, build);
       }
       // solutions go to dXs: dX keeps the right-hand sides; the final solves
       // read F (stage 1's right-hand sides plus the sweep sums, nothing to build)
-      mFactors->solve(plan, st.deferred ? dF : dX, dXs, dR, dG, mXs, mGs, s, prof);
+      mFactors->solve(plan, st.deferred || inF ? dF : dX, dXs, dR, dG, mXs, mGs, s, prof);
       if (st.deferred && mGatherFinal) {
         const size_t need = (size_t)nrhs * st.nSlots;
         LSW_REQUIRE(plan.slotTQ.size() >= need, LSW_ERR_STATE, "final gather: slot map shorter than the step");
