@@ -246,6 +246,7 @@ class DDCheckerboard : public DD {
   std::vector<int> mOwnHost;
   int* dOwn = nullptr;
   bool mGatherFinal = false;  // final solutions gathered straight into the output
+  bool mStage1InF = true;     // stage 1 may build and solve in F (LSW_STAGE1_IN_F=0: never)
   mutable long long mTaskRows = 0;  // rows of dR / dG (task-major solve scratch)
   long long mXs = 0, mGs = 0;
 

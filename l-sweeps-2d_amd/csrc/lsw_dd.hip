@@ -683,7 +683,10 @@ void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalL
   // owner tables of the final gather: the partition must be a product of
   // per-column and per-row ownership (every cell of a board column owns the
   // same global columns at the same local indices, likewise for rows)
-  // (LSW_FINAL_GATHER=0 keeps the accumulate-and-combine path)
+  // (LSW_STAGE1_IN_F=0 builds stage 1 in X; LSW_FINAL_GATHER=0 keeps the
+  // accumulate-and-combine path)
+  const char* s1Env = std::getenv("LSW_STAGE1_IN_F");
+  mStage1InF = !(s1Env && std::atoi(s1Env) == 0);
   const char* gatherEnv = std::getenv("LSW_FINAL_GATHER");
   mGatherFinal = mOwnedPartition && mNR * mNC == (int)mCells.size() && !(gatherEnv && std::atoi(gatherEnv) == 0);
   mOwnHost.clear();
@@ -1554,7 +1557,7 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       };
       // stage 1 in F's layout writes and solves from F alone; X then stays
       // zero off the sweep steps' trace sets and needs no clearing
-      const bool inF = si == 0 && plan.width == planF.width && plan.slotTQ == planF.slotTQ;
+      const bool inF = si == 0 && mStage1InF && plan.width == planF.width && plan.slotTQ == planF.slotTQ;
       if (si == 0) s1InF = inF;
       if (si == 1 && !s1InF) zero(dX, (long long)((size_t)mTaskRows * mXs * es / 8));
       if (sparse) {

@@ -39,13 +39,16 @@ def test_operator_and_precond_match_oracle(ncells, nels, omega):
 
 @pytest.mark.parametrize("ncells,nels,omega", [CASES[2], CASES[4]])
 def test_final_gather_equals_accumulate_and_combine(ncells, nels, omega, monkeypatch):
-    """The one-pass final gather (owned windows partition the grid) gives
-    bit-for-bit the result of the accumulate-then-combine path."""
+    """The one-pass final gather (owned windows partition the grid) and
+    stage 1 built and solved in F give bit-for-bit the result of the
+    accumulate-then-combine path with stage 1 built in X."""
     cb, dd = make_pair(ncells, nels, omega)
     monkeypatch.setenv("LSW_FINAL_GATHER", "0")
+    monkeypatch.setenv("LSW_STAGE1_IN_F", "0")
     _, dd_acc = make_pair(ncells, nels, omega)
     dd_acc.factorize()
     monkeypatch.delenv("LSW_FINAL_GATHER")
+    monkeypatch.delenv("LSW_STAGE1_IN_F")
     cb.factorize()
     dd.factorize()
     rng = np.random.default_rng(11)
