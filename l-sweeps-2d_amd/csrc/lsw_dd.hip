@@ -187,20 +187,32 @@ __global__ void k_build_rhs_groups(const GroupDev* __restrict__ groups, int nGro
         if ((m & 1) == 0) return sl.setCol[s] >= 0 && i == sl.setCol[s] && j >= sl.setColLo[s] && j < sl.setColHi[s];
         return sl.setRow[s] >= 0 && j == sl.setRow[s] && i >= sl.setRowLo[s] && i < sl.setRowHi[s];
       };
+      // the four segments' points as one index space over the workgroup (a
+      // point on several segments gets the same value; only its first
+      // segment adds it into F)
+      int len[4], tot = 0;
+#pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int s = m >> 1;
         const bool col = (m & 1) == 0;
-        if (col ? sl.setCol[s] < 0 : sl.setRow[s] < 0) continue;
+        const bool has = col ? sl.setCol[s] >= 0 : sl.setRow[s] >= 0;
         const int lo = col ? sl.setColLo[s] : sl.setRowLo[s], hi = col ? sl.setColHi[s] : sl.setRowHi[s];
-        for (int t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-          const int i = col ? sl.setCol[s] : t, j = col ? t : sl.setRow[s];
-          const T2 v = sweepRhs<T2>(sl, cd, i, j, coef, Tr);
-          const long long p = (long long)i * ny + j;
-          x[p * W] = v;
-          bool dup = false;
-          for (int mm = 0; mm < m; ++mm) dup = dup || onSeg(mm, i, j);
-          if (!dup) f[p * WF] = cadd(f[p * WF], v);
-        }
+        len[m] = has && hi > lo ? hi - lo : 0;
+        tot += len[m];
+      }
+      for (int u = threadIdx.x; u < tot; u += blockDim.x) {
+        int m = 0, t = u;
+        while (t >= len[m]) t -= len[m++];
+        const int s = m >> 1;
+        const bool col = (m & 1) == 0;
+        t += col ? sl.setColLo[s] : sl.setRowLo[s];
+        const int i = col ? sl.setCol[s] : t, j = col ? t : sl.setRow[s];
+        const T2 v = sweepRhs<T2>(sl, cd, i, j, coef, Tr);
+        const long long p = (long long)i * ny + j;
+        x[p * W] = v;
+        bool dup = false;
+        for (int mm = 0; mm < m; ++mm) dup = dup || onSeg(mm, i, j);
+        if (!dup) f[p * WF] = cadd(f[p * WF], v);
       }
     }
     __syncthreads();
@@ -1561,7 +1573,9 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
       if (si == 0) s1InF = inF;
       if (si == 1 && !s1InF) zero(dX, (long long)((size_t)mTaskRows * mXs * es / 8));
       if (sparse) {
-        hipLaunchKernelGGL(k_build_rhs_groups<T2>, dim3(ng), dim3(256), 0, s, st.dGroups, st.nGroups, dCells,
+        // (one right-hand side: few, latency-bound groups take 1024 threads;
+        // blocks have nrhs times more groups, which fill the chip at 256)
+        hipLaunchKernelGGL(k_build_rhs_groups<T2>, dim3(ng), dim3(nrhs == 1 ? 1024 : 256), 0, s, st.dGroups, st.nGroups, dCells,
                            st.dSlots, st.nSlots, plan.dSlotTQ, W, mTraceEntries, dCoef, (const T2*)dT, (T2*)dX, mXs,
                            (T2*)dF, planF.dSlotTQ, planF.width, stF.nSlots);
       } else if (!st.deferred) {
