@@ -62,6 +62,9 @@ def make_dd(L, cfg, precision):
 def time_single(dd, torch, cfg, mode, tol, max_iter, steps, warmup):
     b = torch.zeros(dd.N, dtype=torch.complex128, device="cuda:0")
     x = torch.zeros_like(b)
+    # torch fills on its own stream; the library's default stream is not
+    # ordered with it
+    torch.cuda.synchronize()
     dd.assemble_rhs_dev(cfg["sources"], b.data_ptr())
     dd.sync()
     for _ in range(warmup):
@@ -88,6 +91,7 @@ def time_batched(dd, torch, cfg, tol, steps, warmup):
     srcs = [(float(v), 0.1) for v in xs]
     B = torch.zeros(BATCH * dd.N, dtype=torch.complex128, device="cuda:0")
     X = torch.zeros_like(B)
+    torch.cuda.synchronize()
     dd.assemble_rhs_block_dev(srcs, B.data_ptr())
     dd.sync()
     for _ in range(warmup):
@@ -110,6 +114,7 @@ def time_batched(dd, torch, cfg, tol, steps, warmup):
 def profile(dd, torch):
     b = torch.randn(dd.N, dtype=torch.complex128, device="cuda:0")
     y = torch.zeros_like(b)
+    torch.cuda.synchronize()
     return dd.profile_precond_dev(b.data_ptr(), y.data_ptr())
 
 
@@ -187,6 +192,7 @@ def plain_request(L, torch, params, sources, tol=1e-6, max_iter=200):
     dd.factorize()
     b = torch.zeros(dd.N, dtype=torch.complex128, device="cuda:0")
     x = torch.zeros_like(b)
+    torch.cuda.synchronize()
     worst, iters = 0.0, 0
     for s in sources:
         dd.assemble_rhs_dev([s], b.data_ptr())
@@ -391,6 +397,7 @@ def layered_bench(args):
     t_fact64 = time.perf_counter() - t0
     b = torch.zeros(dd.N, dtype=torch.complex128, device="cuda:0")
     x = torch.zeros_like(b)
+    torch.cuda.synchronize()
     sample = cfg["sources"][:: max(1, len(cfg["sources"]) // args.plain_sample)][: args.plain_sample]
     ts, worst, its = [], 0.0, []
     for k, src in enumerate(sample):

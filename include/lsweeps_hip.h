@@ -4,7 +4,10 @@
  * Plain pointers and sizes only.  Complex vectors are interleaved
  * (re, im) doubles.  Host-pointer entry points (suffix _z or none) are
  * synchronous; _dev entry points take device pointers and a hipStream_t
- * passed as void* (NULL = the object's own stream) and are asynchronous.
+ * passed as void* and are asynchronous.  NULL means the object's own
+ * non-blocking stream, which is ordered neither with the legacy default
+ * stream nor with any other stream: a caller that fills inputs on another
+ * stream synchronises (or passes that stream) before the call.
  * Every function returns 0 on success or a nonzero status (LSW_ERR_*); the
  * message of the last failure on the calling thread is lsw_last_error().
  * All pointers are borrowed for the duration of the call.  Calls on one
@@ -55,6 +58,13 @@ int lsw_lu_set_csc_z(lsw_lu* lu, int n, int nnz, const int* colptr, const int* r
 int lsw_lu_factorize(lsw_lu* lu);
 /* b, x: n x nrhs column-major, interleaved complex (host). */
 int lsw_lu_solve_z(lsw_lu* lu, int nrhs, const double* b, double* x);
+/* The same on device memory: b, x are n x nrhs column-major interleaved
+ * complex doubles in HBM (b == x allowed), queued on `stream` (NULL: the
+ * solver's own non-blocking stream, not ordered with other streams; wait
+ * with a stream synchronisation).  The solve of
+ * LocalLinearSolverEigenSparseLU::solve (src/LocalLinearSolverEigenSparseLU.cpp:65-76)
+ * without the host round trip. */
+int lsw_lu_solve_dev(lsw_lu* lu, int nrhs, const void* b, void* x, void* stream);
 /* nx, ny of the recovered grid, complex factor entries, tree nodes, height. */
 int lsw_lu_info(lsw_lu* lu, int* nx, int* ny, long long* factor_entries, int* nodes, int* height);
 int lsw_lu_destroy(lsw_lu* lu);

@@ -1262,6 +1262,11 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   // F is the final solves' right-hand-side buffer: stage 1 rewrites every
   // point of every used lane in each application
   LSW_HIP(hipMemsetAsync(dF, 0, (size_t)rows * mXs * es, mStream));
+  // The solves may run on a caller's stream that is not ordered after the
+  // DD's own (non-blocking) stream: wait here so that the clears are done
+  // before any later launch on any stream.  Buffers only grow, so this wait
+  // happens a few times per DD at most.
+  LSW_HIP(hipStreamSynchronize(mStream));
   mBufRhs = nrhs;
 }
 

@@ -144,8 +144,11 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
   LSW_HIP(hipMemsetAsync(X, 0, (size_t)nrhs * n * sizeof(double2), s));
   LSW_HIP(hipMemcpyAsync(R, B, (size_t)nrhs * n * sizeof(double2), hipMemcpyDeviceToDevice, s));
   res = bn;
+  // iters[r] counts the Arnoldi steps of right-hand side r.  A right-hand side
+  // is active only while iters[r] < mMaxIter, every inner step raises iters of
+  // each active one that is not done, and an outer pass runs at least one
+  // inner step, so the loop ends after at most nrhs * mMaxIter steps.
   std::vector<int> iters(nrhs, 0);
-  int it = 0;
   bool first = true;
   while (true) {
     if (!first) {
@@ -174,7 +177,7 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
     std::vector<bool> done(na, false);
     for (int a = 0; a < na; ++a) g[a][0] = res[act[a]];
     int j = 0;
-    for (; j < m && it < mMaxIter; ++j) {
+    for (; j < m; ++j) {
       double2* Vj = dBV + (long long)j * na * n;
       double2* Zj = dBZ + (long long)j * na * n;
       double2* W = dBV + (long long)(j + 1) * na * n;
@@ -197,7 +200,6 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
       LSW_HIP(hipMemcpyAsync(h2.data(), dBH + (m + 2) * nrhs, nv * na * sizeof(double2), hipMemcpyDeviceToHost, s));
       LSW_HIP(hipMemcpyAsync(nr.data(), dBH + 2 * (m + 2) * nrhs - na, na * sizeof(double2), hipMemcpyDeviceToHost, s));
       LSW_HIP(hipStreamSynchronize(s));
-      ++it;
       bool all = true;
       for (int a = 0; a < na; ++a) {
         if (done[a]) continue;

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
 
 #include "lsw_common.h"
@@ -36,8 +37,13 @@ class LocalLinearSolverHIP : public LocalLinearSolver {
   void setMatrix(ComplexSparseMatrix* A) override;
   void factorize() override;
   void solve(ComplexVector& b, ComplexVector& res) const override;
-  // Multiple right-hand sides: b and x are n x nrhs column-major (ld = n).
+  // Multiple right-hand sides: b and x are n x nrhs column-major (ld = n),
+  // host memory; synchronous.
   void solveMany(const Complex* b, Complex* x, int nrhs) const;
+  // The same on device memory (FP64 complex, n x nrhs column-major), queued
+  // on stream s (nullptr: the solver's own non-blocking stream, which is not
+  // ordered with other streams).  b == x is allowed.
+  void solveDev(const double2* b, double2* x, int nrhs, hipStream_t s) const;
   int precision() const { return mPrecision; }
   void setPrecision(int p) { mPrecision = p; }
   int gridNx() const { return mNx; }
@@ -50,11 +56,16 @@ class LocalLinearSolverHIP : public LocalLinearSolver {
   int mNx = 0, mNy = 0;
   std::unique_ptr<FactorSet> mFS;
   hipStream_t mStream = nullptr;
+  // solve scratch for up to mCap right-hand sides, and the launch plan per
+  // right-hand-side count
+  void ensureScratch(int nrhs) const;
+  const SolvePlan& planFor(int nrhs) const;
   mutable void* dX = nullptr;
   mutable void* dR = nullptr;
   mutable void* dG = nullptr;
-  mutable SolveTask* dTasks = nullptr;
+  mutable void* dB = nullptr;
   mutable int mCap = 0;
+  mutable std::map<int, std::unique_ptr<SolvePlan>> mPlans;
 };
 
 }  // namespace lsw

@@ -7,8 +7,9 @@
 namespace lsw {
 
 LocalLinearSolverHIP::~LocalLinearSolverHIP() {
-  for (void* p : {dX, dR, dG}) if (p) (void)devFree(p);
-  if (dTasks) (void)devFree(dTasks);
+  if (mStream) (void)hipStreamSynchronize(mStream);
+  mPlans.clear();
+  for (void* p : {dX, dR, dG, dB}) if (p) (void)devFree(p);
   mFS.reset();
   if (mStream) (void)hipStreamDestroy(mStream);
 }
@@ -44,6 +45,10 @@ void LocalLinearSolverHIP::setMatrix(ComplexSparseMatrix* A) {
   mA = A;
   mNx = nx;
   mNy = ny;
+  mPlans.clear();
+  for (void* p : {dX, dR, dG, dB}) if (p) (void)devFree(p);
+  dX = dR = dG = dB = nullptr;
+  mCap = 0;
   mFS.reset();
 }
 
@@ -70,31 +75,71 @@ void LocalLinearSolverHIP::factorize() {
   mFS->factorize(mPrecision, mStream);
 }
 
-void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) const {
-  LSW_REQUIRE(mFS && mFS->factorized(), LSW_ERR_STATE, "Solve called before factorization");
-  LSW_REQUIRE(nrhs >= 0, LSW_ERR_ARG, "negative right-hand-side count");
-  if (nrhs == 0) return;
+namespace {
+
+// Right-hand sides b (n x nrhs column-major, FP64) into the task-major solve
+// buffer: value of right-hand side r at position p goes to task r / per,
+// lane r % per, i.e. X[((r / per) * n + p) * W + r % per].
+template <typename T>
+__global__ void k_lu_pack(const double2* __restrict__ b, T* __restrict__ X, long long n, int nrhs, int per, int W) {
+  const long long total = n * nrhs;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long r = q / n, p = q - r * n;
+    const double2 v = b[q];
+    T o;
+    o.x = v.x;
+    o.y = v.y;
+    X[((r / per) * n + p) * W + (r % per)] = o;
+  }
+}
+
+template <typename T>
+__global__ void k_lu_unpack(const T* __restrict__ X, double2* __restrict__ x, long long n, int nrhs, int per, int W) {
+  const long long total = n * nrhs;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long r = q / n, p = q - r * n;
+    const T v = X[((r / per) * n + p) * W + (r % per)];
+    x[q] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+inline dim3 gridFor(long long n) { return dim3((unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 8192))); }
+
+}  // namespace
+
+void LocalLinearSolverHIP::ensureScratch(int nrhs) const {
+  if (nrhs <= mCap) return;
   const long long N = mA->rows;
   const long long G = std::max<long long>(mFS->maxG(), 1);
   const size_t es = mPrecision == LSW_FP64 ? sizeof(double2) : sizeof(float2);
   const int per = mFS->maxTaskRhs();
   const int ntasks = (nrhs + per - 1) / per;
-  if (nrhs > mCap) {
-    for (void* p : {dX, dR, dG}) if (p) (void)devFree(p);
-    if (dTasks) (void)devFree(dTasks);
-    // X, R and G are task-major: ntasks rows of up to per right-hand sides
-    const size_t rows = (size_t)ntasks * per;
-    LSW_HIP(devMalloc(&dX, rows * N * es));
-    LSW_HIP(devMalloc(&dR, rows * N * es));
-    LSW_HIP(devMalloc(&dG, rows * G * es));
-    // the solve kernels read (and multiply by zero) entries of X, R and G
-    // they do not use: they must hold finite values from the start
-    LSW_HIP(hipMemsetAsync(dX, 0, rows * N * es, mStream));
-    LSW_HIP(hipMemsetAsync(dR, 0, rows * N * es, mStream));
-    LSW_HIP(hipMemsetAsync(dG, 0, rows * G * es, mStream));
-    LSW_HIP(devMalloc(&dTasks, (size_t)ntasks * sizeof(SolveTask)));
-    mCap = nrhs;
-  }
+  for (void* p : {dX, dR, dG, dB}) if (p) (void)devFree(p);
+  dX = dR = dG = dB = nullptr;
+  mPlans.clear();
+  // X, R and G are task-major: ntasks rows of up to per right-hand sides
+  const size_t rows = (size_t)ntasks * per;
+  LSW_HIP(devMalloc(&dX, rows * N * es));
+  LSW_HIP(devMalloc(&dR, rows * N * es));
+  LSW_HIP(devMalloc(&dG, rows * G * es));
+  LSW_HIP(devMalloc(&dB, (size_t)nrhs * N * sizeof(double2)));  // host-pointer staging
+  // the solve kernels read (and multiply by zero) entries of X, R and G
+  // they do not use: they must hold finite values from the start.  The
+  // solves may run on a caller's stream, so the clears finish here.
+  LSW_HIP(hipMemsetAsync(dX, 0, rows * N * es, mStream));
+  LSW_HIP(hipMemsetAsync(dR, 0, rows * N * es, mStream));
+  LSW_HIP(hipMemsetAsync(dG, 0, rows * G * es, mStream));
+  LSW_HIP(hipStreamSynchronize(mStream));
+  mCap = nrhs;
+}
+
+const SolvePlan& LocalLinearSolverHIP::planFor(int nrhs) const {
+  auto it = mPlans.find(nrhs);
+  if (it != mPlans.end()) return *it->second;
+  const int per = mFS->maxTaskRhs();
+  const int ntasks = (nrhs + per - 1) / per;
   std::vector<SolveTask> tasks(ntasks);
   for (int t = 0; t < ntasks; ++t) {
     std::memset(&tasks[t], 0, sizeof(SolveTask));
@@ -102,41 +147,49 @@ void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) con
     tasks[t].nrhs = std::min(per, nrhs - t * per);
     for (int r = 0; r < tasks[t].nrhs; ++r) tasks[t].slot[r] = t * per + r;
   }
-  SolvePlan plan;
-  mFS->buildPlan(tasks, {}, plan);
-  LSW_REQUIRE(plan.taskRows() <= (long long)((mCap + per - 1) / per) * per, LSW_ERR_STATE,
+  auto plan = std::make_unique<SolvePlan>();
+  mFS->buildPlan(tasks, {}, *plan);
+  LSW_REQUIRE(plan->taskRows() <= (long long)((mCap + per - 1) / per) * per, LSW_ERR_STATE,
               "solve scratch smaller than the plan needs");
+  LSW_REQUIRE(plan->width == per || nrhs < per, LSW_ERR_INTERNAL, "unexpected solve task width");
+  return *mPlans.emplace(nrhs, std::move(plan)).first->second;
+}
+
+void LocalLinearSolverHIP::solveDev(const double2* b, double2* x, int nrhs, hipStream_t s) const {
+  LSW_REQUIRE(mFS && mFS->factorized(), LSW_ERR_STATE, "Solve called before factorization");
+  LSW_REQUIRE(nrhs >= 0, LSW_ERR_ARG, "negative right-hand-side count");
+  if (nrhs == 0) return;
+  if (!s) s = mStream;
+  ensureScratch(nrhs);
+  const SolvePlan& plan = planFor(nrhs);
+  const long long N = mA->rows;
+  const long long G = std::max<long long>(mFS->maxG(), 1);
   // right-hand side r = task r / per, lane r % per of the task-major buffer
-  const int W = plan.width;
-  const size_t cnt = (size_t)plan.taskRows() * N;
-  auto pos = [&](int r, long long p) { return ((size_t)(r / per) * N + p) * W + (r % per); };
+  const int per = mFS->maxTaskRhs(), W = plan.width;
+  const dim3 grid = gridFor(N * nrhs);
   if (mPrecision == LSW_FP64) {
-    std::vector<double2> h(cnt, make_double2(0.0, 0.0));
-    for (int r = 0; r < nrhs; ++r)
-      for (long long p = 0; p < N; ++p) {
-        const Complex v = b[(size_t)r * N + p];
-        h[pos(r, p)] = make_double2(v.real(), v.imag());
-      }
-    LSW_HIP(hipMemcpyAsync(dX, h.data(), cnt * sizeof(double2), hipMemcpyHostToDevice, mStream));
-    mFS->solve(plan, dX, dR, dG, N, G, mStream);
-    LSW_HIP(hipMemcpyAsync(h.data(), dX, cnt * sizeof(double2), hipMemcpyDeviceToHost, mStream));
-    LSW_HIP(hipStreamSynchronize(mStream));
-    for (int r = 0; r < nrhs; ++r)
-      for (long long p = 0; p < N; ++p) x[(size_t)r * N + p] = Complex(h[pos(r, p)].x, h[pos(r, p)].y);
+    hipLaunchKernelGGL(k_lu_pack<double2>, grid, dim3(256), 0, s, b, (double2*)dX, N, nrhs, per, W);
+    mFS->solve(plan, dX, dR, dG, N, G, s);
+    hipLaunchKernelGGL(k_lu_unpack<double2>, grid, dim3(256), 0, s, (const double2*)dX, x, N, nrhs, per, W);
   } else {
-    std::vector<float2> h(cnt, make_float2(0.f, 0.f));
-    for (int r = 0; r < nrhs; ++r)
-      for (long long p = 0; p < N; ++p) {
-        const Complex v = b[(size_t)r * N + p];
-        h[pos(r, p)] = make_float2((float)v.real(), (float)v.imag());
-      }
-    LSW_HIP(hipMemcpyAsync(dX, h.data(), cnt * sizeof(float2), hipMemcpyHostToDevice, mStream));
-    mFS->solve(plan, dX, dR, dG, N, G, mStream);
-    LSW_HIP(hipMemcpyAsync(h.data(), dX, cnt * sizeof(float2), hipMemcpyDeviceToHost, mStream));
-    LSW_HIP(hipStreamSynchronize(mStream));
-    for (int r = 0; r < nrhs; ++r)
-      for (long long p = 0; p < N; ++p) x[(size_t)r * N + p] = Complex(h[pos(r, p)].x, h[pos(r, p)].y);
+    hipLaunchKernelGGL(k_lu_pack<float2>, grid, dim3(256), 0, s, b, (float2*)dX, N, nrhs, per, W);
+    mFS->solve(plan, dX, dR, dG, N, G, s);
+    hipLaunchKernelGGL(k_lu_unpack<float2>, grid, dim3(256), 0, s, (const float2*)dX, x, N, nrhs, per, W);
   }
+  LSW_HIP(hipGetLastError());
+}
+
+void LocalLinearSolverHIP::solveMany(const Complex* b, Complex* x, int nrhs) const {
+  LSW_REQUIRE(mFS && mFS->factorized(), LSW_ERR_STATE, "Solve called before factorization");
+  LSW_REQUIRE(nrhs >= 0, LSW_ERR_ARG, "negative right-hand-side count");
+  if (nrhs == 0) return;
+  ensureScratch(nrhs);
+  const size_t bytes = (size_t)nrhs * mA->rows * sizeof(double2);
+  LSW_HIP(uploadSync(dB, b, bytes));
+  // in place: the staging buffer receives the solutions
+  solveDev(static_cast<const double2*>(dB), static_cast<double2*>(dB), nrhs, mStream);
+  LSW_HIP(hipMemcpyAsync(x, dB, bytes, hipMemcpyDeviceToHost, mStream));
+  LSW_HIP(hipStreamSynchronize(mStream));
 }
 
 void LocalLinearSolverHIP::solve(ComplexVector& b, ComplexVector& res) const {

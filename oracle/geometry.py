@@ -172,36 +172,47 @@ def fd_weights(z, xs, m):
 # --------------------------------------------------------------------------
 # one subdomain (SubdomainStructured + its AssemblerHelmholtz2DFD)
 # --------------------------------------------------------------------------
+def local_grid(int_dom, n, pml_width_in):
+    """AssemblerHelmholtz2DFD::setN / getMeshSize (src/AssemblerHelmholtz2DFD.cpp:67-144):
+    the PML width rounded to the nearest multiple of h = min(h1, h2), the
+    computational domain, intervals per axis and the spacings."""
+    h1 = (int_dom[0][1] - int_dom[0][0]) / n[0]
+    h2 = (int_dom[1][1] - int_dom[1][0]) / n[1]
+    h = min(h1, h2)
+    ratio = pml_width_in / h
+    pml_els = int(pml_width_in / h)
+    if ratio - pml_els > 0.5:
+        pml_els += 1
+    pml = h * pml_els
+    cd = [[int_dom[0][0] - pml, int_dom[0][1] + pml],
+          [int_dom[1][0] - pml, int_dom[1][1] + pml]]
+    mN = [int((cd[0][1] - cd[0][0] + 0.5 * h1) / h1),
+          int((cd[1][1] - cd[1][0] + 0.5 * h2) / h2)]
+    return dict(pml_els=pml_els, pml_width=pml, comp_dom=cd, mN=mN,
+                hx=(cd[0][1] - cd[0][0]) / mN[0], hy=(cd[1][1] - cd[1][0]) / mN[1])
+
+
 class Cell:
     """One checkerboard subdomain with its local FD/PML grid and index sets."""
 
-    def __init__(self, I, J, int_dom, n, lims1, lims2, pml_width_in, omega, C, medium):
+    def __init__(self, I, J, int_dom, n, lims1, lims2, pml_width_in, omega, C, medium, index_sets=True):
         self.I, self.J = I, J
         self.int_dom = [list(int_dom[0]), list(int_dom[1])]
         self.omega, self.C, self.medium = omega, C, medium
-        # AssemblerHelmholtz2DFD::setN (src/AssemblerHelmholtz2DFD.cpp:67-114)
-        h1 = (int_dom[0][1] - int_dom[0][0]) / n[0]
-        h2 = (int_dom[1][1] - int_dom[1][0]) / n[1]
-        h = min(h1, h2)
-        ratio = pml_width_in / h
-        pml_els = int(pml_width_in / h)
-        if ratio - pml_els > 0.5:
-            pml_els += 1
-        pml = h * pml_els
-        self.pml_els, self.pml_width = pml_els, pml
-        cd = [[int_dom[0][0] - pml, int_dom[0][1] + pml],
-              [int_dom[1][0] - pml, int_dom[1][1] + pml]]
-        self.comp_dom = cd
-        self.mN = [int((cd[0][1] - cd[0][0] + 0.5 * h1) / h1),
-                   int((cd[1][1] - cd[1][0] + 0.5 * h2) / h2)]
-        self.hx = (cd[0][1] - cd[0][0]) / self.mN[0]
-        self.hy = (cd[1][1] - cd[1][0]) / self.mN[1]
+        g = local_grid(int_dom, n, pml_width_in)
+        self.pml_els, self.pml_width, self.comp_dom = g["pml_els"], g["pml_width"], g["comp_dom"]
+        self.mN, self.hx, self.hy = g["mN"], g["hx"], g["hy"]
+        cd = self.comp_dom
         self.nx, self.ny = self.mN[0] - 1, self.mN[1] - 1     # DOFs per axis
         self.size = self.nx * self.ny
         # getDOFPts: index i*(ny-1)+j, point (x0c+(i+1)hx, y0c+(j+1)hy)
         self.xs = cd[0][0] + (np.arange(self.nx) + 1).astype(np.float64) * self.hx
         self.ys = cd[1][0] + (np.arange(self.ny) + 1).astype(np.float64) * self.hy
         self.mesh_size = math.sqrt(self.hx * self.hy)
+        self.A = None
+        self.lu = None
+        if not index_sets:
+            return
         # index sets (assignIndexSets / computeVolumeIndexSets)
         self.lims = [list(lims1), list(lims2)]
         self.col_class = self._classify(self.xs, lims1)
@@ -213,8 +224,6 @@ class Cell:
             for slr in (False, True):
                 for sud in (False, True):
                     self.ltr[(t, slr, sud)] = self._ltrace_sets(t, slr, sud)
-        self.A = None
-        self.lu = None
 
     # -- index sets --------------------------------------------------------
     @staticmethod
@@ -352,7 +361,10 @@ class Checkerboard:
     """
 
     def __init__(self, domain, ncells, nels, omega, min_wl_in_pml, medium, order=2,
-                 assemble=True, general=False):
+                 assemble=True, general=False, cells=True):
+        """cells=False keeps only what the global grid needs: the global FD/PML
+        operator, the DD mesh size (from each cell's local grid spacing) and
+        the global right-hand side; no cell index sets or local matrices."""
         if order != 2:
             raise ValueError("only Order 2 is supported")
         tt = int(math.ceil(order / 2.0))
@@ -385,6 +397,7 @@ class Checkerboard:
         self.h = (h1, h2)
         NX, NY = self.nels
         self.cells = []
+        self._cell_mesh = []
         for I in range(self.nR):
             for J in range(self.nC):
                 lims1 = [0.0] * 10
@@ -433,6 +446,10 @@ class Checkerboard:
                 int_dom[1] = [a, b]
                 nloc = [int((int_dom[0][1] - int_dom[0][0] + 0.5 * h1) / h1),
                         int((int_dom[1][1] - int_dom[1][0] + 0.5 * h2) / h2)]
+                if not cells:
+                    g = local_grid(int_dom, nloc, self.pml_width_in)
+                    self._cell_mesh.append(math.sqrt(g["hx"] * g["hy"]))
+                    continue
                 # provisional cell to get the DOF points for the boundary limits
                 probe = Cell(I, J, int_dom, nloc, [0.0] * 10, [0.0] * 10,
                              self.pml_width_in, self.omega, self.C, medium)
@@ -488,13 +505,14 @@ class Checkerboard:
 
     def mesh_size(self):
         """DDCheckerboard::getMeshSize (src/DDCheckerboard.cpp:3536-3561)."""
+        if not self.cells:
+            return min(self._cell_mesh)
         return min(c.mesh_size for c in self.cells)
 
     # ------------------------------------------------------------------
     # global grid: the operator of a 1x1 DD over the whole domain
     # ------------------------------------------------------------------
     def _global_grid(self):
-        c00 = self.cells[0]
         n1, n2 = self.n_el
         h1, h2 = self.h
         dom = self.domain
@@ -506,7 +524,7 @@ class Checkerboard:
         self.pml_els = pml_els
         # global DOF grid = the local grid of a 1x1 DD (src/AssemblerHelmholtz2DFD.cpp:166-177)
         g = Cell(0, 0, dom, [n1, n2], [0.0] * 10, [0.0] * 10, self.pml_width_in,
-                 self.omega, self.C, self.medium)
+                 self.omega, self.C, self.medium, index_sets=False)
         self.Gx, self.Gy = g.nx, g.ny
         self.N = self.Gx * self.Gy
         self.gcell = g
@@ -519,7 +537,6 @@ class Checkerboard:
             ii = np.arange(c.nx)[:, None] + ox
             jj = np.arange(c.ny)[None, :] + oy
             c.gidx = (ii * self.Gy + jj).ravel()   # may be out of range in local PML
-        del c00
 
     def global_operator(self):
         if self.gcell.A is None:
@@ -551,6 +568,10 @@ class Checkerboard:
         return out
 
     def global_rhs(self, sources):
+        if not self.cells:
+            # cell DOF points are global DOF points (grid-aligned cells)
+            X, Y = self.gcell.dof_points()
+            return point_sources_rhs(X, Y, sources, self.mesh_size())
         return self.to_global(self.assemble_rhs(sources))
 
     def factorize(self):

@@ -82,3 +82,20 @@ def test_block_precond_wide_fronts(prec):
         assert relerr(Y[k], y1) < (1e-12 if prec == 0 else 1e-4)
         if prec == 0 and k < 2:
             assert relerr(Y[k], P.apply_global(X[k])) < 1e-9
+
+
+def test_block_gmres_returns_when_right_hand_sides_hit_the_cap():
+    """Block flexible GMRES terminates when right-hand sides stop at different
+    steps: an easy one converges on its estimate, a hard one runs into
+    max_iter across restarts.  Each iteration count stays within max_iter."""
+    cb, dd = make_pair((2, 2), (19, 19), 2 * math.pi * 3.3)
+    dd.factorize()
+    b1 = dd.assemble_rhs([(0.5, 0.5)])
+    B = np.stack([b1, b1 * 0.0 + b1[::-1]])
+    X, infos = dd.gmres_block(B, tol=1e-15, max_iter=5, restart=2)
+    for inf in infos:
+        assert 1 <= inf["iterations"] <= 5
+    X2, infos2 = dd.gmres_block(B, tol=1e-3, max_iter=7, restart=2)
+    for inf in infos2:
+        assert inf["iterations"] <= 7
+        assert inf["true_residual"] <= 1e-3 or inf["iterations"] == 7

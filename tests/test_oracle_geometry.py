@@ -150,3 +150,22 @@ def test_point_source_integrates_to_one():
     theta = np.exp(-np.arange(-30, 31) ** 2.0).sum()
     assert abs(f.real.sum() * h * h - theta ** 2 / np.pi) < 1e-12
     assert np.all(f.imag == 0)
+
+
+@pytest.mark.parametrize("general,dom,nc,nel", [
+    (False, [[0.0, 1.0], [0.0, 1.0]], (3, 3), (12, 12)),
+    (True, [[0.0, 3.0], [0.0, 1.0]], (2, 4), (14, 21)),
+])
+def test_geometry_only_mode_matches_full_checkerboard(general, dom, nc, nel):
+    """The geometry-only checkerboard (no cell index sets) gives the same
+    global operator, DD mesh size and global right-hand side."""
+    omega = 2 * math.pi * 3.3
+    full = Checkerboard(dom, nc, nel, omega, 2.0, ConstantMedium(), general=general)
+    lite = Checkerboard(dom, nc, nel, omega, 2.0, ConstantMedium(), general=general, cells=False)
+    assert lite.mesh_size() == full.mesh_size()
+    assert (lite.Gx, lite.Gy) == (full.Gx, full.Gy)
+    assert (lite.global_operator() != full.global_operator()).nnz == 0
+    srcs = [(0.41 * dom[0][1], 0.37), (0.8 * dom[0][1], 0.61)]
+    a, b = lite.global_rhs(srcs), full.global_rhs(srcs)
+    # owned points of the cells tile the grid; values agree to coordinate rounding
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
