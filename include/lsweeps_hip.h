@@ -65,6 +65,16 @@ int lsw_lu_solve_z(lsw_lu* lu, int nrhs, const double* b, double* x);
  * LocalLinearSolverEigenSparseLU::solve (src/LocalLinearSolverEigenSparseLU.cpp:65-76)
  * without the host round trip. */
 int lsw_lu_solve_dev(lsw_lu* lu, int nrhs, const void* b, void* x, void* stream);
+/* lsw_lu_solve_dev that allocates no device memory (no hipMalloc/hipFree,
+ * no allocating library call): for callers that cannot allocate, such as
+ * code under HIP graph capture or tight serving loops.  No counterpart in
+ * the reference.  The solve's scratch (work lists and solve buffers) lives
+ * in the library's fixed device array of 64 MiB per device, held until the
+ * process ends; a solve that needs more fails with LSW_ERR_CAPACITY.  Every
+ * call on one device shares that device's array: calls that overlap on one
+ * device race.  The solver must be factorised; the results equal
+ * lsw_lu_solve_dev's. */
+int lsw_lu_solve_dev_noalloc(lsw_lu* lu, int nrhs, const void* b, void* x, void* stream);
 /* nx, ny of the recovered grid, complex factor entries, tree nodes, height. */
 int lsw_lu_info(lsw_lu* lu, int* nx, int* ny, long long* factor_entries, int* nodes, int* height);
 int lsw_lu_destroy(lsw_lu* lu);

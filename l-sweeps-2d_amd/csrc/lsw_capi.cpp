@@ -116,6 +116,14 @@ int lsw_lu_solve_dev(lsw_lu* lu, int nrhs, const void* b, void* x, void* stream)
   });
 }
 
+int lsw_lu_solve_dev_noalloc(lsw_lu* lu, int nrhs, const void* b, void* x, void* stream) {
+  return guarded([&] {
+    LSW_REQUIRE(lu && b && x, LSW_ERR_ARG, "null pointer");
+    lu->solver->solveDevNoAlloc(static_cast<const double2*>(b), static_cast<double2*>(x), nrhs,
+                                static_cast<hipStream_t>(stream));
+  });
+}
+
 int lsw_lu_info(lsw_lu* lu, int* nx, int* ny, long long* factor_entries, int* nodes, int* height) {
   return guarded([&] {
     LSW_REQUIRE(lu, LSW_ERR_ARG, "null pointer");

@@ -108,6 +108,13 @@ struct SolvePlan {
   SolveTaskDev* dTasks = nullptr;  // these three live in dBlob
   SolveEntry* dEntries = nullptr;
   void* dBlob = nullptr;
+  // hostOnly: buildPlan keeps the device image of tasks, entries and slot
+  // map in hBlob and allocates nothing; bind() points the plan at a copy of
+  // hBlob that the caller placed in device memory
+  bool hostOnly = false;
+  std::vector<unsigned char> hBlob;
+  size_t oEnt = 0, oTQ = 0;
+  void bind(void* base);
   std::vector<Launch> fwd, bwd;    // in launch order
   double fwdBytes = 0, bwdBytes = 0;
   SolvePlan() = default;

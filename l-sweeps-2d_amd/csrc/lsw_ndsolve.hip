@@ -923,12 +923,22 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   std::memcpy(blob.data(), tdev.data(), bTasks);
   if (bEnt) std::memcpy(blob.data() + oEnt, all.data(), bEnt);
   std::memcpy(blob.data() + oTQ, plan.slotTQ.data(), bTQ);
+  plan.oEnt = oEnt;
+  plan.oTQ = oTQ;
+  if (plan.hostOnly) {
+    plan.hBlob = std::move(blob);
+    return;
+  }
   LSW_HIP(devMalloc(&plan.dBlob, total));
   LSW_HIP(uploadSync(plan.dBlob, blob.data(), total));
-  auto* base = static_cast<unsigned char*>(plan.dBlob);
-  plan.dTasks = reinterpret_cast<SolveTaskDev*>(base);
-  plan.dEntries = reinterpret_cast<SolveEntry*>(base + oEnt);
-  plan.dSlotTQ = reinterpret_cast<int*>(base + oTQ);
+  plan.bind(plan.dBlob);
+}
+
+void SolvePlan::bind(void* base) {
+  auto* b = static_cast<unsigned char*>(base);
+  dTasks = reinterpret_cast<SolveTaskDev*>(b);
+  dEntries = reinterpret_cast<SolveEntry*>(b + oEnt);
+  dSlotTQ = reinterpret_cast<int*>(b + oTQ);
 }
 
 namespace {

@@ -13,6 +13,10 @@
 
 namespace lsw {
 
+// Size of the device scratch array of the allocation-free solve: 64 MiB per
+// device, for the plan's work lists plus the X, R and G solve buffers.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+
 class LocalLinearSolver {
  public:
   virtual ~LocalLinearSolver() = default;
@@ -44,6 +48,14 @@ class LocalLinearSolverHIP : public LocalLinearSolver {
   // on stream s (nullptr: the solver's own non-blocking stream, which is not
   // ordered with other streams).  b == x is allowed.
   void solveDev(const double2* b, double2* x, int nrhs, hipStream_t s) const;
+  // solveDev without any device allocation: the plan's work lists and the
+  // solve buffers use the library's fixed device scratch array
+  // (kNoAllocScratchBytes per device, shared by every call on that device:
+  // calls that overlap on one device race).  Needs a factorised solver;
+  // sizes beyond the array fail with LSW_ERR_CAPACITY.
+  void solveDevNoAlloc(const double2* b, double2* x, int nrhs, hipStream_t s) const;
+  // scratch bytes solveDevNoAlloc needs (plan image counted when given)
+  size_t noAllocBytes(int nrhs, const SolvePlan* plan) const;
   int precision() const { return mPrecision; }
   void setPrecision(int p) { mPrecision = p; }
   int gridNx() const { return mNx; }
@@ -66,6 +78,7 @@ class LocalLinearSolverHIP : public LocalLinearSolver {
   mutable void* dB = nullptr;
   mutable int mCap = 0;
   mutable std::map<int, std::unique_ptr<SolvePlan>> mPlans;
+  mutable std::map<int, std::unique_ptr<SolvePlan>> mPlansNoAlloc;  // host images only
 };
 
 }  // namespace lsw

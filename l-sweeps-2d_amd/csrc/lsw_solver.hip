@@ -3,12 +3,16 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 
 namespace lsw {
 
 LocalLinearSolverHIP::~LocalLinearSolverHIP() {
   if (mStream) (void)hipStreamSynchronize(mStream);
   mPlans.clear();
+  mPlansNoAlloc.clear();
   for (void* p : {dX, dR, dG, dB}) if (p) (void)devFree(p);
   mFS.reset();
   if (mStream) (void)hipStreamDestroy(mStream);
@@ -46,6 +50,7 @@ void LocalLinearSolverHIP::setMatrix(ComplexSparseMatrix* A) {
   mNx = nx;
   mNy = ny;
   mPlans.clear();
+  mPlansNoAlloc.clear();
   for (void* p : {dX, dR, dG, dB}) if (p) (void)devFree(p);
   dX = dR = dG = dB = nullptr;
   mCap = 0;
@@ -166,6 +171,113 @@ void LocalLinearSolverHIP::solveDev(const double2* b, double2* x, int nrhs, hipS
   const long long G = std::max<long long>(mFS->maxG(), 1);
   // right-hand side r = task r / per, lane r % per of the task-major buffer
   const int per = mFS->maxTaskRhs(), W = plan.width;
+  const dim3 grid = gridFor(N * nrhs);
+  if (mPrecision == LSW_FP64) {
+    hipLaunchKernelGGL(k_lu_pack<double2>, grid, dim3(256), 0, s, b, (double2*)dX, N, nrhs, per, W);
+    mFS->solve(plan, dX, dR, dG, N, G, s);
+    hipLaunchKernelGGL(k_lu_unpack<double2>, grid, dim3(256), 0, s, (const double2*)dX, x, N, nrhs, per, W);
+  } else {
+    hipLaunchKernelGGL(k_lu_pack<float2>, grid, dim3(256), 0, s, b, (float2*)dX, N, nrhs, per, W);
+    mFS->solve(plan, dX, dR, dG, N, G, s);
+    hipLaunchKernelGGL(k_lu_unpack<float2>, grid, dim3(256), 0, s, (const float2*)dX, x, N, nrhs, per, W);
+  }
+  LSW_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Solves that allocate nothing.  Their scratch (the plan's work lists and the
+// task-major X, R, G buffers) lives in one file-scope device array of
+// kNoAllocScratchBytes; each device holds its own copy until the process
+// ends.  Host code reaches a device's copy only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host-side
+// stand-in).  Every call on a device shares that device's array and leaves
+// its contents behind, so two such calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex gScratchMu;
+std::map<int, void*> gScratchOf;  // device -> address of its copy of scratch
+
+void* deviceScratch() {
+  int dev = 0;
+  LSW_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(gScratchMu);
+  auto it = gScratchOf.find(dev);
+  if (it != gScratchOf.end()) return it->second;
+  void* ptr = nullptr;
+  LSW_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  LSW_REQUIRE(ptr != nullptr, LSW_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  gScratchOf[dev] = ptr;
+  return ptr;
+}
+
+__global__ void k_clear_words(unsigned long long* __restrict__ p, long long words) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < words;
+       q += (long long)gridDim.x * blockDim.x)
+    p[q] = 0ull;
+}
+
+inline size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
+
+size_t LocalLinearSolverHIP::noAllocBytes(int nrhs, const SolvePlan* plan) const {
+  const long long N = mA->rows;
+  const long long G = std::max<long long>(mFS->maxG(), 1);
+  const size_t es = mPrecision == LSW_FP64 ? sizeof(double2) : sizeof(float2);
+  const int per = mFS->maxTaskRhs();
+  const size_t rows = (size_t)((nrhs + per - 1) / per) * per;
+  const size_t blob = plan ? up256(plan->hBlob.size()) : 0;
+  return blob + 2 * up256(rows * N * es) + up256(rows * G * es);
+}
+
+void LocalLinearSolverHIP::solveDevNoAlloc(const double2* b, double2* x, int nrhs, hipStream_t s) const {
+  LSW_REQUIRE(mFS && mFS->factorized(), LSW_ERR_STATE, "Solve called before factorization");
+  LSW_REQUIRE(nrhs >= 1, LSW_ERR_ARG, "the right-hand-side count must be positive");
+  LSW_REQUIRE(noAllocBytes(nrhs, nullptr) <= kNoAllocScratchBytes, LSW_ERR_CAPACITY,
+              "solve needs " + std::to_string(noAllocBytes(nrhs, nullptr)) + " bytes of scratch, more than the " +
+                  std::to_string(kNoAllocScratchBytes) + " the allocation-free entry point holds");
+  if (!s) s = mStream;
+  void* base = deviceScratch();
+  // the plan's host image, built once per right-hand-side count (host memory only)
+  auto it = mPlansNoAlloc.find(nrhs);
+  if (it == mPlansNoAlloc.end()) {
+    const int per = mFS->maxTaskRhs();
+    const int ntasks = (nrhs + per - 1) / per;
+    std::vector<SolveTask> tasks(ntasks);
+    for (int t = 0; t < ntasks; ++t) {
+      std::memset(&tasks[t], 0, sizeof(SolveTask));
+      tasks[t].nrhs = std::min(per, nrhs - t * per);
+      for (int r = 0; r < tasks[t].nrhs; ++r) tasks[t].slot[r] = t * per + r;
+    }
+    auto plan = std::make_unique<SolvePlan>();
+    plan->hostOnly = true;
+    mFS->buildPlan(tasks, {}, *plan);
+    it = mPlansNoAlloc.emplace(nrhs, std::move(plan)).first;
+  }
+  SolvePlan& plan = *it->second;
+  LSW_REQUIRE(noAllocBytes(nrhs, &plan) <= kNoAllocScratchBytes, LSW_ERR_CAPACITY,
+              "solve needs " + std::to_string(noAllocBytes(nrhs, &plan)) + " bytes of scratch, more than the " +
+                  std::to_string(kNoAllocScratchBytes) + " the allocation-free entry point holds");
+  const long long N = mA->rows;
+  const long long G = std::max<long long>(mFS->maxG(), 1);
+  const size_t es = mPrecision == LSW_FP64 ? sizeof(double2) : sizeof(float2);
+  const int per = mFS->maxTaskRhs(), W = plan.width;
+  const size_t rows = (size_t)((nrhs + per - 1) / per) * per;
+  LSW_REQUIRE((size_t)plan.taskRows() <= rows, LSW_ERR_INTERNAL, "plan needs more rows than the scratch holds");
+  // scratch layout: plan image | X | R | G
+  auto* p = static_cast<unsigned char*>(base);
+  const size_t oX = up256(plan.hBlob.size());
+  const size_t oR = oX + up256(rows * N * es), oG = oR + up256(rows * N * es);
+  const size_t end = oG + up256(rows * G * es);
+  void* dX = p + oX;
+  void* dR = p + oR;
+  void* dG = p + oG;
+  LSW_HIP(hipMemcpyAsync(p, plan.hBlob.data(), plan.hBlob.size(), hipMemcpyHostToDevice, s));
+  plan.bind(p);
+  // X, R and G start at zero: the solve kernels read (and multiply by zero)
+  // entries they do not use, and the array holds whatever the last call left
+  const long long words = (long long)((end - oX) / 8);
+  hipLaunchKernelGGL(k_clear_words, gridFor(words), dim3(256), 0, s, (unsigned long long*)dX, words);
   const dim3 grid = gridFor(N * nrhs);
   if (mPrecision == LSW_FP64) {
     hipLaunchKernelGGL(k_lu_pack<double2>, grid, dim3(256), 0, s, b, (double2*)dX, N, nrhs, per, W);

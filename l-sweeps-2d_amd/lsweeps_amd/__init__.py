@@ -21,7 +21,8 @@ ERRORS = {1: "ARG", 2: "HIP", 3: "STATE", 4: "NUMERIC", 5: "CAPACITY", 6: "UNSUP
 
 EXPORTED = [
     "lsw_last_error", "lsw_version", "lsw_device_count",
-    "lsw_lu_create", "lsw_lu_set_csc_z", "lsw_lu_factorize", "lsw_lu_solve_z", "lsw_lu_solve_dev", "lsw_lu_info",
+    "lsw_lu_create", "lsw_lu_set_csc_z", "lsw_lu_factorize", "lsw_lu_solve_z", "lsw_lu_solve_dev", "lsw_lu_solve_dev_noalloc",
+    "lsw_lu_info",
     "lsw_lu_destroy",
     "lsw_dd_create", "lsw_dd_create_host", "lsw_dd_destroy", "lsw_dd_factorize", "lsw_dd_sizes",
     "lsw_dd_global_coords", "lsw_dd_cell_info", "lsw_dd_assemble_rhs", "lsw_dd_assemble_rhs_dev",
@@ -118,6 +119,7 @@ _sig("lsw_lu_set_csc_z", _P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c
 _sig("lsw_lu_factorize", _P)
 _sig("lsw_lu_solve_z", _P, C.c_int, _D, _D)
 _sig("lsw_lu_solve_dev", _P, C.c_int, _P, _P, _P)
+_sig("lsw_lu_solve_dev_noalloc", _P, C.c_int, _P, _P, _P)
 _sig("lsw_lu_info", _P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong),
      C.POINTER(C.c_int), C.POINTER(C.c_int))
 _sig("lsw_lu_destroy", _P)
@@ -260,6 +262,10 @@ class LocalLinearSolverHIP:
     def solve_dev(self, nrhs, b_ptr, x_ptr, stream=0):
         """Device pointers: n x nrhs column-major complex128 (b_ptr == x_ptr allowed)."""
         _check(_lib.lsw_lu_solve_dev(self._h, int(nrhs), b_ptr, x_ptr, stream or None))
+
+    def solve_dev_noalloc(self, nrhs, b_ptr, x_ptr, stream=0):
+        """solve_dev with no device allocation (the library's fixed scratch array)."""
+        _check(_lib.lsw_lu_solve_dev_noalloc(self._h, int(nrhs), b_ptr, x_ptr, stream or None))
 
     def info(self):
         nx, ny, nodes, h = C.c_int(), C.c_int(), C.c_int(), C.c_int()
