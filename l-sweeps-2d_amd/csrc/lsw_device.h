@@ -20,9 +20,15 @@ namespace lsw {
 // Device memory of the library.  Every allocation is recorded with its size,
 // and each thread keeps the balance of what it allocated minus what it freed:
 // the server charges what a worker allocates while it serves a problem to
-// that problem.
+// that problem.  Freed blocks are cached for reuse (no hipFree, which waits
+// for the whole device); a cached block is reused only after a device
+// synchronisation that follows its free, so kernels queued before the free
+// on any stream have finished with it.
 hipError_t devMallocBytes(void** p, size_t bytes);
 hipError_t devFree(void* p);
+// bytes held in the block cache; return them to the runtime
+long long cachedDeviceBytes();
+void releaseDeviceCache();
 // Copy host memory to the device and return once the data is in device
 // memory (an asynchronous copy on the calling thread's own non-blocking
 // transfer stream, then a wait on that stream): kernels launched afterwards

@@ -757,20 +757,21 @@ void FactorSet::factorize(int precision, hipStream_t s) {
   auto nodeOf = [&](const FrontTask& ft) -> const NdNode& { return mTrees[mCells[ft.cell].shape]->nodes[ft.node]; };
   double2* dWs = nullptr;
   LSW_HIP(devMalloc(&dWs, std::max<long long>(maxWs, 1) * sizeof(double2)));
-  FrontTask* dFt = nullptr;
-  size_t ftCap = 0;
+  // every batch's front list in one upload; the batches then run back to
+  // back on the stream with no host round trip between heights
+  std::vector<FrontTask> allFt;
+  std::vector<size_t> ftOff(batches.size());
+  for (size_t bi = 0; bi < batches.size(); ++bi) {
+    ftOff[bi] = allFt.size();
+    allFt.insert(allFt.end(), batches[bi].begin(), batches[bi].end());
+  }
+  FrontTask* dFtAll = nullptr;
+  LSW_HIP(devMalloc(&dFtAll, std::max<size_t>(allFt.size(), 1) * sizeof(FrontTask)));
+  LSW_HIP(uploadSync(dFtAll, allFt.data(), allFt.size() * sizeof(FrontTask)));
   for (size_t bi = 0; bi < batches.size(); ++bi) {
     const auto& bt = batches[bi];
     if (bt.empty()) continue;
-    if (bt.size() > ftCap) {
-      if (dFt) {
-        LSW_HIP(hipStreamSynchronize(s));
-        (void)devFree(dFt);
-      }
-      ftCap = bt.size();
-      LSW_HIP(devMalloc(&dFt, ftCap * sizeof(FrontTask)));
-    }
-    LSW_HIP(hipMemcpyAsync(dFt, bt.data(), bt.size() * sizeof(FrontTask), hipMemcpyHostToDevice, s));
+    const FrontTask* dFt = dFtAll + ftOff[bi];
     const size_t shm = (size_t)mMaxFront * sizeof(double2);
     const unsigned nf = (unsigned)bt.size();
     if (!batchBlocked[bi]) {
@@ -801,11 +802,10 @@ void FactorSet::factorize(int precision, hipStream_t s) {
       hipLaunchKernelGGL(k_front_store, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dFactor, dSchur, dWs);
     }
     LSW_HIP(hipGetLastError());
-    LSW_HIP(hipStreamSynchronize(s));  // the host task buffer is reused
   }
   LSW_HIP(hipMemcpyAsync(&mInfo, dInfo, sizeof(int), hipMemcpyDeviceToHost, s));
   LSW_HIP(hipStreamSynchronize(s));
-  if (dFt) (void)devFree(dFt);
+  (void)devFree(dFtAll);
   (void)devFree(dWs);
   (void)devFree(dSchur);
   (void)devFree(dInfo);

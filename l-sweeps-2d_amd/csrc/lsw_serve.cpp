@@ -73,6 +73,10 @@ Server::~Server() {
   }
   mCv.notify_all();
   for (auto& t : mWorkers) t.join();
+  for (auto& g : mGrave) {
+    g.first.reset();
+    g.second.reset();
+  }
   for (auto& e : mEntries) {
     e->slots.reset();
     e->dd.reset();
@@ -96,10 +100,12 @@ int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
   auto geo = makeCheckerboard(e->params, false);
   e->Gx = geo->Gx();
   e->Gy = geo->Gy();
+  // factors and sweep buffers (steady) and the factorisation peak; the Krylov
+  // state is reserved on top when work arrives
   geo->deviceBytesEstimate(mOpt.precision, planWidth(mOpt.slots), &e->steadyBytes, &e->peakBytes);
-  e->steadyBytes += (double)GmresSlots::bytesFor(geo->N(), mOpt.slots, mOpt.restart);
-  e->peakBytes = std::max(e->peakBytes, e->steadyBytes);
-  LSW_REQUIRE(e->peakBytes <= mOpt.budgetBytes, LSW_ERR_CAPACITY,
+  e->N = geo->N();
+  const double full = e->steadyBytes + (double)GmresSlots::bytesFor(e->N, mOpt.slots, mOpt.restart);
+  LSW_REQUIRE(std::max(e->peakBytes, full) <= mOpt.budgetBytes, LSW_ERR_CAPACITY,
               "serve: problem needs more device memory than the budget");
   if (Gx) *Gx = e->Gx;
   if (Gy) *Gy = e->Gy;
@@ -111,8 +117,10 @@ int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
 void Server::problemBytes(int problem, double* steady, double* peak) const {
   std::lock_guard<std::mutex> lk(mMu);
   LSW_REQUIRE(problem >= 0 && problem < (int)mEntries.size(), LSW_ERR_ARG, "serve: unknown problem");
-  if (steady) *steady = mEntries[problem]->steadyBytes;
-  if (peak) *peak = mEntries[problem]->peakBytes;
+  const Entry& e = *mEntries[problem];
+  const double full = e.steadyBytes + (double)GmresSlots::bytesFor(e.N, mOpt.slots, mOpt.restart);
+  if (steady) *steady = full;
+  if (peak) *peak = std::max(e.peakBytes, full);
 }
 
 long long Server::submit(int problem, int nsrc, const double* xy, double* xOut) {
@@ -166,19 +174,23 @@ ServeStats Server::stats() const {
 }
 
 void Server::evict(Entry& e) {
-  e.slots.reset();
-  e.dd.reset();
+  mGrave.emplace_back(std::move(e.slots), std::move(e.dd));
   mStats.reservedBytes -= e.reserved;
   e.reserved = 0;
   e.actual = 0;
+  e.slotCap = 0;
+  e.krylov = 0;
   mStats.evictions++;
 }
 
-bool Server::admit(Entry& e) {
-  if (mStats.reservedBytes + e.peakBytes <= mOpt.budgetBytes) {
-    e.reserved = e.peakBytes;
-    mStats.reservedBytes += e.reserved;
+bool Server::admit(Entry& e, double need) {
+  auto take = [&] {
+    e.reserved += need;
+    mStats.reservedBytes += need;
     mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
+  };
+  if (mStats.reservedBytes + need <= mOpt.budgetBytes) {
+    take();
     return true;
   }
   // evict idle resident problems, least recently used first
@@ -189,15 +201,13 @@ bool Server::admit(Entry& e) {
       idle.push_back(o.get());
       freeable += o->reserved;
     }
-  if (mStats.reservedBytes - freeable + e.peakBytes > mOpt.budgetBytes) return false;
+  if (mStats.reservedBytes - freeable + need > mOpt.budgetBytes) return false;
   std::sort(idle.begin(), idle.end(), [](Entry* a, Entry* b) { return a->lastUse < b->lastUse; });
   for (Entry* o : idle) {
-    if (mStats.reservedBytes + e.peakBytes <= mOpt.budgetBytes) break;
+    if (mStats.reservedBytes + need <= mOpt.budgetBytes) break;
     evict(*o);
   }
-  e.reserved = e.peakBytes;
-  mStats.reservedBytes += e.reserved;
-  mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
+  take();
   return true;
 }
 
@@ -212,7 +222,15 @@ Server::Entry* Server::pick() {
     double t = 1e300;
     for (auto& kv : e.inFlight) t = std::min(t, mRequests[kv.second.req].res.submitS);
     if (!e.pending.empty()) t = std::min(t, mRequests[e.pending.front().req].res.submitS);
-    if (!e.dd && e.reserved == 0 && !admit(e)) continue;
+    if (!e.slots && e.slotCap == 0) {
+      // (re)activation: Krylov state for the right-hand sides waiting now,
+      // plus the factorisation peak when the problem is not resident
+      const int cap = std::max(1, std::min(mOpt.slots, (int)e.pending.size()));
+      const double kb = (double)GmresSlots::bytesFor(e.N, cap, mOpt.restart);
+      if (!admit(e, (e.dd ? 0.0 : e.peakBytes) + kb)) continue;
+      e.slotCap = cap;
+      e.krylov = kb;
+    }
     if (!best || t < bestT) {
       best = &e;
       bestT = t;
@@ -242,6 +260,8 @@ void Server::failEntry(Entry& e, int code, const std::string& msg) {
   mStats.reservedBytes -= e.reserved;
   e.reserved = 0;
   e.actual = 0;
+  e.slotCap = 0;
+  e.krylov = 0;
   mDoneCv.notify_all();
 }
 
@@ -251,14 +271,28 @@ void Server::workerLoop() {
   std::unique_lock<std::mutex> lk(mMu);
   while (true) {
     Entry* ep = nullptr;
-    mCv.wait(lk, [&] { return mStop || (ep = pick()) != nullptr; });
+    mCv.wait(lk, [&] { return mStop || (ep = pick()) != nullptr || !mGrave.empty(); });
     if (mStop) return;
+    if (ep) ep->busy = true;  // ours while the lock is dropped below
+    if (!mGrave.empty()) {
+      // evicted problems: destroyed without the lock held
+      auto grave = std::move(mGrave);
+      mGrave.clear();
+      lk.unlock();
+      for (auto& g : grave) {
+        g.first.reset();  // the slot engine uses its DD's stream
+        g.second.reset();
+      }
+      lk.lock();
+      if (!ep) continue;
+    }
     Entry& e = *ep;
-    e.busy = true;
     const bool load = !e.dd;
+    const bool makeSlots = !e.slots;
+    const int slotCap = e.slotCap;
     // right-hand sides to admit in this step
     std::vector<std::pair<long long, Column>> take;
-    const int room = load ? mOpt.slots : e.freeSlots;
+    const int room = makeSlots ? slotCap : e.freeSlots;
     while ((int)take.size() < room && !e.pending.empty()) {
       take.push_back({mNextTag++, e.pending.front()});
       e.pending.pop_front();
@@ -276,6 +310,7 @@ void Server::workerLoop() {
     std::string msg;
     std::vector<std::pair<Column, GmresSlots::Done>> finished;
     std::vector<std::pair<long long, Column>> admitted;
+    bool released = false;
     try {
       if (load) {
         const double t0 = now();
@@ -283,12 +318,14 @@ void Server::workerLoop() {
         dd->setGraphs(false);  // workers run concurrently: no stream capture
         const double t1 = now();
         dd->factorize();
-        const double t2 = now();
-        dd->reserveBlock(planWidth(mOpt.slots));
-        e.slots = std::make_unique<GmresSlots>(dd.get(), mOpt.slots, mOpt.restart, mOpt.tol, mOpt.maxIter);
         e.dd = std::move(dd);
         tSetup = t1 - t0;
-        tFactor = t2 - t1;
+        tFactor = now() - t1;
+      }
+      if (makeSlots) {
+        const double t2 = now();
+        e.dd->reserveBlock(planWidth(slotCap));
+        e.slots = std::make_unique<GmresSlots>(e.dd.get(), slotCap, mOpt.restart, mOpt.tol, mOpt.maxIter);
         tBlock = now() - t2;
       }
       const double ts = now();
@@ -328,6 +365,16 @@ void Server::workerLoop() {
         gs.release(d.slot);
         finished.push_back({c, d});
       }
+      // nothing left to solve: give the Krylov state back (factors stay)
+      bool idle;
+      {
+        std::lock_guard<std::mutex> g(mMu);
+        idle = e.slots->active() == 0 && e.pending.empty();
+      }
+      if (idle) {
+        e.slots.reset();
+        released = true;
+      }
     } catch (const Error& ex) {
       err = ex.code;
       msg = ex.what();
@@ -344,17 +391,20 @@ void Server::workerLoop() {
     e.busy = false;
     e.lastUse = ++mClock;
     e.actual += (double)grown;
-    if (load && err == LSW_OK) {
-      mStats.loads++;
-      // the factorisation workspace is gone: keep the steady estimate or
-      // what is really held, whichever is larger
-      const double keep = std::max(e.steadyBytes, e.actual);
+    if (err == LSW_OK) {
+      if (load) mStats.loads++;
+      if (released) {
+        e.krylov = 0;
+        e.slotCap = 0;
+      }
+      // after a load the factorisation workspace is gone: keep the steady
+      // estimate (plus the Krylov state while it exists) or what is really
+      // held, whichever is larger; plans for new block widths built during
+      // a step raise what is held
+      const double keep = (load || released) ? std::max(e.steadyBytes + e.krylov, e.actual)
+                                             : std::max(e.reserved, e.actual);
       mStats.reservedBytes += keep - e.reserved;
       e.reserved = keep;
-    } else if (err == LSW_OK && e.actual > e.reserved) {
-      // plans for new block widths were built during the step
-      mStats.reservedBytes += e.actual - e.reserved;
-      e.reserved = e.actual;
     }
     mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
     if (err != LSW_OK) {

@@ -116,8 +116,14 @@ class Server {
     lsw_dd_params params{};
     std::vector<float> velocity;
     int Gx = 0, Gy = 0;
+    long long N = 0;
     double steadyBytes = 0, peakBytes = 0, reserved = 0;
     double actual = 0;  // device bytes its loads and steps allocated (recorded)
+    // Krylov state: built when work arrives with room for the right-hand
+    // sides waiting then (at most the server's slots), released when the
+    // problem goes idle; the factors stay cached
+    int slotCap = 0;     // capacity admitted for the next (or current) slot engine
+    double krylov = 0;   // its reserved bytes
     std::unique_ptr<DDCheckerboard> dd;
     std::unique_ptr<GmresSlots> slots;
     std::deque<Column> pending;
@@ -130,8 +136,8 @@ class Server {
 
   void workerLoop();
   Entry* pick();                      // with the lock held
-  bool admit(Entry& e);               // with the lock held: reserve e.peakBytes
-  void evict(Entry& e);               // with the lock held
+  bool admit(Entry& e, double need);  // with the lock held: reserve `need` more bytes for e
+  void evict(Entry& e);               // with the lock held; destruction is deferred to mGrave
   void failEntry(Entry& e, int code, const std::string& msg);  // lock held
   double now() const;
 
@@ -141,6 +147,8 @@ class Server {
   std::condition_variable mDoneCv;  // a request finished
   std::vector<std::unique_ptr<Entry>> mEntries;
   std::map<long long, Request> mRequests;
+  // evicted problems, destroyed by a worker after it drops the lock
+  std::vector<std::pair<std::unique_ptr<GmresSlots>, std::unique_ptr<DDCheckerboard>>> mGrave;
   long long mNextId = 1, mNextTag = 1, mClock = 0;
   ServeStats mStats;
   bool mStop = false;
