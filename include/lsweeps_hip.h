@@ -223,8 +223,9 @@ int lsw_config_problem(const char* path, int id, lsw_problem_spec* spec, double*
 int lsw_problem_run(const char* path, int id, int verbose, double* x, long long x_cap, lsw_problem_result* res);
 
 /* ---- Output (OutputVTK::print2D, src/OutputVTK.cpp:82-366) ---------------
- * Writes <path>.vtk (the reference's legacy-ASCII quad layout over the
- * global DOF grid) and <path>.lswgrid (the same field in full precision:
+ * Writes <path>.vtk (print2D's legacy-ASCII layout of a one-rank run: per
+ * cell its printed elements and their points, boundary points included) and
+ * <path>.lswgrid (the same field on the global grid in full precision:
  * "LSWGRID1", int32 Gx, Gy, Gx x-coordinates, Gy y-coordinates, Gx*Gy
  * complex doubles).  x: global grid vector on the host.  Works on a
  * geometry-only DD. */

@@ -162,6 +162,7 @@ class DDCheckerboard : public DD {
   int precision() const { return mPrecision; }
   const FactorSet& factors() const { return *mFactors; }
   const CellGeom& cellGeom(int i) const { return mCells[i]; }
+  int numCells() const { return (int)mCells.size(); }
   int nRows() const { return mNR; }
   int nCols() const { return mNC; }
   long long localSolvesPerPrecond() const { return mSolvesPerPrecond; }

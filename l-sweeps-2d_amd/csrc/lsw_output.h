@@ -1,14 +1,18 @@
 // Solution output.
 //
-// writeVTK follows the file layout of OutputVTK::print2D
-// (src/OutputVTK.cpp:82-366): a legacy-ASCII "2D Unstuctured Grid of Linear
-// Quads" with POINTS (x y 0.0), CELLS of 4 point indices, CELL_TYPES 9 and two
-// point scalars wavefield_re / wavefield_im printed with the stream's default
-// 6 significant digits.  The reference gathers the owned elements of every
-// cell onto rank 0; here the solution already lives on one global grid, so
-// the points are the global DOF grid (x-major, index gi*Gy + gj) and the
-// quads join neighbouring DOFs.  Values are the same at every node; the point
-// and element order differ.
+// writeVTK writes the file OutputVTK::print2D (src/OutputVTK.cpp:82-366)
+// writes for a one-rank run: a legacy-ASCII "2D Unstuctured Grid of Linear
+// Quads"; cells in order, each contributing the elements whose midpoints lie
+// in its printed window (element classes 3..6 on both axes of
+// SubdomainStructured::computeVolumeElIndexSets) and the physical points
+// those elements use, in the cell's point order (its DOF points, then the
+// Dirichlet boundary points of its computational domain); corner indices as
+// AssemblerHelmholtz2DFD::getElPhysPtIndices numbers them; POINTS (x y 0.0),
+// CELLS of 4 point indices, CELL_TYPES 9, and point scalars wavefield_re /
+// wavefield_im, all at the stream's default 6 significant digits.  Points on
+// the border of two windows appear once per cell, as in the reference; a DOF
+// point carries the solution there and a boundary point zero (setVals'
+// expansion and trace exchange, src/OutputVTK.cpp:65-73).
 //
 // writeGrid writes the full-precision sibling of the same field:
 //   "LSWGRID1", int32 Gx, int32 Gy, Gx doubles (x), Gy doubles (y),

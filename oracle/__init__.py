@@ -12,7 +12,8 @@ It restates, in numpy/scipy, the algorithm of the reference L-Sweeps-2D code:
 * ``sweeps``    -- DDCheckerboard::applyPrecond / applySweep and the nine
   sweep loops, applySystemMatrix;
 * ``gmres``     -- GlobalIterativeSolverGMRES (reference mode, with its
-  quirks) and a textbook flexible GMRES used for the mixed-precision path.
+  quirks) and a textbook flexible GMRES used for the mixed-precision path;
+* ``output``    -- the point, element and value layout of OutputVTK::print2D.
 
 Local sparse LU is SuperLU through ``scipy.sparse.linalg.splu`` with COLAMD
 ordering, the same algorithm family as the Eigen SparseLU the reference calls

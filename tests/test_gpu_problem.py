@@ -49,3 +49,45 @@ def test_problem_run_writes_vtk(tmp_path):
     gx, gy = np.frombuffer(raw[8:16], dtype="<i4")
     x = np.frombuffer(raw[16 + 8 * (gx + gy):], dtype="<c16")
     assert np.array_equal(x, r["x"])
+
+
+def test_problem_run_csv_file_medium(tmp_path):
+    """A reference-format input whose squared slowness is Type = "CSV": the
+    velocities written by workloads.write_csv on a 4 x 16 file grid over the
+    unit square (mH = (1/16, 1/4): the clamp from above on x sits at
+    1 - 1/8, so points past x = 0.875 read column 14, never 15).  The GPU
+    Problem (file read with stof, reference GMRES) against the oracle's
+    FunctionCSV read of the same file."""
+    from lsweeps_amd.workloads import write_csv
+    from oracle.geometry import CSVMedium
+    rng = np.random.default_rng(9)
+    vel = (1.0 + rng.random((4, 16))).astype(np.float32)
+    write_csv(str(tmp_path / "vel.csv"), vel)
+    omega = 2 * math.pi * 3.0
+    cfg = tmp_path / "csv.cfg"
+    cfg.write_text(f"""
+Problems = ( {{ DomainDecomposition = "CheckerboardDD"; DomainDecompositionID = 0;
+               RHS = "src"; Solver = "gmres"; Output = ""; }} );
+Domains = {{ unit = {{ Dim = 2; ax = 0.0; bx = 1.0; ay = 0.0; by = 1.0; }}; }};
+SquaredSlownesses = {{ m = {{ Type = "CSV"; Filename = "{tmp_path / 'vel.csv'}"; Dim = 2; Domain = "unit"; }}; }};
+DomainDecompositions = {{ CheckerboardDD = ( {{
+    Dim = 2; Domain = "unit"; PDE = "Helmholtz"; Omega = {omega!r};
+    MinWavelengthsInPML = 2.0; SquaredSlowness = "m"; Order = 2;
+    NX = 19; NY = 19; NCellsX = 2; NCellsY = 2; Solver = "HIP"; }} ); }};
+RHSs = {{ src = ( {{ Dim = 2; x = 0.6; y = 0.4; }} ); }};
+GlobalSolvers = {{ gmres = {{ Type = "GMRES"; Tol = 1.0e-8; MaxIter = 30; Verbose = false; }}; }};
+""")
+    r = L.problem_run(str(cfg), 0)
+    med = CSVMedium.from_file(str(tmp_path / "vel.csv"), [[0.0, 1.0], [0.0, 1.0]])
+    assert med.mH == [1.0 / 16, 0.25]
+    cb = Checkerboard([[0.0, 1.0], [0.0, 1.0]], (2, 2), (19, 19), omega, 2.0, med)
+    cb.factorize()
+    A = cb.global_operator()
+    xs, _ = cb.gcell.dof_points()
+    assert np.any(xs > 0.9375)  # DOFs the clamp maps from column 15 to 14
+    b = cb.global_rhs([(0.6, 0.4)])
+    xo, log = gmres_reference(lambda v: A @ v, Preconditioner(cb).apply_global, b, tol=1e-8, max_iter=30)
+    assert r["dofs"] == cb.N
+    assert abs(r["iterations"] - (len(log) - 1)) <= 1
+    assert relerr(r["x"], xo) < 1e-6
+    assert np.linalg.norm(A @ r["x"] - b) / np.linalg.norm(b) < 1e-6

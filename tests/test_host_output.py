@@ -1,11 +1,14 @@
-"""Solution output on the host: the VTK file has the reference's layout
-(OutputVTK::print2D, src/OutputVTK.cpp:82-366) over the global DOF grid, and
-the full-precision sibling file round-trips the field exactly."""
+"""Solution output on the host: the VTK file has the layout of
+OutputVTK::print2D (src/OutputVTK.cpp:82-366) for a one-rank run, and the
+full-precision sibling file round-trips the field exactly."""
 import math
 
 import numpy as np
+import pytest
 
 import lsweeps_amd as L
+from oracle.geometry import Checkerboard, ConstantMedium
+from oracle.output import print2d_layout
 
 
 def _read_vtk(path):
@@ -49,26 +52,70 @@ def _read_grid(path):
     return int(gx), int(gy), xs, ys, x
 
 
-def test_vtk_and_grid_files(tmp_path):
+CASES = [
+    dict(ncells=(2, 2), nels=(9, 9), dom=((0.0, 1.0), (0.0, 1.0)), general=False),
+    dict(ncells=(1, 1), nels=(12, 12), dom=((0.0, 1.0), (0.0, 1.0)), general=False),
+    dict(ncells=(3, 3), nels=(7, 7), dom=((0.0, 1.0), (0.0, 1.0)), general=False),
+    dict(ncells=(2, 4), nels=(14, 21), dom=((0.0, 3.0), (0.0, 1.0)), general=True),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_vtk_follows_print2d_layout(case, tmp_path):
+    """The VTK file of a solution equals the oracle's restatement of
+    print2D's layout (cells in order, each cell's printed elements and the
+    points they use, values at 6 significant digits), and has the properties
+    that layout must have: every element an hx x hy square, the elements
+    tiling the global computational domain (PML included) exactly once, and
+    every point carrying the solution there (zero on the Dirichlet edge)."""
+    omega = 2 * math.pi * 3.1
+    p = L.dd_params(domain=case["dom"], ncells=case["ncells"], nels=case["nels"], omega=omega,
+                    min_wl_in_pml=1.0, general=case["general"])
+    dd = L.DDCheckerboard(p, device=False)
+    cb = Checkerboard([list(case["dom"][0]), list(case["dom"][1])], case["ncells"], case["nels"], omega, 1.0,
+                      ConstantMedium(), assemble=False, general=case["general"])
+    assert (dd.Gx, dd.Gy) == (cb.Gx, cb.Gy)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(dd.N) + 1j * rng.standard_normal(dd.N)
+    dd.write_vtk(x, tmp_path / "out")
+    pts, cells, types, vals = _read_vtk(tmp_path / "out.vtk")
+    assert np.all(cells[:, 0] == 4) and set(types) == {9} and np.all(pts[:, 2] == 0.0)
+    # the oracle's restatement of the layout
+    opts, oels, ovals = print2d_layout(cb, x)
+    assert np.array_equal(cells[:, 1:], oels)
+    assert np.allclose(pts[:, :2], opts, rtol=1e-5, atol=1e-6)
+    assert np.allclose(vals, ovals, rtol=1e-5, atol=1e-6)
+    # properties of any correct layout
+    g = cb.gcell
+    (X0, X1), (Y0, Y1) = g.comp_dom
+    h1, h2 = g.hx, g.hy
+    corner = pts[cells[:, 1:], :2]                           # (elements, 4, 2)
+    w = corner[:, :, 0].max(axis=1) - corner[:, :, 0].min(axis=1)
+    t = corner[:, :, 1].max(axis=1) - corner[:, :, 1].min(axis=1)
+    assert np.allclose(w, h1, rtol=1e-3) and np.allclose(t, h2, rtol=1e-3)
+    for k in range(4):                                        # corners are the rectangle's corners
+        on_x = np.isclose(corner[:, k, 0], corner[:, :, 0].min(axis=1), atol=1e-4 * h1) | \
+            np.isclose(corner[:, k, 0], corner[:, :, 0].max(axis=1), atol=1e-4 * h1)
+        assert np.all(on_x)
+    ei = np.rint((corner[:, :, 0].min(axis=1) - X0) / h1).astype(int)
+    ej = np.rint((corner[:, :, 1].min(axis=1) - Y0) / h2).astype(int)
+    assert ei.min() == 0 and ei.max() == g.mN[0] - 1 and ej.min() == 0 and ej.max() == g.mN[1] - 1
+    assert len(cells) == g.mN[0] * g.mN[1] and len(set(zip(ei, ej))) == len(cells)
+    gi = np.rint((pts[:, 0] - X0) / h1).astype(int) - 1
+    gj = np.rint((pts[:, 1] - Y0) / h2).astype(int) - 1
+    edge = (gi < 0) | (gi >= dd.Gx) | (gj < 0) | (gj >= dd.Gy)
+    assert np.all(vals[edge] == 0)
+    inner = ~edge
+    assert np.allclose(vals[inner], x[gi[inner] * dd.Gy + gj[inner]], rtol=1e-5, atol=1e-6)
+
+
+def test_grid_file_round_trips(tmp_path):
     p = L.dd_params(ncells=(2, 2), nels=(9, 9), omega=2 * math.pi * 3.1, min_wl_in_pml=1.0)
     dd = L.DDCheckerboard(p, device=False)
     rng = np.random.default_rng(3)
     x = rng.standard_normal(dd.N) + 1j * rng.standard_normal(dd.N)
     dd.write_vtk(x, tmp_path / "out")
-    pts, cells, types, vals = _read_vtk(tmp_path / "out.vtk")
     xs, ys = dd.coords()
-    assert pts.shape == (dd.N, 3)
-    X, Y = np.meshgrid(xs, ys, indexing="ij")
-    assert np.allclose(pts[:, 0], X.ravel(), rtol=1e-5, atol=1e-6)
-    assert np.allclose(pts[:, 1], Y.ravel(), rtol=1e-5, atol=1e-6)
-    assert np.all(pts[:, 2] == 0.0)
-    assert cells.shape == ((dd.Gx - 1) * (dd.Gy - 1), 5) and np.all(cells[:, 0] == 4)
-    assert set(types) == {9}
-    # every quad joins four grid neighbours counter-clockwise
-    a, b, c, d = cells[:, 1], cells[:, 2], cells[:, 3], cells[:, 4]
-    assert np.all(b - a == dd.Gy) and np.all(c - b == 1) and np.all(d - a == 1)
-    # 6 significant digits, as the reference's default stream precision
-    assert np.allclose(vals, x, rtol=1e-5, atol=1e-6)
     gx, gy, gxs, gys, gv = _read_grid(tmp_path / "out.lswgrid")
     assert (gx, gy) == (dd.Gx, dd.Gy)
     assert np.array_equal(gxs, xs) and np.array_equal(gys, ys) and np.array_equal(gv, x)
