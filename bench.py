@@ -2,24 +2,25 @@
 """Benchmark of the L-sweeps preconditioned Helmholtz solve on one MI355X.
 
 Metric: solves/s and p50/p99 latency at true relative residual <= 1e-6,
-mixed precision / batched against plain FP64.  Paths measured on the same
-model:
+mixed precision / batched against plain FP64.
 
-  plain_ref  FP64 factors and sweeps, the reference GMRES at its tolerance
-             (the parity path; its logged estimate is not the true residual)
-  plain      FP64 factors and sweeps, flexible GMRES to true residual 1e-6
-  mixed      FP32 factors and sweeps, FP64 flexible GMRES to 1e-6  (value)
-  batched    mixed, 16 point sources along y = 0.1 solved as one block
+Default line (python bench.py): the uneven batch of 48 problems from seed 0
+(lsweeps_amd.workloads.config_pack48).
+  value   mixed precision (FP32 factors and sweeps, FP64 flexible GMRES), every
+          problem in flight at once through the continuous-batching server.
+          A step is one pass over the pack: a new server (empty factor
+          cache), every problem registered, set up, factorised and all its
+          sources solved; --warmup passes untimed, then --steps passes timed.
+  plain   the plain FP64 path, one problem at a time as the reference runs a
+          problem (set-up, factorisation, then each source by FP64 flexible
+          GMRES), over all 48 problems.
+Beside it: the 1600^2 lens workload (plain_ref = FP64 with the reference
+GMRES, plain, mixed, a 16-source block), the roofline of the batched local
+solves, and the CPU baselines (the oracle on host processes).
 
-A step is one solve (one right-hand side; for `batched` one block, counted as
-16 solves) with the factors resident in HBM and the right-hand side already
-on the device.  Factorisation is timed separately, as the reference does
-("Time to factorize system" / "Time to run the GMRES", src/Problem.cpp:143-144).
-
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lens1600|const400]
-       python bench.py --workload serve    (request stream through the server)
-       python bench.py --workload pack48   (48 uneven problems at once)
-       python bench.py --workload layered  (2400x800 layered model, 128 sources as one block)
+Other lines: --workload lens1600 | const400 (one problem), pack48 (the pack
+alone), layered (2400x800, 128 sources as one block), serve (a Poisson
+request stream).
 """
 import argparse
 import json
@@ -140,40 +141,26 @@ def load_traffic(workload):
         return json.load(f)
 
 
-def cpu_baseline(cfg, iterations, seconds=20.0):
-    """The oracle's local SuperLU solve on one interior cell, timed on the host
-    (1 core), extrapolated to one reference-mode GMRES solve."""
-    import numpy as np
-    from oracle.geometry import Checkerboard, ConstantMedium, CSVMedium
-    t0 = time.perf_counter()
-    med = ConstantMedium() if cfg["velocity"] is None else CSVMedium(cfg["velocity"],
-                                                                      [list(d) for d in cfg["vel_domain"]])
-    cb = Checkerboard([list(cfg["domain"][0]), list(cfg["domain"][1])], cfg["ncells"], cfg["nels"], cfg["omega"],
-                      cfg["min_wl_in_pml"], med, assemble=False)
-    nR, nC = cfg["ncells"]
-    c = cb.cell(nR // 2, nC // 2)
-    c.assemble()
-    tf = time.perf_counter()
-    c.factorize()
-    t_fact = time.perf_counter() - tf
-    rng = np.random.default_rng(0)
-    b = rng.standard_normal(c.size) + 1j * rng.standard_normal(c.size)
-    ts = []
-    while sum(ts) < seconds * 0.75 and len(ts) < 3000:
-        t1 = time.perf_counter()
-        c.solve(b)
-        ts.append(time.perf_counter() - t1)
-    t_loc = float(np.median(ts))
-    n = max(nR, nC)
-    per_precond = 4 * (3 * n - 2) ** 2
-    t_solve = (iterations + 2) * per_precond * t_loc
-    return dict(value=1.0 / t_solve, unit="solves/s", cores=1, kind="port",
-                sample=(f"oracle (numpy/scipy SuperLU) on cell ({nR // 2},{nC // 2}) of the {cfg['name']} "
-                        f"workload: factorize 1 of {nR * nC} cells ({t_fact:.2f} s) and {len(ts)} local solves "
-                        f"(median {t_loc * 1e3:.1f} ms), extrapolated to {iterations}+2 preconditioner "
-                        f"applications x {per_precond} local solves of the reference GMRES; total sample "
-                        f"{time.perf_counter() - t0:.1f} s"),
-                local_solve_ms=round(t_loc * 1e3, 3), local_factor_s=round(t_fact, 3))
+def cpu_baseline(workload, procs=None, **kw):
+    """The oracle's restatement timed on P host processes (oracle/cpu_run.py,
+    run as a child process so that nothing forks this GPU process): the
+    reference's "factorize" and "GMRES" timers plus set-up, on a bounded
+    sample of the workload.  P = the host cores this job may use, at most 16
+    (one GPU's share of the box)."""
+    import subprocess
+    if procs is None:
+        try:
+            procs = len(os.sched_getaffinity(0))
+        except AttributeError:
+            procs = os.cpu_count() or 1
+        procs = max(1, min(16, procs))
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_run.py"), "--workload", workload, "--procs", str(procs)]
+    for k, v in kw.items():
+        cmd += [f"--{k}", str(v)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def _problem_params(L, p, precision):
@@ -445,49 +432,10 @@ def layered_bench(args):
     print(json.dumps(line))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lens1600")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="skip timing alternative launch shapes per DD and block width before the timed solves")
-    ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
-    ap.add_argument("--serve-models", type=int, default=8)
-    ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-load-requests", type=int, default=6)
-    ap.add_argument("--serve-budget", type=float, default=240e9)
-    ap.add_argument("--serve-slots", type=int, default=16)
-    ap.add_argument("--serve-restart", type=int, default=8)
-    ap.add_argument("--serve-workers", type=int, default=6)
-    ap.add_argument("--plain-sample", type=int, default=4)
-    ap.add_argument("--pack-count", type=int, default=48)
-    ap.add_argument("--layered-sources", type=int, default=128)
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="end with one 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(sharding across GPUs is design-only in this build)", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "serve":
-        serve_bench(args)
-        return
-    if args.workload == "pack48":
-        pack_bench(args)
-        return
-    if args.workload == "layered":
-        layered_bench(args)
-        return
-    import numpy as np
-    import torch
-    import lsweeps_amd as L
-    from lsweeps_amd.workloads import CONFIGS
-
-    cfg = CONFIGS[args.workload]()
-    paths = args.paths.split(",")
+def lens_section(args, L, torch, cfg, paths):
+    """The single-problem figures (plain_ref, plain, mixed, batched) of one
+    workload with per-launch solve profiles; returns (results, solutions,
+    profiles, launch shapes)."""
     log(f"workload {cfg['name']}: grid {cfg['grid']}, board {cfg['ncells']}, omega {cfg['omega']:.3f}")
     out, xs, prof, tune = {}, {}, {}, {}
     if "plain_ref" in paths or "plain" in paths:
@@ -507,7 +455,6 @@ def main():
             out["plain"].update(factorize_s=tf, setup_s=ts, gmres="flexible", tol=1e-6)
             log("  plain", out["plain"])
         prof["fp64"] = profile(dd, torch)
-        st64 = dd.stats()
         del dd
         torch.cuda.empty_cache()
     if "mixed" in paths or "batched" in paths:
@@ -527,66 +474,72 @@ def main():
             out["batched"] = time_batched(dd, torch, cfg, 1e-6, max(1, args.steps // 2), min(args.warmup, 1))
             log("  batched", out["batched"])
         prof["fp32"] = profile(dd, torch)
-        st32 = dd.stats()
         del dd
         torch.cuda.empty_cache()
+    return out, xs, prof, tune
+
+
+def lens_fields(cfg, out, xs, prof, tune):
+    """The single-problem figures as JSON fields."""
+    import numpy as np
+    f = {}
+    for k, v in out.items():
+        f[k] = {kk: (float(f"{vv:.6g}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
+    if "fp32" in prof or "fp64" in prof:
+        f["roofline"] = roofline(prof["fp32"] if "fp32" in prof else prof["fp64"])
+        traffic = load_traffic(cfg["name"])
+        if traffic and "fp32" in prof:
+            f["roofline"]["traffic"] = traffic["bytes_per_launch"]
+            f["roofline"]["traffic_source"] = traffic["source"]
+    if "fp64" in prof:
+        f["roofline_fp64"] = roofline(prof["fp64"])
+    if "plain" in out:
+        base = out["plain"]["solves_per_s"]
+        for k in ("mixed", "batched"):
+            if k in out:
+                f[f"speedup_{k}_vs_plain"] = round(out[k]["solves_per_s"] / base, 3)
+    if "mixed" in xs and "plain" in xs:
+        f["mixed_vs_plain_rel_l2"] = float(np.linalg.norm(xs["mixed"] - xs["plain"]) / np.linalg.norm(xs["plain"]))
+    if "plain_ref" in xs and "plain" in xs:
+        f["plain_ref_vs_plain_rel_l2"] = float(np.linalg.norm(xs["plain_ref"] - xs["plain"]) /
+                                               np.linalg.norm(xs["plain"]))
+    if tune:
+        f["launch_shapes"] = tune
+    f["dofs"] = int(np.prod(cfg["grid"]))
+    return f
+
+
+def lens_bench(args):
+    """One workload's single-problem line (--workload lens1600 or const400)."""
+    import torch
+    import lsweeps_amd as L
+    from lsweeps_amd.workloads import CONFIGS
+    cfg = CONFIGS[args.workload]()
+    out, xs, prof, tune = lens_section(args, L, torch, cfg, args.paths.split(","))
     head_name = next((k for k in ("mixed", "plain", "plain_ref") if k in out), None)
     if head_name is None:
         log("no single-source path in --paths: results above, no bench line")
         return
     head = out[head_name]
-    roof = roofline(prof["fp32"] if "fp32" in prof else prof["fp64"])
     cpu = None
     if not args.no_cpu_baseline:
         iters = out["plain_ref"]["iterations"] if "plain_ref" in out else head["iterations"]
-        log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(cfg, iters)
+        log("cpu baseline (oracle on host processes) ...")
+        cpu = cpu_baseline("lens1600" if cfg["name"].startswith("lens") else "const400", iterations=iters)
         log("  ", cpu)
-    n_dofs = int(np.prod(cfg["grid"]))
     line = {
-        "metric": METRIC,
-        "value": round(head["solves_per_s"], 4),
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(head["solve_s"] * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "c64 factors/sweeps + c128 Krylov" if head_name == "mixed" else "c128",
-        "data": "synthetic",
+        "metric": METRIC, "value": round(head["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["solve_s"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 factors/sweeps + c128 Krylov" if head_name == "mixed" else "c128", "data": "synthetic",
         "config": {"workload": cfg["name"], "grid": list(cfg["grid"]), "board": list(cfg["ncells"]),
                    "omega": round(cfg["omega"], 4), "sources": len(cfg["sources"]), "path": head_name,
                    "residual": "true relative residual <= 1e-6"},
-        "p50_ms": round(head["p50_ms"], 3),
-        "p99_ms": round(head["p99_ms"], 3),
-        "roofline": roof,
-        "cpu_baseline": cpu,
+        "p50_ms": round(head["p50_ms"], 3), "p99_ms": round(head["p99_ms"], 3),
     }
-    for k, v in out.items():
-        line[k] = {kk: (float(f"{vv:.6g}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
-    if "fp64" in prof:
-        line["roofline_fp64"] = roofline(prof["fp64"])
-    if "plain" in out:
-        base = out["plain"]["solves_per_s"]
-        for k in ("mixed", "batched"):
-            if k in out:
-                line[f"speedup_{k}_vs_plain"] = round(out[k]["solves_per_s"] / base, 3)
-    if "mixed" in xs and "plain" in xs:
-        line["mixed_vs_plain_rel_l2"] = float(np.linalg.norm(xs["mixed"] - xs["plain"]) /
-                                              np.linalg.norm(xs["plain"]))
-    if "plain_ref" in xs and "plain" in xs:
-        line["plain_ref_vs_plain_rel_l2"] = float(np.linalg.norm(xs["plain_ref"] - xs["plain"]) /
-                                                  np.linalg.norm(xs["plain"]))
-    if tune:
-        line["launch_shapes"] = tune
-    line["dofs"] = n_dofs
-    line["dofs_per_s"] = round(n_dofs * head["solves_per_s"], 1)
-    traffic = load_traffic(cfg["name"])
-    if traffic:
-        line["roofline"]["traffic"] = traffic["bytes_per_launch"]
-        line["roofline"]["traffic_source"] = traffic["source"]
+    line.update(lens_fields(cfg, out, xs, prof, tune))
+    line["cpu_baseline"] = _cpu_object(cpu) if cpu else None
+    line["dofs_per_s"] = round(line["dofs"] * head["solves_per_s"], 1)
     print(json.dumps(line))
     if args.calibrate_copy:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0")
@@ -594,6 +547,167 @@ def main():
         torch.cuda.synchronize()
         b.copy_(a)
         torch.cuda.synchronize()
+
+
+def _cpu_object(r):
+    """The cpu_baseline object of the bench line from a cpu_run result."""
+    if "gmres_s" in r and "iterations" in r and "solves" not in r:
+        return {"value": round(r["solves_per_s"], 6), "unit": "solves/s", "cores": r["procs"], "kind": "port",
+                "sample": (f"{r['workload']}: the oracle's restatement (SuperLU local solves, reference-mode GMRES) "
+                           f"on {r['procs']} processes of {r['host_cpus']} host CPUs ({r['cpu']}); measured: "
+                           f"{r['measured']}; value = 1 / GMRES seconds"),
+                "factorize_s": round(r["factorize_s"], 3), "gmres_s": round(r["gmres_s"], 3),
+                "setup_s": round(r["setup_s"], 3), "end_to_end_s": round(r["end_to_end_s"], 3),
+                "iterations": r["iterations"], "true_residual": r.get("true_residual")}
+    return {"value": round(r["solves_per_s"], 6), "unit": "solves/s", "cores": r.get("procs"), "kind": "port",
+            "sample": (f"the oracle's restatement (SuperLU local solves, flexible GMRES) on {r.get('procs')} "
+                       f"processes of {r['host_cpus']} host CPUs ({r['cpu']}): {r['measured']}"),
+            "solves": r["solves"], "seconds": round(r["seconds"], 3), "factorize_s": round(r["factorize_s"], 3),
+            "gmres_s": round(r["gmres_s"], 3), "setup_s": round(r["setup_s"], 3)}
+
+
+def pack_pass(L, probs, workers, restart, budget, slots=16):
+    """One pass over the uneven pack: a new server (empty factor cache), every
+    problem registered and its request submitted at once, every result
+    waited for.  Returns (wall seconds, per-request results, server stats)."""
+    t0 = time.perf_counter()
+    srv = L.Server(budget_bytes=budget, slots=slots, restart=restart, tol=1e-6, max_iter=200,
+                   precision=L.LSW_PREC_FP32, workers=workers)
+    try:
+        # each problem's request goes in as soon as the problem is registered
+        ids = [srv.submit(srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)), p["sources"], want_solution=False)
+               for p in probs]
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+    return wall, res, st
+
+
+def north_star_bench(args):
+    """The default line: the 48-problem uneven pack (mixed precision, packed
+    and served by continuous batching) against the plain FP64 path one problem
+    at a time, with the 1600^2 lens figures, the solve kernels' roofline and
+    the CPU baselines beside it."""
+    import numpy as np
+    import torch
+    import lsweeps_amd as L
+    from lsweeps_amd.workloads import config_pack48, config_lens_1600
+    probs = config_pack48(seed=0, count=args.pack_count)
+    nsrc = sum(len(p["sources"]) for p in probs)
+    log(f"pack: {len(probs)} problems, {nsrc} sources, {sum(p['grid'][0] * p['grid'][1] for p in probs)} grid nodes")
+    # plain FP64: every problem as the reference runs one (set-up, factorise,
+    # then each source on its own), one after the other
+    t0 = time.perf_counter()
+    pt, pres, pits = 0.0, 0.0, []
+    for p in probs:
+        t, worst, it = plain_request(L, torch, _problem_params(L, p, L.LSW_PREC_FP64), p["sources"])
+        pt += t
+        pres = max(pres, worst)
+        pits.append(it)
+    torch.cuda.empty_cache()
+    plain = dict(solves_per_s=nsrc / pt, seconds=pt, problems=len(probs), solves=nsrc, worst_true_residual=pres,
+                 max_iterations=max(pits), gmres="flexible FP64, restart 30, one source at a time")
+    log("plain", plain, f"({time.perf_counter() - t0:.1f} s)")
+    # mixed, packed: W untimed passes, then K timed passes
+    for w in range(args.warmup):
+        wall, _, _ = pack_pass(L, probs, args.pack_workers, args.serve_restart, args.serve_budget)
+        log(f"  warm-up pass {w}: {wall:.2f} s")
+    walls, lat, worst, failed, itmax = [], [], 0.0, 0, 0
+    st = None
+    for k in range(args.steps):
+        wall, res, st = pack_pass(L, probs, args.pack_workers, args.serve_restart, args.serve_budget)
+        walls.append(wall)
+        lat += [r["latency_s"] for r in res]
+        failed += sum(r["status"] != 0 for r in res)
+        worst = max(worst, max(r["max_true_residual"] for r in res))
+        itmax = max(itmax, max(r["max_iterations"] for r in res))
+        log(f"  pass {k}: {wall:.2f} s = {nsrc / wall:.2f} solves/s")
+    total = sum(walls)
+    packed = dict(solves_per_s=args.steps * nsrc / total, seconds_per_pass=total / args.steps, passes=args.steps,
+                  failed=failed, worst_true_residual=worst, max_iterations=itmax,
+                  loads=st["loads"], evictions=st["evictions"], steps=st["steps"], merged_steps=st["merged_steps"],
+                  worker_s={k: round(st[k], 3) for k in ("setup_s", "factor_s", "block_s", "step_s")},
+                  workers=args.pack_workers, restart=args.serve_restart, budget_gb=args.serve_budget / 1e9)
+    log("packed", packed)
+    # the 1600^2 lens workload: single-problem figures and the solve roofline
+    cfg = config_lens_1600()
+    out, xs, prof, tune = lens_section(args, L, torch, cfg, args.paths.split(","))
+    lens = lens_fields(cfg, out, xs, prof, tune)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baselines (oracle on host processes) ...")
+        cpu = cpu_baseline("pack48", budget=args.cpu_budget)
+        log("  pack48", cpu)
+        iters = out["plain_ref"]["iterations"] if "plain_ref" in out else 7
+        cpu_lens = cpu_baseline("lens1600", iterations=iters)
+        log("  lens1600", cpu_lens)
+        lens["cpu_baseline"] = _cpu_object(cpu_lens)
+    line = {
+        "metric": METRIC, "value": round(packed["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 factors/sweeps + c128 Krylov", "data": "synthetic",
+        "config": {"workload": f"pack{len(probs)}_seed0", "problems": len(probs), "solves": nsrc,
+                   "grid_nodes": int(sum(p["grid"][0] * p["grid"][1] for p in probs)),
+                   "step": "one pass over the pack: a new server, every problem registered, set up, factorised "
+                           "and all its sources solved",
+                   "path": "FP32 factors and sweeps, FP64 flexible GMRES, continuous batching",
+                   "residual": "true relative residual <= 1e-6"},
+        "p50_ms": round(1e3 * pct(lat, 50), 1), "p99_ms": round(1e3 * pct(lat, 99), 1),
+        "speedup_vs_plain": round(packed["solves_per_s"] / plain["solves_per_s"], 3),
+        "plain": plain, "packed": packed,
+        "roofline": lens.pop("roofline", None),
+        "cpu_baseline": _cpu_object(cpu) if cpu else None,
+        "lens1600": lens,
+    }
+    if line["roofline"] is not None:
+        line["roofline"]["workload"] = "the batched ND solves of one M^-1 application on the 1600^2 lens (FP32)"
+    print(json.dumps(line))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="default",
+                    help="default (the 48-problem pack with the lens figures), lens1600, const400, pack48, "
+                         "layered, serve")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip timing alternative launch shapes per DD and block width before the timed solves")
+    ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
+    ap.add_argument("--pack-workers", type=int, default=8)
+    ap.add_argument("--serve-models", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-load-requests", type=int, default=6)
+    ap.add_argument("--serve-budget", type=float, default=240e9)
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-restart", type=int, default=10)
+    ap.add_argument("--serve-workers", type=int, default=6)
+    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--pack-count", type=int, default=48)
+    ap.add_argument("--layered-sources", type=int, default=128)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with one 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(sharding across GPUs is design-only in this build)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "default":
+        north_star_bench(args)
+    elif args.workload == "serve":
+        serve_bench(args)
+    elif args.workload == "pack48":
+        pack_bench(args)
+    elif args.workload == "layered":
+        layered_bench(args)
+    else:
+        lens_bench(args)
 
 
 if __name__ == "__main__":

@@ -948,6 +948,7 @@ struct LaunchTimer {
   std::vector<hipEvent_t> ev;
   std::vector<int> kind;
   std::vector<double> bytes;
+  std::vector<const SolvePlan::Launch*> what;
   void begin() {
     if (!prof) return;
     hipEvent_t e;
@@ -970,6 +971,12 @@ struct LaunchTimer {
     for (size_t i = 0; i < kind.size(); ++i) {
       float ms = 0;
       LSW_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+      if (std::getenv("LSW_PROFILE_LOG") && i < what.size()) {
+        const auto& L = *what[i];
+        std::fprintf(stderr, "launch %s %zu: entries %d mode %d rp %d wpb %d waves %d cols %d bytes %.0f us %.2f GB/s %.0f\n",
+                     kind[i] ? "bwd" : "fwd", i, L.n, L.mode, L.rp, L.wpb, L.waves, L.cols, L.bytes, 1e3 * ms,
+                     ms > 0 ? L.bytes / (ms * 1e6) : 0.0);
+      }
       if (kind[i] == 0) {
         prof->fwdMs += ms;
         prof->fwdLaunches++;
@@ -1058,11 +1065,13 @@ void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, const 
     tm.begin();
     launch<T2, NR, true>(a, L, factor, X, Xo, R, G, s);
     tm.end(0, L.bytes);
+    if (prof) tm.what.push_back(&L);
   }
   for (const auto& L : plan.bwd) {
     tm.begin();
     launch<T2, NR, false>(a, L, factor, X, Xo, R, G, s);
     tm.end(1, L.bytes);
+    if (prof) tm.what.push_back(&L);
   }
   LSW_HIP(hipGetLastError());
   tm.finish();

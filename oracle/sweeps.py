@@ -38,21 +38,39 @@ FF, TF, FT, TT = (False, False), (True, False), (False, True), (True, True)
 
 def _order(cb, flip_i, flip_j):
     """getSortedIndices on the (optionally mirrored) diagonal index."""
+    return [idx for batch in _diagonals(cb, flip_i, flip_j) for idx in batch]
+
+
+def _diagonals(cb, flip_i, flip_j):
+    """The processing order of getSortedIndices cut into its anti-diagonals:
+    a sweep reads only the traces of cells on earlier diagonals, so the
+    local solves of one diagonal are independent (the reference's ranks run
+    them at the same time)."""
     diag = []
     for idx, c in enumerate(cb.cells):
         I = cb.nR - 1 - c.I if flip_i else c.I
         J = cb.nC - 1 - c.J if flip_j else c.J
         diag.append((float(I + J), idx))
     diag.sort()
-    return [idx for _, idx in diag]
+    out = []
+    for d, idx in diag:
+        if out and out[-1][0] == d:
+            out[-1][1].append(idx)
+        else:
+            out.append((d, [idx]))
+    return [b for _, b in out]
 
 
 class Preconditioner:
     """L-sweeps M^{-1} over one factorized Checkerboard."""
 
-    def __init__(self, cb):
+    def __init__(self, cb, solver=None):
+        """solver (optional): solve_many([(cell index, rhs), ...]) -> list of
+        solutions, e.g. a process pool holding the factors; by default each
+        cell's own factors, one after the other."""
         self.cb = cb
         self.nsolves = 0
+        self.solver = solver
 
     # -- helpers -------------------------------------------------------------
     def expand(self, x, shift=FF):
@@ -66,9 +84,32 @@ class Preconditioner:
     def reduce(self, x, shift=FF):
         return [np.array(v[c.keep[shift]]) for c, v in zip(self.cb.cells, x)]
 
-    def _solve(self, c, rhs):
-        self.nsolves += 1
-        return c.solve(rhs)
+    def _solve_many(self, reqs):
+        self.nsolves += len(reqs)
+        if self.solver is not None:
+            return self.solver.solve_many(reqs)
+        return [self.cb.cells[i].solve(r) for i, r in reqs]
+
+    def _run(self, batches, body):
+        """Runs body(idx) for every cell, a diagonal at a time.  body is a
+        generator that yields the right-hand side of its local solve (or
+        returns without one) and receives the solution; the solves of one
+        diagonal go to _solve_many together."""
+        for batch in batches:
+            gens, reqs = [], []
+            for idx in batch:
+                g = body(idx)
+                try:
+                    rhs = next(g)
+                except StopIteration:
+                    continue
+                gens.append(g)
+                reqs.append((idx, rhs))
+            for g, sol in zip(gens, self._solve_many(reqs) if reqs else []):
+                try:
+                    g.send(sol)
+                except StopIteration:
+                    pass
 
     def _idx(self, I, J):
         return I * self.cb.nC + J
@@ -152,9 +193,15 @@ class Preconditioner:
             r0, r1 = lr[1], lr[2]
         else:
             r0, r1 = c.ltr[(kind, slr, sud)]
-        A = c.A
-        A01 = A[r0][:, r1]
-        A10 = A[r1][:, r0]
+        # the coupling blocks A01 = A[trace0, trace1], A10 = A[trace1, trace0]
+        # are built once per cell, kind and shift, as the reference builds
+        # them at set-up (src/SubdomainStructured.cpp:549-654)
+        cache = c.__dict__.setdefault("_coupling", {})
+        key = (kind, slr, sud)
+        if key not in cache:
+            A = c.A.tocsr()
+            cache[key] = (A[r0][:, r1].tocsr(), A[r1][:, r0].tocsr())
+        A01, A10 = cache[key]
         return A01 @ vals1, -(A10 @ vals0)
 
     @staticmethod
@@ -197,13 +244,17 @@ class Preconditioner:
         n = len(self.cb.cells)
         LR = [[None] * n for _ in range(4)]
         UD = [[None] * n for _ in range(4)]
-        for idx, c in enumerate(self.cb.cells):
-            s = self._solve(c, x[idx])
+
+        def body(idx):
+            c = self.cb.cells[idx]
+            s = yield x[idx]
             res[idx] += s
             lr, ud = c.lr[slr], c.ud[sud]
             for k, si in enumerate((1, 2, 4, 5)):
                 UD[k][idx] = s[ud[si]].copy()
                 LR[k][idx] = s[lr[si]].copy()
+
+        self._run([list(range(n))], body)
         return LR, UD
 
     def stage2_lr(self, res, LRin, slr, sud, rightward):
@@ -212,8 +263,9 @@ class Preconditioner:
         n = len(cb.cells)
         LR = [[None if v is None else v.copy() for v in LRin[k]] for k in range(4)]
         UD = [[None] * n for _ in range(4)]
-        order = _order(cb, False, False) if rightward else _order(cb, True, True)
-        for idx in order:
+        batches = _diagonals(cb, False, False) if rightward else _diagonals(cb, True, True)
+
+        def body(idx):
             c = cb.cells[idx]
             lr, ud = c.lr[slr], c.ud[sud]
             for k, si in enumerate((1, 2, 4, 5)):
@@ -235,8 +287,8 @@ class Preconditioner:
                 rhs[lr[4]] = e1
                 rhs[lr[5]] = e0
             else:
-                continue
-            s = self._solve(c, rhs)
+                return
+            s = yield rhs
             res[idx] += s
             for k, si in enumerate((1, 2, 4, 5)):
                 UD[k][idx] += s[ud[si]]
@@ -246,6 +298,8 @@ class Preconditioner:
             else:
                 LR[0][idx] += s[lr[1]]
                 LR[1][idx] += s[lr[2]]
+
+        self._run(batches, body)
         return UD
 
     def stage2_ud(self, res, UDin, slr, sud, upward):
@@ -254,8 +308,9 @@ class Preconditioner:
         n = len(cb.cells)
         UD = [[None if v is None else v.copy() for v in UDin[k]] for k in range(4)]
         LR = [[None] * n for _ in range(4)]
-        order = _order(cb, False, False) if upward else _order(cb, True, True)
-        for idx in order:
+        batches = _diagonals(cb, False, False) if upward else _diagonals(cb, True, True)
+
+        def body(idx):
             c = cb.cells[idx]
             lr, ud = c.lr[slr], c.ud[sud]
             for k, si in enumerate((1, 2, 4, 5)):
@@ -277,8 +332,8 @@ class Preconditioner:
                 rhs[ud[4]] = e1
                 rhs[ud[5]] = e0
             else:
-                continue
-            s = self._solve(c, rhs)
+                return
+            s = yield rhs
             res[idx] += s
             for k, si in enumerate((1, 2, 4, 5)):
                 LR[k][idx] += s[lr[si]]
@@ -288,6 +343,8 @@ class Preconditioner:
             else:
                 UD[0][idx] += s[ud[1]]
                 UD[1][idx] += s[ud[2]]
+
+        self._run(batches, body)
         return LR
 
     def stage3(self, res, LRin, UDin, slr, sud, kind):
@@ -298,14 +355,15 @@ class Preconditioner:
         from_left = kind in ("BottomLeft", "TopLeft")
         from_bottom = kind in ("BottomLeft", "BottomRight")
         # diagonal index mirrors (BL2TR: none; TR2BL: both; BR2TL: J; TL2BR: I)
-        order = _order(cb, not from_bottom, not from_left)
-        for idx in order:
+        batches = _diagonals(cb, not from_bottom, not from_left)
+
+        def body(idx):
             c = cb.cells[idx]
             I, J = c.I, c.J
             okI = I > 0 if from_bottom else I < cb.nR - 1
             okJ = J > 0 if from_left else J < cb.nC - 1
             if not (okI and okJ):
-                continue
+                return
             lr, ud = c.lr[slr], c.ud[sud]
             if from_left:
                 li = self._idx(I, J - 1)
@@ -333,7 +391,7 @@ class Preconditioner:
             rhs = np.zeros(c.size, dtype=np.complex128)
             rhs[s0] = e0
             rhs[s1] = e1
-            s = self._solve(c, rhs)
+            s = yield rhs
             if kind != "BottomLeft":
                 res[idx] += s
             # overwrite the incoming outer traces into the solution
@@ -360,6 +418,8 @@ class Preconditioner:
             else:
                 UD[0][idx] += s[ud[1]]
                 UD[1][idx] += s[ud[2]]
+
+        self._run(batches, body)
 
     # -- M^{-1} ----------------------------------------------------------------
     def apply(self, x):

@@ -54,15 +54,15 @@ def test_lu_multi_rhs_and_fp32():
     # depend on the block width through the LDS budget and prefetch depth
     for k in range(B.shape[1]):
         assert relerr(s.solve(B[:, k]), X[:, k]) < 1e-12
-    # FP32 factors on this grid are conditioning-limited (about 7e-2 forward
-    # error measured); the FP32 accuracy check is on an interior cell below
+    # FP32 factors (FP64 factorisation, FP32 storage and solves): measured
+    # forward error 5.0e-5 and normwise backward error 1.9e-7 on this grid
     s32 = L.LocalLinearSolverHIP(L.LSW_PREC_FP32)
     s32.set_matrix(A)
     s32.factorize()
     X32 = s32.solve(B)
     bwd = np.linalg.norm(A @ X32 - B) / (spla.norm(A) * np.linalg.norm(X32) + np.linalg.norm(B))
     print(f"FP32 all-PML grid: forward {relerr(X32, ref):.3e}, normwise backward {bwd:.3e}", flush=True)
-    assert np.all(np.isfinite(X32)) and relerr(X32, ref) < 0.5
+    assert relerr(X32, ref) < 5e-4 and bwd < 2e-6
 
 
 def test_lu_fp32_interior_cell():

@@ -148,3 +148,20 @@ def test_block_rhs_superposition():
         x, _, _ = fgmres(lambda v: A @ v, P.apply_global, cb.global_rhs(srcs), tol=1e-12, restart=20)
         u.append(x)
     assert np.linalg.norm(u[0] + u[1] - u[2]) / np.linalg.norm(u[2]) < 1e-9
+
+
+def test_solves_on_worker_processes_give_the_same_bits():
+    """The CPU baseline runs the oracle's local solves on worker processes,
+    an anti-diagonal at a time (cells of one diagonal read only traces of
+    earlier diagonals); the result is the sequential oracle's, bit for bit,
+    for both cell-to-process mappings."""
+    from oracle.parallel import CellPool
+    cb = _cb((3, 3), (12, 12))
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(cb.N) + 1j * rng.standard_normal(cb.N)
+    ref = Preconditioner(cb).apply_global(x)
+    for assign in ("cells", "rows"):
+        with CellPool(cb, 3, assign) as pool:
+            P = Preconditioner(cb, solver=pool)
+            assert np.array_equal(P.apply_global(x), ref)
+            assert P.nsolves == 4 * (3 * 3 - 2) ** 2
