@@ -221,8 +221,11 @@ def serve_bench(args):
     """Serving workload: a Poisson stream over layered models (seeds 0..M-1)
     x 4 frequencies (6, 8, 10, 12 points per wavelength), 1-64 sources per
     request near the surface.  Measures the plain FP64 path one request at a
-    time on a sample of the stream, then the server (FP32 factors, FP64
-    GMRES, continuous batching) at 0.8x the plain capacity and at saturation."""
+    time on a sample of the stream (its capacity in requests/s), then one
+    server (FP32 factors, FP64 GMRES, continuous batching, 288 GB admission
+    budget) taking the first --serve-load-requests requests as a Poisson
+    stream at 0.8x that capacity (value, p50/p99), and the rest as one burst
+    (saturation, reported separately)."""
     import numpy as np
     import torch
     import lsweeps_amd as L
@@ -297,7 +300,10 @@ def serve_bench(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "c64 factors/sweeps + c128 Krylov", "data": "synthetic",
         "config": {"workload": f"serve_{M}models_x{F}freq_{len(stream)}req", "budget_gb": args.serve_budget / 1e9,
                    "slots": args.serve_slots, "restart": args.serve_restart, "workers": args.serve_workers},
-        "p50_ms": round(1e3 * out["saturated"]["p50_s"], 1), "p99_ms": round(1e3 * out["saturated"]["p99_s"], 1),
+        # latency at the offered load of 0.8x the plain capacity (Poisson
+        # arrivals); throughput (value) of the saturated burst
+        "p50_ms": round(1e3 * out["load0.8"]["p50_s"], 1), "p99_ms": round(1e3 * out["load0.8"]["p99_s"], 1),
+        "latency_phase": f"Poisson stream of {out['load0.8']['requests']} requests at 0.8x the plain capacity",
         "plain": plain, "served": out,
         "speedup_served_vs_plain": round(out["saturated"]["solves_per_s"] / plain["solves_per_s"], 2),
     }
@@ -682,9 +688,11 @@ def main():
     ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
     ap.add_argument("--pack-workers", type=int, default=8)
     ap.add_argument("--serve-models", type=int, default=8)
-    ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-load-requests", type=int, default=6)
-    ap.add_argument("--serve-budget", type=float, default=240e9)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-load-requests", type=int, default=32)
+    ap.add_argument("--serve-budget", type=float, default=None,
+                    help="device bytes the server may reserve (default: 288e9 for the serving stream, the HBM "
+                         "the serving workload states; 240e9 for the pack)")
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-restart", type=int, default=10)
     ap.add_argument("--serve-workers", type=int, default=6)
@@ -694,6 +702,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.serve_budget is None:
+        args.serve_budget = 288e9 if args.workload == "serve" else 240e9
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(sharding across GPUs is design-only in this build)", file=sys.stderr)

@@ -99,6 +99,12 @@ typedef struct lsw_dd_params {
 } lsw_dd_params;
 
 int lsw_dd_create(const lsw_dd_params* p, lsw_dd** out);
+/* The DD of problem `id` of a reference-format input file, built from what
+ * Input::readCheckerboardDD reads (src/Input.cpp:470-623, through
+ * Input::readProblemDD and Problem::read, src/Problem.cpp:70-78): domain,
+ * board, Omega, PML, the "Constant" or "CSV" squared slowness (the CSV file
+ * is read) and CheckerboardDD.Precision.  Not yet factorised. */
+int lsw_dd_create_config(const char* path, int id, lsw_dd** out);
 /* Geometry only (no device memory): sizes, coordinates and cell_info work;
  * every compute entry point returns LSW_ERR_STATE. */
 int lsw_dd_create_host(const lsw_dd_params* p, lsw_dd** out);

@@ -141,6 +141,21 @@ int lsw_lu_destroy(lsw_lu* lu) {
 }
 
 // ------------------------------------------------------------------ DD
+int lsw_dd_create_config(const char* path, int id, lsw_dd** out) {
+  return guarded([&] {
+    LSW_REQUIRE(path && out, LSW_ERR_ARG, "null pointer");
+    Input in;
+    in.openFile(path);
+    LSW_REQUIRE(id >= 0 && (UInt)id < in.getNumProblems(), LSW_ERR_ARG, "problem id out of range");
+    Problem pr;
+    pr.read(in, (UInt)id);
+    auto h = std::make_unique<lsw_dd>();
+    h->dd = pr.takeDD();
+    h->gmres.setParent(h->dd.get());
+    *out = h.release();
+  });
+}
+
 static int ddCreate(const lsw_dd_params* p, lsw_dd** out, bool device) {
   return guarded([&] {
     LSW_REQUIRE(p && out, LSW_ERR_ARG, "null pointer");

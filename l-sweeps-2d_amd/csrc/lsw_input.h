@@ -103,6 +103,8 @@ class Problem {
   ProblemResult solve(bool verbose = true);
   const std::vector<Complex>& solution() const { return mSolution; }
   const DDCheckerboard& dd() const { return *mDD; }
+  // hands the DD over to the caller (the Problem is then empty)
+  std::unique_ptr<DDCheckerboard> takeDD() { return std::move(mDD); }
   const ProblemSpec& spec() const { return mSpec; }
 
  private:

@@ -24,7 +24,7 @@ EXPORTED = [
     "lsw_lu_create", "lsw_lu_set_csc_z", "lsw_lu_factorize", "lsw_lu_solve_z", "lsw_lu_solve_dev", "lsw_lu_solve_dev_noalloc",
     "lsw_lu_info",
     "lsw_lu_destroy",
-    "lsw_dd_create", "lsw_dd_create_host", "lsw_dd_destroy", "lsw_dd_factorize", "lsw_dd_sizes",
+    "lsw_dd_create", "lsw_dd_create_config", "lsw_dd_create_host", "lsw_dd_destroy", "lsw_dd_factorize", "lsw_dd_sizes",
     "lsw_dd_global_coords", "lsw_dd_cell_info", "lsw_dd_assemble_rhs", "lsw_dd_assemble_rhs_dev",
     "lsw_dd_apply_precond", "lsw_dd_apply_precond_dev", "lsw_dd_apply_a", "lsw_dd_apply_a_dev",
     "lsw_dd_stats", "lsw_dd_sync", "lsw_dd_profile_precond",
@@ -125,6 +125,7 @@ _sig("lsw_lu_info", _P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_lo
 _sig("lsw_lu_destroy", _P)
 _sig("lsw_dd_create", C.POINTER(DDParams), C.POINTER(_P))
 _sig("lsw_dd_create_host", C.POINTER(DDParams), C.POINTER(_P))
+_sig("lsw_dd_create_config", C.c_char_p, C.c_int, C.POINTER(_P))
 _sig("lsw_dd_destroy", _P)
 _sig("lsw_dd_factorize", _P)
 _sig("lsw_dd_sizes", _P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _D)
@@ -310,11 +311,16 @@ def dd_params(domain=((0.0, 1.0), (0.0, 1.0)), ncells=(4, 4), nels=(9, 9), omega
 class DDCheckerboard:
     """A checkerboard DD with every cell resident on the GPU."""
 
-    def __init__(self, params, device=True):
+    def __init__(self, params, device=True, config=None):
+        """params: DDParams; or config = (input file path, problem id) for the
+        DD a reference-format input file describes (params is then None)."""
         self.params = params
         self._h = _P()
-        fn = _lib.lsw_dd_create if device else _lib.lsw_dd_create_host
-        _check(fn(C.byref(params), C.byref(self._h)))
+        if config is not None:
+            _check(_lib.lsw_dd_create_config(str(config[0]).encode(), int(config[1]), C.byref(self._h)))
+        else:
+            fn = _lib.lsw_dd_create if device else _lib.lsw_dd_create_host
+            _check(fn(C.byref(params), C.byref(self._h)))
         gx, gy, nc = C.c_int(), C.c_int(), C.c_int()
         ms = C.c_double()
         _check(_lib.lsw_dd_sizes(self._h, C.byref(gx), C.byref(gy), C.byref(nc), C.byref(ms)))

@@ -91,3 +91,25 @@ GlobalSolvers = {{ gmres = {{ Type = "GMRES"; Tol = 1.0e-8; MaxIter = 30; Verbos
     assert abs(r["iterations"] - (len(log) - 1)) <= 1
     assert relerr(r["x"], xo) < 1e-6
     assert np.linalg.norm(A @ r["x"] - b) / np.linalg.norm(b) < 1e-6
+
+
+def test_dd_from_config_file_equals_dd_from_parameters():
+    """lsw_dd_create_config builds the DD the input file describes: the same
+    operator and preconditioner as the DD built from the same parameters, and
+    the reference GMRES on it gives Problem's solution."""
+    path = os.path.join(GOLD, "config_const_3x3.cfg")
+    spec = L.config_problem(path, 0)
+    a = L.DDCheckerboard(None, config=(path, 0))
+    b = L.DDCheckerboard(L.dd_params(domain=spec["domain"], ncells=spec["ncells"], nels=spec["nels"],
+                                     omega=spec["omega"], min_wl_in_pml=spec["min_wl_in_pml"]))
+    assert (a.Gx, a.Gy, a.ncells) == (b.Gx, b.Gy, b.ncells)
+    a.factorize()
+    b.factorize()
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(a.N) + 1j * rng.standard_normal(a.N)
+    assert np.array_equal(a.apply_a(v), b.apply_a(v))
+    assert np.array_equal(a.apply_precond(v), b.apply_precond(v))
+    rhs = a.assemble_rhs(spec["sources"])
+    x, info = a.gmres(rhs, mode=0, tol=spec["tol"], max_iter=spec["max_iter"])
+    r = L.problem_run(path, 0)
+    assert relerr(x, r["x"]) < 1e-13 and info["iterations"] == r["iterations"]
