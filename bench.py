@@ -576,13 +576,16 @@ def pack_pass(L, probs, workers, restart, budget, slots=16):
     """One pass over the uneven pack: a new server (empty factor cache), every
     problem registered and its request submitted at once, every result
     waited for.  Returns (wall seconds, per-request results, server stats)."""
+    from concurrent.futures import ThreadPoolExecutor
     t0 = time.perf_counter()
     srv = L.Server(budget_bytes=budget, slots=slots, restart=restart, tol=1e-6, max_iter=200,
                    precision=L.LSW_PREC_FP32, workers=workers)
     try:
-        # each problem's request goes in as soon as the problem is registered
-        ids = [srv.submit(srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)), p["sources"], want_solution=False)
-               for p in probs]
+        # registration (host geometry and the device-bytes estimate) on a few
+        # threads (the library call releases the GIL), then every request
+        with ThreadPoolExecutor(8) as ex:
+            pids = list(ex.map(lambda p: srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)), probs))
+        ids = [srv.submit(pids[k], p["sources"], want_solution=False) for k, p in enumerate(probs)]
         res = [srv.wait(i) for i in ids]
         wall = time.perf_counter() - t0
         st = srv.stats()

@@ -87,7 +87,8 @@ struct SolvePlan {
     int mode = 0;       // 0 packed: P = 64/RP entries side by side per wavefront,
                         // `wpb` independent wavefronts per workgroup, `rounds`
                         // entries per lane group; 1 wide: `waves` wavefronts
-                        // split one entry's columns
+                        // split one entry's columns; 2 packed and 3 wide on the
+                        // matrix cores (FP32 tasks of 8 right-hand sides)
     int rp = 64;        // lanes per entry (rows per pass) in packed mode
     int rounds = 1;
     int wpb = 1;

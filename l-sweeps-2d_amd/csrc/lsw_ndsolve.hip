@@ -669,6 +669,124 @@ __global__ __launch_bounds__(256) void k_nd_mfma(const SolveTaskDev* __restrict_
   else
     bwdStore<float2, NR>(it, ok, dof, res, Xo, xs);
 }
+
+// Few large entries of FP32 tasks of 8 right-hand sides on the matrix cores:
+// one workgroup per entry and W wavefronts splitting its columns in
+// interleaved k-steps of 4 (wavefront w takes k-steps w, w + W, w + 2W, ...).
+// The right-hand side is gathered once for all of them (as k_nd_wide); each
+// wavefront forms its part of the 64 x 16 product in four 16 x 16 tiles with
+// k_nd_mfma's operand layout; after a barrier the staging holds the W
+// partial products, which wavefront 0 sums in wavefront order (a fixed order
+// of additions) before the store.  LDS: max(NR x cols staging, W x 64 x
+// kMfmaRow floats).
+template <bool FWD, int W>
+__global__ __launch_bounds__(64 * W) void k_nd_mfma_wide(const SolveTaskDev* __restrict__ tasks,
+                                                         const SolveNode* __restrict__ nodes,
+                                                         const SolveEntry* __restrict__ ent, int nent, int cols,
+                                                         const int* __restrict__ poolE, const int* __restrict__ poolB,
+                                                         const int* __restrict__ poolMap,
+                                                         const float2* __restrict__ factor, const float2* X,
+                                                         float2* Xo, float2* __restrict__ Rb, float2* __restrict__ G,
+                                                         long long xs, long long gs) {
+  constexpr int NR = 8;
+  extern __shared__ char smem_raw[];
+  float2* v = reinterpret_cast<float2*>(smem_raw);
+  const int pad = cols - 1;
+  if (threadIdx.x < NR) v[pad * NR + threadIdx.x] = czero<float2>();  // ordered by the barrier after the gather
+  const long long idx = xcdRemap(blockIdx.x, gridDim.x);
+  if (idx >= nent) return;  // the whole workgroup
+  const Item it = loadItem(ent, tasks, nodes, idx);
+  const int e = it.nd.e, F = e + it.nd.b;
+  const int k0 = FWD ? 0 : ((it.flags & kEntVisited) ? 0 : e);
+  const int rows = it.rows;
+  // (a forward root entry of rows = 0 only keeps r_E: no product)
+  const int K = rows > 0 ? (FWD ? e : F) : k0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const long long ld = FWD ? it.nd.b : e;
+  // an entry without columns reads the factor base (valid)
+  const float2* M0 = factor + (K > k0 ? it.tk.facBase + (FWD ? it.nd.fwdOff : it.nd.bwdOff) + it.r0 : 0);
+  const float2* tM[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int row = 16 * t + li;
+    tM[t] = M0 + (row < rows ? row : 0);
+  }
+  const bool storer = w == 0 && lane < rows;
+  float2 add[NR];
+  int dof = 0;
+  if constexpr (FWD)
+    fwdRowAdd<float2, NR>(it, storer, lane, poolMap, G, gs, add);
+  else
+    dof = poolE[it.nd.E + (storer ? it.r0 + lane : 0)];
+  if constexpr (FWD)
+    fwdGather<float2, NR>(it, true, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
+  else
+    bwdGather<float2, NR>(it, true, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
+  __syncthreads();
+  lswF4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = lswF4{0.f, 0.f, 0.f, 0.f};
+  const bool lo = li < 8;
+  const int nsteps = (K - k0 + 3) / 4;
+  const int nw = nsteps > w ? (nsteps - w + W - 1) / W : 0;  // this wavefront's k-steps (uniform)
+  constexpr int KS = 4;
+  float2 ca[KS][4], na[KS][4];
+  auto colOf = [&](int s) { return k0 + 4 * (s * W + w) + lk; };
+  auto loadA = [&](int s0, float2 (&dst)[KS][4]) {
+#pragma unroll
+    for (int st = 0; st < KS; ++st) {
+      const int col = colOf(s0 + st);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) dst[st][t] = tM[t][(long long)(col < K ? col : 0) * ld];
+    }
+  };
+  loadA(0, ca);
+  for (int s0 = 0; s0 < nw; s0 += KS) {
+    const bool more = s0 + KS < nw;
+    if (more) loadA(s0 + KS, na);
+#pragma unroll
+    for (int st = 0; st < KS; ++st) {
+      const int col = colOf(s0 + st);
+      const float2 x = v[(col < K ? col : pad) * NR + (li & 7)];
+      const float b1 = lo ? x.x : x.y;
+      const float b2 = lo ? -x.y : x.x;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[st][t].x, b1, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[st][t].y, b2, acc[t], 0, 0, 0);
+      }
+    }
+    if (!more) break;
+#pragma unroll
+    for (int st = 0; st < KS; ++st)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ca[st][t] = na[st][t];
+  }
+  __syncthreads();  // every wavefront is done with the staging
+  float* red = reinterpret_cast<float*>(smem_raw);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) red[(w * 64 + 16 * t + 4 * lk + rr) * kMfmaRow + li] = acc[t][rr];
+  __syncthreads();
+  if (w != 0) return;
+  float2 res[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    float re = red[lane * kMfmaRow + q], im = red[lane * kMfmaRow + 8 + q];
+    for (int ww = 1; ww < W; ++ww) {
+      re += red[(ww * 64 + lane) * kMfmaRow + q];
+      im += red[(ww * 64 + lane) * kMfmaRow + 8 + q];
+    }
+    res[q] = make_float2(re, im);
+    if constexpr (FWD) res[q] = cadd(res[q], add[q]);
+  }
+  if constexpr (FWD)
+    fwdStore<float2, NR>(it, storer, lane, res, G, gs);
+  else
+    bwdStore<float2, NR>(it, storer, dof, res, Xo, xs);
+}
 }  // namespace
 
 // ------------------------------------------------------------------ plans
@@ -852,12 +970,18 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     // FP32 tasks of 8 right-hand sides: packed entries on the matrix cores
     const int rpm = std::max(16, rp);
     auto ldsMfma = [&](int wpb) { return wpb * ((64.0 / rpm) * S * nrk * stage + 8.0 * kMfmaOut); };
+    auto ldsMfmaWide = [&](int w) { return std::max(S * nrk * stage, 4.0 * w * 64 * kMfmaRow); };
     if (W == 0 && S == 8.0 && nrk == kMaxRhs && shp.mfma && ldsMfma(1) <= kMaxLds) {
       L.mode = 2;
       L.rp = rpm;
       int wpb = 4;
       while (wpb > 1 && ldsMfma(wpb) > kMaxLds) wpb /= 2;
       L.wpb = wpb;
+    } else if (W > 0 && S == 8.0 && nrk == kMaxRhs && shp.mfma && ldsMfmaWide(W) <= kMaxLds) {
+      // few large entries of FP32 tasks of 8 right-hand sides: their columns
+      // split over W wavefronts on the matrix cores
+      L.mode = 3;
+      L.waves = W;
     } else if (W > 0 && ldsWide(W) <= kMaxLds) {
       L.mode = 1;
       L.waves = W;
@@ -1013,6 +1137,35 @@ void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, co
                          ceilLog2(L.rp), L.cols, a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
     } else {
       LSW_REQUIRE(false, LSW_ERR_STATE, "matrix-core solve launch for a non-FP32 or narrow task plan");
+    }
+    return;
+  }
+  if (L.mode == 3) {
+    if constexpr (std::is_same<T2, float2>::value && NR == kMaxRhs) {
+      const size_t shm = std::max((size_t)NR * L.cols * sizeof(T2), (size_t)L.waves * 64 * kMfmaRow * sizeof(float));
+      const dim3 grid((unsigned)L.n);
+      switch (L.waves) {
+        case 1:
+          hipLaunchKernelGGL((k_nd_mfma_wide<FWD, 1>), grid, dim3(64), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                             a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_nd_mfma_wide<FWD, 2>), grid, dim3(128), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                             a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+          break;
+        case 4:
+          hipLaunchKernelGGL((k_nd_mfma_wide<FWD, 4>), grid, dim3(256), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                             a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+          break;
+        case 8:
+          hipLaunchKernelGGL((k_nd_mfma_wide<FWD, 8>), grid, dim3(512), shm, s, a.tasks, a.nodes, ent, L.n, L.cols,
+                             a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+          break;
+        default:
+          LSW_REQUIRE(false, LSW_ERR_STATE, "matrix-core wide solve launch of an unsupported wavefront count");
+      }
+    } else {
+      LSW_REQUIRE(false, LSW_ERR_STATE, "matrix-core wide solve launch for a non-FP32 or narrow task plan");
     }
     return;
   }
