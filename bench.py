@@ -112,11 +112,11 @@ def time_batched(dd, torch, cfg, tol, steps, warmup):
                 true_residual=max(i["true_residual"] for i in infos))
 
 
-def profile(dd, torch):
-    b = torch.randn(dd.N, dtype=torch.complex128, device="cuda:0")
+def profile(dd, torch, nrhs=1):
+    b = torch.randn(nrhs * dd.N, dtype=torch.complex128, device="cuda:0")
     y = torch.zeros_like(b)
     torch.cuda.synchronize()
-    return dd.profile_precond_dev(b.data_ptr(), y.data_ptr())
+    return dd.profile_precond_dev(b.data_ptr(), y.data_ptr(), nrhs=nrhs)
 
 
 def roofline(prof):
@@ -480,6 +480,8 @@ def lens_section(args, L, torch, cfg, paths):
             out["batched"] = time_batched(dd, torch, cfg, 1e-6, max(1, args.steps // 2), min(args.warmup, 1))
             log("  batched", out["batched"])
         prof["fp32"] = profile(dd, torch)
+        if "batched" in paths:
+            prof["fp32_x16"] = profile(dd, torch, BATCH)
         del dd
         torch.cuda.empty_cache()
     return out, xs, prof, tune
@@ -499,6 +501,10 @@ def lens_fields(cfg, out, xs, prof, tune):
             f["roofline"]["traffic_source"] = traffic["source"]
     if "fp64" in prof:
         f["roofline_fp64"] = roofline(prof["fp64"])
+    if "fp32_x16" in prof:
+        f["roofline_block16"] = roofline(prof["fp32_x16"])
+        f["roofline_block16"]["kernel"] = ("k_nd_mfma + k_nd_mfma_wide + k_nd_packed + k_nd_wide (batched ND local "
+                                           "solves of one M^-1 application to a block of 16 sources, FP32)")
     if "plain" in out:
         base = out["plain"]["solves_per_s"]
         for k in ("mixed", "batched"):

@@ -155,6 +155,9 @@ int lsw_dd_sync(lsw_dd* dd);
  * backward: summed ms, launches, algorithmic bytes.  No counterpart in the
  * reference: it serves the roofline measurement. */
 int lsw_dd_profile_precond(lsw_dd* dd, const void* x, void* y, double* out);
+/* The same for a block of nrhs right-hand sides (x, y: nrhs consecutive
+ * global vectors on the device).  No counterpart in the reference. */
+int lsw_dd_profile_precond_block(lsw_dd* dd, int nrhs, const void* x, void* y, double* out);
 
 /* ---- GlobalIterativeSolver (include/GlobalIterativeSolver.h:36-48) -------
  * mode 0: the reference GMRES (src/GlobalIterativeSolverGMRES.cpp:65-348),

@@ -350,6 +350,21 @@ int lsw_dd_profile_precond(lsw_dd* dd, const void* x, void* y, double* out) {
   });
 }
 
+int lsw_dd_profile_precond_block(lsw_dd* dd, int nrhs, const void* x, void* y, double* out) {
+  return guarded([&] {
+    LSW_REQUIRE(dd && x && y && out, LSW_ERR_ARG, "null pointer");
+    SolveProfile prof;
+    LSW_REQUIRE(nrhs >= 1, LSW_ERR_ARG, "nrhs must be positive");
+    dd->dd->applyPrecondProfiled((const double2*)x, (double2*)y, prof, nrhs);
+    out[0] = prof.fwdMs;
+    out[1] = prof.fwdLaunches;
+    out[2] = prof.fwdBytes;
+    out[3] = prof.bwdMs;
+    out[4] = prof.bwdLaunches;
+    out[5] = prof.bwdBytes;
+  });
+}
+
 int lsw_dd_set_graphs(lsw_dd* dd, int on) {
   return guarded([&] {
     LSW_REQUIRE(dd, LSW_ERR_ARG, "null pointer");

@@ -27,7 +27,7 @@ EXPORTED = [
     "lsw_dd_create", "lsw_dd_create_config", "lsw_dd_create_host", "lsw_dd_destroy", "lsw_dd_factorize", "lsw_dd_sizes",
     "lsw_dd_global_coords", "lsw_dd_cell_info", "lsw_dd_assemble_rhs", "lsw_dd_assemble_rhs_dev",
     "lsw_dd_apply_precond", "lsw_dd_apply_precond_dev", "lsw_dd_apply_a", "lsw_dd_apply_a_dev",
-    "lsw_dd_stats", "lsw_dd_sync", "lsw_dd_profile_precond",
+    "lsw_dd_stats", "lsw_dd_sync", "lsw_dd_profile_precond", "lsw_dd_profile_precond_block",
     "lsw_dd_apply_precond_block", "lsw_dd_apply_precond_block_dev",
     "lsw_gmres_solve", "lsw_gmres_solve_dev",
     "lsw_config_num_problems", "lsw_config_problem", "lsw_problem_run",
@@ -142,6 +142,7 @@ _sig("lsw_dd_sync", _P)
 _sig("lsw_dd_set_graphs", _P, C.c_int)
 _sig("lsw_dd_autotune", _P, C.c_int, _D)
 _sig("lsw_dd_profile_precond", _P, _P, _P, _D)
+_sig("lsw_dd_profile_precond_block", _P, C.c_int, _P, _P, _D)
 _sig("lsw_dd_apply_precond_block", _P, C.c_int, _D, _D)
 _sig("lsw_dd_apply_precond_block_dev", _P, C.c_int, _P, _P, _P)
 _sig("lsw_gmres_solve", _P, C.POINTER(GmresParams), _D, _D, C.POINTER(GmresResult), _D, C.c_int)
@@ -455,10 +456,14 @@ class DDCheckerboard:
     def sync(self):
         _check(_lib.lsw_dd_sync(self._h))
 
-    def profile_precond_dev(self, x_ptr, y_ptr):
-        """Per-launch HIP-event timing of the batched local solves of one M^{-1}."""
+    def profile_precond_dev(self, x_ptr, y_ptr, nrhs=1):
+        """Per-launch HIP-event timing of the batched local solves of one M^{-1}
+        (x_ptr, y_ptr: nrhs consecutive global vectors on the device)."""
         out = np.zeros(6)
-        _check(_lib.lsw_dd_profile_precond(self._h, x_ptr, y_ptr, _dptr(out)))
+        if nrhs == 1:
+            _check(_lib.lsw_dd_profile_precond(self._h, x_ptr, y_ptr, _dptr(out)))
+        else:
+            _check(_lib.lsw_dd_profile_precond_block(self._h, int(nrhs), x_ptr, y_ptr, _dptr(out)))
         return dict(fwd_ms=out[0], fwd_launches=int(out[1]), fwd_bytes=out[2],
                     bwd_ms=out[3], bwd_launches=int(out[4]), bwd_bytes=out[5])
 
