@@ -1488,6 +1488,9 @@ DDCheckerboard::TuneResult DDCheckerboard::autotune(int nrhs) const {
     PlanShape c = best;
     c.mfma = !c.mfma;
     consider(c);
+    c = best;
+    c.mfmaWideCols = c.mfmaWideCols >= (1 << 30) ? PlanShape().mfmaWideCols : (1 << 30);
+    consider(c);
   }
   mShapes[nrhs] = best;
   dropPlans(nrhs);

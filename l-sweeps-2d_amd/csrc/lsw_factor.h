@@ -76,6 +76,8 @@ struct PlanShape {
   int wideCap = 8;                // most wavefronts one entry's columns are split over (2..16; 16 only
                                   // for FP32 tasks of up to 4 right-hand sides)
   bool mfma = true;               // FP32 8-right-hand-side packed launches on the matrix cores
+  int mfmaWideCols = 192;         // ... and of more than 32 rows and at least this many columns split
+                                  // them over 2 wavefronts on the matrix cores even when there are many
   static PlanShape defaults();
 };
 

@@ -819,6 +819,7 @@ PlanShape PlanShape::defaults() {
     PlanShape p;
     if (const char* e = std::getenv("LSW_LATENCY_WAVES")) p.latencyWaves = std::max(0LL, std::atoll(e));
     if (const char* e = std::getenv("LSW_SOLVE_MFMA")) p.mfma = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LSW_MFMA_WIDE_COLS")) p.mfmaWideCols = std::max(0, std::atoi(e));
     return p;
   }();
   return d;
@@ -964,6 +965,13 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       // few large entries: split their columns over W wavefronts
       int w = 1;
       while (w < shp.wideCap && (double)v.size() * w * 2 <= (double)shp.wideBudget && per >= 1024.0 * w * 2) w *= 2;
+      // Long rows of FP32 8-right-hand-side tasks: a packed matrix-core
+      // wavefront walks a 64-row entry's 200-400 columns alone, while two
+      // wavefronts on halves of them keep more loads in flight (measured on
+      // the 16-source lens block: 2.9-3.6 TB/s packed, 5.3-6.4 TB/s split;
+      // at 127 columns packed was faster, 5.0 against 4.2 TB/s).
+      if (w == 1 && S == 8.0 && nrk == kMaxRhs && shp.mfma && maxRows > 32 && cols >= shp.mfmaWideCols)
+        w = std::min(2, shp.wideCap);
       if (w > 1) W = w;
     }
     while (W > 1 && ldsWide(W) > kMaxLds) W /= 2;
