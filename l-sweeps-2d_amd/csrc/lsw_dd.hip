@@ -1349,8 +1349,12 @@ void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady
   const double s = entries * es + fixed + r * perRhs + plans;
   double ws = 0;
   for (double w : wsPerHeight) ws = std::max(ws, w);
-  ws = std::min(ws, (double)(512LL << 20)) * 16.0;
-  const double f = fixed + perRhs + entries * 16.0 + schur * 16.0 + ws + (precision == LSW_FP32 ? entries * 8.0 : 0.0);
+  // fronts, Schur complements and factors while factorising: FP32 work
+  // stores its factors directly; FP64 work for FP32 solves adds the copy
+  const double wes = factorWorkFp32(precision) ? 8.0 : 16.0;
+  ws = std::min(ws, (double)kFactorWsEntries) * wes;
+  const double f = fixed + perRhs + (entries + schur) * wes + ws +
+                   (precision == LSW_FP32 && wes == 16.0 ? entries * 8.0 : 0.0);
   if (steady) *steady = s;
   if (peak) *peak = std::max(s, f);
 }

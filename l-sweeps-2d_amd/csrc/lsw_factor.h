@@ -24,6 +24,13 @@
 
 namespace lsw {
 
+// Fronts of a factorisation in precision `precision` are reduced in FP32
+// (LSW_FP32 with LSW_FACTOR_FP32=1) or in FP64 (otherwise).
+bool factorWorkFp32(int precision);
+// Workspace budget of one factorisation: complex entries of the fronts of a
+// height (and their panel buffers) held at once.
+constexpr long long kFactorWsEntries = 512LL << 20;
+
 // Per-launch timing of the batched solve kernels (HIP events on the launch
 // stream) and their algorithmic bytes.
 struct SolveProfile {
@@ -143,8 +150,9 @@ class FactorSet {
   void setSolvePrecision(int precision) { mPrecision = precision; }
   // Build ND trees, upload descriptors, allocate, assemble stencils.
   void setup(hipStream_t s);
-  // Multifrontal factorisation (FP64 on the device); FP32 copies of the
-  // factors are made when precision == LSW_FP32.
+  // Multifrontal factorisation on the device in FP64 fronts; for LSW_FP32
+  // the factors are then rounded to FP32 (LSW_FACTOR_FP32=1: FP32 fronts
+  // with FP32-accumulated matrix-core updates storing FP32 factors).
   void factorize(int precision, hipStream_t s);
   bool factorized() const { return mFactorized; }
   int precision() const { return mPrecision; }
@@ -226,6 +234,8 @@ class FactorSet {
   int mPrecision = LSW_FP64;
   bool mFactorized = false;
   int mInfo = 0;
+  template <typename W2>
+  void factorizeAs(hipStream_t s, W2*& factor);
 };
 
 }  // namespace lsw

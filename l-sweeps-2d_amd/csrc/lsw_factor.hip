@@ -50,15 +50,18 @@ __global__ void k_assemble(const CellDev* __restrict__ cells, const double* __re
 // (one workgroup per front) each followed by a batched rank-kPanel update of
 // the rest of every front of the height, tiled over many workgroups (the
 // bulk of the arithmetic, and no longer one CU per front).
+template <typename W2>
 struct FrontView {
   NdNodeDev nd;
   int e, b, R, Cc, N;
-  double2* M;
+  W2* M;
 };
 
-__device__ __forceinline__ FrontView frontView(const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes,
-                                               const FrontTask& ft, double2* __restrict__ ws) {
-  FrontView f;
+template <typename W2>
+__device__ __forceinline__ FrontView<W2> frontView(const CellDev* __restrict__ cells,
+                                                   const ShapeDev* __restrict__ shapes, const FrontTask& ft,
+                                                   W2* __restrict__ ws) {
+  FrontView<W2> f;
   const CellDev& cd = cells[ft.cell];
   const ShapeDev& sh = shapes[cd.shape];
   f.nd = sh.nodes[ft.node];
@@ -71,22 +74,32 @@ __device__ __forceinline__ FrontView frontView(const CellDev* __restrict__ cells
   return f;
 }
 
-// zero the front, put the identity block, add the original entries and the
-// children's Schur complements (one workgroup)
-__device__ void frontAssemble(const CellDev& cd, const ShapeDev& sh, const FrontView& f,
-                              const double2* __restrict__ coef, const double2* __restrict__ schur) {
+template <typename W2>
+__device__ __forceinline__ W2 cinvT(W2 a) {
+  using R = typename CTraits<W2>::R;
+  const R d = a.x * a.x + a.y * a.y;
+  return CTraits<W2>::make(a.x / d, -a.y / d);
+}
+
+// zero the front, put the identity block, add the original entries (the
+// FP64 stencil, rounded to the work type) and the children's Schur
+// complements (one workgroup)
+template <typename W2>
+__device__ void frontAssemble(const CellDev& cd, const ShapeDev& sh, const FrontView<W2>& f,
+                              const double2* __restrict__ coef, const W2* __restrict__ schur) {
+  using R = typename CTraits<W2>::R;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const double2 zero = make_double2(0.0, 0.0);
-  double2* M = f.M;
+  const W2 zero = czero<W2>();
+  W2* M = f.M;
   const int e = f.e, Cc = f.Cc, b = f.b;
   for (long long idx = tid; idx < (long long)f.R * Cc; idx += nt) M[idx] = zero;
   __syncthreads();
-  for (int k = tid; k < e; k += nt) M[(long long)k * Cc + e + b + k] = make_double2(1.0, 0.0);
+  for (int k = tid; k < e; k += nt) M[(long long)k * Cc + e + b + k] = CTraits<W2>::make(1, 0);
   for (int a = tid; a < f.nd.asmCnt; a += nt) {
     const AsmEntry en = sh.asmList[f.nd.asmOff + a];
     const double2 v = coef[cd.coefOff + (long long)en.dir * f.N + en.dof];
-    double2* t = &M[(long long)en.fr * Cc + en.fc];
-    *t = cadd(*t, v);
+    W2* t = &M[(long long)en.fr * Cc + en.fc];
+    *t = cadd(*t, CTraits<W2>::make((R)v.x, (R)v.y));
   }
   __syncthreads();
   const int cs[2] = {f.nd.c1, f.nd.c2};
@@ -95,11 +108,11 @@ __device__ void frontAssemble(const CellDev& cd, const ShapeDev& sh, const Front
     if (c < 0) continue;
     const NdNodeDev cn = sh.nodes[c];
     const int bc = cn.b;
-    const double2* S = schur + cd.schurOff + cn.sOff;
+    const W2* S = schur + cd.schurOff + cn.sOff;
     const int* up = sh.up + cn.upOff;
     for (long long idx = tid; idx < (long long)bc * bc; idx += nt) {
       const int t1 = (int)(idx / bc), t2 = (int)(idx - (long long)t1 * bc);
-      double2* t = &M[(long long)up[t1] * Cc + up[t2]];
+      W2* t = &M[(long long)up[t1] * Cc + up[t2]];
       *t = cadd(*t, S[idx]);
     }
     __syncthreads();
@@ -107,13 +120,14 @@ __device__ void frontAssemble(const CellDev& cd, const ShapeDev& sh, const Front
 }
 
 // write W, K and S from the reduced front (one workgroup)
-__device__ void frontStore(const CellDev& cd, const FrontView& f, double2* __restrict__ factor,
-                           double2* __restrict__ schur) {
+template <typename W2>
+__device__ void frontStore(const CellDev& cd, const FrontView<W2>& f, W2* __restrict__ factor,
+                           W2* __restrict__ schur) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  const double2* M = f.M;
+  const W2* M = f.M;
   const int e = f.e, b = f.b, R = f.R, Cc = f.Cc;
-  double2* W = factor + cd.factorOff + f.nd.fwdOff;
-  double2* K = factor + cd.factorOff + f.nd.bwdOff;
+  W2* W = factor + cd.factorOff + f.nd.fwdOff;
+  W2* K = factor + cd.factorOff + f.nd.bwdOff;
   for (long long idx = tid; idx < (long long)b * e; idx += nt) {
     const int k = (int)(idx / b), t = (int)(idx - (long long)k * b);
     W[idx] = cneg(M[(long long)(e + t) * Cc + e + b + k]);
@@ -123,7 +137,7 @@ __device__ void frontStore(const CellDev& cd, const FrontView& f, double2* __res
     K[idx] = (k < e) ? M[(long long)r * Cc + e + b + k] : cneg(M[(long long)r * Cc + k]);
   }
   if (b > 0) {
-    double2* S = schur + cd.schurOff + f.nd.sOff;
+    W2* S = schur + cd.schurOff + f.nd.sOff;
     for (long long idx = tid; idx < (long long)b * b; idx += nt) {
       const int t1 = (int)(idx / b), t2 = (int)(idx - (long long)t1 * b);
       S[idx] = M[(long long)(e + t1) * Cc + e + t2];
@@ -133,14 +147,15 @@ __device__ void frontStore(const CellDev& cd, const FrontView& f, double2* __res
 
 // Pivot of column k: the largest modulus among rows k..e-1 (lowest index on
 // ties), reduced over the workgroup (256 threads).
-__device__ __forceinline__ int pivotRow(const double2* __restrict__ M, int Cc, int k, int e, double* redv, int* redi,
+template <typename W2>
+__device__ __forceinline__ int pivotRow(const W2* __restrict__ M, int Cc, int k, int e, double* redv, int* redi,
                                         double& pmag) {
   const int tid = threadIdx.x, nt = blockDim.x;
   double best = -1.0;
   int bi = e;
   for (int i = k + tid; i < e; i += nt) {
-    const double2 v = M[(long long)i * Cc + k];
-    const double a = v.x * v.x + v.y * v.y;
+    const W2 v = M[(long long)i * Cc + k];
+    const double a = (double)(v.x * v.x + v.y * v.y);
     if (a > best) {
       best = a;
       bi = i;
@@ -167,56 +182,58 @@ __device__ __forceinline__ int pivotRow(const double2* __restrict__ M, int Cc, i
 }
 
 // One workgroup per front: assemble, full Gauss-Jordan, store.
+template <typename W2>
 __global__ __launch_bounds__(256) void k_front_factor(
     const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes,
     const FrontTask* __restrict__ fts, const double2* __restrict__ coef,
-    double2* __restrict__ factor, double2* __restrict__ schur, double2* __restrict__ ws,
+    W2* __restrict__ factor, W2* __restrict__ schur, W2* __restrict__ ws,
     int* __restrict__ info) {
-  extern __shared__ double2 colk[];
+  extern __shared__ double2 lswDynLds[];
+  W2* colk = reinterpret_cast<W2*>(lswDynLds);
   __shared__ double redv[256];
   __shared__ int redi[256];
   const FrontTask ft = fts[blockIdx.x];
   const CellDev& cd = cells[ft.cell];
   const ShapeDev& sh = shapes[cd.shape];
-  const FrontView f = frontView(cells, shapes, ft, ws);
+  const FrontView<W2> f = frontView(cells, shapes, ft, ws);
   frontAssemble(cd, sh, f, coef, schur);
-  double2* M = f.M;
+  W2* M = f.M;
   const int e = f.e, R = f.R, Cc = f.Cc;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const double2 zero = make_double2(0.0, 0.0);
+  const W2 zero = czero<W2>();
   for (int k = 0; k < e; ++k) {
     double pmag;
     const int p = pivotRow(M, Cc, k, e, redv, redi, pmag);
     if (p != k) {
       for (int j = tid; j < Cc; j += nt) {
-        const double2 t = M[(long long)k * Cc + j];
+        const W2 t = M[(long long)k * Cc + j];
         M[(long long)k * Cc + j] = M[(long long)p * Cc + j];
         M[(long long)p * Cc + j] = t;
       }
     }
     __syncthreads();
-    double2 piv = M[(long long)k * Cc + k];
+    W2 piv = M[(long long)k * Cc + k];
     if (!(pmag > 0.0) || !isfinite(pmag)) {
       if (tid == 0) atomicMax(info, 1);
-      piv = make_double2(1.0, 0.0);
+      piv = CTraits<W2>::make(1, 0);
     }
-    const double2 inv = cinv(piv);
+    const W2 inv = cinvT(piv);
     for (int j = k + 1 + tid; j < Cc; j += nt) {
-      double2* t = &M[(long long)k * Cc + j];
+      W2* t = &M[(long long)k * Cc + j];
       *t = cmul(*t, inv);
     }
     for (int i = tid; i < R; i += nt) colk[i] = (i == k) ? zero : M[(long long)i * Cc + k];
     __syncthreads();
     for (int j = k + 1 + tid; j < Cc; j += nt) {
-      const double2 rk = M[(long long)k * Cc + j];
-      if (rk.x == 0.0 && rk.y == 0.0) continue;
+      const W2 rk = M[(long long)k * Cc + j];
+      if (rk.x == 0 && rk.y == 0) continue;
       for (int i = 0; i < R; ++i) {
         if (i == k) continue;
-        double2* t = &M[(long long)i * Cc + j];
+        W2* t = &M[(long long)i * Cc + j];
         *t = cfms(colk[i], rk, *t);
       }
     }
-    for (int i = tid; i < R; i += nt) M[(long long)i * Cc + k] = (i == k) ? make_double2(1.0, 0.0) : zero;
+    for (int i = tid; i < R; i += nt) M[(long long)i * Cc + k] = (i == k) ? CTraits<W2>::make(1, 0) : zero;
     __syncthreads();
   }
   frontStore(cd, f, factor, schur);
@@ -227,20 +244,22 @@ __global__ __launch_bounds__(256) void k_front_factor(
 // kPanel, row-major) and B (kPanel x Cc, row-major).
 constexpr int kPanel = 16;
 
+template <typename W2>
 __global__ __launch_bounds__(256) void k_front_assemble(const CellDev* __restrict__ cells,
                                                         const ShapeDev* __restrict__ shapes,
                                                         const FrontTask* __restrict__ fts,
                                                         const double2* __restrict__ coef,
-                                                        const double2* __restrict__ schur, double2* __restrict__ ws) {
+                                                        const W2* __restrict__ schur, W2* __restrict__ ws) {
   const FrontTask ft = fts[blockIdx.x];
   const CellDev& cd = cells[ft.cell];
   frontAssemble(cd, shapes[cd.shape], frontView(cells, shapes, ft, ws), coef, schur);
 }
 
+template <typename W2>
 __global__ __launch_bounds__(256) void k_front_store(const CellDev* __restrict__ cells,
                                                      const ShapeDev* __restrict__ shapes,
-                                                     const FrontTask* __restrict__ fts, double2* __restrict__ factor,
-                                                     double2* __restrict__ schur, double2* __restrict__ ws) {
+                                                     const FrontTask* __restrict__ fts, W2* __restrict__ factor,
+                                                     W2* __restrict__ schur, W2* __restrict__ ws) {
   const FrontTask ft = fts[blockIdx.x];
   frontStore(cells[ft.cell], frontView(cells, shapes, ft, ws), factor, schur);
 }
@@ -255,23 +274,25 @@ __global__ __launch_bounds__(256) void k_front_store(const CellDev* __restrict__
 //   new = old + (T - E) B,  B = old pivot rows (columns after the panel),
 // E = 1 at (pivot row c, c): the batched update below.  B is copied here,
 // before that update changes the pivot rows.
+template <typename W2>
 __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__ cells,
                                                      const ShapeDev* __restrict__ shapes,
-                                                     const FrontTask* __restrict__ fts, double2* __restrict__ ws,
+                                                     const FrontTask* __restrict__ fts, W2* __restrict__ ws,
                                                      int k0, int* __restrict__ info) {
-  extern __shared__ double2 colk[];
+  extern __shared__ double2 lswDynLds[];
+  W2* colk = reinterpret_cast<W2*>(lswDynLds);
   __shared__ double redv[256];
   __shared__ int redi[256];
   const FrontTask ft = fts[blockIdx.x];
-  const FrontView f = frontView(cells, shapes, ft, ws);
+  const FrontView<W2> f = frontView(cells, shapes, ft, ws);
   const int e = f.e, R = f.R, Cc = f.Cc;
   const int nbk = min(kPanel, e - k0);
   if (nbk <= 0) return;
-  double2* M = f.M;
-  double2* T = M + (long long)R * Cc;
-  double2* B = T + (long long)R * kPanel;
+  W2* M = f.M;
+  W2* T = M + (long long)R * Cc;
+  W2* B = T + (long long)R * kPanel;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const double2 zero = make_double2(0.0, 0.0);
+  const W2 zero = czero<W2>();
   const int pend = k0 + nbk;  // panel columns [k0, pend)
   for (int idx = tid; idx < R * kPanel; idx += nt) T[idx] = zero;
   __syncthreads();
@@ -281,28 +302,28 @@ __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__
     const int p = pivotRow(M, Cc, k, e, redv, redi, pmag);
     if (p != k) {
       for (int j = k0 + tid; j < Cc; j += nt) {
-        const double2 t = M[(long long)k * Cc + j];
+        const W2 t = M[(long long)k * Cc + j];
         M[(long long)k * Cc + j] = M[(long long)p * Cc + j];
         M[(long long)p * Cc + j] = t;
       }
       for (int c = tid; c < kk; c += nt) {
-        const double2 t = T[k * kPanel + c];
+        const W2 t = T[k * kPanel + c];
         T[k * kPanel + c] = T[p * kPanel + c];
         T[p * kPanel + c] = t;
       }
     }
     __syncthreads();
-    if (tid == 0) T[k * kPanel + kk] = make_double2(1.0, 0.0);
-    double2 piv = M[(long long)k * Cc + k];
+    if (tid == 0) T[k * kPanel + kk] = CTraits<W2>::make(1, 0);
+    W2 piv = M[(long long)k * Cc + k];
     if (!(pmag > 0.0) || !isfinite(pmag)) {
       if (tid == 0) atomicMax(info, 1);
-      piv = make_double2(1.0, 0.0);
+      piv = CTraits<W2>::make(1, 0);
     }
-    const double2 inv = cinv(piv);
+    const W2 inv = cinvT(piv);
     __syncthreads();
     // scale the pivot row (panel columns after k, and T)
     for (int j = tid; j < (pend - k - 1) + (kk + 1); j += nt) {
-      double2* t = j < pend - k - 1 ? &M[(long long)k * Cc + k + 1 + j] : &T[k * kPanel + (j - (pend - k - 1))];
+      W2* t = j < pend - k - 1 ? &M[(long long)k * Cc + k + 1 + j] : &T[k * kPanel + (j - (pend - k - 1))];
       *t = cmul(*t, inv);
     }
     for (int i = tid; i < R; i += nt) colk[i] = (i == k) ? zero : M[(long long)i * Cc + k];
@@ -312,18 +333,18 @@ __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__
     for (long long idx = tid; idx < (long long)R * w; idx += nt) {
       const int i = (int)(idx / w), j = (int)(idx - (long long)i * w);
       if (i == k) continue;
-      const double2 fi = colk[i];
-      if (fi.x == 0.0 && fi.y == 0.0) continue;
+      const W2 fi = colk[i];
+      if (fi.x == 0 && fi.y == 0) continue;
       if (j < pend - k - 1) {
-        double2* t = &M[(long long)i * Cc + k + 1 + j];
+        W2* t = &M[(long long)i * Cc + k + 1 + j];
         *t = cfms(fi, M[(long long)k * Cc + k + 1 + j], *t);
       } else {
         const int c = j - (pend - k - 1);
-        double2* t = &T[i * kPanel + c];
+        W2* t = &T[i * kPanel + c];
         *t = cfms(fi, T[k * kPanel + c], *t);
       }
     }
-    for (int i = tid; i < R; i += nt) M[(long long)i * Cc + k] = (i == k) ? make_double2(1.0, 0.0) : zero;
+    for (int i = tid; i < R; i += nt) M[(long long)i * Cc + k] = (i == k) ? CTraits<W2>::make(1, 0) : zero;
     __syncthreads();
   }
   // B = the pivot rows from the first column after the panel; T -= E
@@ -333,14 +354,15 @@ __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__
     B[(long long)c * Cc + j] = M[(long long)(k0 + c) * Cc + pend + j];
   }
   for (int c = tid; c < nbk; c += nt) {
-    double2* t = &T[(k0 + c) * kPanel + c];
-    t->x -= 1.0;
+    W2* t = &T[(k0 + c) * kPanel + c];
+    t->x -= 1;
   }
 }
 
 // new = old + (T - E) B over rows [0, R) and columns [pend, Cc) of every
 // front: 64 x 64 output tiles, 4 x 4 per thread, the panel's T and B slices
-// staged in LDS.  blockIdx.y = front, blockIdx.x = tile.
+// staged in LDS.  blockIdx.y = front, blockIdx.x = tile.  (FP64 fronts; kept
+// for comparison with the matrix-core kernel.)
 __global__ __launch_bounds__(256) void k_front_trail(const CellDev* __restrict__ cells,
                                                      const ShapeDev* __restrict__ shapes,
                                                      const FrontTask* __restrict__ fts, double2* __restrict__ ws,
@@ -348,7 +370,7 @@ __global__ __launch_bounds__(256) void k_front_trail(const CellDev* __restrict__
   __shared__ double2 sT[64][kPanel + 1];
   __shared__ double2 sB[kPanel][64 + 1];
   const FrontTask ft = fts[blockIdx.y];
-  const FrontView f = frontView(cells, shapes, ft, ws);
+  const FrontView<double2> f = frontView(cells, shapes, ft, ws);
   const int e = f.e, R = f.R, Cc = f.Cc;
   const int nbk = min(kPanel, e - k0);
   if (nbk <= 0) return;
@@ -404,21 +426,42 @@ __global__ __launch_bounds__(256) void k_front_trail(const CellDev* __restrict__
 
 // The same update on the matrix cores: each wavefront owns a 32 x 32
 // quadrant of the 64 x 64 tile as 2 x 2 blocks of 16 x 16, and every complex
-// product-accumulate is four real v_mfma_f64_16x16x4f64 (Cr += Tr Br - Ti Bi,
-// Ci += Tr Bi + Ti Br) over k-steps of 4 panel columns.  Operand layout of
-// the 16x16x4 f64 instruction (checked on gfx950): lane l supplies
-// A[l % 16][l / 16] and B[l / 16][l % 16]; it holds D[l / 16 + 4 r][l % 16]
-// in result register r (the f32 variant holds D[4 (l / 16) + r][l % 16]).
+// product-accumulate is four real 16x16x4 matrix-core products (Cr += Tr Br -
+// Ti Bi, Ci += Tr Bi + Ti Br) over k-steps of 4 panel columns:
+// v_mfma_f64_16x16x4f64 on FP64 fronts, v_mfma_f32_16x16x4f32 (FP32
+// accumulation) on FP32 fronts.  Operand layout of both (checked on gfx950):
+// lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; the f64 result
+// register r of lane l holds D[l / 16 + 4 r][l % 16], the f32 one
+// D[4 (l / 16) + r][l % 16].
 typedef double lswD4 __attribute__((ext_vector_type(4)));
+typedef float lswS4 __attribute__((ext_vector_type(4)));
 
+template <typename W2>
+struct TrailAcc;
+template <>
+struct TrailAcc<double2> {
+  using V = lswD4;
+  __device__ static V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct TrailAcc<float2> {
+  using V = lswS4;
+  __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lk, int r) { return 4 * lk + r; }
+};
+
+template <typename W2>
 __global__ __launch_bounds__(256) void k_front_trail_mfma(const CellDev* __restrict__ cells,
                                                           const ShapeDev* __restrict__ shapes,
                                                           const FrontTask* __restrict__ fts,
-                                                          double2* __restrict__ ws, int k0) {
-  __shared__ double2 sT[64][kPanel + 1];
-  __shared__ double2 sB[kPanel][64 + 1];
+                                                          W2* __restrict__ ws, int k0) {
+  using A = TrailAcc<W2>;
+  using V = typename A::V;
+  __shared__ W2 sT[64][kPanel + 1];
+  __shared__ W2 sB[kPanel][64 + 1];
   const FrontTask ft = fts[blockIdx.y];
-  const FrontView f = frontView(cells, shapes, ft, ws);
+  const FrontView<W2> f = frontView(cells, shapes, ft, ws);
   const int e = f.e, R = f.R, Cc = f.Cc;
   const int nbk = min(kPanel, e - k0);
   if (nbk <= 0) return;
@@ -427,11 +470,11 @@ __global__ __launch_bounds__(256) void k_front_trail_mfma(const CellDev* __restr
   const int tile = blockIdx.x;
   if (tile >= tilesC * tilesR) return;
   const int r0 = (tile / tilesC) * 64, c0 = (tile % tilesC) * 64;
-  double2* M = f.M;
-  const double2* T = M + (long long)R * Cc;
-  const double2* B = T + (long long)R * kPanel;
+  W2* M = f.M;
+  const W2* T = M + (long long)R * Cc;
+  const W2* B = T + (long long)R * kPanel;
   const int tid = threadIdx.x;
-  const double2 zero = make_double2(0.0, 0.0);
+  const W2 zero = czero<W2>();
   for (int idx = tid; idx < 64 * kPanel; idx += 256) {
     const int i = idx / kPanel, c = idx - i * kPanel;
     sT[i][c] = (r0 + i < R && c < nbk) ? T[(r0 + i) * kPanel + c] : zero;
@@ -444,16 +487,16 @@ __global__ __launch_bounds__(256) void k_front_trail_mfma(const CellDev* __restr
   const int wave = tid >> 6, lane = tid & 63;
   const int qr = (wave >> 1) * 32, qc = (wave & 1) * 32;  // the wavefront's quadrant
   const int li = lane & 15, lk = lane >> 4;
-  lswD4 cr[2][2], ci[2][2];
+  V cr[2][2], ci[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      cr[a][b] = lswD4{0.0, 0.0, 0.0, 0.0};
-      ci[a][b] = lswD4{0.0, 0.0, 0.0, 0.0};
+      cr[a][b] = V{0, 0, 0, 0};
+      ci[a][b] = V{0, 0, 0, 0};
     }
   for (int kk = 0; kk < nbk; kk += 4) {  // columns past nbk are zero in the staging
-    double2 ta[2], tb[2];
+    W2 ta[2], tb[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) ta[a] = sT[qr + a * 16 + li][kk + lk];
 #pragma unroll
@@ -462,10 +505,10 @@ __global__ __launch_bounds__(256) void k_front_trail_mfma(const CellDev* __restr
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a].x, tb[b].x, cr[a][b], 0, 0, 0);
-        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ta[a].y, tb[b].y, cr[a][b], 0, 0, 0);
-        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a].x, tb[b].y, ci[a][b], 0, 0, 0);
-        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a].y, tb[b].x, ci[a][b], 0, 0, 0);
+        cr[a][b] = A::mma(ta[a].x, tb[b].x, cr[a][b]);
+        cr[a][b] = A::mma(-ta[a].y, tb[b].y, cr[a][b]);
+        ci[a][b] = A::mma(ta[a].x, tb[b].y, ci[a][b]);
+        ci[a][b] = A::mma(ta[a].y, tb[b].x, ci[a][b]);
       }
   }
 #pragma unroll
@@ -475,10 +518,10 @@ __global__ __launch_bounds__(256) void k_front_trail_mfma(const CellDev* __restr
       const int j = c0 + qc + b * 16 + li;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = r0 + qr + a * 16 + lk + 4 * r;
+        const int i = r0 + qr + a * 16 + A::row(lk, r);
         if (i < R && j < cw) {
-          double2* t = &M[(long long)i * Cc + pend + j];
-          *t = cadd(*t, make_double2(cr[a][b][r], ci[a][b][r]));
+          W2* t = &M[(long long)i * Cc + pend + j];
+          *t = cadd(*t, CTraits<W2>::make(cr[a][b][r], ci[a][b][r]));
         }
       }
     }
@@ -679,7 +722,8 @@ int blockedMinRows() {
 }
 
 // Trailing updates on the matrix cores (k_front_trail_mfma) unless
-// LSW_FACTOR_MFMA=0 selects the vector-unit kernel (kept for comparison).
+// LSW_FACTOR_MFMA=0 selects the vector-unit kernel (FP64 fronts; kept for
+// comparison).
 bool trailMfma() {
   static const bool v = [] {
     const char* e = std::getenv("LSW_FACTOR_MFMA");
@@ -687,12 +731,57 @@ bool trailMfma() {
   }();
   return v;
 }
+
+void launchTrail(dim3 grid, hipStream_t s, const CellDev* cells, const ShapeDev* shapes, const FrontTask* fts,
+                 double2* ws, int k0) {
+  if (trailMfma())
+    hipLaunchKernelGGL(k_front_trail_mfma<double2>, grid, dim3(256), 0, s, cells, shapes, fts, ws, k0);
+  else
+    hipLaunchKernelGGL(k_front_trail, grid, dim3(256), 0, s, cells, shapes, fts, ws, k0);
+}
+
+void launchTrail(dim3 grid, hipStream_t s, const CellDev* cells, const ShapeDev* shapes, const FrontTask* fts,
+                 float2* ws, int k0) {
+  hipLaunchKernelGGL(k_front_trail_mfma<float2>, grid, dim3(256), 0, s, cells, shapes, fts, ws, k0);
+}
 }  // namespace
+
+// FP32 fronts are opt-in: the explicit inverses F11^{-1} and the Schur
+// complements passed up the tree amplify FP32 rounding by the fronts'
+// condition numbers.  Measured: normwise backward error 3.7e-4 against
+// 1.9e-7 (FP64 fronts, FP32 factors) on an all-PML 59 x 59 cell, and on the
+// 48-problem pack flexible GMRES ran into its 200-iteration cap (7.6
+// solves/s against 88.6).
+bool factorWorkFp32(int precision) {
+  static const bool fp32Work = [] {
+    const char* e = std::getenv("LSW_FACTOR_FP32");
+    return e && std::atoi(e) != 0;
+  }();
+  return precision == LSW_FP32 && fp32Work;
+}
 
 void FactorSet::factorize(int precision, hipStream_t s) {
   LSW_REQUIRE(dCells != nullptr, LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(precision == LSW_FP64 || precision == LSW_FP32, LSW_ERR_ARG, "unknown precision");
-  if (!dFactor) LSW_HIP(devMalloc(&dFactor, std::max<long long>(mFactorEntries, 1) * sizeof(double2)));
+  if (factorWorkFp32(precision))
+    factorizeAs<float2>(s, dFactor32);
+  else
+    factorizeAs<double2>(s, dFactor);
+  mPrecision = precision;
+  if (precision == LSW_FP32 && dFactor) {
+    if (!dFactor32) LSW_HIP(devMalloc(&dFactor32, std::max<long long>(mFactorEntries, 1) * sizeof(float2)));
+    hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, dFactor, dFactor32, mFactorEntries);
+    LSW_HIP(hipGetLastError());
+    LSW_HIP(hipStreamSynchronize(s));
+    (void)devFree(dFactor);
+    dFactor = nullptr;
+  }
+  mFactorized = true;
+}
+
+template <typename W2>
+void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
+  if (!factor) LSW_HIP(devMalloc(&factor, std::max<long long>(mFactorEntries, 1) * sizeof(W2)));
   // Schur workspace for all cells
   std::vector<long long> schurOff(mCells.size());
   long long so = 0;
@@ -702,15 +791,15 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     mCells[i].schurOff = so - mTrees[mCells[i].shape]->schurSize;
   }
   LSW_HIP(uploadSync(dCells, mCells.data(), mCells.size() * sizeof(CellDev)));
-  double2* dSchur = nullptr;
-  LSW_HIP(devMalloc(&dSchur, std::max<long long>(so, 1) * sizeof(double2)));
+  W2* dSchur = nullptr;
+  LSW_HIP(devMalloc(&dSchur, std::max<long long>(so, 1) * sizeof(W2)));
   int* dInfo = nullptr;
   LSW_HIP(devMalloc(&dInfo, sizeof(int)));
   LSW_HIP(hipMemsetAsync(dInfo, 0, sizeof(int), s));
   // front batches per height, bounded by a workspace budget; a height whose
   // largest front has at least blockedMinRows() rows takes the blocked
   // schedule (panels + batched updates), which also needs T and B per front
-  const long long budget = 512LL << 20;  // complex entries (8 GiB)
+  const long long budget = kFactorWsEntries;
   std::vector<std::vector<FrontTask>> batches;
   std::vector<long long> batchWs;
   std::vector<char> batchBlocked;
@@ -755,8 +844,8 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     batchBlocked.push_back(0);
   }
   auto nodeOf = [&](const FrontTask& ft) -> const NdNode& { return mTrees[mCells[ft.cell].shape]->nodes[ft.node]; };
-  double2* dWs = nullptr;
-  LSW_HIP(devMalloc(&dWs, std::max<long long>(maxWs, 1) * sizeof(double2)));
+  W2* dWs = nullptr;
+  LSW_HIP(devMalloc(&dWs, std::max<long long>(maxWs, 1) * sizeof(W2)));
   // every batch's front list in one upload; the batches then run back to
   // back on the stream with no host round trip between heights
   std::vector<FrontTask> allFt;
@@ -772,17 +861,17 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     const auto& bt = batches[bi];
     if (bt.empty()) continue;
     const FrontTask* dFt = dFtAll + ftOff[bi];
-    const size_t shm = (size_t)mMaxFront * sizeof(double2);
+    const size_t shm = (size_t)mMaxFront * sizeof(W2);
     const unsigned nf = (unsigned)bt.size();
     if (!batchBlocked[bi]) {
-      hipLaunchKernelGGL(k_front_factor, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dCoef, dFactor, dSchur,
-                         dWs, dInfo);
+      hipLaunchKernelGGL(k_front_factor<W2>, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dCoef, factor,
+                         dSchur, dWs, dInfo);
     } else {
-      hipLaunchKernelGGL(k_front_assemble, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dCoef, dSchur, dWs);
+      hipLaunchKernelGGL(k_front_assemble<W2>, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dCoef, dSchur, dWs);
       int maxE = 0;
       for (const auto& ft : bt) maxE = std::max(maxE, nodeOf(ft).e);
       for (int k0 = 0; k0 < maxE; k0 += kPanel) {
-        hipLaunchKernelGGL(k_front_panel, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dWs, k0, dInfo);
+        hipLaunchKernelGGL(k_front_panel<W2>, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dWs, k0, dInfo);
         long long tiles = 0;
         for (const auto& ft : bt) {
           const NdNode& nd = nodeOf(ft);
@@ -790,16 +879,9 @@ void FactorSet::factorize(int precision, hipStream_t s) {
           const long long R = nd.e + nd.b, cw = 2LL * nd.e + nd.b - std::min(k0 + kPanel, nd.e);
           tiles = std::max(tiles, ((R + 63) / 64) * ((cw + 63) / 64));
         }
-        if (tiles > 0) {
-          if (trailMfma())
-            hipLaunchKernelGGL(k_front_trail_mfma, dim3((unsigned)tiles, nf), dim3(256), 0, s, dCells, dShapes, dFt,
-                               dWs, k0);
-          else
-            hipLaunchKernelGGL(k_front_trail, dim3((unsigned)tiles, nf), dim3(256), 0, s, dCells, dShapes, dFt, dWs,
-                               k0);
-        }
+        if (tiles > 0) launchTrail(dim3((unsigned)tiles, nf), s, dCells, dShapes, dFt, dWs, k0);
       }
-      hipLaunchKernelGGL(k_front_store, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dFactor, dSchur, dWs);
+      hipLaunchKernelGGL(k_front_store<W2>, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, factor, dSchur, dWs);
     }
     LSW_HIP(hipGetLastError());
   }
@@ -810,16 +892,6 @@ void FactorSet::factorize(int precision, hipStream_t s) {
   (void)devFree(dSchur);
   (void)devFree(dInfo);
   LSW_REQUIRE(mInfo == 0, LSW_ERR_NUMERIC, "factorization failed: zero or non-finite pivot");
-  mPrecision = precision;
-  if (precision == LSW_FP32) {
-    if (!dFactor32) LSW_HIP(devMalloc(&dFactor32, std::max<long long>(mFactorEntries, 1) * sizeof(float2)));
-    hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, dFactor, dFactor32, mFactorEntries);
-    LSW_HIP(hipGetLastError());
-    LSW_HIP(hipStreamSynchronize(s));
-    (void)devFree(dFactor);
-    dFactor = nullptr;
-  }
-  mFactorized = true;
 }
 
 }  // namespace lsw

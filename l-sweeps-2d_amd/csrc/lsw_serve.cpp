@@ -223,11 +223,15 @@ Server::Entry* Server::pick() {
     for (auto& kv : e.inFlight) t = std::min(t, mRequests[kv.second.req].res.submitS);
     if (!e.pending.empty()) t = std::min(t, mRequests[e.pending.front().req].res.submitS);
     if (!e.slots && e.slotCap == 0) {
-      // (re)activation: Krylov state for the right-hand sides waiting now,
-      // plus the factorisation peak when the problem is not resident
+      // (re)activation: Krylov state for the right-hand sides waiting now;
+      // a problem that is not resident is loaded first, and its
+      // factorisation workspace is gone before the Krylov state is made, so
+      // it needs the larger of the two.  Both stay within what addProblem
+      // checked against the budget, so every problem can be admitted once
+      // the idle ones are evicted.
       const int cap = std::max(1, std::min(mOpt.slots, (int)e.pending.size()));
       const double kb = (double)GmresSlots::bytesFor(e.N, cap, mOpt.restart);
-      if (!admit(e, (e.dd ? 0.0 : e.peakBytes) + kb)) continue;
+      if (!admit(e, e.dd ? kb : std::max(e.peakBytes, e.steadyBytes + kb))) continue;
       e.slotCap = cap;
       e.krylov = kb;
     }
