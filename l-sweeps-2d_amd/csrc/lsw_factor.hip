@@ -359,6 +359,151 @@ __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__
   }
 }
 
+// The same panel with its columns, T and the elimination column held in
+// LDS (fronts of up to kPanelLdsRows rows): the 16 pivot steps then run on
+// LDS instead of chains of L2 round trips, and the pivot search reduces
+// within each wavefront by shuffles (one barrier instead of a 256-wide LDS
+// tree).  The row swaps of the columns after the panel are recorded and
+// applied at the end, column by column (each thread owns columns and
+// replays the swaps in order), together with the copy of B.  Same
+// arithmetic in the same order per entry as k_front_panel, and the same
+// pivots (largest modulus, lowest row on ties, an order-free choice).
+constexpr int kPanelLdsRows = 280;
+constexpr int kPanelLd = kPanel + 1;  // padded LDS row (bank spread)
+
+__host__ __device__ constexpr size_t panelLdsBytes(int rows) {
+  return (size_t)rows * (2 * kPanelLd + 1) * sizeof(double2);
+}
+
+__global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restrict__ cells,
+                                                         const ShapeDev* __restrict__ shapes,
+                                                         const FrontTask* __restrict__ fts,
+                                                         double2* __restrict__ ws, int k0, int* __restrict__ info) {
+  extern __shared__ double2 lswDynLds[];
+  __shared__ double wbest[4];
+  __shared__ int wrow[4];
+  __shared__ int perm[kPanel];
+  const FrontTask ft = fts[blockIdx.x];
+  const FrontView<double2> f = frontView(cells, shapes, ft, ws);
+  const int e = f.e, R = f.R, Cc = f.Cc;
+  const int nbk = min(kPanel, e - k0);
+  if (nbk <= 0) return;
+  double2* sP = lswDynLds;            // R x kPanelLd: panel columns k0 .. k0+nbk-1
+  double2* sT = sP + R * kPanelLd;    // R x kPanelLd
+  double2* sC = sT + R * kPanelLd;    // R: the column being eliminated
+  double2* M = f.M;
+  double2* T = M + (long long)R * Cc;
+  double2* B = T + (long long)R * kPanel;
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  const double2 zero = make_double2(0.0, 0.0), one = make_double2(1.0, 0.0);
+  const int pend = k0 + nbk;
+  for (int idx = tid; idx < R * kPanel; idx += nt) {
+    const int i = idx / kPanel, c = idx - i * kPanel;
+    sP[i * kPanelLd + c] = c < nbk ? M[(long long)i * Cc + k0 + c] : zero;
+    sT[i * kPanelLd + c] = zero;
+  }
+  __syncthreads();
+  for (int kk = 0; kk < nbk; ++kk) {
+    const int k = k0 + kk;
+    double best = -1.0;
+    int bi = e;
+    for (int i = k + tid; i < e; i += nt) {
+      const double2 v = sP[i * kPanelLd + kk];
+      const double a = v.x * v.x + v.y * v.y;
+      if (a > best) {
+        best = a;
+        bi = i;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      wbest[wave] = best;
+      wrow[wave] = bi;
+    }
+    __syncthreads();
+    best = wbest[0];
+    bi = wrow[0];
+    for (int w = 1; w < nt / 64; ++w)
+      if (wbest[w] > best || (wbest[w] == best && wrow[w] < bi)) {
+        best = wbest[w];
+        bi = wrow[w];
+      }
+    const int p = bi;
+    const double pmag = best;
+    if (tid == 0) perm[kk] = p;
+    // columns before kk are unit columns of rows above k: zero in rows k, p
+    if (p != k) {
+      for (int j = tid; j < (nbk - kk) + kk; j += nt) {
+        double2* a = j < nbk - kk ? &sP[k * kPanelLd + kk + j] : &sT[k * kPanelLd + (j - (nbk - kk))];
+        double2* b = j < nbk - kk ? &sP[p * kPanelLd + kk + j] : &sT[p * kPanelLd + (j - (nbk - kk))];
+        const double2 t = *a;
+        *a = *b;
+        *b = t;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) sT[k * kPanelLd + kk] = one;
+    double2 piv = sP[k * kPanelLd + kk];
+    if (!(pmag > 0.0) || !isfinite(pmag)) {
+      if (tid == 0) atomicMax(info, 1);
+      piv = one;
+    }
+    const double2 inv = cinvT(piv);
+    __syncthreads();
+    for (int j = tid; j < (nbk - kk - 1) + (kk + 1); j += nt) {
+      double2* t = j < nbk - kk - 1 ? &sP[k * kPanelLd + kk + 1 + j] : &sT[k * kPanelLd + (j - (nbk - kk - 1))];
+      *t = cmul(*t, inv);
+    }
+    for (int i = tid; i < R; i += nt) sC[i] = (i == k) ? zero : sP[i * kPanelLd + kk];
+    __syncthreads();
+    const int w = (nbk - kk - 1) + (kk + 1);
+    for (int idx = tid; idx < R * w; idx += nt) {
+      const int i = idx / w, j = idx - i * w;
+      if (i == k) continue;
+      const double2 fi = sC[i];
+      if (fi.x == 0.0 && fi.y == 0.0) continue;
+      if (j < nbk - kk - 1) {
+        double2* t = &sP[i * kPanelLd + kk + 1 + j];
+        *t = cfms(fi, sP[k * kPanelLd + kk + 1 + j], *t);
+      } else {
+        const int c = j - (nbk - kk - 1);
+        double2* t = &sT[i * kPanelLd + c];
+        *t = cfms(fi, sT[k * kPanelLd + c], *t);
+      }
+    }
+    for (int i = tid; i < R; i += nt) sP[i * kPanelLd + kk] = (i == k) ? one : zero;
+    __syncthreads();
+  }
+  // panel columns and T (minus E) back to the workspace
+  for (int idx = tid; idx < R * kPanel; idx += nt) {
+    const int i = idx / kPanel, c = idx - i * kPanel;
+    if (c < nbk) M[(long long)i * Cc + k0 + c] = sP[i * kPanelLd + c];
+    double2 t = sT[i * kPanelLd + c];
+    if (c < nbk && i == k0 + c) t.x -= 1.0;
+    T[i * kPanel + c] = t;
+  }
+  // the recorded swaps on the columns after the panel, then B
+  for (int j = pend + tid; j < Cc; j += nt) {
+    for (int kk = 0; kk < nbk; ++kk) {
+      const int k = k0 + kk, p = perm[kk];
+      if (p != k) {
+        const double2 t = M[(long long)k * Cc + j];
+        M[(long long)k * Cc + j] = M[(long long)p * Cc + j];
+        M[(long long)p * Cc + j] = t;
+      }
+    }
+    for (int c = 0; c < nbk; ++c) B[(long long)c * Cc + (j - pend)] = M[(long long)(k0 + c) * Cc + j];
+  }
+}
+
 // new = old + (T - E) B over rows [0, R) and columns [pend, Cc) of every
 // front: 64 x 64 output tiles, 4 x 4 per thread, the panel's T and B slices
 // staged in LDS.  blockIdx.y = front, blockIdx.x = tile.  (FP64 fronts; kept
@@ -732,6 +877,30 @@ bool trailMfma() {
   return v;
 }
 
+// LDS-resident panels for fronts of up to kPanelLdsRows rows (FP64 fronts)
+// unless LSW_PANEL_LDS=0.
+bool panelLds() {
+  static const bool v = [] {
+    const char* e = std::getenv("LSW_PANEL_LDS");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return v;
+}
+
+void launchPanel(unsigned nf, int maxR, size_t shm, hipStream_t s, const CellDev* cells, const ShapeDev* shapes,
+                 const FrontTask* fts, double2* ws, int k0, int* info) {
+  if (panelLds() && maxR > 0 && maxR <= kPanelLdsRows)
+    hipLaunchKernelGGL(k_front_panel_lds, dim3(nf), dim3(256), panelLdsBytes(maxR), s, cells, shapes, fts, ws, k0,
+                       info);
+  else
+    hipLaunchKernelGGL(k_front_panel<double2>, dim3(nf), dim3(256), shm, s, cells, shapes, fts, ws, k0, info);
+}
+
+void launchPanel(unsigned nf, int, size_t shm, hipStream_t s, const CellDev* cells, const ShapeDev* shapes,
+                 const FrontTask* fts, float2* ws, int k0, int* info) {
+  hipLaunchKernelGGL(k_front_panel<float2>, dim3(nf), dim3(256), shm, s, cells, shapes, fts, ws, k0, info);
+}
+
 void launchTrail(dim3 grid, hipStream_t s, const CellDev* cells, const ShapeDev* shapes, const FrontTask* fts,
                  double2* ws, int k0) {
   if (trailMfma())
@@ -871,7 +1040,12 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
       int maxE = 0;
       for (const auto& ft : bt) maxE = std::max(maxE, nodeOf(ft).e);
       for (int k0 = 0; k0 < maxE; k0 += kPanel) {
-        hipLaunchKernelGGL(k_front_panel<W2>, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dWs, k0, dInfo);
+        int maxR = 0;
+        for (const auto& ft : bt) {
+          const NdNode& nd = nodeOf(ft);
+          if (nd.e > k0) maxR = std::max(maxR, nd.e + nd.b);
+        }
+        launchPanel(nf, maxR, shm, s, dCells, dShapes, dFt, dWs, k0, dInfo);
         long long tiles = 0;
         for (const auto& ft : bt) {
           const NdNode& nd = nodeOf(ft);
