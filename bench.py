@@ -503,8 +503,13 @@ def lens_fields(cfg, out, xs, prof, tune):
         f["roofline_fp64"] = roofline(prof["fp64"])
     if "fp32_x16" in prof:
         f["roofline_block16"] = roofline(prof["fp32_x16"])
-        f["roofline_block16"]["kernel"] = ("k_nd_mfma + k_nd_mfma_wide + k_nd_packed + k_nd_wide (batched ND local "
-                                           "solves of one M^-1 application to a block of 16 sources, FP32)")
+        f["roofline_block16"]["kernel"] = ("k_nd_mfma + k_nd_mfma_wide (batched ND local solves of one M^-1 "
+                                           "application to a block of 16 sources: FP32 tasks of 8 right-hand "
+                                           "sides on the matrix cores)")
+        traffic = load_traffic(cfg["name"] + "_block16")
+        if traffic:
+            f["roofline_block16"]["traffic"] = traffic["bytes_per_launch"]
+            f["roofline_block16"]["traffic_source"] = traffic["source"]
     if "plain" in out:
         base = out["plain"]["solves_per_s"]
         for k in ("mixed", "batched"):
@@ -566,7 +571,9 @@ def _cpu_object(r):
     if "gmres_s" in r and "iterations" in r and "solves" not in r:
         return {"value": round(r["solves_per_s"], 6), "unit": "solves/s", "cores": r["procs"], "kind": "port",
                 "sample": (f"{r['workload']}: the oracle's restatement (SuperLU local solves, reference-mode GMRES) "
-                           f"on {r['procs']} processes of {r['host_cpus']} host CPUs ({r['cpu']}); measured: "
+                           f"on {r['procs']} processes ("
+                           f"{'one per board row, the reference rank layout' if r.get('assign') == 'rows' else 'cells dealt round-robin'}"
+                           f") of {r['host_cpus']} host CPUs ({r['cpu']}); measured: "
                            f"{r['measured']}; value = 1 / GMRES seconds"),
                 "factorize_s": round(r["factorize_s"], 3), "gmres_s": round(r["gmres_s"], 3),
                 "setup_s": round(r["setup_s"], 3), "end_to_end_s": round(r["end_to_end_s"], 3),
@@ -659,6 +666,11 @@ def north_star_bench(args):
         cpu_lens = cpu_baseline("lens1600", iterations=iters)
         log("  lens1600", cpu_lens)
         lens["cpu_baseline"] = _cpu_object(cpu_lens)
+        # the reference's own rank layout: one process per checkerboard row
+        rows = int(cfg["ncells"][0])
+        cpu_rows = cpu_baseline("lens1600", procs=rows, iterations=iters, assign="rows")
+        log("  lens1600, one process per row", cpu_rows)
+        lens["cpu_baseline_rows"] = _cpu_object(cpu_rows)
     line = {
         "metric": METRIC, "value": round(packed["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * total / args.steps, 3),
@@ -673,12 +685,22 @@ def north_star_bench(args):
         "p50_ms": round(1e3 * pct(lat, 50), 1), "p99_ms": round(1e3 * pct(lat, 99), 1),
         "speedup_vs_plain": round(packed["solves_per_s"] / plain["solves_per_s"], 3),
         "plain": plain, "packed": packed,
-        "roofline": lens.pop("roofline", None),
+        "roofline": None,
         "cpu_baseline": _cpu_object(cpu) if cpu else None,
         "lens1600": lens,
     }
-    if line["roofline"] is not None:
-        line["roofline"]["workload"] = "the batched ND solves of one M^-1 application on the 1600^2 lens (FP32)"
+    # the dominant kernel of the packed path: the FP32 8-right-hand-side
+    # solve tasks on the matrix cores, measured on one 16-source application
+    # on the 1600^2 lens; the single-source figure beside it
+    single, block = lens.pop("roofline", None), lens.pop("roofline_block16", None)
+    if single is not None:
+        single["workload"] = "the batched ND solves of one single-source M^-1 application on the 1600^2 lens (FP32)"
+    if block is not None:
+        block["workload"] = ("the batched ND solves of one M^-1 application to 16 sources on the 1600^2 lens "
+                             "(FP32), the launch shape the packed and served paths run")
+        line["roofline"], line["roofline_single"] = block, single
+    else:
+        line["roofline"] = single
     print(json.dumps(line))
 
 

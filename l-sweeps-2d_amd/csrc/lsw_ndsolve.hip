@@ -847,6 +847,18 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   std::vector<double> fbytes(mMaxHeight + 1, 0.0), bbytes(mMaxHeight + 1, 0.0);
   std::vector<double> fwork(mMaxHeight + 1, 0.0), bwork(mMaxHeight + 1, 0.0);
   std::vector<char> on, need;
+  // Algorithmic bytes: the right-hand-side and update vectors of every task,
+  // and each factor block once per launch however many tasks of its cell
+  // read it (they run as neighbouring entries, and the block comes from HBM
+  // once): per (cell, node), the widest factor read counted so far.
+  std::map<std::pair<int, int>, std::pair<double, double>> facSeen;
+  auto facBytes = [&](int cell, int node, bool fwd, double bytes) {
+    auto& seen = facSeen[{cell, node}];
+    double& c = fwd ? seen.first : seen.second;
+    const double add = std::max(0.0, bytes - c);
+    c = std::max(c, bytes);
+    return add;
+  };
   for (int t = 0; t < nt; ++t) {
     const SolveTask& tk = tasks[t];
     LSW_REQUIRE(tk.nrhs >= 1 && tk.nrhs <= kMaxRhs, LSW_ERR_ARG, "bad task right-hand-side count");
@@ -896,7 +908,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
           }
         }
         fcols[h] = std::max(fcols[h], nd.e);
-        fbytes[h] += S * (b * e + r * (2 * e + 2 * b));
+        fbytes[h] += facBytes(tk.cell, p, true, S * b * e) + S * r * (2 * e + 2 * b);
         fwork[h] += b * e + e;
       }
       if (!need[p]) continue;
@@ -907,7 +919,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       }
       bcols[h] = std::max(bcols[h], nd.e + nd.b);
       const double kcols = on[p] ? e + b : b;
-      bbytes[h] += S * (e * kcols + r * (kcols + e));
+      bbytes[h] += facBytes(tk.cell, p, false, S * e * kcols) + S * r * (kcols + e);
       bwork[h] += e * kcols;
     }
   }
