@@ -142,11 +142,11 @@ int lsw_dd_set_graphs(lsw_dd* dd, int on);
 /* Launch-shape autotuning for blocks of nrhs right-hand sides: times M^-1
  * under the default shape and under alternatives (column-split thresholds
  * and caps, matrix cores on/off), one choice at a time, and keeps the
- * fastest for that block width (synchronous).  out[0..8] = default ms, best
+ * fastest for that block width (synchronous).  out[0..9] = default ms, best
  * ms per application, shapes tried, chosen latency-wave threshold, wide
  * budget, wide cap, matrix cores (0/1), latency row limit, fewest columns
- * of a matrix-core 2-wavefront split (2^30: off).  No counterpart in the
- * reference. */
+ * of a matrix-core 2-wavefront split (2^30: off), paired matrix-core
+ * entries (0/1).  No counterpart in the reference. */
 int lsw_dd_autotune(lsw_dd* dd, int nrhs, double* out);
 /* Wait for all work queued on the DD's own stream. */
 int lsw_dd_sync(lsw_dd* dd);

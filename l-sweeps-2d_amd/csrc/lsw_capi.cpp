@@ -385,6 +385,7 @@ int lsw_dd_autotune(lsw_dd* dd, int nrhs, double* out) {
     out[6] = r.shape.mfma ? 1.0 : 0.0;
     out[7] = (double)r.shape.latencyRows;
     out[8] = (double)r.shape.mfmaWideCols;
+    out[9] = r.shape.mfmaPair ? 1.0 : 0.0;
   });
 }
 

@@ -1495,6 +1495,9 @@ DDCheckerboard::TuneResult DDCheckerboard::autotune(int nrhs) const {
     c = best;
     c.mfmaWideCols = c.mfmaWideCols >= (1 << 30) ? PlanShape().mfmaWideCols : (1 << 30);
     consider(c);
+    c = best;
+    c.mfmaPair = !c.mfmaPair;
+    consider(c);
   }
   mShapes[nrhs] = best;
   dropPlans(nrhs);

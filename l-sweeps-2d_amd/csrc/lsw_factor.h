@@ -85,6 +85,8 @@ struct PlanShape {
   bool mfma = true;               // FP32 8-right-hand-side packed launches on the matrix cores
   int mfmaWideCols = 192;         // ... and of more than 32 rows and at least this many columns split
                                   // them over 2 wavefronts on the matrix cores even when there are many
+  bool mfmaPair = true;           // packed matrix-core launches: two tasks' entries of the same node
+                                  // rows share a wavefront and its factor loads
   static PlanShape defaults();
 };
 
@@ -102,6 +104,7 @@ struct SolvePlan {
     int rounds = 1;
     int wpb = 1;
     int waves = 1;
+    int pair = 0;       // matrix-core packed: entries paired by factor rows (1: RP 16, 2: RP 32)
     int cols = 0;       // largest column count (sizes the LDS)
     double bytes = 0;   // algorithmic bytes
   };

@@ -344,11 +344,12 @@ class DDCheckerboard:
     def autotune(self, nrhs=1):
         """Time M^{-1} for blocks of nrhs right-hand sides under alternative
         launch shapes and keep the fastest for that block width."""
-        out = np.zeros(9)
+        out = np.zeros(10)
         _check(_lib.lsw_dd_autotune(self._h, int(nrhs), _dptr(out)))
         return {"default_ms": float(out[0]), "best_ms": float(out[1]), "tried": int(out[2]),
                 "latency_waves": int(out[3]), "wide_budget": int(out[4]), "wide_cap": int(out[5]),
-                "mfma": bool(out[6]), "latency_rows": int(out[7]), "mfma_wide_cols": int(out[8])}
+                "mfma": bool(out[6]), "latency_rows": int(out[7]), "mfma_wide_cols": int(out[8]),
+                "mfma_pair": bool(out[9])}
 
     def write_vtk(self, x, path):
         """<path>.vtk (OutputVTK layout) and <path>.lswgrid (full precision)."""
