@@ -336,7 +336,7 @@ def pack_bench(args):
     plain = dict(solves_per_s=pn / pt, problems=len(sample), solves=pn, worst_true_residual=pres)
     log("plain", plain)
     srv = L.Server(budget_bytes=args.serve_budget, slots=16, restart=args.serve_restart, tol=1e-6, max_iter=200,
-                   precision=L.LSW_PREC_FP32, workers=args.serve_workers)
+                   precision=L.LSW_PREC_FP32, workers=args.serve_workers, batch_order=True)
     pids = [srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)) for p in probs]
     stream = [(0.0, k, p["sources"]) for k, p in enumerate(probs)]
     res, wall = run_stream(L, srv, pids, stream)
@@ -592,7 +592,7 @@ def pack_pass(L, probs, workers, restart, budget, slots=16):
     from concurrent.futures import ThreadPoolExecutor
     t0 = time.perf_counter()
     srv = L.Server(budget_bytes=budget, slots=slots, restart=restart, tol=1e-6, max_iter=200,
-                   precision=L.LSW_PREC_FP32, workers=workers)
+                   precision=L.LSW_PREC_FP32, workers=workers, batch_order=True)
     try:
         # registration (host geometry and the device-bytes estimate) on a few
         # threads (the library call releases the GIL), then every request

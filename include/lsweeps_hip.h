@@ -46,6 +46,13 @@ const char* lsw_last_error(void);
 const char* lsw_version(void);
 /* Number of visible HIP devices; selects nothing. */
 int lsw_device_count(int* count);
+/* Device memory the library holds: live = in use by its objects, cached =
+ * freed blocks it keeps for reuse (either pointer may be NULL).  No
+ * counterpart in the reference (the CPU code has no device memory): the
+ * library keeps freed device blocks for the life of the process, and a
+ * caller that shares the device with another allocator (torch's, say) hands
+ * them back with release != 0 (after a device synchronisation). */
+int lsw_device_memory(long long* live, long long* cached, int release);
 
 /* ---- LocalLinearSolver (include/LocalLinearSolver.h:27-40) -------------
  * Replaces LocalLinearSolverEigenSparseLU (src/LocalLinearSolverEigenSparseLU.cpp:33-76):
@@ -262,7 +269,13 @@ typedef struct lsw_server_options {
   int max_iter;        /* Arnoldi steps per right-hand side */
   int precision;       /* LSW_PREC_FP32 (mixed) or LSW_PREC_FP64 */
   int workers;         /* host threads (1..16) */
+  int order;           /* LSW_ORDER_OLDEST: the problem with the oldest waiting
+                        * request first; LSW_ORDER_BATCH (a batch submitted at
+                        * once): problems already in flight first, then the
+                        * one with the most estimated work (longest job first) */
 } lsw_server_options;
+#define LSW_ORDER_OLDEST 0
+#define LSW_ORDER_BATCH 1
 
 typedef struct lsw_request_result {
   int status;          /* LSW_OK or the error that failed the request */

@@ -69,6 +69,14 @@ int lsw_device_count(int* count) {
   });
 }
 
+int lsw_device_memory(long long* live, long long* cached, int release) {
+  return guarded([&] {
+    if (release) releaseDeviceCache();
+    if (live) *live = totalDeviceBytes();
+    if (cached) *cached = cachedDeviceBytes();
+  });
+}
+
 // ------------------------------------------------------------------ LU
 int lsw_lu_create(int precision, lsw_lu** out) {
   return guarded([&] {
@@ -607,6 +615,8 @@ int lsw_server_create(const lsw_server_options* o, lsw_server** out) {
     so.maxIter = o->max_iter;
     so.precision = o->precision;
     so.workers = o->workers;
+    LSW_REQUIRE(o->order == LSW_ORDER_OLDEST || o->order == LSW_ORDER_BATCH, LSW_ERR_ARG, "unknown server order");
+    so.batchOrder = o->order == LSW_ORDER_BATCH;
     auto h = std::make_unique<lsw_server>();
     h->srv = std::make_unique<Server>(so);
     *out = h.release();

@@ -148,7 +148,10 @@ class DDCheckerboard : public DD {
   // cell shapes; valid after setUpGeometry): steady = factors in the solve
   // precision + stencils + sweep buffers; peak adds what factorisation holds
   // for a while (FP64 factors before the FP32 copy, front workspace).
-  void deviceBytesEstimate(int precision, int nrhs, double* steady, double* peak) const;
+  // The trees it builds for the estimate are shared (sharedTree); keep, if
+  // given, receives them so that the caller can hold them for a later set-up.
+  void deviceBytesEstimate(int precision, int nrhs, double* steady, double* peak,
+                           std::vector<std::shared_ptr<const NdTree>>* keep = nullptr) const;
   // y = A x for nrhs vectors
   void applyADevBlock(const double2* x, double2* y, int nrhs, hipStream_t s) const;
   // y = A x (global operator)
@@ -166,8 +169,9 @@ class DDCheckerboard : public DD {
   int nRows() const { return mNR; }
   int nCols() const { return mNC; }
   long long localSolvesPerPrecond() const { return mSolvesPerPrecond; }
-  // algorithmic bytes moved by the local solves of one applyPrecond
-  double solveBytesPerPrecond() const { return mSolveBytesPerPrecond; }
+  // algorithmic bytes moved by the local solves of one applyPrecond of one
+  // right-hand side (builds the single-right-hand-side plans if need be)
+  double solveBytesPerPrecond() const;
   // owned global indices of cell c (unshifted), in local DOF order
   void ownedGlobalIndices(int c, std::vector<long long>& idx) const;
   // global vector <-> per-cell owned vectors
@@ -194,6 +198,15 @@ class DDCheckerboard : public DD {
   CellGeom mGlobal;
   std::unique_ptr<FactorSet> mFactors;
   hipStream_t mStream = nullptr;
+  // the stream of a call (its own for NULL); notes a caller's stream other
+  // than its own, which the destructor then drains with the whole device
+  hipStream_t useStream(hipStream_t s) const {
+    if (!s) return mStream;
+    if (s != mStream) mForeignStream = true;
+    return s;
+  }
+  mutable bool mForeignStream = false;
+  DevArena mArena;  // step slot, group and copy lists, trace classes, owner tables
 
   // global operator
   CellDev mGlobalDev{};
@@ -206,7 +219,6 @@ class DDCheckerboard : public DD {
   struct Step {
     SlotDev* dSlots = nullptr;
     int nSlots = 0;
-    int nTasks = 0, maxNrhs = 1;
     std::vector<SlotDev> hSlots;               // host copy (single right-hand side)
     std::vector<std::vector<Rect>> hSupport;   // right-hand-side support per slot
     // solve plans per block width (built on first use)
@@ -228,7 +240,7 @@ class DDCheckerboard : public DD {
   int mTmax = 0;
   long long mTraceEntries = 0;
   long long mSolvesPerPrecond = 0;
-  double mSolveBytesPerPrecond = 0;
+  mutable double mSolveBytesPerPrecond = -1;  // < 0: not computed yet
 
   // buffers (element type follows precision), sized for mBufRhs right-hand sides
   mutable void* dX = nullptr;   // right-hand sides of the local solves (slot-major)

@@ -433,20 +433,20 @@ __global__ void k_point_sources(const double* __restrict__ src, int nsrc, double
 DDCheckerboard::DDCheckerboard() = default;
 
 DDCheckerboard::~DDCheckerboard() {
-  if (mStream) (void)hipStreamSynchronize(mStream);
+  // drain every stream that used the DD's memory (a caller's own stream: the
+  // whole device), so that its blocks can be reused at once
+  if (mForeignStream)
+    (void)hipDeviceSynchronize();
+  else if (mStream)
+    (void)hipStreamSynchronize(mStream);
+  IdleFrees idle;
   for (auto& kv : mGraphs) (void)hipGraphExecDestroy(kv.second);
   mGraphs.clear();
-  for (auto& st : mSteps) {
-    if (st.dSlots) (void)devFree(st.dSlots);
-    if (st.dGroups) (void)devFree(st.dGroups);
-    if (st.dCopies) (void)devFree(st.dCopies);
-  }
   for (void* p : {dX, dXs, dR, dG, dT, dF}) if (p) (void)devFree(p);
   if (dY) (void)devFree(dY);
-  if (dOwn) (void)devFree(dOwn);
   if (dGlobalCoef) (void)devFree(dGlobalCoef);
-  if (dColTrace) (void)devFree(dColTrace);
-  if (dRowTrace) (void)devFree(dRowTrace);
+  mSteps.clear();  // their solve plans
+  mArena.release();
   mFactors.reset();
   if (mStream) (void)hipStreamDestroy(mStream);
 }
@@ -489,11 +489,18 @@ void DDCheckerboard::setUp(const Assembler* inputAssembler, const LocalLinearSol
                            std::vector<DoubleVector> domain, std::vector<UInt> nCells,
                            std::vector<UInt> nElsInCell) {
   const auto t0 = std::chrono::steady_clock::now();
+  double up0, al0;
+  long long nu0, na0;
+  threadMemTimes(&up0, &nu0, &al0, &na0);
   setUpGeometry(inputAssembler, solver, domain, nCells, nElsInCell);
   mSetupLog.emplace_back("geometry", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   initDevice();
   if (std::getenv("LSW_SETUP_LOG")) {
+    double up1, al1;
+    long long nu1, na1;
+    threadMemTimes(&up1, &nu1, &al1, &na1);
     for (const auto& kv : mSetupLog) std::fprintf(stderr, "setup %-10s %.4f s\n", kv.first.c_str(), kv.second);
+    std::fprintf(stderr, "setup uploads %lld %.4f s allocs %lld %.4f s\n", nu1 - nu0, up1 - up0, na1 - na0, al1 - al0);
   }
 }
 
@@ -750,10 +757,8 @@ void DDCheckerboard::initDevice() {
   // legacy stream, from other server workers) must not wait for this
   // problem's solves
   if (!mStream) LSW_HIP(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking));
-  LSW_HIP(devMalloc(&dColTrace, mGx));
-  LSW_HIP(devMalloc(&dRowTrace, mGy));
-  LSW_HIP(uploadSync(dColTrace, mColT.data(), mGx));
-  LSW_HIP(uploadSync(dRowTrace, mRowT.data(), mGy));
+  dColTrace = mArena.put(mColT);
+  dRowTrace = mArena.put(mRowT);
 
   auto toDev = [&](const CellGeom& c) {
     CellDev d;
@@ -812,11 +817,10 @@ void DDCheckerboard::initDevice() {
     for (size_t k = 0; ok && k < fin.hSlots.size(); ++k)
       ok = fin.hSlots[k].d == (int)k / nc && fin.hSlots[k].cell == (int)k % nc;
     mGatherFinal = ok;
-    if (ok) {
-      LSW_HIP(devMalloc(&dOwn, mOwnHost.size() * sizeof(int)));
-      LSW_HIP(hipMemcpy(dOwn, mOwnHost.data(), mOwnHost.size() * sizeof(int), hipMemcpyHostToDevice));
-    }
+    if (ok) dOwn = mArena.put(mOwnHost);
   }
+  // every descriptor of the schedule, the trace classes and owner tables
+  mArena.flush();
   allocBuffers();
   lap("buffers", t);
 }
@@ -1095,7 +1099,7 @@ void DDCheckerboard::buildSchedule() {
   // slot buffers, tasks (same cell -> one task of up to kMaxRhs), groups
   mMaxSlots = 0;
   mSolvesPerPrecond = 0;
-  mSolveBytesPerPrecond = 0;
+  mSolveBytesPerPrecond = -1;
   const double S = (mPrecision == LSW_FP64) ? 16.0 : 8.0;
   for (auto& st : steps) mMaxSlots = std::max(mMaxSlots, (int)st.slots.size());
   mSteps.clear();
@@ -1147,22 +1151,27 @@ void DDCheckerboard::buildSchedule() {
     st.nGroups = (int)groups.size();
     st.nCopies = (int)hst.copies.size();
     LSW_REQUIRE(st.nSlots <= 65535 && st.nCopies <= 65535 * 8, LSW_ERR_CAPACITY, "board too large for one launch");
-    LSW_HIP(devMalloc(&st.dSlots, std::max<size_t>(1, hst.slots.size()) * sizeof(SlotDev)));
-    LSW_HIP(uploadSync(st.dSlots, hst.slots.data(), hst.slots.size() * sizeof(SlotDev)));
-    {
-      const SolvePlan& pl = planFor(st, 1);
-      mSolveBytesPerPrecond += pl.fwdBytes + pl.bwdBytes;
-      st.nTasks = pl.ntasks;
-      st.maxNrhs = pl.maxNrhs;
-    }
-    LSW_HIP(devMalloc(&st.dGroups, std::max<size_t>(1, groups.size()) * sizeof(GroupDev)));
-    LSW_HIP(uploadSync(st.dGroups, groups.data(), groups.size() * sizeof(GroupDev)));
-    if (!hst.copies.empty()) {
-      LSW_HIP(devMalloc(&st.dCopies, hst.copies.size() * sizeof(CopyOp)));
-      LSW_HIP(uploadSync(st.dCopies, hst.copies.data(), hst.copies.size() * sizeof(CopyOp)));
-    }
+    // (uploaded by the arena's flush at the end of the device set-up; the
+    // solve plans are built when a block width is first solved)
+    if (hst.slots.empty()) hst.slots.emplace_back();
+    if (groups.empty()) groups.emplace_back();
+    st.dSlots = mArena.put(hst.slots);
+    st.dGroups = mArena.put(groups);
+    st.dCopies = mArena.put(hst.copies);
     mSteps.push_back(std::move(st));
   }
+}
+
+double DDCheckerboard::solveBytesPerPrecond() const {
+  if (mSolveBytesPerPrecond < 0) {
+    double b = 0;
+    for (const auto& st : mSteps) {
+      const SolvePlan& pl = planFor(st, 1);
+      b += pl.fwdBytes + pl.bwdBytes;
+    }
+    mSolveBytesPerPrecond = b;
+  }
+  return mSolveBytesPerPrecond;
 }
 
 // Solve plan of one step for a block of nrhs right-hand sides.  Slots are
@@ -1235,8 +1244,15 @@ long long DDCheckerboard::taskRowsFor(int nrhs) const {
 void DDCheckerboard::ensureBuffers(int nrhs) const {
   if (nrhs <= mBufRhs) return;
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
-  for (void* p : {dX, dXs, dR, dG, dT, dF}) if (p) (void)devFree(p);
-  if (dY) (void)devFree(dY);
+  {
+    // buffers used on the DD's own stream only: drain it, and the blocks are
+    // reusable at once
+    const bool own = !mForeignStream && mStream && (dX || dY);
+    if (own) LSW_HIP(hipStreamSynchronize(mStream));
+    std::unique_ptr<IdleFrees> idle(own ? new IdleFrees : nullptr);
+    for (void* p : {dX, dXs, dR, dG, dT, dF}) if (p) (void)devFree(p);
+    if (dY) (void)devFree(dY);
+  }
   dX = dXs = dR = dG = dT = dF = nullptr;
   dY = nullptr;
   mBufRhs = 0;
@@ -1310,16 +1326,17 @@ void DDCheckerboard::allocBuffers() {
   ensureBuffers(1);
 }
 
-void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady, double* peak) const {
+void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady, double* peak,
+                                        std::vector<std::shared_ptr<const NdTree>>* keep) const {
   LSW_REQUIRE(!mCells.empty(), LSW_ERR_STATE, "deviceBytesEstimate before setUp");
-  std::map<std::pair<int, int>, std::unique_ptr<NdTree>> trees;
+  std::map<std::pair<int, int>, std::shared_ptr<const NdTree>> trees;
   double entries = 0, schur = 0, local = 0;
   std::vector<double> wsPerHeight(kMaxHeight + 1, 0.0);
   for (const auto& c : mCells) {
     auto& t = trees[{c.nx, c.ny}];
     if (!t) {
-      t = std::make_unique<NdTree>();
-      t->build(c.nx, c.ny);
+      t = sharedTree(c.nx, c.ny);
+      if (keep) keep->push_back(t);
     }
     entries += (double)t->factorSize;
     schur += (double)t->schurSize;
@@ -1367,7 +1384,7 @@ void DDCheckerboard::factorize() {
 // ------------------------------------------------------------- operations
 
 void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s, int nrhs, bool replay) const {
-  if (!s) s = mStream;
+  s = useStream(s);
   if (!mUseGraphs || !replay) {
     applyPrecondImpl(x, y, s, nullptr, nrhs);
     return;
@@ -1529,7 +1546,7 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   LSW_REQUIRE(mFactors && mFactors->factorized(), LSW_ERR_STATE, "applyPrecond before factorize");
   LSW_REQUIRE(nrhs >= 1, LSW_ERR_ARG, "applyPrecond: nrhs must be positive");
   LSW_REQUIRE((long long)mMaxSlots * nrhs <= 65535, LSW_ERR_CAPACITY, "too many right-hand sides in one block");
-  if (!s) s = mStream;
+  s = useStream(s);
   ensureBuffers(nrhs);
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
   const long long n = N();
@@ -1649,7 +1666,7 @@ void DDCheckerboard::applyADev(const double2* x, double2* y, hipStream_t s) cons
 
 void DDCheckerboard::applyADevBlock(const double2* x, double2* y, int nrhs, hipStream_t s) const {
   LSW_REQUIRE(dGlobalCoef != nullptr, LSW_ERR_STATE, "DD has no device state");
-  if (!s) s = mStream;
+  s = useStream(s);
   const long long n = N();
   if (nrhs > 0)
     hipLaunchKernelGGL(k_matvec, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s,
@@ -1659,7 +1676,7 @@ void DDCheckerboard::applyADevBlock(const double2* x, double2* y, int nrhs, hipS
 
 void DDCheckerboard::assembleRHSDev(const std::vector<DoubleVector>& srcs, double2* b, hipStream_t s) const {
   LSW_REQUIRE(deviceReady(), LSW_ERR_STATE, "DD has no device state");
-  if (!s) s = mStream;
+  s = useStream(s);
   std::vector<double> xy;
   for (auto& p : srcs) {
     xy.push_back(p[0]);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "lsw_common.h"
 #include "lsw_nd.h"
@@ -26,6 +27,17 @@ namespace lsw {
 // on any stream have finished with it.
 hipError_t devMallocBytes(void** p, size_t bytes);
 hipError_t devFree(void* p);
+// While an IdleFrees object lives on a thread, that thread's devFree calls
+// return blocks that no queued device work can touch any more (the caller
+// has synchronised every stream that used them): they are reusable at once,
+// without the device-wide wait a plain free needs.
+class IdleFrees {
+ public:
+  IdleFrees();
+  ~IdleFrees();
+  IdleFrees(const IdleFrees&) = delete;
+  IdleFrees& operator=(const IdleFrees&) = delete;
+};
 // bytes held in the block cache; return them to the runtime
 long long cachedDeviceBytes();
 void releaseDeviceCache();
@@ -38,10 +50,44 @@ void releaseDeviceCache();
 hipError_t uploadSync(void* dst, const void* src, size_t bytes);
 long long threadDeviceBytes();
 long long totalDeviceBytes();
+// this thread's running totals of uploadSync and devMallocBytes calls and
+// the seconds spent in them
+void threadMemTimes(double* uploadS, long long* uploads, double* allocS, long long* allocs);
 template <typename T>
 inline hipError_t devMalloc(T** p, size_t bytes) {
   return devMallocBytes(reinterpret_cast<void**>(p), bytes);
 }
+
+// The small device arrays of one owner (tree descriptors, index pools, step
+// slot lists), sub-allocated from a few blocks and uploaded together: put()
+// returns an array's device address at once and keeps its bytes on the host,
+// flush() uploads everything put since the last flush with one uploadSync per
+// block.  Set-up then costs a handful of allocations and waits for the device
+// instead of one per array.  Nothing put may be read on the device before the
+// next flush(); the blocks are freed with the arena.
+class DevArena {
+ public:
+  DevArena() = default;
+  DevArena(const DevArena&) = delete;
+  DevArena& operator=(const DevArena&) = delete;
+  ~DevArena();
+  void* put(const void* src, size_t bytes);
+  template <typename T>
+  T* put(const std::vector<T>& v) {
+    return v.empty() ? nullptr : static_cast<T*>(put(v.data(), v.size() * sizeof(T)));
+  }
+  void flush();
+  void release();  // frees every block (arrays put earlier become invalid)
+
+ private:
+  struct Block {
+    unsigned char* dev = nullptr;
+    size_t cap = 0, used = 0, flushed = 0;
+    std::vector<unsigned char> host;
+  };
+  std::vector<Block> mBlocks;
+  int mOpen = -1;  // block that takes small arrays
+};
 
 constexpr int kMaxHeight = 40;
 

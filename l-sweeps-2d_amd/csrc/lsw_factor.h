@@ -217,7 +217,7 @@ class FactorSet {
   std::vector<CellDev> mCells;
   std::vector<std::vector<double>> mM;
   std::vector<std::vector<double2>> mCoefHost;
-  std::vector<std::unique_ptr<NdTree>> mTrees;
+  std::vector<std::shared_ptr<const NdTree>> mTrees;
   std::map<std::pair<int, int>, int> mShapeOf;
   CellDev* dCells = nullptr;
   ShapeDev* dShapes = nullptr;
@@ -227,7 +227,7 @@ class FactorSet {
   int* dPoolE = nullptr;
   int* dPoolB = nullptr;
   int* dPoolMap = nullptr;
-  std::vector<void*> mShapeAllocs;
+  DevArena mArena;  // cells, shapes, trees' index arrays, solve descriptors
   double2* dCoef = nullptr;
   double* dM = nullptr;
   double2* dFactor = nullptr;

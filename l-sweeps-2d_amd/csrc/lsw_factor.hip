@@ -679,22 +679,8 @@ __global__ void k_to_f32(const double2* __restrict__ a, float2* __restrict__ o, 
 }
 
 // --------------------------------------------------------------- host side
-namespace {
-template <typename T>
-T* devUpload(const std::vector<T>& h, std::vector<void*>& allocs) {
-  if (h.empty()) return nullptr;
-  T* d = nullptr;
-  LSW_HIP(devMalloc(&d, h.size() * sizeof(T)));
-  LSW_HIP(uploadSync(d, h.data(), h.size() * sizeof(T)));
-  allocs.push_back(d);
-  return d;
-}
-}  // namespace
-
 FactorSet::~FactorSet() {
-  for (void* p : mShapeAllocs) (void)devFree(p);
-  if (dCells) (void)devFree(dCells);
-  if (dShapes) (void)devFree(dShapes);
+  mArena.release();
   if (dCoef) (void)devFree(dCoef);
   if (dM) (void)devFree(dM);
   if (dFactor) (void)devFree(dFactor);
@@ -729,8 +715,7 @@ void FactorSet::setup(hipStream_t s) {
     auto key = std::make_pair(c.nx, c.ny);
     auto it = mShapeOf.find(key);
     if (it == mShapeOf.end()) {
-      auto t = std::make_unique<NdTree>();
-      t->build(c.nx, c.ny);
+      auto t = sharedTree(c.nx, c.ny);
       LSW_REQUIRE(t->maxHeight < kMaxHeight, LSW_ERR_ARG, "FactorSet: tree too deep");
       mShapeOf[key] = (int)mTrees.size();
       mTrees.push_back(std::move(t));
@@ -770,12 +755,12 @@ void FactorSet::setup(hipStream_t s) {
     const NdTree& t = *mTrees[si];
     ShapeDev& d = shapes[si];
     std::memset(&d, 0, sizeof(d));
-    d.nodes = devUpload(t.deviceNodes(), mShapeAllocs);
-    d.E = devUpload(t.E, mShapeAllocs);
-    d.B = devUpload(t.B, mShapeAllocs);
-    d.cmap = devUpload(t.cmap, mShapeAllocs);
-    d.up = devUpload(t.up, mShapeAllocs);
-    d.asmList = devUpload(t.asmList, mShapeAllocs);
+    d.nodes = mArena.put(t.deviceNodes());
+    d.E = mArena.put(t.E);
+    d.B = mArena.put(t.B);
+    d.cmap = mArena.put(t.cmap);
+    d.up = mArena.put(t.up);
+    d.asmList = mArena.put(t.asmList);
     std::vector<SolveItem> fi, bi;
     for (int h = 0; h <= t.maxHeight; ++h) {
       d.fwdStart[h] = (int)fi.size();
@@ -785,8 +770,8 @@ void FactorSet::setup(hipStream_t s) {
       d.bwdCount[h] = (int)t.bwdItems[h].size();
       bi.insert(bi.end(), t.bwdItems[h].begin(), t.bwdItems[h].end());
     }
-    d.fwdItems = devUpload(fi, mShapeAllocs);
-    d.bwdItems = devUpload(bi, mShapeAllocs);
+    d.fwdItems = mArena.put(fi);
+    d.bwdItems = mArena.put(bi);
     d.maxHeight = t.maxHeight;
     d.nodeCount = (int)t.nodes.size();
   }
@@ -818,15 +803,14 @@ void FactorSet::setup(hipStream_t s) {
         flat.push_back(sn);
       }
     }
-    dSolveNodes = devUpload(flat, mShapeAllocs);
-    dPoolE = devUpload(pe, mShapeAllocs);
-    dPoolB = devUpload(pb, mShapeAllocs);
-    dPoolMap = devUpload(pm, mShapeAllocs);
+    dSolveNodes = mArena.put(flat);
+    dPoolE = mArena.put(pe);
+    dPoolB = mArena.put(pb);
+    dPoolMap = mArena.put(pm);
   }
-  LSW_HIP(devMalloc(&dShapes, shapes.size() * sizeof(ShapeDev)));
-  LSW_HIP(uploadSync(dShapes, shapes.data(), shapes.size() * sizeof(ShapeDev)));
-  LSW_HIP(devMalloc(&dCells, mCells.size() * sizeof(CellDev)));
-  LSW_HIP(uploadSync(dCells, mCells.data(), mCells.size() * sizeof(CellDev)));
+  dShapes = mArena.put(shapes);
+  dCells = mArena.put(mCells);
+  mArena.flush();
   // stencils
   LSW_HIP(devMalloc(&dCoef, co * sizeof(double2)));
   for (size_t i = 0; i < mCells.size(); ++i)
@@ -834,10 +818,14 @@ void FactorSet::setup(hipStream_t s) {
       LSW_HIP(uploadSync(dCoef + mCells[i].coefOff, mCoefHost[i].data(),
                         mCoefHost[i].size() * sizeof(double2)));
   if (mo > 0) {
+    // every cell's squared slowness in one upload
     LSW_HIP(devMalloc(&dM, mo * sizeof(double)));
-    for (size_t i = 0; i < mCells.size(); ++i)
-      if (!mM[i].empty())
-        LSW_HIP(uploadSync(dM + mCells[i].mOff, mM[i].data(), mM[i].size() * sizeof(double)));
+    {
+      std::vector<double> all((size_t)mo);
+      for (size_t i = 0; i < mCells.size(); ++i)
+        if (!mM[i].empty()) std::copy(mM[i].begin(), mM[i].end(), all.begin() + mCells[i].mOff);
+      LSW_HIP(uploadSync(dM, all.data(), (size_t)mo * sizeof(double)));
+    }
     int* dBad = nullptr;
     LSW_HIP(devMalloc(&dBad, sizeof(int)));
     LSW_HIP(hipMemsetAsync(dBad, 0, sizeof(int), s));
@@ -942,6 +930,7 @@ void FactorSet::factorize(int precision, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, dFactor, dFactor32, mFactorEntries);
     LSW_HIP(hipGetLastError());
     LSW_HIP(hipStreamSynchronize(s));
+    IdleFrees idle;  // the FP64 factors were only used on s
     (void)devFree(dFactor);
     dFactor = nullptr;
   }
@@ -1061,6 +1050,7 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
   }
   LSW_HIP(hipMemcpyAsync(&mInfo, dInfo, sizeof(int), hipMemcpyDeviceToHost, s));
   LSW_HIP(hipStreamSynchronize(s));
+  IdleFrees idle;  // the workspace was only used on s
   (void)devFree(dFtAll);
   (void)devFree(dWs);
   (void)devFree(dSchur);
@@ -1095,6 +1085,7 @@ double2* assembleStencilDev(const CellDev& c, const std::vector<double>& m, hipS
   int bad = 0;
   LSW_HIP(hipMemcpyAsync(&bad, dBad, sizeof(int), hipMemcpyDeviceToHost, s));
   LSW_HIP(hipStreamSynchronize(s));
+  IdleFrees idle;
   (void)devFree(dC);
   (void)devFree(dM);
   (void)devFree(dBad);

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -205,6 +207,29 @@ void NdTree::build(int nx_, int ny_) {
         bwdItems[h].push_back({p, r0, std::min(kSolveRows, nd.e - r0), 0});
     }
   }
+}
+
+std::shared_ptr<const NdTree> sharedTree(int nx, int ny) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, std::weak_ptr<const NdTree>> cache;
+  const auto key = std::make_pair(nx, ny);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end())
+      if (auto t = it->second.lock()) return t;
+  }
+  // built without the lock (threads set up different shapes at once); when
+  // two threads build the same shape, the first one stored is kept
+  auto t = std::make_shared<NdTree>();
+  t->build(nx, ny);
+  std::lock_guard<std::mutex> lk(mu);
+  auto& slot = cache[key];
+  if (auto had = slot.lock()) return had;
+  slot = t;
+  if (cache.size() > 4096)
+    for (auto it = cache.begin(); it != cache.end();) it = it->second.expired() ? cache.erase(it) : std::next(it);
+  return t;
 }
 
 std::vector<NdNodeDev> NdTree::deviceNodes() const {

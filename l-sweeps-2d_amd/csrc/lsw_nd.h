@@ -14,6 +14,7 @@
 // so both triangular solves become dense matrix-vector products.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 namespace lsw {
@@ -77,5 +78,11 @@ struct NdTree {
  private:
   int rec(int i0, int i1, int j0, int j1, int parent);
 };
+
+// The tree of an nx x ny grid.  A tree never changes once built, so grids of
+// one shape share it: a process-wide cache of weak references hands out the
+// tree while any holder keeps it (a problem whose memory estimate built its
+// trees at registration reuses them when it is set up on the device).
+std::shared_ptr<const NdTree> sharedTree(int nx, int ny);
 
 }  // namespace lsw

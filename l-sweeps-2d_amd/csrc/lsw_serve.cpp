@@ -102,7 +102,7 @@ int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
   e->Gy = geo->Gy();
   // factors and sweep buffers (steady) and the factorisation peak; the Krylov
   // state is reserved on top when work arrives
-  geo->deviceBytesEstimate(mOpt.precision, planWidth(mOpt.slots), &e->steadyBytes, &e->peakBytes);
+  geo->deviceBytesEstimate(mOpt.precision, planWidth(mOpt.slots), &e->steadyBytes, &e->peakBytes, &e->trees);
   e->N = geo->N();
   const double full = e->steadyBytes + (double)GmresSlots::bytesFor(e->N, mOpt.slots, mOpt.restart);
   LSW_REQUIRE(std::max(e->peakBytes, full) <= mOpt.budgetBytes, LSW_ERR_CAPACITY,
@@ -212,9 +212,15 @@ bool Server::admit(Entry& e, double need) {
 }
 
 Server::Entry* Server::pick() {
-  // the problem whose oldest outstanding right-hand side is the oldest
-  Entry* best = nullptr;
-  double bestT = 0;
+  // candidates: problems with work and no worker on them, best first --
+  // oldest waiting request first, or (batch order) problems in flight first
+  // by age, then the most estimated work first
+  struct Cand {
+    int rank;
+    double key;
+    Entry* e;
+  };
+  std::vector<Cand> cands;
   for (auto& up : mEntries) {
     Entry& e = *up;
     if (e.busy || e.failed) continue;
@@ -222,6 +228,20 @@ Server::Entry* Server::pick() {
     double t = 1e300;
     for (auto& kv : e.inFlight) t = std::min(t, mRequests[kv.second.req].res.submitS);
     if (!e.pending.empty()) t = std::min(t, mRequests[e.pending.front().req].res.submitS);
+    if (mOpt.batchOrder && !e.slots && e.slotCap == 0) {
+      // work still to do: the factors (their bytes) and every waiting
+      // right-hand side's sweeps over them
+      const double work = e.steadyBytes * (1.0 + (double)(e.pending.size() + e.active) / 4.0);
+      cands.push_back({1, -work, &e});
+    } else {
+      cands.push_back({0, t, &e});
+    }
+  }
+  std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
+    return a.rank != b.rank ? a.rank < b.rank : a.key < b.key;
+  });
+  for (const Cand& c : cands) {
+    Entry& e = *c.e;
     if (!e.slots && e.slotCap == 0) {
       // (re)activation: Krylov state for the right-hand sides waiting now;
       // a problem that is not resident is loaded first, and its
@@ -235,12 +255,9 @@ Server::Entry* Server::pick() {
       e.slotCap = cap;
       e.krylov = kb;
     }
-    if (!best || t < bestT) {
-      best = &e;
-      bestT = t;
-    }
+    return &e;
   }
-  return best;
+  return nullptr;
 }
 
 void Server::failEntry(Entry& e, int code, const std::string& msg) {

@@ -55,6 +55,10 @@ struct ServeOptions {
   int maxIter = 200;
   int precision = LSW_PREC_FP32;
   int workers = 4;
+  // false: the problem with the oldest waiting request first; true (a batch
+  // submitted at once): problems in flight first, then the most estimated
+  // work first, so that the longest jobs do not start last
+  bool batchOrder = false;
 };
 
 struct ServeResult {
@@ -126,6 +130,9 @@ class Server {
     double krylov = 0;   // its reserved bytes
     std::unique_ptr<DDCheckerboard> dd;
     std::unique_ptr<GmresSlots> slots;
+    // the ND trees of its cell shapes, built for the memory estimate at
+    // registration and held so that every load reuses them
+    std::vector<std::shared_ptr<const NdTree>> trees;
     std::deque<Column> pending;
     std::map<long long, Column> inFlight;  // slot tag -> column
     int active = 0, freeSlots = 0;

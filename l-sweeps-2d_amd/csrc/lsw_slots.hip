@@ -191,6 +191,7 @@ GmresSlots::GmresSlots(const DDCheckerboard* dd, int capacity, int restart, doub
 
 GmresSlots::~GmresSlots() {
   (void)hipStreamSynchronize(mDD->stream());
+  IdleFrees idle;  // the engine only runs on its DD's stream
   for (double2* p : {dV, dZ, dB, dX, dIn, dPZ, dW, dStage, dPart, dH}) if (p) (void)devFree(p);
 }
 

@@ -17,10 +17,12 @@ _lib = C.CDLL(_LIB_PATH)
 
 LSW_PREC_FP64 = 0
 LSW_PREC_FP32 = 1
+LSW_ORDER_OLDEST = 0
+LSW_ORDER_BATCH = 1
 ERRORS = {1: "ARG", 2: "HIP", 3: "STATE", 4: "NUMERIC", 5: "CAPACITY", 6: "UNSUPPORTED", 7: "INTERNAL"}
 
 EXPORTED = [
-    "lsw_last_error", "lsw_version", "lsw_device_count",
+    "lsw_last_error", "lsw_version", "lsw_device_count", "lsw_device_memory",
     "lsw_lu_create", "lsw_lu_set_csc_z", "lsw_lu_factorize", "lsw_lu_solve_z", "lsw_lu_solve_dev", "lsw_lu_solve_dev_noalloc",
     "lsw_lu_info",
     "lsw_lu_destroy",
@@ -84,7 +86,7 @@ class ProblemResult(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("budget_bytes", C.c_double), ("slots", C.c_int), ("restart", C.c_int), ("tol", C.c_double),
-                ("max_iter", C.c_int), ("precision", C.c_int), ("workers", C.c_int)]
+                ("max_iter", C.c_int), ("precision", C.c_int), ("workers", C.c_int), ("order", C.c_int)]
 
 
 class RequestResult(C.Structure):
@@ -114,6 +116,7 @@ def _sig(name, *args):
 
 
 _sig("lsw_device_count", C.POINTER(C.c_int))
+_sig("lsw_device_memory", C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int)
 _sig("lsw_lu_create", C.c_int, C.POINTER(_P))
 _sig("lsw_lu_set_csc_z", _P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _D)
 _sig("lsw_lu_factorize", _P)
@@ -229,6 +232,14 @@ def device_count():
 
 
 # --------------------------------------------------------------------------
+def device_memory(release=False):
+    """(live, cached) device bytes the library holds; release=True first
+    hands the cached blocks back to the HIP runtime."""
+    live, cached = C.c_longlong(), C.c_longlong()
+    _check(_lib.lsw_device_memory(C.byref(live), C.byref(cached), int(release)))
+    return live.value, cached.value
+
+
 class LocalLinearSolverHIP:
     """setMatrix / factorize / solve of one 5-point structured-grid matrix."""
 
@@ -484,9 +495,9 @@ class Server:
     the request is waited for."""
 
     def __init__(self, budget_bytes=250e9, slots=16, restart=16, tol=1e-6, max_iter=200,
-                 precision=LSW_PREC_FP32, workers=4):
+                 precision=LSW_PREC_FP32, workers=4, batch_order=False):
         o = ServerOptions(float(budget_bytes), int(slots), int(restart), float(tol), int(max_iter),
-                          int(precision), int(workers))
+                          int(precision), int(workers), LSW_ORDER_BATCH if batch_order else LSW_ORDER_OLDEST)
         self._h = _P()
         _check(_lib.lsw_server_create(C.byref(o), C.byref(self._h)))
         self._grids = {}
