@@ -120,15 +120,27 @@ def profile(dd, torch, nrhs=1):
 
 
 def roofline(prof):
+    """HBM roofline of the batched local solves of one M^-1 application
+    (algorithmic bytes over the summed HIP-event launch times), with the
+    flops beside it: frac_combined = summed per-launch roofline time (the
+    larger of its bytes at 8 TB/s and its flops at the matrix-core peak) over
+    the summed launch time."""
     ms = prof["fwd_ms"] + prof["bwd_ms"]
     nl = prof["fwd_launches"] + prof["bwd_launches"]
     byts = prof["fwd_bytes"] + prof["bwd_bytes"]
+    flops = prof.get("fwd_flops", 0.0) + prof.get("bwd_flops", 0.0)
     achieved = (byts / nl) / (ms / nl * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_nd_packed + k_nd_wide (batched ND local solves, one M^-1 application)",
-            "bytes_per_launch": round(byts / nl, 1), "ms_per_launch": round(ms / nl, 5),
-            "launches_per_precond": nl, "fwd": prof}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "kernel": "k_nd_packed + k_nd_wide (batched ND local solves, one M^-1 application)",
+           "bytes_per_launch": round(byts / nl, 1), "ms_per_launch": round(ms / nl, 5),
+           "launches_per_precond": nl, "fwd": prof}
+    if flops > 0 and ms > 0:
+        out["flops_per_launch"] = round(flops / nl, 1)
+        out["achieved_tflops"] = round(flops / (ms * 1e-3) / 1e12, 2)
+        out["roofline_ms"] = round(prof["ideal_ms"], 4)
+        out["frac_combined"] = round(prof["ideal_ms"] / ms, 4)
+    return out
 
 
 def load_traffic(workload):
@@ -653,6 +665,10 @@ def north_star_bench(args):
                   worker_s={k: round(st[k], 3) for k in ("setup_s", "factor_s", "block_s", "step_s")},
                   workers=args.pack_workers, restart=args.serve_restart, budget_gb=args.serve_budget / 1e9)
     log("packed", packed)
+    # the blocks the pack left in the library's cache go back to the runtime
+    # before the next section
+    L.device_memory(release=True)
+    torch.cuda.empty_cache()
     # the 1600^2 lens workload: single-problem figures and the solve roofline
     cfg = config_lens_1600()
     out, xs, prof, tune = lens_section(args, L, torch, cfg, args.paths.split(","))

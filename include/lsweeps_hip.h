@@ -149,19 +149,23 @@ int lsw_dd_set_graphs(lsw_dd* dd, int on);
 /* Launch-shape autotuning for blocks of nrhs right-hand sides: times M^-1
  * under the default shape and under alternatives (column-split thresholds
  * and caps, matrix cores on/off), one choice at a time, and keeps the
- * fastest for that block width (synchronous).  out[0..9] = default ms, best
+ * fastest for that block width (synchronous).  out[0..10] = default ms, best
  * ms per application, shapes tried, chosen latency-wave threshold, wide
  * budget, wide cap, matrix cores (0/1), latency row limit, fewest columns
  * of a matrix-core 2-wavefront split (2^30: off), paired matrix-core
- * entries (0/1).  No counterpart in the reference. */
+ * entries (0/1), grouped matrix-core entries (0/1).  No counterpart in the
+ * reference. */
 int lsw_dd_autotune(lsw_dd* dd, int nrhs, double* out);
 /* Wait for all work queued on the DD's own stream. */
 int lsw_dd_sync(lsw_dd* dd);
 /* One applyPrecond on device vectors with every batched local-solve launch
  * timed by HIP events on the DD's stream (synchronous; not for timed
- * regions).  out[0..5] = forward: summed ms, launches, algorithmic bytes;
- * backward: summed ms, launches, algorithmic bytes.  No counterpart in the
- * reference: it serves the roofline measurement. */
+ * regions).  out[0..8] = forward: summed ms, launches, algorithmic bytes;
+ * backward: summed ms, launches, algorithmic bytes; forward and backward
+ * algorithmic flops; the summed roofline time in ms (per launch the larger of
+ * its bytes at 8 TB/s and its flops at the matrix-core peak of the solve
+ * precision).  No counterpart in the reference: it serves the roofline
+ * measurement. */
 int lsw_dd_profile_precond(lsw_dd* dd, const void* x, void* y, double* out);
 /* The same for a block of nrhs right-hand sides (x, y: nrhs consecutive
  * global vectors on the device).  No counterpart in the reference. */

@@ -355,6 +355,9 @@ int lsw_dd_profile_precond(lsw_dd* dd, const void* x, void* y, double* out) {
     out[3] = prof.bwdMs;
     out[4] = prof.bwdLaunches;
     out[5] = prof.bwdBytes;
+    out[6] = prof.fwdFlops;
+    out[7] = prof.bwdFlops;
+    out[8] = prof.idealMs;
   });
 }
 
@@ -370,6 +373,9 @@ int lsw_dd_profile_precond_block(lsw_dd* dd, int nrhs, const void* x, void* y, d
     out[3] = prof.bwdMs;
     out[4] = prof.bwdLaunches;
     out[5] = prof.bwdBytes;
+    out[6] = prof.fwdFlops;
+    out[7] = prof.bwdFlops;
+    out[8] = prof.idealMs;
   });
 }
 
@@ -394,6 +400,7 @@ int lsw_dd_autotune(lsw_dd* dd, int nrhs, double* out) {
     out[7] = (double)r.shape.latencyRows;
     out[8] = (double)r.shape.mfmaWideCols;
     out[9] = r.shape.mfmaPair ? 1.0 : 0.0;
+    out[10] = r.shape.mfmaGroup ? 1.0 : 0.0;
   });
 }
 

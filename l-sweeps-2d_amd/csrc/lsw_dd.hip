@@ -1515,6 +1515,15 @@ DDCheckerboard::TuneResult DDCheckerboard::autotune(int nrhs) const {
     c = best;
     c.mfmaPair = !c.mfmaPair;
     consider(c);
+    c = best;
+    c.mfmaGroup = !c.mfmaGroup;
+    consider(c);
+    if (best.mfmaGroup)
+      for (int rows : {0, 16, 32}) {
+        c = best;
+        c.grpMinRowsFwd = c.grpMinRowsBwd = rows;
+        consider(c);
+      }
   }
   mShapes[nrhs] = best;
   dropPlans(nrhs);

@@ -37,7 +37,16 @@ struct SolveProfile {
   double fwdMs = 0, bwdMs = 0;        // summed launch durations
   int fwdLaunches = 0, bwdLaunches = 0;
   double fwdBytes = 0, bwdBytes = 0;  // summed algorithmic bytes
+  double fwdFlops = 0, bwdFlops = 0;  // summed algorithmic flops (real)
+  // summed roofline times of the launches: each launch's larger of its bytes
+  // at the HBM peak and its flops at the matrix-core peak of its precision
+  double idealMs = 0;
 };
+// MI355X peaks the roofline times use: HBM3E bandwidth and dense
+// matrix-core FP32 / FP64 rates
+constexpr double kPeakHbmBytesPerS = 8.0e12;
+constexpr double kPeakMfmaFp32 = 157.3e12;
+constexpr double kPeakMfmaFp64 = 78.6e12;
 
 struct Rect {
   int i0, i1, j0, j1;
@@ -60,6 +69,19 @@ struct SolveTaskDev {
   int nrhs;
   int slot[kMaxRhs];
   int pad;
+};
+
+// One group of a grouped matrix-core launch: the entries of one node row
+// chunk of up to four tasks of one cell, with everything the kernel needs
+// in one record (one load).
+struct GroupRec {
+  SolveNode nd;
+  long long facBase;
+  int r0rows;  // r0 | rows << 16
+  int ntasks;
+  int task[4];
+  int flags[4];
+  int pad[2];
 };
 
 // One work item: rows [r0, r0+rows) of one node of one task.
@@ -87,6 +109,12 @@ struct PlanShape {
                                   // them over 2 wavefronts on the matrix cores even when there are many
   bool mfmaPair = true;           // packed matrix-core launches: two tasks' entries of the same node
                                   // rows share a wavefront and its factor loads
+  bool mfmaGroup = true;          // packed matrix-core launches as groups: the entries of one node row
+                                  // chunk of up to 4 tasks in one wavefront, columns in pipelined chunks,
+  int grpMinRowsFwd = 32;         // ... for launches whose entries have more rows than this (forward)
+  int grpMinRowsBwd = 8;          // ... (backward); fewer rows: the paired packed kernel
+  bool grpLatency = true;        // ... also for the latency-bound launches of short rows (instead of
+                                  // splitting their columns)
   static PlanShape defaults();
 };
 
@@ -105,8 +133,11 @@ struct SolvePlan {
     int wpb = 1;
     int waves = 1;
     int pair = 0;       // matrix-core packed: entries paired by factor rows (1: RP 16, 2: RP 32)
+    long long goff = 0; // mode 4 (grouped matrix-core): the launch's groups in the plan's group list
+    int ngroups = 0;
     int cols = 0;       // largest column count (sizes the LDS)
     double bytes = 0;   // algorithmic bytes
+    double flops = 0;   // algorithmic real flops (8 per complex multiply-add)
   };
   int ntasks = 0, maxNrhs = 1;
   int width = 1;            // right-hand sides per task in the kernels (taskWidth(maxNrhs))
@@ -120,16 +151,18 @@ struct SolvePlan {
   std::vector<SolveTask> hTasks;
   SolveTaskDev* dTasks = nullptr;  // these three live in dBlob
   SolveEntry* dEntries = nullptr;
+  GroupRec* dGroups = nullptr;  // the groups of mode-4 launches
   void* dBlob = nullptr;
   // hostOnly: buildPlan keeps the device image of tasks, entries and slot
   // map in hBlob and allocates nothing; bind() points the plan at a copy of
   // hBlob that the caller placed in device memory
   bool hostOnly = false;
   std::vector<unsigned char> hBlob;
-  size_t oEnt = 0, oTQ = 0;
+  size_t oEnt = 0, oTQ = 0, oGrp = 0;
   void bind(void* base);
   std::vector<Launch> fwd, bwd;    // in launch order
   double fwdBytes = 0, bwdBytes = 0;
+  double fwdFlops = 0, bwdFlops = 0;
   SolvePlan() = default;
   SolvePlan(const SolvePlan&) = delete;
   SolvePlan& operator=(const SolvePlan&) = delete;
@@ -223,6 +256,7 @@ class FactorSet {
   ShapeDev* dShapes = nullptr;
   // flattened solve descriptors
   std::vector<int> mShapeNodeBase;
+  std::vector<SolveNode> mSolveNodes;  // host copy
   SolveNode* dSolveNodes = nullptr;
   int* dPoolE = nullptr;
   int* dPoolB = nullptr;

@@ -804,6 +804,7 @@ void FactorSet::setup(hipStream_t s) {
       }
     }
     dSolveNodes = mArena.put(flat);
+    mSolveNodes = flat;
     dPoolE = mArena.put(pe);
     dPoolB = mArena.put(pb);
     dPoolMap = mArena.put(pm);

@@ -862,6 +862,269 @@ __global__ __launch_bounds__(64 * W) void k_nd_mfma_wide(const SolveTaskDev* __r
   else
     bwdStore<float2, NR>(it, storer, dof, res, Xo, xs);
 }
+
+// ------------------------------------------------------------------ grouped
+// Matrix-core solves of FP32 tasks of 8 right-hand sides with the entries of
+// one (cell, node, row chunk) of up to kGrpTasks tasks in one wavefront (a
+// group): the column indices, the children's maps and the factor tiles are
+// loaded once for all of them, and every factor operand feeds the products
+// of every task.  The columns are walked in chunks of kGrpCols: the gather of
+// chunk c+1 (index loads one chunk earlier still) and its factor operands are
+// in flight while the products of chunk c run, from LDS staging that holds
+// the chunk of every task (double-buffered).  The products are moved to one
+// row per lane through LDS for the stores, one task at a time (a forward
+// row's children's updates for the next task are loaded meanwhile).
+constexpr int kGrpTasks = 4;
+constexpr int kGrpCols = 16;
+constexpr int kGrpWaves = 4;  // wavefronts per workgroup
+// LDS per wavefront (float2 words): two staging buffers of kGrpTasks x
+// kGrpCols x 8, reused after the columns for the 64 x kMfmaRow float rows
+constexpr int kGrpStage = kGrpTasks * kGrpCols * 8;
+constexpr int kGrpLds = (2 * kGrpStage > 64 * kMfmaRow / 2) ? 2 * kGrpStage : 64 * kMfmaRow / 2;
+
+// float offset of component li (0..7: real parts, 8..15: imaginary parts of
+// the 8 right-hand sides) in a row of 8 interleaved complex values
+__device__ __forceinline__ int grpComp(int li) { return li < 8 ? 2 * li : 2 * (li - 8) + 1; }
+
+template <bool FWD>
+__global__ __launch_bounds__(64 * kGrpWaves) void k_nd_mfma_grp(
+    const GroupRec* __restrict__ grp, int ngrp, const int* __restrict__ poolE, const int* __restrict__ poolB,
+    const int* __restrict__ poolMap, const float2* __restrict__ factor, const float2* X, float2* Xo,
+    float2* __restrict__ Rb, float2* __restrict__ G, long long xs, long long gs) {
+  constexpr int NR = 8, TM = kGrpTasks, KC = kGrpCols;
+  __shared__ float2 lds[kGrpWaves][kGrpLds];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const long long gid = (long long)xcdRemap(blockIdx.x, gridDim.x) * kGrpWaves + wave;
+  if (gid >= ngrp) return;  // this wavefront only (no workgroup barriers below)
+  float2* st = lds[wave];
+  // everything about the group in one record
+  const GroupRec& gr = grp[gid];
+  const SolveNode nd = gr.nd;
+  const int T = gr.ntasks;
+  const int r0 = gr.r0rows & 0xffff, rows = gr.r0rows >> 16;
+  const int e = nd.e, F = e + nd.b;
+  int tTask[TM], tFlags[TM];
+  int anyVis = 0;
+#pragma unroll
+  for (int t = 0; t < TM; ++t) {
+    tTask[t] = gr.task[t];
+    tFlags[t] = gr.flags[t];
+    anyVis |= tFlags[t] & (kEntVisited | kEntRootRhs);
+  }
+  const long long facBase = gr.facBase;  // one cell
+  // columns: forward r_E (e of them); backward [r_E; x_B], from e on when no
+  // task has r_E there
+  const int kb = FWD ? 0 : (anyVis ? 0 : e);
+  const int ke = FWD ? e : F;
+  const int nt = (rows + 15) >> 4;  // 16-row tiles with rows
+  const int li = lane & 15, lk = lane >> 4;
+  const bool lo = li < 8;
+  const long long ld = FWD ? nd.b : e;
+  const float2* M0 = factor + facBase + (FWD ? nd.fwdOff : nd.bwdOff) + r0;
+  const float2* tM[4];
+#pragma unroll
+  for (int tl = 0; tl < 4; ++tl) {
+    const int row = 16 * tl + li;
+    tM[tl] = M0 + (row < rows ? row : 0);
+  }
+  // this lane's gather pair of every chunk: column kk of task tg
+  const int kk = lane & (KC - 1), tg = lane / KC;
+  const bool tgOk = tg < T;
+  const int myTask = tg == 0 ? tTask[0] : tg == 1 ? tTask[1] : tg == 2 ? tTask[2] : tTask[3];
+  const int myFlags = tg == 0 ? tFlags[0] : tg == 1 ? tFlags[1] : tg == 2 ? tFlags[2] : tFlags[3];
+  const bool myRoot = !FWD && (myFlags & kEntRootRhs);
+  const bool myVis = FWD || (myFlags & (kEntVisited | kEntRootRhs));
+  const bool myC1 = (myFlags & kEntChild1) != 0, myC2 = (myFlags & kEntChild2) != 0;
+  const bool keepR = FWD && r0 == 0;
+  // children's maps are read for a forward gather, or a backward one of a
+  // root forming its r_E (uniform)
+  const bool kidsG = waveAny(tgOk && (FWD || myRoot) && (myC1 || myC2));
+  const int* Ep = poolE + nd.E;
+  const int* Bp = poolB + nd.B;
+  const int* m1 = poolMap + nd.map;
+
+  struct Idx {
+    int dof, a1, a2;
+    bool act;
+  };
+  auto loadIdx = [&](int kbase) __attribute__((always_inline)) {
+    Idx ix;
+    const int k = kbase + kk;
+    ix.act = tgOk && k < ke;
+    const int kc = k < ke ? k : kb;  // a valid column (kb < ke inside the loop)
+    ix.dof = kc < e ? Ep[kc] : Bp[kc - e];
+    ix.a1 = -1;
+    ix.a2 = -1;
+    if (kidsG) {
+      const int km = kc < e ? kc : 0;
+      ix.a1 = m1[km];
+      ix.a2 = m1[F + km];
+      if (kc >= e) ix.a1 = ix.a2 = -1;
+    }
+    // a pair that is past the columns, or a backward r_E column of a task
+    // without r_E, stages zero
+    if (!FWD && kc < e && !myVis) ix.act = false;
+    return ix;
+  };
+  struct Val {
+    float2 x[NR], g1[NR], g2[NR];
+    bool h1, h2, r;
+  };
+  auto loadVal = [&](const Idx& ix, int kbase, Val& v) __attribute__((always_inline)) {
+    const int k = kbase + kk;
+    const bool isR = k < e;  // an r_E column (past ke: any, masked)
+    // forward, or a backward root: X; backward r_E: the kept R; x_B: Xo
+    // (selected as integers: a select of pointers went through scratch)
+    unsigned long long sa = (unsigned long long)X;
+    if (!FWD) {
+      sa = (isR && !myRoot) ? (unsigned long long)Rb : sa;
+      sa = !isR ? (unsigned long long)Xo : sa;
+    }
+    const float2* src = reinterpret_cast<const float2*>(sa);
+    loadNR<float2, NR>(src + ((long long)myTask * xs + ix.dof) * NR, v.x);
+    v.h1 = (FWD || myRoot) && isR && myC1 && ix.a1 >= 0;
+    v.h2 = (FWD || myRoot) && isR && myC2 && ix.a2 >= 0;
+    v.r = ix.act;
+    if (kidsG) {
+      loadNR<float2, NR>(G + ((long long)myTask * gs + (v.h1 ? nd.g1 + ix.a1 : 0)) * NR, v.g1);
+      loadNR<float2, NR>(G + ((long long)myTask * gs + (v.h2 ? nd.g2 + ix.a2 : 0)) * NR, v.g2);
+    }
+  };
+  auto putVal = [&](const Idx& ix, const Val& v, float2* buf) __attribute__((always_inline)) {
+    float2 o[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      float2 w = v.x[q];
+      if (kidsG) w = csub(csub(w, cmask(v.g1[q], v.h1)), cmask(v.g2[q], v.h2));
+      o[q] = cmask(w, v.r);
+    }
+    // every lane writes its pair (a task past T writes its own, unread region)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) buf[(tg * KC + kk) * NR + q] = o[q];
+    if (keepR && v.r) storeNR<float2, NR>(Rb + ((long long)myTask * xs + ix.dof) * NR, o);
+  };
+  auto loadA = [&](int kbase, float2 (&a)[4][4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+      const int col = kbase + 4 * s4 + lk;
+      const long long cc = col < ke ? col : kb;
+#pragma unroll
+      for (int tl = 0; tl < 4; ++tl)
+        if (tl < nt) a[s4][tl] = tM[tl][cc * ld];  // (nt is uniform)
+    }
+  };
+
+  // the stores: one row per lane (row `lane` of the chunk); their
+  // destinations (backward) or the positions of the children's updates
+  // they add (forward) are loaded now, while the products run
+  const bool ok = lane < rows;
+  float* cb = reinterpret_cast<float*>(st);
+  int dof = 0, ra1 = -1, ra2 = -1;
+  if (nt > 0) {
+    if constexpr (FWD) {
+      const int f = e + r0 + (ok ? lane : 0);
+      ra1 = m1[f];
+      ra2 = m1[F + f];
+    } else {
+      dof = Ep[r0 + (ok ? lane : 0)];
+    }
+  }
+  auto loadAdd = [&](int t, float2 (&c1)[NR], float2 (&c2)[NR], bool& h1, bool& h2) __attribute__((always_inline)) {
+    const int fl = tFlags[t];
+    h1 = ok && (fl & kEntChild1) && ra1 >= 0;
+    h2 = ok && (fl & kEntChild2) && ra2 >= 0;
+    loadNR<float2, NR>(G + ((long long)tTask[t] * gs + (h1 ? nd.g1 + ra1 : 0)) * NR, c1);
+    loadNR<float2, NR>(G + ((long long)tTask[t] * gs + (h2 ? nd.g2 + ra2 : 0)) * NR, c2);
+  };
+  lswF4 acc[TM][4];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int tl = 0; tl < 4; ++tl) acc[t][tl] = lswF4{0.f, 0.f, 0.f, 0.f};
+
+  if (kb < ke) {
+    Idx ixC = loadIdx(kb);
+    Val vC;
+    loadVal(ixC, kb, vC);
+    Idx ixN = loadIdx(kb + KC);
+    float2 aC[4][4], aN[4][4];
+    loadA(kb, aC);
+    int bi = 0;
+    for (int kbase = kb;; kbase += KC) {
+      const bool more = kbase + KC < ke;
+      putVal(ixC, vC, st + bi * kGrpStage);
+      if (more) {
+        loadVal(ixN, kbase + KC, vC);
+        ixC = ixN;
+        ixN = loadIdx(kbase + 2 * KC);
+        loadA(kbase + KC, aN);
+      }
+      waveSync();
+      const float2* buf = st + bi * kGrpStage;
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+#pragma unroll
+        for (int t = 0; t < TM; ++t) {
+          if (t >= T) break;
+          const float2 w = buf[(t * KC + 4 * s4 + lk) * NR + (li & 7)];
+          const float b1 = lo ? w.x : w.y;
+          const float b2 = lo ? -w.y : w.x;
+#pragma unroll
+          for (int tl = 0; tl < 4; ++tl) {
+            if (tl >= nt) break;
+            acc[t][tl] = __builtin_amdgcn_mfma_f32_16x16x4f32(aC[s4][tl].x, b1, acc[t][tl], 0, 0, 0);
+            acc[t][tl] = __builtin_amdgcn_mfma_f32_16x16x4f32(aC[s4][tl].y, b2, acc[t][tl], 0, 0, 0);
+          }
+        }
+      }
+      if (!more) break;
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+        for (int tl = 0; tl < 4; ++tl) aC[s4][tl] = aN[s4][tl];
+      bi ^= 1;
+    }
+  }
+  if (nt == 0) return;  // a forward root entry only keeps r_E
+  waveSync();  // the staging is reused for the rows below
+  float2 c1C[NR], c2C[NR];
+  bool h1C = false, h2C = false;
+  if constexpr (FWD) loadAdd(0, c1C, c2C, h1C, h2C);
+#pragma unroll
+  for (int t = 0; t < TM; ++t) {
+    if (t >= T) break;
+    float2 c1N[NR], c2N[NR];
+    bool h1N = false, h2N = false;
+    if constexpr (FWD) {
+      if (t + 1 < T) loadAdd(t + 1, c1N, c2N, h1N, h2N);
+    }
+#pragma unroll
+    for (int tl = 0; tl < 4; ++tl)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) cb[(16 * tl + 4 * lk + rr) * kMfmaRow + li] = acc[t][tl][rr];
+    waveSync();
+    float2 res[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      res[q] = make_float2(cb[lane * kMfmaRow + q], cb[lane * kMfmaRow + 8 + q]);
+      if constexpr (FWD) res[q] = cadd(res[q], cadd(cmask(c1C[q], h1C), cmask(c2C[q], h2C)));
+    }
+    waveSync();  // the rows are read before the next task's are written
+    if constexpr (FWD) {
+      float2* dst = ok ? G + ((long long)tTask[t] * gs + nd.gOff + r0 + lane) * NR : trashRow<float2, NR>();
+      storeNR<float2, NR>(dst, res);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        c1C[q] = c1N[q];
+        c2C[q] = c2N[q];
+      }
+      h1C = h1N;
+      h2C = h2N;
+    } else {
+      storeNR<float2, NR>(ok ? Xo + ((long long)tTask[t] * xs + dof) * NR : trashRow<float2, NR>(), res);
+    }
+  }
+}
 }  // namespace
 
 // ------------------------------------------------------------------ plans
@@ -896,6 +1159,10 @@ PlanShape PlanShape::defaults() {
     if (const char* e = std::getenv("LSW_SOLVE_MFMA")) p.mfma = std::atoi(e) != 0;
     if (const char* e = std::getenv("LSW_MFMA_WIDE_COLS")) p.mfmaWideCols = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LSW_MFMA_PAIR")) p.mfmaPair = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LSW_MFMA_GROUP")) p.mfmaGroup = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LSW_GRP_ROWS_FWD")) p.grpMinRowsFwd = std::atoi(e);
+    if (const char* e = std::getenv("LSW_GRP_ROWS_BWD")) p.grpMinRowsBwd = std::atoi(e);
+    if (const char* e = std::getenv("LSW_GRP_LATENCY")) p.grpLatency = std::atoi(e) != 0;
     return p;
   }();
   return d;
@@ -922,6 +1189,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   std::vector<int> frows(mMaxHeight + 1, 0), brows(mMaxHeight + 1, 0);
   std::vector<double> fbytes(mMaxHeight + 1, 0.0), bbytes(mMaxHeight + 1, 0.0);
   std::vector<double> fwork(mMaxHeight + 1, 0.0), bwork(mMaxHeight + 1, 0.0);
+  std::vector<double> fflops(mMaxHeight + 1, 0.0), bflops(mMaxHeight + 1, 0.0);
   std::vector<char> on, need;
   // Algorithmic bytes: the right-hand-side and update vectors of every task,
   // and each factor block once per launch however many tasks of its cell
@@ -986,6 +1254,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
         fcols[h] = std::max(fcols[h], nd.e);
         fbytes[h] += facBytes(tk.cell, p, true, S * b * e) + S * r * (2 * e + 2 * b);
         fwork[h] += b * e + e;
+        fflops[h] += 8.0 * r * b * e;
       }
       if (!need[p]) continue;
       for (int r0 = 0; r0 < nd.e; r0 += kSolveRows) {
@@ -997,6 +1266,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       const double kcols = on[p] ? e + b : b;
       bbytes[h] += facBytes(tk.cell, p, false, S * e * kcols) + S * r * (kcols + e);
       bwork[h] += e * kcols;
+      bflops[h] += 8.0 * r * e * kcols;
     }
   }
   plan.width = taskWidth(plan.maxNrhs);
@@ -1016,12 +1286,14 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   for (auto& v : fw) std::stable_sort(v.begin(), v.end(), byNode);
   for (auto& v : bw) std::stable_sort(v.begin(), v.end(), byNode);
   std::vector<SolveEntry> all;
+  std::vector<GroupRec> allGrp;
   auto addLaunch = [&](std::vector<SolvePlan::Launch>& out, const std::vector<SolveEntry>& v, int cols, int maxRows,
-                       double bytes, double work) {
+                       double bytes, double work, double flops, bool fwdLaunch) {
     if (v.empty()) return;
     SolvePlan::Launch L;
     L.off = (long long)all.size();
     L.n = (int)v.size();
+    L.flops = flops;
     // staging columns: the front's widest r_E / [r_E; x_B] plus a zero pad
     const int stage = cols + 1;
     L.cols = stage;
@@ -1035,7 +1307,8 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     while (rp < 64 && ldsWave(rp) > kMaxLds) rp *= 2;
     const long long packedWaves = ((long long)v.size() * rp + 63) / 64;
     int W = 0;
-    if (packedWaves < shp.latencyWaves && maxRows > 0 && maxRows <= shp.latencyRows) {
+    const bool latency = packedWaves < shp.latencyWaves && maxRows > 0 && maxRows <= shp.latencyRows;
+    if (latency) {
       // Few wavefronts of short rows: the launch is a chain of memory round
       // trips, and a packed lane walks its whole row while most lanes of an
       // entry's wavefront share its few rows.  Split the columns instead (W
@@ -1069,7 +1342,31 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     auto ldsMfmaWide = [&](int w) { return std::max(S * nrk * stage, 4.0 * w * 64 * kMfmaRow); };
     const std::vector<SolveEntry>* ev = &v;
     std::vector<SolveEntry> paired;
-    if (W == 0 && S == 8.0 && nrk == kMaxRhs && shp.mfma && ldsMfma(1) <= kMaxLds) {
+    if ((W == 0 || (latency && shp.grpLatency)) && S == 8.0 && nrk == kMaxRhs && shp.mfma && shp.mfmaGroup &&
+        maxRows > (fwdLaunch ? shp.grpMinRowsFwd : shp.grpMinRowsBwd)) {
+      // groups: the entries of one (cell, node, row chunk), kGrpTasks at most
+      L.mode = 4;
+      L.goff = (long long)allGrp.size();
+      size_t i = 0;
+      while (i < v.size()) {
+        size_t j = i + 1;
+        while (j < v.size() && j - i < (size_t)kGrpTasks && !byNode(v[i], v[j]) && v[i].r0rows == v[j].r0rows) ++j;
+        GroupRec g;
+        std::memset(&g, 0, sizeof(g));
+        g.nd = mSolveNodes[v[i].node];
+        g.facBase = tdev[v[i].task].facBase;
+        g.r0rows = v[i].r0rows;
+        g.ntasks = (int)(j - i);
+        for (int t = 0; t < kGrpTasks; ++t) {
+          const SolveEntry& en = v[i + std::min<size_t>(t, j - i - 1)];
+          g.task[t] = en.task;
+          g.flags[t] = t < g.ntasks ? en.flags : 0;
+        }
+        allGrp.push_back(g);
+        i = j;
+      }
+      L.ngroups = (int)(allGrp.size() - L.goff);
+    } else if (W == 0 && S == 8.0 && nrk == kMaxRhs && shp.mfma && ldsMfma(1) <= kMaxLds) {
       L.mode = 2;
       L.rp = rpm;
       // Entries of two tasks of one cell for the same node rows read the same
@@ -1131,6 +1428,8 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   plan.bwd.clear();
   plan.fwdBytes = 0;
   plan.bwdBytes = 0;
+  plan.fwdFlops = 0;
+  plan.bwdFlops = 0;
   // The forward pass only feeds the backward pass: with no solution value
   // read (sweep solves whose traces nobody uses), there is nothing to do.
   bool anyBwd = false;
@@ -1138,14 +1437,17 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   if (!anyBwd) {
     for (auto& v : fw) v.clear();
     std::fill(fbytes.begin(), fbytes.end(), 0.0);
+    std::fill(fflops.begin(), fflops.end(), 0.0);
   }
   for (int h = 0; h <= mMaxHeight; ++h) {
-    addLaunch(plan.fwd, fw[h], fcols[h], frows[h], fbytes[h], fwork[h]);
+    addLaunch(plan.fwd, fw[h], fcols[h], frows[h], fbytes[h], fwork[h], fflops[h], true);
     plan.fwdBytes += fbytes[h];
+    if (!fw[h].empty()) plan.fwdFlops += fflops[h];
   }
   for (int h = mMaxHeight; h >= 0; --h) {
-    addLaunch(plan.bwd, bw[h], bcols[h], brows[h], bbytes[h], bwork[h]);
+    addLaunch(plan.bwd, bw[h], bcols[h], brows[h], bbytes[h], bwork[h], bflops[h], false);
     plan.bwdBytes += bbytes[h];
+    if (!bw[h].empty()) plan.bwdFlops += bflops[h];
   }
   if (std::getenv("LSW_PLAN_LOG")) {
     for (int pass = 0; pass < 2; ++pass)
@@ -1169,14 +1471,17 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   // served problem builds plans for many block widths)
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t bTasks = tdev.size() * sizeof(SolveTaskDev), bEnt = all.size() * sizeof(SolveEntry);
-  const size_t bTQ = plan.slotTQ.size() * sizeof(int);
-  const size_t oEnt = up(bTasks), oTQ = oEnt + up(std::max<size_t>(bEnt, 1)), total = oTQ + up(bTQ);
+  const size_t bTQ = plan.slotTQ.size() * sizeof(int), bGrp = allGrp.size() * sizeof(GroupRec);
+  const size_t oEnt = up(bTasks), oTQ = oEnt + up(std::max<size_t>(bEnt, 1)), oGrp = oTQ + up(bTQ);
+  const size_t total = oGrp + up(std::max<size_t>(bGrp, 1));
   std::vector<unsigned char> blob(total, 0);
   std::memcpy(blob.data(), tdev.data(), bTasks);
   if (bEnt) std::memcpy(blob.data() + oEnt, all.data(), bEnt);
   std::memcpy(blob.data() + oTQ, plan.slotTQ.data(), bTQ);
+  if (bGrp) std::memcpy(blob.data() + oGrp, allGrp.data(), bGrp);
   plan.oEnt = oEnt;
   plan.oTQ = oTQ;
+  plan.oGrp = oGrp;
   if (plan.hostOnly) {
     plan.hBlob = std::move(blob);
     return;
@@ -1191,6 +1496,7 @@ void SolvePlan::bind(void* base) {
   dTasks = reinterpret_cast<SolveTaskDev*>(b);
   dEntries = reinterpret_cast<SolveEntry*>(b + oEnt);
   dSlotTQ = reinterpret_cast<int*>(b + oTQ);
+  dGroups = reinterpret_cast<GroupRec*>(b + oGrp);
 }
 
 namespace {
@@ -1199,7 +1505,8 @@ struct LaunchTimer {
   hipStream_t s;
   std::vector<hipEvent_t> ev;
   std::vector<int> kind;
-  std::vector<double> bytes;
+  std::vector<double> bytes, flops;
+  double flopPeak = kPeakMfmaFp32;
   std::vector<const SolvePlan::Launch*> what;
   void begin() {
     if (!prof) return;
@@ -1208,7 +1515,7 @@ struct LaunchTimer {
     LSW_HIP(hipEventRecord(e, s));
     ev.push_back(e);
   }
-  void end(int k, double b) {
+  void end(int k, double b, double f) {
     if (!prof) return;
     hipEvent_t e;
     LSW_HIP(hipEventCreate(&e));
@@ -1216,6 +1523,7 @@ struct LaunchTimer {
     ev.push_back(e);
     kind.push_back(k);
     bytes.push_back(b);
+    flops.push_back(f);
   }
   void finish() {
     if (!prof) return;
@@ -1225,18 +1533,23 @@ struct LaunchTimer {
       LSW_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
       if (std::getenv("LSW_PROFILE_LOG") && i < what.size()) {
         const auto& L = *what[i];
-        std::fprintf(stderr, "launch %s %zu: entries %d mode %d rp %d wpb %d waves %d cols %d bytes %.0f us %.2f GB/s %.0f\n",
+        std::fprintf(stderr, "launch %s %zu: entries %d mode %d rp %d wpb %d waves %d cols %d bytes %.0f us %.2f GB/s %.0f "
+                     "flops %.0f ideal_us %.2f\n",
                      kind[i] ? "bwd" : "fwd", i, L.n, L.mode, L.rp, L.wpb, L.waves, L.cols, L.bytes, 1e3 * ms,
-                     ms > 0 ? L.bytes / (ms * 1e6) : 0.0);
+                     ms > 0 ? L.bytes / (ms * 1e6) : 0.0, L.flops,
+                     1e6 * std::max(L.bytes / kPeakHbmBytesPerS, L.flops / flopPeak));
       }
+      prof->idealMs += 1e3 * std::max(bytes[i] / kPeakHbmBytesPerS, flops[i] / flopPeak);
       if (kind[i] == 0) {
         prof->fwdMs += ms;
         prof->fwdLaunches++;
         prof->fwdBytes += bytes[i];
+        prof->fwdFlops += flops[i];
       } else {
         prof->bwdMs += ms;
         prof->bwdLaunches++;
         prof->bwdBytes += bytes[i];
+        prof->bwdFlops += flops[i];
       }
     }
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -1247,6 +1560,7 @@ struct SolveArgs {
   const SolveTaskDev* tasks;
   const SolveNode* nodes;
   const SolveEntry* ent;
+  const GroupRec* grp;
   const int *poolE, *poolB, *poolMap;
   long long xs, gs;
 };
@@ -1255,6 +1569,16 @@ template <typename T2, int NR, bool FWD>
 void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, const T2* X, T2* Xo, T2* R, T2* G,
             hipStream_t s) {
   const SolveEntry* ent = a.ent + L.off;
+  if (L.mode == 4) {
+    if constexpr (std::is_same<T2, float2>::value && NR == kMaxRhs) {
+      const unsigned grid = (unsigned)((L.ngroups + kGrpWaves - 1) / kGrpWaves);
+      hipLaunchKernelGGL((k_nd_mfma_grp<FWD>), dim3(grid), dim3(64 * kGrpWaves), 0, s, a.grp + L.goff, L.ngroups,
+                         a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+    } else {
+      LSW_REQUIRE(false, LSW_ERR_STATE, "grouped matrix-core solve launch for a non-FP32 or narrow task plan");
+    }
+    return;
+  }
   if (L.mode == 2) {
     if constexpr (std::is_same<T2, float2>::value && NR == kMaxRhs) {
       const int P = 64 / L.rp;
@@ -1349,16 +1673,17 @@ template <typename T2, int NR>
 void runPlan(const SolvePlan& plan, const SolveArgs& a, const T2* factor, const T2* X, T2* Xo, T2* R, T2* G,
              hipStream_t s, SolveProfile* prof) {
   LaunchTimer tm{prof, s, {}, {}, {}};
+  tm.flopPeak = std::is_same<T2, double2>::value ? kPeakMfmaFp64 : kPeakMfmaFp32;
   for (const auto& L : plan.fwd) {
     tm.begin();
     launch<T2, NR, true>(a, L, factor, X, Xo, R, G, s);
-    tm.end(0, L.bytes);
+    tm.end(0, L.bytes, L.flops);
     if (prof) tm.what.push_back(&L);
   }
   for (const auto& L : plan.bwd) {
     tm.begin();
     launch<T2, NR, false>(a, L, factor, X, Xo, R, G, s);
-    tm.end(1, L.bytes);
+    tm.end(1, L.bytes, L.flops);
     if (prof) tm.what.push_back(&L);
   }
   LSW_HIP(hipGetLastError());
@@ -1376,7 +1701,7 @@ void FactorSet::solve(const SolvePlan& plan, const void* X, void* Xo, void* R, v
   LSW_REQUIRE(mFactorized, LSW_ERR_STATE, "solve called before factorize");
   LSW_REQUIRE(Xo != X || !plan.outOfPlace, LSW_ERR_ARG, "plan built for separate right-hand side and solution buffers");
   if (plan.ntasks <= 0) return;
-  const SolveArgs a{plan.dTasks, dSolveNodes, plan.dEntries, dPoolE, dPoolB, dPoolMap, xs, gs};
+  const SolveArgs a{plan.dTasks, dSolveNodes, plan.dEntries, plan.dGroups, dPoolE, dPoolB, dPoolMap, xs, gs};
   if (mPrecision == LSW_FP64) {
     const auto* x = (const double2*)X;
     auto* xo = (double2*)Xo;

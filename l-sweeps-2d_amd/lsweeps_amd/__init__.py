@@ -355,12 +355,12 @@ class DDCheckerboard:
     def autotune(self, nrhs=1):
         """Time M^{-1} for blocks of nrhs right-hand sides under alternative
         launch shapes and keep the fastest for that block width."""
-        out = np.zeros(10)
+        out = np.zeros(11)
         _check(_lib.lsw_dd_autotune(self._h, int(nrhs), _dptr(out)))
         return {"default_ms": float(out[0]), "best_ms": float(out[1]), "tried": int(out[2]),
                 "latency_waves": int(out[3]), "wide_budget": int(out[4]), "wide_cap": int(out[5]),
                 "mfma": bool(out[6]), "latency_rows": int(out[7]), "mfma_wide_cols": int(out[8]),
-                "mfma_pair": bool(out[9])}
+                "mfma_pair": bool(out[9]), "mfma_group": bool(out[10])}
 
     def write_vtk(self, x, path):
         """<path>.vtk (OutputVTK layout) and <path>.lswgrid (full precision)."""
@@ -471,13 +471,14 @@ class DDCheckerboard:
     def profile_precond_dev(self, x_ptr, y_ptr, nrhs=1):
         """Per-launch HIP-event timing of the batched local solves of one M^{-1}
         (x_ptr, y_ptr: nrhs consecutive global vectors on the device)."""
-        out = np.zeros(6)
+        out = np.zeros(9)
         if nrhs == 1:
             _check(_lib.lsw_dd_profile_precond(self._h, x_ptr, y_ptr, _dptr(out)))
         else:
             _check(_lib.lsw_dd_profile_precond_block(self._h, int(nrhs), x_ptr, y_ptr, _dptr(out)))
         return dict(fwd_ms=out[0], fwd_launches=int(out[1]), fwd_bytes=out[2],
-                    bwd_ms=out[3], bwd_launches=int(out[4]), bwd_bytes=out[5])
+                    bwd_ms=out[3], bwd_launches=int(out[4]), bwd_bytes=out[5],
+                    fwd_flops=out[6], bwd_flops=out[7], ideal_ms=out[8])
 
     def __del__(self):
         # (at interpreter exit the module's globals may already be cleared)
