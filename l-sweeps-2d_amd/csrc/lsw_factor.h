@@ -135,6 +135,7 @@ struct SolvePlan {
     int pair = 0;       // matrix-core packed: entries paired by factor rows (1: RP 16, 2: RP 32)
     long long goff = 0; // mode 4 (grouped matrix-core): the launch's groups in the plan's group list
     int ngroups = 0;
+    bool one = false;   // mode 4 with at most one column chunk per entry
     int cols = 0;       // largest column count (sizes the LDS)
     double bytes = 0;   // algorithmic bytes
     double flops = 0;   // algorithmic real flops (8 per complex multiply-add)

@@ -794,14 +794,6 @@ __global__ __launch_bounds__(64 * W) void k_nd_mfma_wide(const SolveTaskDev* __r
     fwdRowAdd<float2, NR>(it, storer, lane, poolMap, G, gs, add);
   else
     dof = poolE[it.nd.E + (storer ? it.r0 + lane : 0)];
-  if constexpr (FWD)
-    fwdGather<float2, NR>(it, true, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
-  else
-    bwdGather<float2, NR>(it, true, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
-  __syncthreads();
-  lswF4 acc[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = lswF4{0.f, 0.f, 0.f, 0.f};
   const bool lo = li < 8;
   const int nsteps = (K - k0 + 3) / 4;
   const int nw = nsteps > w ? (nsteps - w + W - 1) / W : 0;  // this wavefront's k-steps (uniform)
@@ -816,7 +808,16 @@ __global__ __launch_bounds__(64 * W) void k_nd_mfma_wide(const SolveTaskDev* __r
       for (int t = 0; t < 4; ++t) dst[st][t] = tM[t][(long long)(col < K ? col : 0) * ld];
     }
   };
+  // the first factor operands do not depend on the gather: in flight with it
   loadA(0, ca);
+  if constexpr (FWD)
+    fwdGather<float2, NR>(it, true, threadIdx.x, 64 * W, poolE, poolMap, X, Rb, G, xs, gs, v);
+  else
+    bwdGather<float2, NR>(it, true, k0 + threadIdx.x, 64 * W, poolE, poolB, poolMap, X, Xo, Rb, G, xs, gs, v);
+  __syncthreads();
+  lswF4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = lswF4{0.f, 0.f, 0.f, 0.f};
   for (int s0 = 0; s0 < nw; s0 += KS) {
     const bool more = s0 + KS < nw;
     if (more) loadA(s0 + KS, na);
@@ -880,13 +881,20 @@ constexpr int kGrpWaves = 4;  // wavefronts per workgroup
 // LDS per wavefront (float2 words): two staging buffers of kGrpTasks x
 // kGrpCols x 8, reused after the columns for the 64 x kMfmaRow float rows
 constexpr int kGrpStage = kGrpTasks * kGrpCols * 8;
-constexpr int kGrpLds = (2 * kGrpStage > 64 * kMfmaRow / 2) ? 2 * kGrpStage : 64 * kMfmaRow / 2;
+constexpr int kGrpRows = 64 * kMfmaRow / 2;  // the 64 product rows of one task (float2 words)
+// two staging buffers, the rows reusing them; one staging buffer and the
+// rows beside it (single-chunk launches)
+constexpr int kGrpLds = (2 * kGrpStage > kGrpStage + kGrpRows) ? 2 * kGrpStage : kGrpStage + kGrpRows;
 
 // float offset of component li (0..7: real parts, 8..15: imaginary parts of
 // the 8 right-hand sides) in a row of 8 interleaved complex values
 __device__ __forceinline__ int grpComp(int li) { return li < 8 ? 2 * li : 2 * (li - 8) + 1; }
 
-template <bool FWD>
+// ONE: every entry of the launch has at most kGrpCols columns.  Its columns
+// are staged at once, and the tasks then run one after the other (products,
+// rows, stores) with one task's accumulators, which leaves the registers for
+// twice the wavefronts per SIMD.
+template <bool FWD, bool ONE>
 __global__ __launch_bounds__(64 * kGrpWaves) void k_nd_mfma_grp(
     const GroupRec* __restrict__ grp, int ngrp, const int* __restrict__ poolE, const int* __restrict__ poolB,
     const int* __restrict__ poolMap, const float2* __restrict__ factor, const float2* X, float2* Xo,
@@ -1036,6 +1044,76 @@ __global__ __launch_bounds__(64 * kGrpWaves) void k_nd_mfma_grp(
     loadNR<float2, NR>(G + ((long long)tTask[t] * gs + (h1 ? nd.g1 + ra1 : 0)) * NR, c1);
     loadNR<float2, NR>(G + ((long long)tTask[t] * gs + (h2 ? nd.g2 + ra2 : 0)) * NR, c2);
   };
+  if constexpr (ONE) {
+    if (nt == 0 && kb >= ke) return;
+    float2 a[4][4];
+    if (kb < ke) {
+      const Idx ix = loadIdx(kb);
+      Val v;
+      loadVal(ix, kb, v);
+      loadA(kb, a);
+      putVal(ix, v, st);
+    }
+    if (nt == 0) return;  // a forward root entry only keeps r_E
+    float* rws = reinterpret_cast<float*>(st + kGrpStage);
+    float2 c1C[NR], c2C[NR];
+    bool h1C = false, h2C = false;
+    if constexpr (FWD) loadAdd(0, c1C, c2C, h1C, h2C);
+    waveSync();
+#pragma unroll
+    for (int t = 0; t < TM; ++t) {
+      if (t >= T) break;
+      float2 c1N[NR], c2N[NR];
+      bool h1N = false, h2N = false;
+      if constexpr (FWD) {
+        if (t + 1 < T) loadAdd(t + 1, c1N, c2N, h1N, h2N);
+      }
+      lswF4 acc1[4];
+#pragma unroll
+      for (int tl = 0; tl < 4; ++tl) acc1[tl] = lswF4{0.f, 0.f, 0.f, 0.f};
+      if (kb < ke) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const float2 w = st[(t * KC + 4 * s4 + lk) * NR + (li & 7)];
+          const float b1 = lo ? w.x : w.y;
+          const float b2 = lo ? -w.y : w.x;
+#pragma unroll
+          for (int tl = 0; tl < 4; ++tl) {
+            if (tl >= nt) break;
+            acc1[tl] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4][tl].x, b1, acc1[tl], 0, 0, 0);
+            acc1[tl] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4][tl].y, b2, acc1[tl], 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int tl = 0; tl < 4; ++tl)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) rws[(16 * tl + 4 * lk + rr) * kMfmaRow + li] = acc1[tl][rr];
+      waveSync();
+      float2 res[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        res[q] = make_float2(rws[lane * kMfmaRow + q], rws[lane * kMfmaRow + 8 + q]);
+        if constexpr (FWD) res[q] = cadd(res[q], cadd(cmask(c1C[q], h1C), cmask(c2C[q], h2C)));
+      }
+      waveSync();  // the rows are read before the next task's are written
+      if constexpr (FWD) {
+        float2* dst = ok ? G + ((long long)tTask[t] * gs + nd.gOff + r0 + lane) * NR : trashRow<float2, NR>();
+        storeNR<float2, NR>(dst, res);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          c1C[q] = c1N[q];
+          c2C[q] = c2N[q];
+        }
+        h1C = h1N;
+        h2C = h2N;
+      } else {
+        storeNR<float2, NR>(ok ? Xo + ((long long)tTask[t] * xs + dof) * NR : trashRow<float2, NR>(), res);
+      }
+    }
+    return;
+  }
+
   lswF4 acc[TM][4];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
@@ -1366,6 +1444,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
         i = j;
       }
       L.ngroups = (int)(allGrp.size() - L.goff);
+      L.one = cols <= kGrpCols;
     } else if (W == 0 && S == 8.0 && nrk == kMaxRhs && shp.mfma && ldsMfma(1) <= kMaxLds) {
       L.mode = 2;
       L.rp = rpm;
@@ -1572,8 +1651,12 @@ void launch(const SolveArgs& a, const SolvePlan::Launch& L, const T2* factor, co
   if (L.mode == 4) {
     if constexpr (std::is_same<T2, float2>::value && NR == kMaxRhs) {
       const unsigned grid = (unsigned)((L.ngroups + kGrpWaves - 1) / kGrpWaves);
-      hipLaunchKernelGGL((k_nd_mfma_grp<FWD>), dim3(grid), dim3(64 * kGrpWaves), 0, s, a.grp + L.goff, L.ngroups,
-                         a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+      if (L.one)
+        hipLaunchKernelGGL((k_nd_mfma_grp<FWD, true>), dim3(grid), dim3(64 * kGrpWaves), 0, s, a.grp + L.goff,
+                           L.ngroups, a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
+      else
+        hipLaunchKernelGGL((k_nd_mfma_grp<FWD, false>), dim3(grid), dim3(64 * kGrpWaves), 0, s, a.grp + L.goff,
+                           L.ngroups, a.poolE, a.poolB, a.poolMap, factor, X, Xo, R, G, a.xs, a.gs);
     } else {
       LSW_REQUIRE(false, LSW_ERR_STATE, "grouped matrix-core solve launch for a non-FP32 or narrow task plan");
     }
