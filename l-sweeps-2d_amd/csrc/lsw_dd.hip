@@ -130,22 +130,71 @@ __global__ void k_build_rhs_tasks(const CellDev* __restrict__ cells, const SlotD
                                   int writeX) {
   const int t = blockIdx.y;
   const SolveTaskDev& tk = tasks[t];
-  const CellDev& cd = cells[slots[tk.slot[0] % nSlots1].cell];
+  const int s0 = tk.slot[0] % nSlots1;
+  const SlotDev& sl0 = slots[s0];
+  const CellDev& cd = cells[sl0.cell];
   const int ny = cd.ny, N = cd.nx * ny;
   T2* x = X + (long long)t * xs * W;
+  // A stage-1 task carries one (d, cell) for several right-hand sides: one
+  // window test per point serves every lane, and when the lanes' places in
+  // F are one task row of F's layout, the point's lanes are one vector store
+  // (all W of them when the task fills the row or the row is this task's own).
+  bool one = sl0.kind == 0;
+  long long fb[W];
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    fb[q] = 0;
+    if (q < tk.nrhs) {
+      const int sidx = tk.slot[q];
+      one = one && sidx % nSlots1 == s0;
+      fb[q] = slotBase(tqF, (sidx / nSlots1) * nSlotsF + slots[sidx % nSlots1].grp, WF, xs);
+    }
+  }
+  bool fvec = one && WF == W && (tk.nrhs == W || writeX == 0) && fb[0] % W == 0;
+#pragma unroll
+  for (int q = 1; q < W; ++q)
+    if (q < tk.nrhs) fvec = fvec && fb[q] == fb[0] + q;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
     const int i = p / ny, j = p - i * ny;
     T2 out[W];
+    if (one) {
+      const int slr = sl0.d & 1, sud = sl0.d >> 1;
+      const bool own = i >= cd.vlr[slr][2] && i < cd.vlr[slr][5] && j >= cd.vud[sud][2] && j < cd.vud[sud][5];
+      const int gi = i + cd.ox, gj = j + cd.oy;
+      bool keep = false;
+      if (own) {
+        const bool ct = colT[gi] != 0, rt = rowT[gj] != 0;
+        keep = (sl0.d == 0) ? (!ct && !rt) : (sl0.d == 1) ? (ct && !rt) : (sl0.d == 2) ? (!ct && rt) : (ct && rt);
+      }
 #pragma unroll
-    for (int q = 0; q < W; ++q) {
-      out[q] = CTraits<T2>::make(0, 0);
-      if (q < tk.nrhs) {
-        const int sidx = tk.slot[q];
-        const SlotDev& sl = slots[sidx % nSlots1];
-        const int rhs = sidx / nSlots1;
-        out[q] = slotRhs<T2>(sl, cd, i, j, xin + (long long)rhs * Nglob, colT, rowT, Gy, coef,
-                             T + (long long)rhs * traceStride);
-        F[slotBase(tqF, rhs * nSlotsF + sl.grp, WF, xs) + (long long)p * WF] = out[q];
+      for (int q = 0; q < W; ++q) {
+        out[q] = CTraits<T2>::make(0, 0);
+        if (keep && q < tk.nrhs) {
+          const double2 v = xin[(long long)(tk.slot[q] / nSlots1) * Nglob + (long long)gi * Gy + gj];
+          out[q] = CTraits<T2>::make(v.x, v.y);
+        }
+      }
+      if (fvec) {
+        T2* f = F + fb[0] + (long long)p * WF;
+#pragma unroll
+        for (int q = 0; q < W; ++q) f[q] = out[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+          if (q < tk.nrhs) F[fb[q] + (long long)p * WF] = out[q];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        out[q] = CTraits<T2>::make(0, 0);
+        if (q < tk.nrhs) {
+          const int sidx = tk.slot[q];
+          const SlotDev& sl = slots[sidx % nSlots1];
+          const int rhs = sidx / nSlots1;
+          out[q] = slotRhs<T2>(sl, cd, i, j, xin + (long long)rhs * Nglob, colT, rowT, Gy, coef,
+                               T + (long long)rhs * traceStride);
+          F[fb[q] + (long long)p * WF] = out[q];
+        }
       }
     }
     if (writeX) {
@@ -337,7 +386,7 @@ __global__ void k_combine(const double2* __restrict__ Y, double2* __restrict__ o
 // as res1 + (res2 + res3 + res4) (src/DDCheckerboard.cpp:733-735).  Owner
 // tables: colJ / colLi [2][Gx], rowI / rowLj [2][Gy], cellAt [nR][nC].
 template <typename T2, int W>
-__global__ void k_gather_final(const CellDev* __restrict__ cells, const int* __restrict__ own,
+__global__ __launch_bounds__(256) void k_gather_final(const CellDev* __restrict__ cells, const int* __restrict__ own,
                                const int* __restrict__ tq, long long xs, int nSlots1, int nc, int nC,
                                const T2* __restrict__ Xs, double2* __restrict__ out, int Gx, int Gy, int nrhs) {
   const long long N = (long long)Gx * Gy;
@@ -358,10 +407,12 @@ __global__ void k_gather_final(const CellDev* __restrict__ cells, const int* __r
       sl[d] = d * nc + c;
       pw[d] = ((long long)colLi[slr * Gx + gi] * cells[c].ny + rowLj[sud * Gy + gj]) * W;
     }
-    // a task's lanes of one point are one contiguous W-vector: load it once
-    // and take every (right-hand side, DD) value it holds from registers
-    T2 lanes[W];
-    long long at = -1;
+    // a task's lanes of one point are one contiguous W-vector: each DD keeps
+    // the vector it loaded last, and every (right-hand side, DD) value it
+    // holds is taken from registers (the right-hand sides of one (d, cell)
+    // fill a task in turn, so each vector is loaded once)
+    T2 lanes[4][W];
+    long long at[4] = {-1, -1, -1, -1};
     for (int r = 0; r < nrhs; ++r) {
       double2 v[4];
 #pragma unroll
@@ -369,15 +420,15 @@ __global__ void k_gather_final(const CellDev* __restrict__ cells, const int* __r
         const int tqv = tq[r * nSlots1 + sl[d]];
         const int t = tqv / W, q = tqv - t * W;
         const long long base = (long long)t * xs * W + pw[d];
-        if (base != at) {
+        if (base != at[d]) {
 #pragma unroll
-          for (int k = 0; k < W; ++k) lanes[k] = Xs[base + k];
-          at = base;
+          for (int k = 0; k < W; ++k) lanes[d][k] = Xs[base + k];
+          at[d] = base;
         }
-        T2 a = lanes[0];
+        T2 a = lanes[d][0];
 #pragma unroll
         for (int k = 1; k < W; ++k)
-          if (q == k) a = lanes[k];
+          if (q == k) a = lanes[d][k];
         v[d] = make_double2((double)a.x, (double)a.y);
       }
       out[r * N + g] = cadd(v[0], cadd(cadd(v[1], v[2]), v[3]));
