@@ -665,10 +665,6 @@ def north_star_bench(args):
                   worker_s={k: round(st[k], 3) for k in ("setup_s", "factor_s", "block_s", "step_s")},
                   workers=args.pack_workers, restart=args.serve_restart, budget_gb=args.serve_budget / 1e9)
     log("packed", packed)
-    # the blocks the pack left in the library's cache go back to the runtime
-    # before the next section
-    L.device_memory(release=True)
-    torch.cuda.empty_cache()
     # the 1600^2 lens workload: single-problem figures and the solve roofline
     cfg = config_lens_1600()
     out, xs, prof, tune = lens_section(args, L, torch, cfg, args.paths.split(","))
