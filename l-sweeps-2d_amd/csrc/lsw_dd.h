@@ -205,7 +205,15 @@ class DDCheckerboard : public DD {
     if (s != mStream) mForeignStream = true;
     return s;
   }
+  // Work the DD queues on its own stream for later calls (the set-up's
+  // stencil assembly, the clears of grown buffers) is followed by an event;
+  // a call on another stream waits for it on the host first (ownWorkDone),
+  // which happens at most a few times per DD.
+  void markOwnWork() const;
+  void ownWorkDone(hipStream_t s) const;
   mutable bool mForeignStream = false;
+  mutable hipEvent_t mOwnEvent = nullptr;
+  mutable bool mOwnPending = false;
   DevArena mArena;  // step slot, group and copy lists, trace classes, owner tables
 
   // global operator

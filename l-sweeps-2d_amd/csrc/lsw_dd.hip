@@ -493,6 +493,7 @@ DDCheckerboard::~DDCheckerboard() {
   IdleFrees idle;
   for (auto& kv : mGraphs) (void)hipGraphExecDestroy(kv.second);
   mGraphs.clear();
+  if (mOwnEvent) (void)hipEventDestroy(mOwnEvent);
   for (void* p : {dX, dXs, dR, dG, dT, dF}) if (p) (void)devFree(p);
   if (dY) (void)devFree(dY);
   if (dGlobalCoef) (void)devFree(dGlobalCoef);
@@ -873,6 +874,7 @@ void DDCheckerboard::initDevice() {
   // every descriptor of the schedule, the trace classes and owner tables
   mArena.flush();
   allocBuffers();
+  markOwnWork();  // the stencil assembly and the buffer clears
   lap("buffers", t);
 }
 
@@ -1330,11 +1332,23 @@ void DDCheckerboard::ensureBuffers(int nrhs) const {
   // point of every used lane in each application
   LSW_HIP(hipMemsetAsync(dF, 0, (size_t)rows * mXs * es, mStream));
   // The solves may run on a caller's stream that is not ordered after the
-  // DD's own (non-blocking) stream: wait here so that the clears are done
-  // before any later launch on any stream.  Buffers only grow, so this wait
-  // happens a few times per DD at most.
-  LSW_HIP(hipStreamSynchronize(mStream));
+  // DD's own (non-blocking) stream: such a call waits for the clears first
+  // (ownWorkDone).  Buffers only grow, so this happens a few times per DD.
+  markOwnWork();
   mBufRhs = nrhs;
+}
+
+void DDCheckerboard::markOwnWork() const {
+  if (!mOwnEvent) LSW_HIP(hipEventCreateWithFlags(&mOwnEvent, hipEventDisableTiming));
+  LSW_HIP(hipEventRecord(mOwnEvent, mStream));
+  mOwnPending = true;
+}
+
+void DDCheckerboard::ownWorkDone(hipStream_t s) const {
+  if (s != mStream && mOwnPending) {
+    LSW_HIP(hipEventSynchronize(mOwnEvent));
+    mOwnPending = false;
+  }
 }
 
 // Rectangles of local grid points where the right-hand side of a slot can be
@@ -1448,6 +1462,7 @@ void DDCheckerboard::applyPrecondDev(const double2* x, double2* y, hipStream_t s
   LSW_REQUIRE(nrhs >= 1, LSW_ERR_ARG, "applyPrecond: nrhs must be positive");
   const int before = mBufRhs;
   ensureBuffers(nrhs);
+  ownWorkDone(s);  // (a host wait, so not inside the capture)
   if (mBufRhs != before) clearGraphs();  // captured graphs point at the old buffers
   for (const auto& st : mSteps) (void)planFor(st, nrhs);
   const GraphKey key{x, y, nrhs, s};
@@ -1608,6 +1623,7 @@ void DDCheckerboard::applyPrecondImpl(const double2* x, double2* y, hipStream_t 
   LSW_REQUIRE((long long)mMaxSlots * nrhs <= 65535, LSW_ERR_CAPACITY, "too many right-hand sides in one block");
   s = useStream(s);
   ensureBuffers(nrhs);
+  ownWorkDone(s);
   const size_t es = (mPrecision == LSW_FP64) ? sizeof(double2) : sizeof(float2);
   const long long n = N();
   auto zero = [&](void* p, long long words) {
@@ -1727,6 +1743,7 @@ void DDCheckerboard::applyADev(const double2* x, double2* y, hipStream_t s) cons
 void DDCheckerboard::applyADevBlock(const double2* x, double2* y, int nrhs, hipStream_t s) const {
   LSW_REQUIRE(dGlobalCoef != nullptr, LSW_ERR_STATE, "DD has no device state");
   s = useStream(s);
+  ownWorkDone(s);
   const long long n = N();
   if (nrhs > 0)
     hipLaunchKernelGGL(k_matvec, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s,

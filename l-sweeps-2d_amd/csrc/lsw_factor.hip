@@ -16,7 +16,7 @@ namespace lsw {
 // One thread per DOF: cubic-PML stencil (AssemblerHelmholtz2DFD::
 // assembleSystemMatrix, src/AssemblerHelmholtz2DFD.cpp:371-457).
 __global__ void k_assemble(const CellDev* __restrict__ cells, const double* __restrict__ m,
-                           double2* __restrict__ coef, int* __restrict__ bad) {
+                           double2* __restrict__ coef) {
   const CellDev& cd = cells[blockIdx.y];
   if (cd.mOff < 0) return;
   const int N = cd.nx * cd.ny;
@@ -31,8 +31,8 @@ __global__ void k_assemble(const CellDev* __restrict__ cells, const double* __re
     const double x = cd.x0 + (double)(i + 1) * cd.hx;
     const double y = cd.y0 + (double)(j + 1) * cd.hy;
     double cf[10];
-    const bool ok = helmholtzStencil(x, y, g, cd.C, cd.omega, m[cd.mOff + d], cd.hx, cd.hy, cf);
-    if (!ok) atomicOr(bad, 1);
+    // (every DOF inside the PML's domain: checked on the host, gridInsidePml)
+    (void)helmholtzStencil(x, y, g, cd.C, cd.omega, m[cd.mOff + d], cd.hx, cd.hy, cf);
     for (int dir = 0; dir < 5; ++dir)
       coef[cd.coefOff + (long long)dir * N + d] = make_double2(cf[2 * dir], cf[2 * dir + 1]);
   }
@@ -827,17 +827,22 @@ void FactorSet::setup(hipStream_t s) {
         if (!mM[i].empty()) std::copy(mM[i].begin(), mM[i].end(), all.begin() + mCells[i].mOff);
       LSW_HIP(uploadSync(dM, all.data(), (size_t)mo * sizeof(double)));
     }
-    int* dBad = nullptr;
-    LSW_HIP(devMalloc(&dBad, sizeof(int)));
-    LSW_HIP(hipMemsetAsync(dBad, 0, sizeof(int), s));
+    // the PML evaluation range on the host (no device round trip), then the
+    // stencils on the stream: the factorisation reads them in order
+    for (const CellDev& c : mCells) {
+      if (c.mOff < 0) continue;
+      Geo2 g;
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          g.intDom[a][b] = c.intDom[a][b];
+          g.compDom[a][b] = c.compDom[a][b];
+        }
+      LSW_REQUIRE(gridInsidePml(g, c.x0, c.y0, c.hx, c.hy, c.nx, c.ny), LSW_ERR_ARG,
+                  "PML: Evaluation point out of range!");
+    }
     dim3 grid((unsigned)std::min<long long>((mMaxDofs + 255) / 256, 4096), (unsigned)mCells.size());
-    hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, s, dCells, dM, dCoef, dBad);
+    hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, s, dCells, dM, dCoef);
     LSW_HIP(hipGetLastError());
-    int bad = 0;
-    LSW_HIP(hipMemcpyAsync(&bad, dBad, sizeof(int), hipMemcpyDeviceToHost, s));
-    LSW_HIP(hipStreamSynchronize(s));
-    (void)devFree(dBad);
-    LSW_REQUIRE(bad == 0, LSW_ERR_ARG, "PML: Evaluation point out of range!");
   }
 }
 
@@ -1069,31 +1074,28 @@ double2* assembleStencilDev(const CellDev& c, const std::vector<double>& m, hipS
   CellDev cd = c;
   cd.mOff = 0;
   cd.coefOff = 0;
+  Geo2 g;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      g.intDom[a][b] = c.intDom[a][b];
+      g.compDom[a][b] = c.compDom[a][b];
+    }
+  LSW_REQUIRE(gridInsidePml(g, c.x0, c.y0, c.hx, c.hy, c.nx, c.ny), LSW_ERR_ARG, "PML: Evaluation point out of range!");
   CellDev* dC = nullptr;
   double* dM = nullptr;
   double2* dCoef = nullptr;
-  int* dBad = nullptr;
   LSW_HIP(devMalloc(&dC, sizeof(CellDev)));
   LSW_HIP(devMalloc(&dM, N * sizeof(double)));
   LSW_HIP(devMalloc(&dCoef, 5 * N * sizeof(double2)));
-  LSW_HIP(devMalloc(&dBad, sizeof(int)));
   LSW_HIP(uploadSync(dC, &cd, sizeof(CellDev)));
   LSW_HIP(uploadSync(dM, m.data(), N * sizeof(double)));
-  LSW_HIP(hipMemsetAsync(dBad, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_assemble, dim3((unsigned)std::min<long long>((N + 255) / 256, 8192), 1), dim3(256), 0, s,
-                     dC, dM, dCoef, dBad);
+                     dC, dM, dCoef);
   LSW_HIP(hipGetLastError());
-  int bad = 0;
-  LSW_HIP(hipMemcpyAsync(&bad, dBad, sizeof(int), hipMemcpyDeviceToHost, s));
-  LSW_HIP(hipStreamSynchronize(s));
-  IdleFrees idle;
+  // kernels queued before a free keep their blocks: the cache reuses them
+  // only after a device synchronisation
   (void)devFree(dC);
   (void)devFree(dM);
-  (void)devFree(dBad);
-  if (bad) {
-    (void)devFree(dCoef);
-    throw Error(LSW_ERR_ARG, "PML: Evaluation point out of range!");
-  }
   return dCoef;
 }
 }  // namespace lsw

@@ -63,6 +63,23 @@ __host__ __device__ inline void pmlAxis(double s, double ds, double& Lr, double&
   Bi = dr * ai + di * ar;
 }
 
+// Whether every DOF of a grid (i, j) at (x0 + (i+1) hx, y0 + (j+1) hy),
+// 0 <= i < nx, 0 <= j < ny, lies inside the computational domain, the
+// condition under which helmholtzStencil evaluates the PML; it is separable,
+// so a host check of the nx + ny coordinates stands for the nx * ny DOFs.
+inline bool gridInsidePml(const Geo2& g, double x0, double y0, double hx, double hy, int nx, int ny) {
+  double s, ds;
+  for (int i = 0; i < nx; ++i)
+    if (!pmlSigma(x0 + (double)(i + 1) * hx, g.compDom[0][0], g.intDom[0][0], g.intDom[0][1], g.compDom[0][1], 1.0,
+                  s, ds))
+      return false;
+  for (int j = 0; j < ny; ++j)
+    if (!pmlSigma(y0 + (double)(j + 1) * hy, g.compDom[1][0], g.intDom[1][0], g.intDom[1][1], g.compDom[1][1], 1.0,
+                  s, ds))
+      return false;
+  return true;
+}
+
 // coef: 5 complex entries as (re, im) pairs.  Returns false if (x, y) is
 // outside the computational domain.
 __host__ __device__ inline bool helmholtzStencil(double x, double y, const Geo2& g, double C,
