@@ -143,6 +143,8 @@ class DDCheckerboard : public DD {
   // Sizes the sweep buffers for blocks of up to nrhs right-hand sides now
   // (otherwise they grow on first use).
   void reserveBlock(int nrhs) const { ensureBuffers(nrhs); }
+  // The narrowest block width in [lo, hi] whose solve plans are built, or 0.
+  int builtWidth(int lo, int hi) const;
   // Device bytes of this DD once factorised, serving blocks of nrhs
   // right-hand sides (a host estimate from the nested-dissection trees of the
   // cell shapes; valid after setUpGeometry): steady = factors in the solve

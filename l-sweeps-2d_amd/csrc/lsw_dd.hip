@@ -1261,17 +1261,35 @@ const SolvePlan& DDCheckerboard::planFor(const Step& st, int nrhs) const {
       const auto& nd = st.hNeed[s % n1];
       need.insert(need.end(), nd.begin(), nd.end());
     }
+    // (the right-hand sides of one slot geometry repeat its rectangles)
+    auto dedupe = [](std::vector<Rect>& v) {
+      auto key = [](const Rect& r) { return std::make_tuple(r.i0, r.i1, r.j0, r.j1); };
+      std::sort(v.begin(), v.end(), [&](const Rect& a, const Rect& b) { return key(a) < key(b); });
+      v.erase(std::unique(v.begin(), v.end(), [&](const Rect& a, const Rect& b) { return key(a) == key(b); }), v.end());
+    };
+    dedupe(sup);
+    dedupe(need);
     tasks.push_back(tk);
-    supports.push_back(sup);
-    needs.push_back(need);
+    supports.push_back(std::move(sup));
+    needs.push_back(std::move(need));
   }
   auto plan = std::make_unique<SolvePlan>();
+  if (std::getenv("LSW_PLAN_TIME"))
+    std::fprintf(stderr, "planFor nrhs %d slots %d tasks %zu\n", nrhs, n1, tasks.size());
   // the final solves need the whole solution; the sweep solves only their traces
   mFactors->buildPlan(tasks, supports, *plan, st.deferred ? nullptr : &needs, /*outOfPlace=*/true,
                       &shapeFor(nrhs));
   const SolvePlan& ref = *plan;
   st.plans[nrhs] = std::move(plan);
   return ref;
+}
+
+int DDCheckerboard::builtWidth(int lo, int hi) const {
+  if (mSteps.empty()) return 0;
+  // (every step's plan for a width is built by the first application at it)
+  const auto& plans = mSteps.back().plans;
+  auto it = plans.lower_bound(lo);
+  return it != plans.end() && it->first <= hi ? it->first : 0;
 }
 
 // Rows of the task-major solve scratch (R, G) a block of nrhs right-hand
@@ -1768,6 +1786,7 @@ void DDCheckerboard::assembleRHSDev(const std::vector<DoubleVector>& srcs, doubl
                      mGlobal.asmb.hy(), mGx, mGy, b);
   LSW_HIP(hipGetLastError());
   LSW_HIP(hipStreamSynchronize(s));
+  IdleFrees idle;  // only the kernel above, done, read it
   (void)devFree(dsrc);
 }
 

@@ -239,6 +239,222 @@ __global__ __launch_bounds__(256) void k_front_factor(
   frontStore(cd, f, factor, schur);
 }
 
+// ---- small fronts in registers
+// Fronts of at most kSmallRows rows (the leaves and the first heights above
+// them): one workgroup of 512 threads per front, the R x R front held in
+// registers (thread (ti, tj) of a 32 x 16 grid owns rows ti + 32a and
+// columns tj + 16c), no workspace.  Gauss-Jordan exchange steps in place:
+// pivot (p, k) exchanges equation p and unknown k,
+//   M[p][k] <- 1/M[p][k],  M[p][j] <- -M[p][j]/M[p][k],
+//   M[i][k] <- M[i][k]/M[p][k],  M[i][j] <- M[i][j] - M[i][k] M[p][j]/M[p][k],
+// for k = 0..e-1 with p the largest-modulus row among the fully-summed rows
+// not yet exchanged (partial pivoting, as the workspace schedules; rows are
+// not moved, the pivot order is kept instead).  After the e steps, with p_k
+// the pivot row of column k,
+//   M[p_k][l]     = (F11^{-1})[k][p_l],     M[p_k][e+t] = -(F11^{-1} F12)[k][t],
+//   M[e+t][l]     = (F21 F11^{-1})[t][p_l], M[e+t][e+u] = S[t][u],
+// which are W, K and S in the workspace schedules' layouts.  Column k's
+// owners are the 32 lanes of one half wavefront: they pick the pivot with
+// lane shuffles and publish it with the column; one workgroup barrier per
+// step, the row going round by lane shuffles.
+constexpr int kSmallRows = 96;
+constexpr int kSmallRT = 32, kSmallCT = 16;  // thread grid
+constexpr int kSmallPR = kSmallRows / kSmallRT, kSmallPC = kSmallRows / kSmallCT;
+
+template <typename W2>
+__global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
+    const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes, const FrontTask* __restrict__ fts,
+    const double2* __restrict__ coef, W2* __restrict__ factor, W2* __restrict__ schur, int* __restrict__ info) {
+  constexpr int RT = kSmallRT, CT = kSmallCT, PR = kSmallPR, PC = kSmallPC, NT = RT * CT;
+  __shared__ int rowCnt[kSmallRows];
+  __shared__ int rowCol[kSmallRows][5];  // a row's original entries: column
+  __shared__ int rowAsm[kSmallRows][5];  // ... and assembly-list index
+  __shared__ int inv1[kSmallRows], inv2[kSmallRows];  // front row -> child boundary index
+  __shared__ int pivOf[kSmallRows], stepOf[kSmallRows];
+  __shared__ W2 colK[2][kSmallRows];
+  __shared__ double sMag[2];
+  __shared__ int sP[2];
+  const FrontTask ft = fts[blockIdx.x];
+  const CellDev& cd = cells[ft.cell];
+  const ShapeDev& sh = shapes[cd.shape];
+  const NdNodeDev nd = sh.nodes[ft.node];
+  const int e = nd.e, b = nd.b, R = e + b;
+  const int N = cd.nx * cd.ny;
+  const int tid = threadIdx.x, ti = tid % RT, tj = tid / RT;
+  // ---- assembly: original entries, then child 1's and child 2's Schur
+  // complements, each position in that order (as the workspace schedules)
+  for (int i = tid; i < kSmallRows; i += NT) {
+    rowCnt[i] = 0;
+    inv1[i] = -1;
+    inv2[i] = -1;
+  }
+  __syncthreads();
+  for (int a = tid; a < nd.asmCnt; a += NT) {
+    const AsmEntry en = sh.asmList[nd.asmOff + a];
+    const int s = atomicAdd(&rowCnt[en.fr], 1);  // at most 5 entries a row, one per column
+    rowCol[en.fr][s] = en.fc;
+    rowAsm[en.fr][s] = a;
+  }
+  int bc1 = 0, bc2 = 0;
+  const W2* S1 = nullptr;
+  const W2* S2 = nullptr;
+  if (nd.c1 >= 0) {
+    const NdNodeDev cn = sh.nodes[nd.c1];
+    bc1 = cn.b;
+    S1 = schur + cd.schurOff + cn.sOff;
+    for (int t = tid; t < bc1; t += NT) inv1[sh.up[cn.upOff + t]] = t;
+  }
+  if (nd.c2 >= 0) {
+    const NdNodeDev cn = sh.nodes[nd.c2];
+    bc2 = cn.b;
+    S2 = schur + cd.schurOff + cn.sOff;
+    for (int t = tid; t < bc2; t += NT) inv2[sh.up[cn.upOff + t]] = t;
+  }
+  __syncthreads();
+  W2 M[PR][PC];
+#pragma unroll
+  for (int a = 0; a < PR; ++a) {
+    const int i = ti + RT * a;
+    const int cnt = i < R ? rowCnt[i] : 0;
+    const int r1 = i < R ? inv1[i] : -1, r2 = i < R ? inv2[i] : -1;
+#pragma unroll
+    for (int c = 0; c < PC; ++c) {
+      const int j = tj + CT * c;
+      W2 v = czero<W2>();
+      if (i < R && j < R) {
+        for (int s2 = 0; s2 < cnt; ++s2)
+          if (rowCol[i][s2] == j) {
+            const AsmEntry en = sh.asmList[nd.asmOff + rowAsm[i][s2]];
+            const double2 cf = coef[cd.coefOff + (long long)en.dir * N + en.dof];
+            v = cadd(v, CTraits<W2>::make((typename CTraits<W2>::R)cf.x, (typename CTraits<W2>::R)cf.y));
+          }
+        const int q1 = inv1[j], q2 = inv2[j];
+        if (r1 >= 0 && q1 >= 0) v = cadd(v, S1[(long long)r1 * bc1 + q1]);
+        if (r2 >= 0 && q2 >= 0) v = cadd(v, S2[(long long)r2 * bc2 + q2]);
+      }
+      M[a][c] = v;
+    }
+  }
+  // ---- the e exchange steps (one workgroup barrier each)
+  // Column k's owners (tj == k % CT: the 32 lanes of one half wavefront)
+  // publish it and its pivot row p in LDS; row p's values reach the other
+  // threads of their column group by lane shuffles within the half
+  // wavefront (thread (p % RT, tj) owns them), so no second barrier.
+  unsigned usedMask = 0;  // this thread's rows already exchanged
+  for (int k = 0; k < e; ++k) {
+    const int kb = k / CT, kc = k % CT;
+    W2* ck = colK[k & 1];
+    if (tj == kc) {
+      double best = -1.0;
+      int bi = e;
+#pragma unroll
+      for (int c = 0; c < PC; ++c)
+        if (c == kb) {
+#pragma unroll
+          for (int a = 0; a < PR; ++a) {
+            const int i = ti + RT * a;
+            if (i < R) ck[i] = M[a][c];
+            if (i < e && !((usedMask >> a) & 1u)) {
+              const double m = (double)(M[a][c].x * M[a][c].x + M[a][c].y * M[a][c].y);
+              if (m > best) {
+                best = m;
+                bi = i;
+              }
+            }
+          }
+        }
+      // largest modulus over the 32 lanes (lowest row on ties)
+#pragma unroll
+      for (int o = 1; o < RT; o <<= 1) {
+        const double om = __shfl_xor(best, o, RT);
+        const int oi = __shfl_xor(bi, o, RT);
+        if (om > best || (om == best && oi < bi)) {
+          best = om;
+          bi = oi;
+        }
+      }
+      if (ti == 0) {
+        sMag[k & 1] = best;
+        sP[k & 1] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = sP[k & 1];
+    const double pmag = sMag[k & 1];
+    const bool bad = !(pmag > 0.0) || !isfinite(pmag) || p >= e;
+    const int pp = bad ? 0 : p;
+    const int pa = pp / RT, pr = pp % RT;
+    W2 pv = ck[pp];
+    if (bad) {
+      if (tid == 0) atomicMax(info, 1);
+      pv = CTraits<W2>::make(1, 0);
+    }
+    const W2 inv = cinvT(pv);
+    if (ti == pr) usedMask |= 1u << pa;
+    if (tid == 0) pivOf[k] = pp;
+    W2 ca[PR];
+#pragma unroll
+    for (int a = 0; a < PR; ++a) {
+      const int i = ti + RT * a;
+      ca[a] = ck[i < R ? i : 0];
+    }
+    const int srcLane = (tid & ~(RT - 1) & 63) + pr;  // lane of (pr, tj) in this wavefront
+#pragma unroll
+    for (int c = 0; c < PC; ++c) {
+      // row p's value in this column (before this step's update)
+      W2 mine = M[0][c];
+#pragma unroll
+      for (int a = 1; a < PR; ++a)
+        if (a == pa) mine = M[a][c];
+      W2 rowv;
+      rowv.x = __shfl(mine.x, srcLane, 64);
+      rowv.y = __shfl(mine.y, srcLane, 64);
+      const W2 rs = cmul(rowv, inv);
+#pragma unroll
+      for (int a = 0; a < PR; ++a) M[a][c] = cfms(ca[a], rs, M[a][c]);
+      if (c == kb && tj == kc) {
+#pragma unroll
+        for (int a = 0; a < PR; ++a) M[a][c] = cmul(ca[a], inv);
+      }
+      if (ti == pr) {
+        const W2 rowNew = (c == kb && tj == kc) ? inv : cneg(rs);
+#pragma unroll
+        for (int a = 0; a < PR; ++a)
+          if (a == pa) M[a][c] = rowNew;
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < e; k += NT) stepOf[pivOf[k]] = k;
+  __syncthreads();
+  // ---- W, K and S
+  W2* Wp = factor + cd.factorOff + nd.fwdOff;
+  W2* Kp = factor + cd.factorOff + nd.bwdOff;
+  W2* Sp = schur + cd.schurOff + nd.sOff;
+#pragma unroll
+  for (int a = 0; a < PR; ++a) {
+    const int i = ti + RT * a;
+    if (i >= R) continue;
+    const int ki = i < e ? stepOf[i] : 0;
+#pragma unroll
+    for (int c = 0; c < PC; ++c) {
+      const int j = tj + CT * c;
+      if (j >= R) continue;
+      const W2 v = M[a][c];
+      if (i < e) {
+        if (j < e)
+          Kp[(long long)pivOf[j] * e + ki] = v;
+        else
+          Kp[(long long)j * e + ki] = v;
+      } else if (j < e) {
+        Wp[(long long)pivOf[j] * b + (i - e)] = v;
+      } else {
+        Sp[(long long)(i - e) * b + (j - e)] = v;
+      }
+    }
+  }
+}
+
 // ---- blocked schedule for the large fronts
 // Per front, after the front itself (R x Cc) the workspace holds T (R x
 // kPanel, row-major) and B (kPanel x Cc, row-major).
@@ -852,6 +1068,16 @@ namespace {
 // factorisation took 0.39 s with one workgroup per front everywhere, 0.20 s
 // with the blocked schedule from 192 rows up and 0.149 s with it on every
 // height.  LSW_FACTOR_BLOCKED_ROWS overrides it for tuning.
+// Heights whose fronts all have at most kSmallRows rows take k_front_small
+// (LSW_FACTOR_SMALL=0: the workspace schedules, for comparison).
+bool smallFronts() {
+  static const bool v = [] {
+    const char* e = std::getenv("LSW_FACTOR_SMALL");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return v;
+}
+
 int blockedMinRows() {
   static const int v = [] {
     const char* e = std::getenv("LSW_FACTOR_BLOCKED_ROWS");
@@ -975,7 +1201,11 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
       if (h > t.maxHeight) continue;
       for (int p : t.byHeight[h]) maxR = std::max(maxR, t.nodes[p].e + t.nodes[p].b);
     }
+    // heights of small fronts only: one register-resident workgroup per
+    // front, no workspace (state 2)
+    const bool small = smallFronts() && maxR <= kSmallRows;
     const bool blocked = maxR >= blockedMinRows();
+    const char state = small ? 2 : blocked ? 1 : 0;
     std::vector<FrontTask> cur;
     long long ws = 0;
     for (size_t ci = 0; ci < mCells.size(); ++ci) {
@@ -984,11 +1214,11 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
       for (int p : t.byHeight[h]) {
         const NdNode& nd = t.nodes[p];
         const long long R = nd.e + nd.b, Cc = 2 * nd.e + nd.b;
-        const long long need = R * Cc + (blocked ? R * kPanel + kPanel * Cc : 0);
+        const long long need = small ? 0 : R * Cc + (blocked ? R * kPanel + kPanel * Cc : 0);
         if (!cur.empty() && ws + need > budget) {
           batches.push_back(cur);
           batchWs.push_back(ws);
-          batchBlocked.push_back(blocked);
+          batchBlocked.push_back(state);
           maxWs = std::max(maxWs, ws);
           cur.clear();
           ws = 0;
@@ -1000,7 +1230,7 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
     if (!cur.empty()) {
       batches.push_back(cur);
       batchWs.push_back(ws);
-      batchBlocked.push_back(blocked);
+      batchBlocked.push_back(state);
       maxWs = std::max(maxWs, ws);
     }
     batches.push_back({});  // height separator (keeps order explicit)
@@ -1027,7 +1257,10 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
     const FrontTask* dFt = dFtAll + ftOff[bi];
     const size_t shm = (size_t)mMaxFront * sizeof(W2);
     const unsigned nf = (unsigned)bt.size();
-    if (!batchBlocked[bi]) {
+    if (batchBlocked[bi] == 2) {
+      hipLaunchKernelGGL(k_front_small<W2>, dim3(nf), dim3(kSmallRT * kSmallCT), 0, s, dCells, dShapes, dFt, dCoef,
+                         factor, dSchur, dInfo);
+    } else if (!batchBlocked[bi]) {
       hipLaunchKernelGGL(k_front_factor<W2>, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dCoef, factor,
                          dSchur, dWs, dInfo);
     } else {
@@ -1092,8 +1325,11 @@ double2* assembleStencilDev(const CellDev& c, const std::vector<double>& m, hipS
   hipLaunchKernelGGL(k_assemble, dim3((unsigned)std::min<long long>((N + 255) / 256, 8192), 1), dim3(256), 0, s,
                      dC, dM, dCoef);
   LSW_HIP(hipGetLastError());
-  // kernels queued before a free keep their blocks: the cache reuses them
-  // only after a device synchronisation
+  // the temporaries are free for reuse once the assembly is done (a wait for
+  // this stream, which set-up has just filled, instead of a later wait for
+  // the whole device by the allocator)
+  LSW_HIP(hipStreamSynchronize(s));
+  IdleFrees idle;
   (void)devFree(dC);
   (void)devFree(dM);
   return dCoef;

@@ -5,9 +5,10 @@
 // spend its time in those waits.  Freed blocks are therefore kept:
 //   * a freed block first goes to the pending list: kernels queued before
 //     the free (on any stream) may still use it;
-//   * a request that no ready block fits, but a pending one would, waits
-//     once for the device (hipDeviceSynchronize) and moves every pending
-//     block to the ready list;
+//   * a request above 64 MiB (or one that would pass the cap below) that no
+//     ready block fits, but a pending one would, waits once for the device
+//     (hipDeviceSynchronize) and moves every pending block to the ready
+//     list; a smaller one takes a new block instead;
 //   * a ready block serves a request of at most its size and at least half
 //     of it (best fit), so a cached block wastes at most half its bytes;
 //   * live and cached bytes together stay below what the device had free at
@@ -68,6 +69,9 @@ size_t blockSize(size_t bytes) {
 // blocks, a quarter more above 64 MiB (a large block handed out for a much
 // smaller request would hold device memory nobody accounted for)
 size_t maxFit(size_t size) { return size <= ((size_t)64 << 20) ? 2 * size : size + size / 4; }
+
+// requests above this size wait for pending blocks rather than take new ones
+constexpr size_t kWaitMin = (size_t)64 << 20;
 
 // best fit among cached blocks of [size, maxFit(size)]; the lock is held
 void* takeFrom(std::multimap<size_t, void*>& m, size_t size, size_t* got) {
@@ -170,7 +174,11 @@ hipError_t devMallocBytes(void** p, size_t bytes) {
   std::unique_lock<std::mutex> lk(gMu);
   size_t got = 0;
   void* q = takeFrom(gReady, size, &got);
-  if (!q && fitsIn(gPending, size)) {
+  // A small request that only a pending block fits takes a new block while
+  // the cap allows: the wait for the device would wait for every stream's
+  // queued work (tens of milliseconds on a device other threads keep busy).
+  const bool wait = size > kWaitMin || gTotal.load() + gCached.load() + (long long)size > deviceCap();
+  if (!q && wait && fitsIn(gPending, size)) {
     // kernels queued before those frees may still read the blocks: wait for
     // the device without the lock (other threads allocate and free in the
     // meantime; what they free during the wait stays pending), then make the

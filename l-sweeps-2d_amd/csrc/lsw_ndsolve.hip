@@ -33,10 +33,10 @@
 // Factors are column-major, so the lanes of an entry read consecutive rows of
 // one column (coalesced); right-hand-side values are broadcast from LDS.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <map>
 #include <type_traits>
 
 #include "lsw_cplx.h"
@@ -1241,6 +1241,9 @@ PlanShape PlanShape::defaults() {
     if (const char* e = std::getenv("LSW_GRP_ROWS_FWD")) p.grpMinRowsFwd = std::atoi(e);
     if (const char* e = std::getenv("LSW_GRP_ROWS_BWD")) p.grpMinRowsBwd = std::atoi(e);
     if (const char* e = std::getenv("LSW_GRP_LATENCY")) p.grpLatency = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LSW_LATENCY_ROWS")) p.latencyRows = std::atoi(e);
+    if (const char* e = std::getenv("LSW_WIDE_BUDGET")) p.wideBudget = std::max(0LL, std::atoll(e));
+    if (const char* e = std::getenv("LSW_WIDE_CAP")) p.wideCap = std::max(1, std::atoi(e));
     return p;
   }();
   return d;
@@ -1250,6 +1253,10 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
                           SolvePlan& plan, const std::vector<std::vector<Rect>>* bwdSupports, bool outOfPlace,
                           const PlanShape* shapeIn) const {
   PlanShape shp = shapeIn ? *shapeIn : PlanShape::defaults();
+  static const bool ptime = std::getenv("LSW_PLAN_TIME") != nullptr;
+  auto clk = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+  const double pt0 = ptime ? clk() : 0;
+  double pt1 = 0, pt2 = 0, pt3 = 0;
   LSW_REQUIRE(!mTrees.empty(), LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(supports.empty() || supports.size() == tasks.size(), LSW_ERR_ARG, "one support per task");
   LSW_REQUIRE(!bwdSupports || bwdSupports->size() == tasks.size(), LSW_ERR_ARG, "one backward support per task");
@@ -1273,80 +1280,122 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   // and each factor block once per launch however many tasks of its cell
   // read it (they run as neighbouring entries, and the block comes from HBM
   // once): per (cell, node), the widest factor read counted so far.
-  std::map<std::pair<int, int>, std::pair<double, double>> facSeen;
+  // (flat: cell c's nodes from cellNode[c] on)
+  std::vector<long long> cellNode(mCells.size() + 1, 0);
+  for (size_t c = 0; c < mCells.size(); ++c)
+    cellNode[c + 1] = cellNode[c] + (long long)mTrees[mCells[c].shape]->nodes.size();
+  std::vector<std::pair<double, double>> facSeen((size_t)cellNode.back(), {0.0, 0.0});
   auto facBytes = [&](int cell, int node, bool fwd, double bytes) {
-    auto& seen = facSeen[{cell, node}];
+    auto& seen = facSeen[(size_t)(cellNode[cell] + node)];
     double& c = fwd ? seen.first : seen.second;
     const double add = std::max(0.0, bytes - c);
     c = std::max(c, bytes);
     return add;
   };
-  for (int t = 0; t < nt; ++t) {
-    const SolveTask& tk = tasks[t];
-    LSW_REQUIRE(tk.nrhs >= 1 && tk.nrhs <= kMaxRhs, LSW_ERR_ARG, "bad task right-hand-side count");
-    LSW_REQUIRE(tk.cell >= 0 && tk.cell < (int)mCells.size(), LSW_ERR_ARG, "bad task cell");
-    plan.maxNrhs = std::max(plan.maxNrhs, tk.nrhs);
-    SolveTaskDev& td = tdev[t];
-    std::memset(&td, 0, sizeof(td));
-    td.facBase = mCells[tk.cell].factorOff;
-    td.nrhs = tk.nrhs;
-    for (int q = 0; q < kMaxRhs; ++q) td.slot[q] = q < tk.nrhs ? tk.slot[q] : 0;
-    const int shape = mCells[tk.cell].shape;
+  // Entries are generated cell by cell (a run of tasks of one cell), node by
+  // node, row chunk by row chunk, task by task: the order the launches want
+  // (the entries of one node and row chunk for all tasks of a cell adjacent),
+  // so the lists need no sort when the tasks come cell by cell.
+  std::vector<int> gFlags;
+  for (int t0 = 0; t0 < nt;) {
+    int t1 = t0 + 1;
+    while (t1 < nt && tasks[t1].cell == tasks[t0].cell) ++t1;
+    const int ng = t1 - t0;
+    const int cell = tasks[t0].cell;
+    LSW_REQUIRE(cell >= 0 && cell < (int)mCells.size(), LSW_ERR_ARG, "bad task cell");
+    const int shape = mCells[cell].shape;
     const NdTree& tr = *mTrees[shape];
+    const int nn = (int)tr.nodes.size();
     const int base = mShapeNodeBase[shape];
-    const std::vector<Rect> none;
-    const std::vector<Rect>& sup = supports.empty() ? none : supports[t];
-    const double r = tk.nrhs;
-    on.assign(tr.nodes.size(), 0);
-    for (int p = 0; p < (int)tr.nodes.size(); ++p) on[p] = meets(tr.nodes[p], sup) ? 1 : 0;
-    // nodes whose solution values are needed
-    need.assign(tr.nodes.size(), 1);
-    if (bwdSupports) {
-      const std::vector<Rect>& bs = (*bwdSupports)[t];
-      for (int p = 0; p < (int)tr.nodes.size(); ++p) need[p] = !bs.empty() && meets(tr.nodes[p], bs) ? 1 : 0;
+    on.assign((size_t)ng * nn, 0);
+    need.assign((size_t)ng * nn, 1);
+    gFlags.assign((size_t)ng, 0);
+    for (int t = t0; t < t1; ++t) {
+      const SolveTask& tk = tasks[t];
+      LSW_REQUIRE(tk.nrhs >= 1 && tk.nrhs <= kMaxRhs, LSW_ERR_ARG, "bad task right-hand-side count");
+      plan.maxNrhs = std::max(plan.maxNrhs, tk.nrhs);
+      SolveTaskDev& td = tdev[t];
+      std::memset(&td, 0, sizeof(td));
+      td.facBase = mCells[cell].factorOff;
+      td.nrhs = tk.nrhs;
+      for (int q = 0; q < kMaxRhs; ++q) td.slot[q] = q < tk.nrhs ? tk.slot[q] : 0;
+      char* onT = on.data() + (size_t)(t - t0) * nn;
+      char* needT = need.data() + (size_t)(t - t0) * nn;
+      if (supports.empty() || supports[t].empty()) {
+        std::fill(onT, onT + nn, 1);
+      } else {
+        for (int p = 0; p < nn; ++p) onT[p] = meets(tr.nodes[p], supports[t]) ? 1 : 0;
+      }
+      // nodes whose solution values are needed
+      if (bwdSupports) {
+        const std::vector<Rect>& bs = (*bwdSupports)[t];
+        for (int p = 0; p < nn; ++p) needT[p] = !bs.empty() && meets(tr.nodes[p], bs) ? 1 : 0;
+      }
     }
-    for (int p = 0; p < (int)tr.nodes.size(); ++p) {
+    for (int p = 0; p < nn; ++p) {
       const NdNode& nd = tr.nodes[p];
       const int h = nd.height;
       const double e = nd.e, b = nd.b;
-      int flags = 0;
-      if (nd.c1 >= 0 && on[nd.c1]) flags |= kEntChild1;
-      if (nd.c2 >= 0 && on[nd.c2]) flags |= kEntChild2;
-      if (on[p]) flags |= kEntVisited;
-      // A root (no boundary rows) has no forward product, only r_E to form.
-      // Its backward entries form it themselves when the solution goes to
-      // another buffer, or when the root is one row chunk; otherwise (in
-      // place, several chunks) a forward entry stores r_E, because the chunks
-      // of its backward entries overwrite X[E] while the others read it.
-      if (nd.b == 0 && (outOfPlace || nd.e <= kSolveRows)) flags |= kEntRootRhs;
-      if (on[p] && !(flags & kEntRootRhs)) {
+      bool anyFwd = false, anyBwd = false;
+      for (int g = 0; g < ng; ++g) {
+        const char* onT = on.data() + (size_t)g * nn;
+        int flags = 0;
+        if (nd.c1 >= 0 && onT[nd.c1]) flags |= kEntChild1;
+        if (nd.c2 >= 0 && onT[nd.c2]) flags |= kEntChild2;
+        if (onT[p]) flags |= kEntVisited;
+        // A root (no boundary rows) has no forward product, only r_E to
+        // form.  Its backward entries form it themselves when the solution
+        // goes to another buffer, or when the root is one row chunk;
+        // otherwise (in place, several chunks) a forward entry stores r_E,
+        // because the chunks of its backward entries overwrite X[E] while
+        // the others read it.
+        if (nd.b == 0 && (outOfPlace || nd.e <= kSolveRows)) flags |= kEntRootRhs;
+        gFlags[g] = flags;
+        anyFwd |= onT[p] && !(flags & kEntRootRhs);
+        anyBwd |= need[(size_t)g * nn + p] != 0;
+      }
+      if (anyFwd) {
         if (nd.b == 0) {
-          fw[h].push_back({t, base + p, 0, flags});
+          for (int g = 0; g < ng; ++g)
+            if (on[(size_t)g * nn + p] && !(gFlags[g] & kEntRootRhs)) fw[h].push_back({t0 + g, base + p, 0, gFlags[g]});
         } else {
           for (int r0 = 0; r0 < nd.b; r0 += kSolveRows) {
             const int rows = std::min(kSolveRows, nd.b - r0);
-            fw[h].push_back({t, base + p, r0 | (rows << 16), flags});
+            for (int g = 0; g < ng; ++g)
+              if (on[(size_t)g * nn + p] && !(gFlags[g] & kEntRootRhs))
+                fw[h].push_back({t0 + g, base + p, r0 | (rows << 16), gFlags[g]});
             frows[h] = std::max(frows[h], rows);
           }
         }
         fcols[h] = std::max(fcols[h], nd.e);
-        fbytes[h] += facBytes(tk.cell, p, true, S * b * e) + S * r * (2 * e + 2 * b);
-        fwork[h] += b * e + e;
-        fflops[h] += 8.0 * r * b * e;
+        for (int g = 0; g < ng; ++g) {
+          if (!on[(size_t)g * nn + p] || (gFlags[g] & kEntRootRhs)) continue;
+          const double r = tasks[t0 + g].nrhs;
+          fbytes[h] += facBytes(cell, p, true, S * b * e) + S * r * (2 * e + 2 * b);
+          fwork[h] += b * e + e;
+          fflops[h] += 8.0 * r * b * e;
+        }
       }
-      if (!need[p]) continue;
+      if (!anyBwd) continue;
       for (int r0 = 0; r0 < nd.e; r0 += kSolveRows) {
         const int rows = std::min(kSolveRows, nd.e - r0);
-        bw[h].push_back({t, base + p, r0 | (rows << 16), flags});
+        for (int g = 0; g < ng; ++g)
+          if (need[(size_t)g * nn + p]) bw[h].push_back({t0 + g, base + p, r0 | (rows << 16), gFlags[g]});
         brows[h] = std::max(brows[h], rows);
       }
       bcols[h] = std::max(bcols[h], nd.e + nd.b);
-      const double kcols = on[p] ? e + b : b;
-      bbytes[h] += facBytes(tk.cell, p, false, S * e * kcols) + S * r * (kcols + e);
-      bwork[h] += e * kcols;
-      bflops[h] += 8.0 * r * e * kcols;
+      for (int g = 0; g < ng; ++g) {
+        if (!need[(size_t)g * nn + p]) continue;
+        const double r = tasks[t0 + g].nrhs;
+        const double kcols = on[(size_t)g * nn + p] ? e + b : b;
+        bbytes[h] += facBytes(cell, p, false, S * e * kcols) + S * r * (kcols + e);
+        bwork[h] += e * kcols;
+        bflops[h] += 8.0 * r * e * kcols;
+      }
     }
+    t0 = t1;
   }
+  if (ptime) pt1 = clk();
   plan.width = taskWidth(plan.maxNrhs);
   const double nrk = plan.width;
   // 16-wavefront splits exist for FP32 tasks of up to 4 right-hand sides
@@ -1361,10 +1410,32 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
     if (a.node != b.node) return a.node < b.node;
     return (a.r0rows & 0xffff) < (b.r0rows & 0xffff);
   };
-  for (auto& v : fw) std::stable_sort(v.begin(), v.end(), byNode);
-  for (auto& v : bw) std::stable_sort(v.begin(), v.end(), byNode);
+  // (already in this order when the tasks come cell by cell in ascending
+  // cells; otherwise sorted here)
+  for (auto& v : fw)
+    if (!std::is_sorted(v.begin(), v.end(), byNode)) std::stable_sort(v.begin(), v.end(), byNode);
+  for (auto& v : bw)
+    if (!std::is_sorted(v.begin(), v.end(), byNode)) std::stable_sort(v.begin(), v.end(), byNode);
+  if (ptime) pt2 = clk();
   std::vector<SolveEntry> all;
   std::vector<GroupRec> allGrp;
+  {
+    // room for every entry and every group a launch could form (no
+    // reallocation of the large lists while they fill)
+    size_t nEnt = 0, nGrp = 0;
+    for (const auto* lists : {&fw, &bw})
+      for (const auto& v : *lists) {
+        nEnt += v.size();
+        for (size_t i = 0; i < v.size();) {
+          size_t j = i + 1;
+          while (j < v.size() && j - i < (size_t)kGrpTasks && !byNode(v[i], v[j]) && v[i].r0rows == v[j].r0rows) ++j;
+          ++nGrp;
+          i = j;
+        }
+      }
+    all.reserve(nEnt);
+    allGrp.reserve(nGrp);
+  }
   auto addLaunch = [&](std::vector<SolvePlan::Launch>& out, const std::vector<SolveEntry>& v, int cols, int maxRows,
                        double bytes, double work, double flops, bool fwdLaunch) {
     if (v.empty()) return;
@@ -1500,7 +1571,8 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
       const long long groups = (long long)v.size() * rp / 64 + 1;
       L.rounds = (int)std::min<long long>(16, std::max<long long>(1, groups / packedWaveTarget()));
     }
-    all.insert(all.end(), ev->begin(), ev->end());
+    // (a grouped launch reads its group records, not the entries)
+    if (L.mode != 4) all.insert(all.end(), ev->begin(), ev->end());
     out.push_back(L);
   };
   plan.fwd.clear();
@@ -1535,6 +1607,7 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
                      "bytes %.0f\n", pass ? "bwd" : "fwd", nt, plan.maxNrhs, L.n, L.mode, L.rp, L.rounds, L.wpb,
                      L.waves, L.cols, L.bytes);
   }
+  if (ptime) pt3 = clk();
   if (plan.dBlob) (void)devFree(plan.dBlob);
   plan.dBlob = nullptr;
   plan.dTasks = nullptr;
@@ -1553,21 +1626,29 @@ void FactorSet::buildPlan(const std::vector<SolveTask>& tasks, const std::vector
   const size_t bTQ = plan.slotTQ.size() * sizeof(int), bGrp = allGrp.size() * sizeof(GroupRec);
   const size_t oEnt = up(bTasks), oTQ = oEnt + up(std::max<size_t>(bEnt, 1)), oGrp = oTQ + up(bTQ);
   const size_t total = oGrp + up(std::max<size_t>(bGrp, 1));
-  std::vector<unsigned char> blob(total, 0);
-  std::memcpy(blob.data(), tdev.data(), bTasks);
-  if (bEnt) std::memcpy(blob.data() + oEnt, all.data(), bEnt);
-  std::memcpy(blob.data() + oTQ, plan.slotTQ.data(), bTQ);
-  if (bGrp) std::memcpy(blob.data() + oGrp, allGrp.data(), bGrp);
   plan.oEnt = oEnt;
   plan.oTQ = oTQ;
   plan.oGrp = oGrp;
   if (plan.hostOnly) {
+    std::vector<unsigned char> blob(total, 0);
+    std::memcpy(blob.data(), tdev.data(), bTasks);
+    if (bEnt) std::memcpy(blob.data() + oEnt, all.data(), bEnt);
+    std::memcpy(blob.data() + oTQ, plan.slotTQ.data(), bTQ);
+    if (bGrp) std::memcpy(blob.data() + oGrp, allGrp.data(), bGrp);
     plan.hBlob = std::move(blob);
     return;
   }
+  // (the four parts uploaded from where they lie; the gaps are never read)
   LSW_HIP(devMalloc(&plan.dBlob, total));
-  LSW_HIP(uploadSync(plan.dBlob, blob.data(), total));
+  auto* dst = static_cast<unsigned char*>(plan.dBlob);
+  LSW_HIP(uploadSync(dst, tdev.data(), bTasks));
+  if (bEnt) LSW_HIP(uploadSync(dst + oEnt, all.data(), bEnt));
+  LSW_HIP(uploadSync(dst + oTQ, plan.slotTQ.data(), bTQ));
+  if (bGrp) LSW_HIP(uploadSync(dst + oGrp, allGrp.data(), bGrp));
   plan.bind(plan.dBlob);
+  if (ptime)
+    std::fprintf(stderr, "buildPlan tasks %d entries %zu groups %zu: entries_ms %.2f sort_ms %.2f launch_ms %.2f upload_ms %.2f\n",
+                 nt, all.size(), allGrp.size(), 1e3 * (pt1 - pt0), 1e3 * (pt2 - pt1), 1e3 * (pt3 - pt2), 1e3 * (clk() - pt3));
 }
 
 void SolvePlan::bind(void* base) {

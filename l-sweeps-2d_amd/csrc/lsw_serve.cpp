@@ -4,9 +4,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <set>
 
+#include "lsw_device.h"
 #include "lsw_model.h"
 #include "lsw_solver.h"
 
@@ -286,6 +289,14 @@ void Server::failEntry(Entry& e, int code, const std::string& msg) {
   mDoneCv.notify_all();
 }
 
+namespace {
+// LSW_STEP_LOG=1: one line per worker step on stderr (times, allocations)
+bool stepLog() {
+  static const bool on = std::getenv("LSW_STEP_LOG") != nullptr;
+  return on;
+}
+}  // namespace
+
 void Server::workerLoop() {
   (void)hipSetDevice(0);
   std::vector<GmresSlots::Done> done;
@@ -326,6 +337,9 @@ void Server::workerLoop() {
     }
     lk.unlock();
     const long long bytes0 = threadDeviceBytes();
+    double upS0 = 0, allocS0 = 0;
+    long long ups0 = 0, allocs0 = 0;
+    threadMemTimes(&upS0, &ups0, &allocS0, &allocs0);
     double tSetup = 0, tFactor = 0, tBlock = 0, tStep = 0;
     int err = LSW_OK;
     std::string msg;
@@ -404,6 +418,17 @@ void Server::workerLoop() {
       msg = ex.what();
     }
     const long long grown = threadDeviceBytes() - bytes0;
+    if (stepLog()) {
+      double upS = 0, allocS = 0;
+      long long ups = 0, allocs = 0;
+      threadMemTimes(&upS, &ups, &allocS, &allocs);
+      std::fprintf(stderr,
+                   "step entry %p N %lld load %d slots %d admitted %zu done %zu setup_ms %.2f factor_ms %.2f "
+                   "block_ms %.2f step_ms %.2f allocs %lld alloc_ms %.2f uploads %lld upload_ms %.2f\n",
+                   (void*)&e, e.dd ? (long long)e.dd->N() : 0LL, (int)load, (int)makeSlots, admitted.size(),
+                   finished.size(), 1e3 * tSetup, 1e3 * tFactor, 1e3 * tBlock, 1e3 * tStep, allocs - allocs0,
+                   1e3 * (allocS - allocS0), ups - ups0, 1e3 * (upS - upS0));
+    }
     lk.lock();
     mStats.setupS += tSetup;
     mStats.factorS += tFactor;

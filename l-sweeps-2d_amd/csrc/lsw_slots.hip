@@ -2,7 +2,10 @@
 #include "lsw_slots.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "lsw_cplx.h"
@@ -311,12 +314,28 @@ void GmresSlots::step(std::vector<Done>& done) {
   const int na = L.n;
   const unsigned gx = gridN(mN);
   hipLaunchKernelGGL(k_sgather, dim3(gx, na), dim3(256), 0, s, dIn, dV, L, mM, mN);
-  const int nb = planWidth(na);
+  // A width whose plans exist already is used for fewer active slots too:
+  // building a narrower width's plans costs tens to hundreds of applications'
+  // worth of host time on a large board, and the slots that remain at the
+  // end of a problem usually finish within a few steps.
+  const int built = mDD->builtWidth(na, planWidth(mCap));
+  const int nb = built ? built : planWidth(na);
   if (nb > na)
     LSW_HIP(hipMemsetAsync(dIn + (long long)na * mN, 0, (size_t)(nb - na) * mN * sizeof(double2), s));
+  static const bool log = std::getenv("LSW_STEP_LOG") != nullptr;
+  auto lap = [&](double& t) {  // (LSW_STEP_LOG: phases timed with a wait after each)
+    if (!log) return;
+    (void)hipStreamSynchronize(s);
+    const double n = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    t = n;
+  };
+  double tg = 0, tp = 0, ta = 0, tgs = 0;
+  lap(tg);
   mDD->applyPrecondDev(dIn, dPZ, s, nb);
+  lap(tp);
   hipLaunchKernelGGL(k_sscatter, dim3(gx, na), dim3(256), 0, s, dZ, dPZ, L, mM, mN);
   mDD->applyADevBlock(dPZ, dW, na, s);
+  lap(ta);
   ++mBlockApplies;
   // two passes of classical Gram-Schmidt against each slot's own basis
   const int np = na * L.maxnv;
@@ -338,6 +357,10 @@ void GmresSlots::step(std::vector<Done>& done) {
   LSW_HIP(hipMemcpyAsync(c3.data(), nr, na * sizeof(double2), hipMemcpyDeviceToHost, s));
   LSW_HIP(hipStreamSynchronize(s));
   LSW_HIP(hipGetLastError());
+  lap(tgs);
+  if (log)
+    std::fprintf(stderr, "slots N %lld na %d nb %d maxnv %d precond_ms %.2f scatter_A_ms %.2f gs_ms %.2f\n", (long long)mN,
+                 na, nb, L.maxnv, 1e3 * (tp - tg), 1e3 * (ta - tp), 1e3 * (tgs - ta));
   ++mSteps;
   for (int a = 0; a < na; ++a) {
     const int k = L.slot[a];
