@@ -273,7 +273,7 @@ class FactorSet {
   bool mFactorized = false;
   int mInfo = 0;
   template <typename W2>
-  void factorizeAs(hipStream_t s, W2*& factor);
+  void factorizeAs(hipStream_t s, W2*& factor, bool smallFp32);
 };
 
 }  // namespace lsw

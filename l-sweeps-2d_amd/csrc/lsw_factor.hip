@@ -261,10 +261,18 @@ constexpr int kSmallRows = 96;
 constexpr int kSmallRT = 32, kSmallCT = 16;  // thread grid
 constexpr int kSmallPR = kSmallRows / kSmallRT, kSmallPC = kSmallRows / kSmallCT;
 
-template <typename W2>
+// W2: the arithmetic; O2: the factor and Schur-complement storage (a
+// mixed-precision factorisation computes its small fronts in FP32 and keeps
+// FP64 storage for the heights above).
+template <typename O2, typename W2>
+__device__ __forceinline__ O2 ccast(W2 v) {
+  return CTraits<O2>::make((typename CTraits<O2>::R)v.x, (typename CTraits<O2>::R)v.y);
+}
+
+template <typename W2, typename O2>
 __global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
     const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes, const FrontTask* __restrict__ fts,
-    const double2* __restrict__ coef, W2* __restrict__ factor, W2* __restrict__ schur, int* __restrict__ info) {
+    const double2* __restrict__ coef, O2* __restrict__ factor, O2* __restrict__ schur, int* __restrict__ info) {
   constexpr int RT = kSmallRT, CT = kSmallCT, PR = kSmallPR, PC = kSmallPC, NT = RT * CT;
   __shared__ int rowCnt[kSmallRows];
   __shared__ int rowCol[kSmallRows][5];  // a row's original entries: column
@@ -296,8 +304,8 @@ __global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
     rowAsm[en.fr][s] = a;
   }
   int bc1 = 0, bc2 = 0;
-  const W2* S1 = nullptr;
-  const W2* S2 = nullptr;
+  const O2* S1 = nullptr;
+  const O2* S2 = nullptr;
   if (nd.c1 >= 0) {
     const NdNodeDev cn = sh.nodes[nd.c1];
     bc1 = cn.b;
@@ -329,8 +337,8 @@ __global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
             v = cadd(v, CTraits<W2>::make((typename CTraits<W2>::R)cf.x, (typename CTraits<W2>::R)cf.y));
           }
         const int q1 = inv1[j], q2 = inv2[j];
-        if (r1 >= 0 && q1 >= 0) v = cadd(v, S1[(long long)r1 * bc1 + q1]);
-        if (r2 >= 0 && q2 >= 0) v = cadd(v, S2[(long long)r2 * bc2 + q2]);
+        if (r1 >= 0 && q1 >= 0) v = cadd(v, ccast<W2>(S1[(long long)r1 * bc1 + q1]));
+        if (r2 >= 0 && q2 >= 0) v = cadd(v, ccast<W2>(S2[(long long)r2 * bc2 + q2]));
       }
       M[a][c] = v;
     }
@@ -428,9 +436,9 @@ __global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
   for (int k = tid; k < e; k += NT) stepOf[pivOf[k]] = k;
   __syncthreads();
   // ---- W, K and S
-  W2* Wp = factor + cd.factorOff + nd.fwdOff;
-  W2* Kp = factor + cd.factorOff + nd.bwdOff;
-  W2* Sp = schur + cd.schurOff + nd.sOff;
+  O2* Wp = factor + cd.factorOff + nd.fwdOff;
+  O2* Kp = factor + cd.factorOff + nd.bwdOff;
+  O2* Sp = schur + cd.schurOff + nd.sOff;
 #pragma unroll
   for (int a = 0; a < PR; ++a) {
     const int i = ti + RT * a;
@@ -440,7 +448,7 @@ __global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
     for (int c = 0; c < PC; ++c) {
       const int j = tj + CT * c;
       if (j >= R) continue;
-      const W2 v = M[a][c];
+      const O2 v = ccast<O2>(M[a][c]);
       if (i < e) {
         if (j < e)
           Kp[(long long)pivOf[j] * e + ki] = v;
@@ -1078,6 +1086,19 @@ bool smallFronts() {
   return v;
 }
 
+// FP32-factor sets compute their small fronts (k_front_small) in FP32, the
+// heights above in FP64 (LSW_FACTOR_SMALL_FP32=0: FP64 everywhere).  The
+// small fronts are the leaves and the first separators above them: a few
+// grid lines across, far better conditioned than the top fronts, whose
+// FP32 rounding made an all-FP32 factorisation unusable.
+bool smallFrontsFp32() {
+  static const bool v = [] {
+    const char* e = std::getenv("LSW_FACTOR_SMALL_FP32");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return v;
+}
+
 int blockedMinRows() {
   static const int v = [] {
     const char* e = std::getenv("LSW_FACTOR_BLOCKED_ROWS");
@@ -1153,9 +1174,9 @@ void FactorSet::factorize(int precision, hipStream_t s) {
   LSW_REQUIRE(dCells != nullptr, LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(precision == LSW_FP64 || precision == LSW_FP32, LSW_ERR_ARG, "unknown precision");
   if (factorWorkFp32(precision))
-    factorizeAs<float2>(s, dFactor32);
+    factorizeAs<float2>(s, dFactor32, false);
   else
-    factorizeAs<double2>(s, dFactor);
+    factorizeAs<double2>(s, dFactor, precision == LSW_FP32 && smallFrontsFp32());
   mPrecision = precision;
   if (precision == LSW_FP32 && dFactor) {
     if (!dFactor32) LSW_HIP(devMalloc(&dFactor32, std::max<long long>(mFactorEntries, 1) * sizeof(float2)));
@@ -1170,7 +1191,7 @@ void FactorSet::factorize(int precision, hipStream_t s) {
 }
 
 template <typename W2>
-void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
+void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
   if (!factor) LSW_HIP(devMalloc(&factor, std::max<long long>(mFactorEntries, 1) * sizeof(W2)));
   // Schur workspace for all cells
   std::vector<long long> schurOff(mCells.size());
@@ -1258,8 +1279,12 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor) {
     const size_t shm = (size_t)mMaxFront * sizeof(W2);
     const unsigned nf = (unsigned)bt.size();
     if (batchBlocked[bi] == 2) {
-      hipLaunchKernelGGL(k_front_small<W2>, dim3(nf), dim3(kSmallRT * kSmallCT), 0, s, dCells, dShapes, dFt, dCoef,
-                         factor, dSchur, dInfo);
+      if (smallFp32)
+        hipLaunchKernelGGL((k_front_small<float2, W2>), dim3(nf), dim3(kSmallRT * kSmallCT), 0, s, dCells, dShapes,
+                           dFt, dCoef, factor, dSchur, dInfo);
+      else
+        hipLaunchKernelGGL((k_front_small<W2, W2>), dim3(nf), dim3(kSmallRT * kSmallCT), 0, s, dCells, dShapes, dFt,
+                           dCoef, factor, dSchur, dInfo);
     } else if (!batchBlocked[bi]) {
       hipLaunchKernelGGL(k_front_factor<W2>, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dCoef, factor,
                          dSchur, dWs, dInfo);
