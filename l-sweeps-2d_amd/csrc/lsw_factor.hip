@@ -240,10 +240,10 @@ __global__ __launch_bounds__(256) void k_front_factor(
 }
 
 // ---- small fronts in registers
-// Fronts of at most kSmallRows rows (the leaves and the first heights above
-// them): one workgroup of 512 threads per front, the R x R front held in
-// registers (thread (ti, tj) of a 32 x 16 grid owns rows ti + 32a and
-// columns tj + 16c), no workspace.  Gauss-Jordan exchange steps in place:
+// Fronts of at most kMidRows rows (the leaves and the first heights above
+// them): one workgroup per front, the R x R front held in registers
+// (thread (ti, tj) of a 32 x CT grid owns rows ti + 32a and columns
+// tj + CT c), no workspace.  Gauss-Jordan exchange steps in place:
 // pivot (p, k) exchanges equation p and unknown k,
 //   M[p][k] <- 1/M[p][k],  M[p][j] <- -M[p][j]/M[p][k],
 //   M[i][k] <- M[i][k]/M[p][k],  M[i][j] <- M[i][j] - M[i][k] M[p][j]/M[p][k],
@@ -257,9 +257,10 @@ __global__ __launch_bounds__(256) void k_front_factor(
 // owners are the 32 lanes of one half wavefront: they pick the pivot with
 // lane shuffles and publish it with the column; one workgroup barrier per
 // step, the row going round by lane shuffles.
+// Two shapes on 512 threads: 96 rows (3 rows x 6 columns a thread) and 128
+// rows (4 x 8); the thread grid is 32 rows of threads by ROWS / PC columns.
 constexpr int kSmallRows = 96;
-constexpr int kSmallRT = 32, kSmallCT = 16;  // thread grid
-constexpr int kSmallPR = kSmallRows / kSmallRT, kSmallPC = kSmallRows / kSmallCT;
+constexpr int kMidRows = 128;
 
 // W2: the arithmetic; O2: the factor and Schur-complement storage (a
 // mixed-precision factorisation computes its small fronts in FP32 and keeps
@@ -269,17 +270,18 @@ __device__ __forceinline__ O2 ccast(W2 v) {
   return CTraits<O2>::make((typename CTraits<O2>::R)v.x, (typename CTraits<O2>::R)v.y);
 }
 
-template <typename W2, typename O2>
-__global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
+template <typename W2, typename O2, int PR, int PC>
+__global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
     const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes, const FrontTask* __restrict__ fts,
     const double2* __restrict__ coef, O2* __restrict__ factor, O2* __restrict__ schur, int* __restrict__ info) {
-  constexpr int RT = kSmallRT, CT = kSmallCT, PR = kSmallPR, PC = kSmallPC, NT = RT * CT;
-  __shared__ int rowCnt[kSmallRows];
-  __shared__ int rowCol[kSmallRows][5];  // a row's original entries: column
-  __shared__ int rowAsm[kSmallRows][5];  // ... and assembly-list index
-  __shared__ int inv1[kSmallRows], inv2[kSmallRows];  // front row -> child boundary index
-  __shared__ int pivOf[kSmallRows], stepOf[kSmallRows];
-  __shared__ W2 colK[2][kSmallRows];
+  constexpr int RT = 32, ROWS = RT * PR, CT = ROWS / PC, NT = RT * CT;
+  static_assert(CT * PC == ROWS && CT <= 32, "thread grid");
+  __shared__ int rowCnt[ROWS];
+  __shared__ int rowCol[ROWS][5];  // a row's original entries: column
+  __shared__ int rowAsm[ROWS][5];  // ... and assembly-list index
+  __shared__ int inv1[ROWS], inv2[ROWS];  // front row -> child boundary index
+  __shared__ int pivOf[ROWS], stepOf[ROWS];
+  __shared__ W2 colK[2][ROWS];
   __shared__ double sMag[2];
   __shared__ int sP[2];
   const FrontTask ft = fts[blockIdx.x];
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(kSmallRT * kSmallCT) void k_front_small(
   const int tid = threadIdx.x, ti = tid % RT, tj = tid / RT;
   // ---- assembly: original entries, then child 1's and child 2's Schur
   // complements, each position in that order (as the workspace schedules)
-  for (int i = tid; i < kSmallRows; i += NT) {
+  for (int i = tid; i < ROWS; i += NT) {
     rowCnt[i] = 0;
     inv1[i] = -1;
     inv2[i] = -1;
@@ -1076,7 +1078,7 @@ namespace {
 // factorisation took 0.39 s with one workgroup per front everywhere, 0.20 s
 // with the blocked schedule from 192 rows up and 0.149 s with it on every
 // height.  LSW_FACTOR_BLOCKED_ROWS overrides it for tuning.
-// Heights whose fronts all have at most kSmallRows rows take k_front_small
+// Heights whose fronts all have at most kMidRows rows take k_front_small
 // (LSW_FACTOR_SMALL=0: the workspace schedules, for comparison).
 bool smallFronts() {
   static const bool v = [] {
@@ -1225,8 +1227,9 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
     // heights of small fronts only: one register-resident workgroup per
     // front, no workspace (state 2)
     const bool small = smallFronts() && maxR <= kSmallRows;
+    const bool mid = !small && smallFronts() && maxR <= kMidRows;
     const bool blocked = maxR >= blockedMinRows();
-    const char state = small ? 2 : blocked ? 1 : 0;
+    const char state = small ? 2 : mid ? 3 : blocked ? 1 : 0;
     std::vector<FrontTask> cur;
     long long ws = 0;
     for (size_t ci = 0; ci < mCells.size(); ++ci) {
@@ -1235,7 +1238,7 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
       for (int p : t.byHeight[h]) {
         const NdNode& nd = t.nodes[p];
         const long long R = nd.e + nd.b, Cc = 2 * nd.e + nd.b;
-        const long long need = small ? 0 : R * Cc + (blocked ? R * kPanel + kPanel * Cc : 0);
+        const long long need = (small || mid) ? 0 : R * Cc + (blocked ? R * kPanel + kPanel * Cc : 0);
         if (!cur.empty() && ws + need > budget) {
           batches.push_back(cur);
           batchWs.push_back(ws);
@@ -1280,11 +1283,14 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
     const unsigned nf = (unsigned)bt.size();
     if (batchBlocked[bi] == 2) {
       if (smallFp32)
-        hipLaunchKernelGGL((k_front_small<float2, W2>), dim3(nf), dim3(kSmallRT * kSmallCT), 0, s, dCells, dShapes,
-                           dFt, dCoef, factor, dSchur, dInfo);
+        hipLaunchKernelGGL((k_front_small<float2, W2, 3, 6>), dim3(nf), dim3(512), 0, s, dCells, dShapes, dFt, dCoef,
+                           factor, dSchur, dInfo);
       else
-        hipLaunchKernelGGL((k_front_small<W2, W2>), dim3(nf), dim3(kSmallRT * kSmallCT), 0, s, dCells, dShapes, dFt,
-                           dCoef, factor, dSchur, dInfo);
+        hipLaunchKernelGGL((k_front_small<W2, W2, 3, 6>), dim3(nf), dim3(512), 0, s, dCells, dShapes, dFt, dCoef,
+                           factor, dSchur, dInfo);
+    } else if (batchBlocked[bi] == 3) {
+      hipLaunchKernelGGL((k_front_small<W2, W2, 4, 8>), dim3(nf), dim3(512), 0, s, dCells, dShapes, dFt, dCoef,
+                         factor, dSchur, dInfo);
     } else if (!batchBlocked[bi]) {
       hipLaunchKernelGGL(k_front_factor<W2>, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dCoef, factor,
                          dSchur, dWs, dInfo);
