@@ -257,8 +257,9 @@ __global__ __launch_bounds__(256) void k_front_factor(
 // owners are the 32 lanes of one half wavefront: they pick the pivot with
 // lane shuffles and publish it with the column; one workgroup barrier per
 // step, the row going round by lane shuffles.
-// Two shapes on 512 threads: 96 rows (3 rows x 6 columns a thread) and 128
-// rows (4 x 8); the thread grid is 32 rows of threads by ROWS / PC columns.
+// Shapes: 96 rows as 3 rows x 6 columns a thread (512 threads; FP32
+// arithmetic) or 3 x 3 (1024 threads; FP64), 128 rows as 4 x 8 (512
+// threads); the thread grid is 32 rows of threads by ROWS / PC columns.
 constexpr int kSmallRows = 96;
 constexpr int kMidRows = 128;
 
@@ -1285,8 +1286,8 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
       if (smallFp32)
         hipLaunchKernelGGL((k_front_small<float2, W2, 3, 6>), dim3(nf), dim3(512), 0, s, dCells, dShapes, dFt, dCoef,
                            factor, dSchur, dInfo);
-      else
-        hipLaunchKernelGGL((k_front_small<W2, W2, 3, 6>), dim3(nf), dim3(512), 0, s, dCells, dShapes, dFt, dCoef,
+      else  // (FP64: 3 x 3 a thread on 1024 threads, 116 VGPRs; 3 x 6 on 512 needed 154)
+        hipLaunchKernelGGL((k_front_small<W2, W2, 3, 3>), dim3(nf), dim3(1024), 0, s, dCells, dShapes, dFt, dCoef,
                            factor, dSchur, dInfo);
     } else if (batchBlocked[bi] == 3) {
       hipLaunchKernelGGL((k_front_small<W2, W2, 4, 8>), dim3(nf), dim3(512), 0, s, dCells, dShapes, dFt, dCoef,
