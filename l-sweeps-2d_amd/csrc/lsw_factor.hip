@@ -262,6 +262,7 @@ __global__ __launch_bounds__(256) void k_front_factor(
 // threads); the thread grid is 32 rows of threads by ROWS / PC columns.
 constexpr int kSmallRows = 96;
 constexpr int kMidRows = 128;
+constexpr int kBig32Rows = 192;  // FP32 arithmetic only
 
 // W2: the arithmetic; O2: the factor and Schur-complement storage (a
 // mixed-precision factorisation computes its small fronts in FP32 and keeps
@@ -1229,8 +1230,11 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
     // front, no workspace (state 2)
     const bool small = smallFronts() && maxR <= kSmallRows;
     const bool mid = !small && smallFronts() && maxR <= kMidRows;
+    // FP32 arithmetic: fronts of up to 192 rows fit the registers of one
+    // 1024-thread workgroup (6 x 6 entries a thread)
+    const bool big32 = !small && !mid && smallFronts() && smallFp32 && maxR <= kBig32Rows;
     const bool blocked = maxR >= blockedMinRows();
-    const char state = small ? 2 : mid ? 3 : blocked ? 1 : 0;
+    const char state = small ? 2 : mid ? 3 : big32 ? 4 : blocked ? 1 : 0;
     std::vector<FrontTask> cur;
     long long ws = 0;
     for (size_t ci = 0; ci < mCells.size(); ++ci) {
@@ -1239,7 +1243,7 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
       for (int p : t.byHeight[h]) {
         const NdNode& nd = t.nodes[p];
         const long long R = nd.e + nd.b, Cc = 2 * nd.e + nd.b;
-        const long long need = (small || mid) ? 0 : R * Cc + (blocked ? R * kPanel + kPanel * Cc : 0);
+        const long long need = (small || mid || big32) ? 0 : R * Cc + (blocked ? R * kPanel + kPanel * Cc : 0);
         if (!cur.empty() && ws + need > budget) {
           batches.push_back(cur);
           batchWs.push_back(ws);
@@ -1289,6 +1293,9 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
       else  // (FP64: 3 x 3 a thread on 1024 threads, 116 VGPRs; 3 x 6 on 512 needed 154)
         hipLaunchKernelGGL((k_front_small<W2, W2, 3, 3>), dim3(nf), dim3(1024), 0, s, dCells, dShapes, dFt, dCoef,
                            factor, dSchur, dInfo);
+    } else if (batchBlocked[bi] == 4) {
+      hipLaunchKernelGGL((k_front_small<float2, W2, 6, 6>), dim3(nf), dim3(1024), 0, s, dCells, dShapes, dFt, dCoef,
+                         factor, dSchur, dInfo);
     } else if (batchBlocked[bi] == 3) {
       hipLaunchKernelGGL((k_front_small<W2, W2, 4, 8>), dim3(nf), dim3(512), 0, s, dCells, dShapes, dFt, dCoef,
                          factor, dSchur, dInfo);
