@@ -1446,7 +1446,11 @@ void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady
                         64.0 * n;
   // solve plans: about 32 bytes per local DOF per block width, a dozen widths
   const double plans = 12.0 * 32.0 * local;
-  const double s = entries * es + fixed + r * perRhs + plans;
+  // allocation granularity: the 4 MiB descriptor blocks of the DD's and the
+  // factor set's arenas and the 2 MiB pages of every larger buffer (measured:
+  // up to 32 MB over the byte counts above on cells of ~16k points)
+  const double grain = 40.0 * (1 << 20);
+  const double s = entries * es + fixed + r * perRhs + plans + grain;
   double ws = 0;
   for (double w : wsPerHeight) ws = std::max(ws, w);
   // fronts, Schur complements and factors while factorising: FP32 work
@@ -1454,7 +1458,7 @@ void DDCheckerboard::deviceBytesEstimate(int precision, int nrhs, double* steady
   const double wes = factorWorkFp32(precision) ? 8.0 : 16.0;
   ws = std::min(ws, (double)kFactorWsEntries) * wes;
   const double f = fixed + perRhs + (entries + schur) * wes + ws +
-                   (precision == LSW_FP32 && wes == 16.0 ? entries * 8.0 : 0.0);
+                   (precision == LSW_FP32 && wes == 16.0 ? entries * 8.0 : 0.0) + grain;
   if (steady) *steady = s;
   if (peak) *peak = std::max(s, f);
 }

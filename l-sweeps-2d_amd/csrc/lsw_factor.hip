@@ -81,12 +81,20 @@ __device__ __forceinline__ W2 cinvT(W2 a) {
   return CTraits<W2>::make(a.x / d, -a.y / d);
 }
 
+// W2: the arithmetic; O2: the factor and Schur-complement storage (a
+// mixed-precision factorisation computes some heights in FP32 and keeps FP64
+// storage for the heights above).
+template <typename O2, typename W2>
+__device__ __forceinline__ O2 ccast(W2 v) {
+  return CTraits<O2>::make((typename CTraits<O2>::R)v.x, (typename CTraits<O2>::R)v.y);
+}
+
 // zero the front, put the identity block, add the original entries (the
 // FP64 stencil, rounded to the work type) and the children's Schur
-// complements (one workgroup)
-template <typename W2>
+// complements (stored as O2; one workgroup)
+template <typename W2, typename O2>
 __device__ void frontAssemble(const CellDev& cd, const ShapeDev& sh, const FrontView<W2>& f,
-                              const double2* __restrict__ coef, const W2* __restrict__ schur) {
+                              const double2* __restrict__ coef, const O2* __restrict__ schur) {
   using R = typename CTraits<W2>::R;
   const int tid = threadIdx.x, nt = blockDim.x;
   const W2 zero = czero<W2>();
@@ -108,39 +116,39 @@ __device__ void frontAssemble(const CellDev& cd, const ShapeDev& sh, const Front
     if (c < 0) continue;
     const NdNodeDev cn = sh.nodes[c];
     const int bc = cn.b;
-    const W2* S = schur + cd.schurOff + cn.sOff;
+    const O2* S = schur + cd.schurOff + cn.sOff;
     const int* up = sh.up + cn.upOff;
     for (long long idx = tid; idx < (long long)bc * bc; idx += nt) {
       const int t1 = (int)(idx / bc), t2 = (int)(idx - (long long)t1 * bc);
       W2* t = &M[(long long)up[t1] * Cc + up[t2]];
-      *t = cadd(*t, S[idx]);
+      *t = cadd(*t, ccast<W2>(S[idx]));
     }
     __syncthreads();
   }
 }
 
 // write W, K and S from the reduced front (one workgroup)
-template <typename W2>
-__device__ void frontStore(const CellDev& cd, const FrontView<W2>& f, W2* __restrict__ factor,
-                           W2* __restrict__ schur) {
+template <typename W2, typename O2>
+__device__ void frontStore(const CellDev& cd, const FrontView<W2>& f, O2* __restrict__ factor,
+                           O2* __restrict__ schur) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const W2* M = f.M;
   const int e = f.e, b = f.b, R = f.R, Cc = f.Cc;
-  W2* W = factor + cd.factorOff + f.nd.fwdOff;
-  W2* K = factor + cd.factorOff + f.nd.bwdOff;
+  O2* W = factor + cd.factorOff + f.nd.fwdOff;
+  O2* K = factor + cd.factorOff + f.nd.bwdOff;
   for (long long idx = tid; idx < (long long)b * e; idx += nt) {
     const int k = (int)(idx / b), t = (int)(idx - (long long)k * b);
-    W[idx] = cneg(M[(long long)(e + t) * Cc + e + b + k]);
+    W[idx] = ccast<O2>(cneg(M[(long long)(e + t) * Cc + e + b + k]));
   }
   for (long long idx = tid; idx < (long long)e * R; idx += nt) {
     const int k = (int)(idx / e), r = (int)(idx - (long long)k * e);
-    K[idx] = (k < e) ? M[(long long)r * Cc + e + b + k] : cneg(M[(long long)r * Cc + k]);
+    K[idx] = ccast<O2>((k < e) ? M[(long long)r * Cc + e + b + k] : cneg(M[(long long)r * Cc + k]));
   }
   if (b > 0) {
-    W2* S = schur + cd.schurOff + f.nd.sOff;
+    O2* S = schur + cd.schurOff + f.nd.sOff;
     for (long long idx = tid; idx < (long long)b * b; idx += nt) {
       const int t1 = (int)(idx / b), t2 = (int)(idx - (long long)t1 * b);
-      S[idx] = M[(long long)(e + t1) * Cc + e + t2];
+      S[idx] = ccast<O2>(M[(long long)(e + t1) * Cc + e + t2]);
     }
   }
 }
@@ -263,14 +271,6 @@ __global__ __launch_bounds__(256) void k_front_factor(
 constexpr int kSmallRows = 96;
 constexpr int kMidRows = 128;
 constexpr int kBig32Rows = 192;  // FP32 arithmetic only
-
-// W2: the arithmetic; O2: the factor and Schur-complement storage (a
-// mixed-precision factorisation computes its small fronts in FP32 and keeps
-// FP64 storage for the heights above).
-template <typename O2, typename W2>
-__device__ __forceinline__ O2 ccast(W2 v) {
-  return CTraits<O2>::make((typename CTraits<O2>::R)v.x, (typename CTraits<O2>::R)v.y);
-}
 
 template <typename W2, typename O2, int PR, int PC>
 __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
@@ -472,22 +472,22 @@ __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
 // kPanel, row-major) and B (kPanel x Cc, row-major).
 constexpr int kPanel = 16;
 
-template <typename W2>
+template <typename W2, typename O2>
 __global__ __launch_bounds__(256) void k_front_assemble(const CellDev* __restrict__ cells,
                                                         const ShapeDev* __restrict__ shapes,
                                                         const FrontTask* __restrict__ fts,
                                                         const double2* __restrict__ coef,
-                                                        const W2* __restrict__ schur, W2* __restrict__ ws) {
+                                                        const O2* __restrict__ schur, W2* __restrict__ ws) {
   const FrontTask ft = fts[blockIdx.x];
   const CellDev& cd = cells[ft.cell];
   frontAssemble(cd, shapes[cd.shape], frontView(cells, shapes, ft, ws), coef, schur);
 }
 
-template <typename W2>
+template <typename W2, typename O2>
 __global__ __launch_bounds__(256) void k_front_store(const CellDev* __restrict__ cells,
                                                      const ShapeDev* __restrict__ shapes,
-                                                     const FrontTask* __restrict__ fts, W2* __restrict__ factor,
-                                                     W2* __restrict__ schur, W2* __restrict__ ws) {
+                                                     const FrontTask* __restrict__ fts, O2* __restrict__ factor,
+                                                     O2* __restrict__ schur, W2* __restrict__ ws) {
   const FrontTask ft = fts[blockIdx.x];
   frontStore(cells[ft.cell], frontView(cells, shapes, ft, ws), factor, schur);
 }
@@ -499,9 +499,12 @@ __global__ __launch_bounds__(256) void k_front_store(const CellDev* __restrict__
 // later scaling, swap and elimination, so that at the end, in the swapped
 // row order, T = [A11^{-1}; -A21 A11^{-1}] (A11 = the pivot rows' panel
 // block, A21 = the other rows').  The rest of the front is then
-//   new = old + (T - E) B,  B = old pivot rows (columns after the panel),
-// E = 1 at (pivot row c, c): the batched update below.  B is copied here,
-// before that update changes the pivot rows.
+//   pivot rows: new = T B,  other rows: new = old + T B,
+// B = old pivot rows (columns after the panel): the batched update below.
+// (Not old + (T - E) B with E = 1 at (pivot row c, c), which is the same
+// in exact arithmetic: T - E rounds A11^{-1}'s diagonal against 1, which
+// lost most FP32 digits where |A11^{-1}| << 1.)  B is copied here, before
+// that update changes the pivot rows.
 template <typename W2>
 __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__ cells,
                                                      const ShapeDev* __restrict__ shapes,
@@ -575,15 +578,11 @@ __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__
     for (int i = tid; i < R; i += nt) M[(long long)i * Cc + k] = (i == k) ? CTraits<W2>::make(1, 0) : zero;
     __syncthreads();
   }
-  // B = the pivot rows from the first column after the panel; T -= E
+  // B = the pivot rows from the first column after the panel
   const int cw = Cc - pend;
   for (long long idx = tid; idx < (long long)nbk * cw; idx += nt) {
     const int c = (int)(idx / cw), j = (int)(idx - (long long)c * cw);
     B[(long long)c * Cc + j] = M[(long long)(k0 + c) * Cc + pend + j];
-  }
-  for (int c = tid; c < nbk; c += nt) {
-    W2* t = &T[(k0 + c) * kPanel + c];
-    t->x -= 1;
   }
 }
 
@@ -710,13 +709,11 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
     for (int i = tid; i < R; i += nt) sP[i * kPanelLd + kk] = (i == k) ? one : zero;
     __syncthreads();
   }
-  // panel columns and T (minus E) back to the workspace
+  // panel columns and T back to the workspace
   for (int idx = tid; idx < R * kPanel; idx += nt) {
     const int i = idx / kPanel, c = idx - i * kPanel;
     if (c < nbk) M[(long long)i * Cc + k0 + c] = sP[i * kPanelLd + c];
-    double2 t = sT[i * kPanelLd + c];
-    if (c < nbk && i == k0 + c) t.x -= 1.0;
-    T[i * kPanel + c] = t;
+    T[i * kPanel + c] = sT[i * kPanelLd + c];
   }
   // the recorded swaps on the columns after the panel, then B
   for (int j = pend + tid; j < Cc; j += nt) {
@@ -732,8 +729,9 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
   }
 }
 
-// new = old + (T - E) B over rows [0, R) and columns [pend, Cc) of every
-// front: 64 x 64 output tiles, 4 x 4 per thread, the panel's T and B slices
+// new = T B (the panel's pivot rows) or old + T B (the other rows) over rows
+// [0, R) and columns [pend, Cc) of every front: 64 x 64 output tiles, 4 x 4
+// per thread, the panel's T and B slices
 // staged in LDS.  blockIdx.y = front, blockIdx.x = tile.  (FP64 fronts; kept
 // for comparison with the matrix-core kernel.)
 __global__ __launch_bounds__(256) void k_front_trail(const CellDev* __restrict__ cells,
@@ -792,7 +790,7 @@ __global__ __launch_bounds__(256) void k_front_trail(const CellDev* __restrict__
       const int j = c0 + tx * 4 + q;
       if (j >= cw) continue;
       double2* t = &M[(long long)i * Cc + pend + j];
-      *t = cadd(*t, acc[a][q]);
+      *t = (i >= k0 && i < pend) ? acc[a][q] : cadd(*t, acc[a][q]);
     }
   }
 }
@@ -894,7 +892,8 @@ __global__ __launch_bounds__(256) void k_front_trail_mfma(const CellDev* __restr
         const int i = r0 + qr + a * 16 + A::row(lk, r);
         if (i < R && j < cw) {
           W2* t = &M[(long long)i * Cc + pend + j];
-          *t = cadd(*t, CTraits<W2>::make(cr[a][b][r], ci[a][b][r]));
+          const W2 v = CTraits<W2>::make(cr[a][b][r], ci[a][b][r]);
+          *t = (i >= k0 && i < pend) ? v : cadd(*t, v);
         }
       }
     }
@@ -1090,11 +1089,9 @@ bool smallFronts() {
   return v;
 }
 
-// FP32-factor sets compute their small fronts (k_front_small) in FP32, the
-// heights above in FP64 (LSW_FACTOR_SMALL_FP32=0: FP64 everywhere).  The
-// small fronts are the leaves and the first separators above them: a few
-// grid lines across, far better conditioned than the top fronts, whose
-// FP32 rounding made an all-FP32 factorisation unusable.
+// With FP64 fronts for an FP32-factor set (LSW_FACTOR_FP32=0), the small
+// fronts (k_front_small) still run in FP32 arithmetic unless
+// LSW_FACTOR_SMALL_FP32=0.
 bool smallFrontsFp32() {
   static const bool v = [] {
     const char* e = std::getenv("LSW_FACTOR_SMALL_FP32");
@@ -1160,16 +1157,16 @@ void launchTrail(dim3 grid, hipStream_t s, const CellDev* cells, const ShapeDev*
 }
 }  // namespace
 
-// FP32 fronts are opt-in: the explicit inverses F11^{-1} and the Schur
-// complements passed up the tree amplify FP32 rounding by the fronts'
-// condition numbers.  Measured: normwise backward error 3.7e-4 against
-// 1.9e-7 (FP64 fronts, FP32 factors) on an all-PML 59 x 59 cell, and on the
-// 48-problem pack flexible GMRES ran into its 200-iteration cap (7.6
-// solves/s against 88.6).
+// FP32-factor sets factorise in FP32 (LSW_FACTOR_FP32=0: FP64 fronts,
+// FP32 copy of the factors).  M^{-1} against the FP64-factor one: 2.9e-5
+// relative on a 3 x 3 board of 40 x 40 cells (FP64 fronts: 5.1e-5), and the
+// same GMRES iteration counts on the 48-problem pack.  (The panel update
+// once formed old + (T - E) B for the pivot rows, which cancels A11^{-1}'s
+// diagonal against 1: FP32 fronts then gave 0.11 on that board.)
 bool factorWorkFp32(int precision) {
   static const bool fp32Work = [] {
     const char* e = std::getenv("LSW_FACTOR_FP32");
-    return e && std::atoi(e) != 0;
+    return !(e && std::atoi(e) == 0);
   }();
   return precision == LSW_FP32 && fp32Work;
 }
@@ -1178,7 +1175,7 @@ void FactorSet::factorize(int precision, hipStream_t s) {
   LSW_REQUIRE(dCells != nullptr, LSW_ERR_STATE, "FactorSet: setup() not called");
   LSW_REQUIRE(precision == LSW_FP64 || precision == LSW_FP32, LSW_ERR_ARG, "unknown precision");
   if (factorWorkFp32(precision))
-    factorizeAs<float2>(s, dFactor32, false);
+    factorizeAs<float2>(s, dFactor32, true);
   else
     factorizeAs<double2>(s, dFactor, precision == LSW_FP32 && smallFrontsFp32());
   mPrecision = precision;
@@ -1303,26 +1300,33 @@ void FactorSet::factorizeAs(hipStream_t s, W2*& factor, bool smallFp32) {
       hipLaunchKernelGGL(k_front_factor<W2>, dim3(nf), dim3(256), shm, s, dCells, dShapes, dFt, dCoef, factor,
                          dSchur, dWs, dInfo);
     } else {
-      hipLaunchKernelGGL(k_front_assemble<W2>, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dCoef, dSchur, dWs);
-      int maxE = 0;
-      for (const auto& ft : bt) maxE = std::max(maxE, nodeOf(ft).e);
-      for (int k0 = 0; k0 < maxE; k0 += kPanel) {
-        int maxR = 0;
-        for (const auto& ft : bt) {
-          const NdNode& nd = nodeOf(ft);
-          if (nd.e > k0) maxR = std::max(maxR, nd.e + nd.b);
+      // the workspace schedule
+      auto blockedRun = [&](auto* wsT) {
+        using WT = typename std::remove_pointer<decltype(wsT)>::type;
+        hipLaunchKernelGGL((k_front_assemble<WT, W2>), dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, dCoef,
+                           dSchur, wsT);
+        int maxE = 0;
+        for (const auto& ft : bt) maxE = std::max(maxE, nodeOf(ft).e);
+        for (int k0 = 0; k0 < maxE; k0 += kPanel) {
+          int maxR = 0;
+          for (const auto& ft : bt) {
+            const NdNode& nd = nodeOf(ft);
+            if (nd.e > k0) maxR = std::max(maxR, nd.e + nd.b);
+          }
+          launchPanel(nf, maxR, shm, s, dCells, dShapes, dFt, wsT, k0, dInfo);
+          long long tiles = 0;
+          for (const auto& ft : bt) {
+            const NdNode& nd = nodeOf(ft);
+            if (nd.e <= k0) continue;
+            const long long R = nd.e + nd.b, cw = 2LL * nd.e + nd.b - std::min(k0 + kPanel, nd.e);
+            tiles = std::max(tiles, ((R + 63) / 64) * ((cw + 63) / 64));
+          }
+          if (tiles > 0) launchTrail(dim3((unsigned)tiles, nf), s, dCells, dShapes, dFt, wsT, k0);
         }
-        launchPanel(nf, maxR, shm, s, dCells, dShapes, dFt, dWs, k0, dInfo);
-        long long tiles = 0;
-        for (const auto& ft : bt) {
-          const NdNode& nd = nodeOf(ft);
-          if (nd.e <= k0) continue;
-          const long long R = nd.e + nd.b, cw = 2LL * nd.e + nd.b - std::min(k0 + kPanel, nd.e);
-          tiles = std::max(tiles, ((R + 63) / 64) * ((cw + 63) / 64));
-        }
-        if (tiles > 0) launchTrail(dim3((unsigned)tiles, nf), s, dCells, dShapes, dFt, dWs, k0);
-      }
-      hipLaunchKernelGGL(k_front_store<W2>, dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, factor, dSchur, dWs);
+        hipLaunchKernelGGL((k_front_store<WT, W2>), dim3(nf), dim3(256), 0, s, dCells, dShapes, dFt, factor, dSchur,
+                           wsT);
+      };
+      blockedRun(dWs);
     }
     LSW_HIP(hipGetLastError());
   }

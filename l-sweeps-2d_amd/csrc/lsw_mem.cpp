@@ -21,7 +21,9 @@
 //     allocating them again later stalled single hipMalloc calls for
 //     1-2 seconds on a busy device.
 // The per-thread balance (what a thread allocated minus what it freed) counts
-// block sizes, so the server charges a problem the device memory it holds.
+// the requested sizes in whole blocks (256 B, or 2 MiB pages above 2 MiB), so
+// the server charges a problem the memory it asked for, independent of which
+// larger cached block served a request.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -38,7 +40,10 @@ namespace lsw {
 
 namespace {
 std::mutex gMu;
-std::unordered_map<void*, size_t> gSizes;     // live blocks (handed out) -> block size
+struct Live {
+  size_t block, charged;  // the block's size; what the owner's balance counts
+};
+std::unordered_map<void*, Live> gSizes;       // live blocks (handed out)
 std::multimap<size_t, void*> gReady;          // cached blocks safe to reuse, by size
 std::multimap<size_t, void*> gPending;        // cached blocks freed since the last device sync
 std::atomic<long long> gTotal{0};             // bytes handed out
@@ -105,10 +110,10 @@ long long deviceCap() {
   return cap;
 }
 
-void handOut(void* p, size_t size) {
-  gSizes[p] = size;
+void handOut(void* p, size_t size, size_t charged) {
+  gSizes[p] = Live{size, charged};
   gTotal += (long long)size;
-  tBalance += (long long)size;
+  tBalance += (long long)charged;
 }
 
 // LSW_MEM_LOG=1 prints the slow allocator events (waits, runtime calls)
@@ -196,7 +201,7 @@ hipError_t devMallocBytes(void** p, size_t bytes) {
   }
   if (q) {
     gCached -= (long long)got;
-    handOut(q, got);
+    handOut(q, got, size);
     *p = q;
     return hipSuccess;
   }
@@ -215,7 +220,7 @@ hipError_t devMallocBytes(void** p, size_t bytes) {
     e = hipMalloc(&q, size);
   }
   if (e == hipSuccess && q) {
-    handOut(q, size);
+    handOut(q, size, size);
     *p = q;
   }
   return e;
@@ -226,10 +231,10 @@ hipError_t devFree(void* p) {
   std::lock_guard<std::mutex> lk(gMu);
   auto it = gSizes.find(p);
   if (it == gSizes.end()) return hipFree(p);  // not ours
-  const size_t size = it->second;
+  const size_t size = it->second.block;
+  tBalance -= (long long)it->second.charged;
   gSizes.erase(it);
   gTotal -= (long long)size;
-  tBalance -= (long long)size;
   (tIdleFrees > 0 ? gReady : gPending).insert({size, p});
   gCached += (long long)size;
   return hipSuccess;

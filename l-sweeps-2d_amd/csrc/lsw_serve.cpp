@@ -15,6 +15,14 @@
 
 namespace lsw {
 
+namespace {
+// LSW_STEP_LOG=1: one line per worker step on stderr (times, allocations)
+bool stepLog() {
+  static const bool on = std::getenv("LSW_STEP_LOG") != nullptr;
+  return on;
+}
+}  // namespace
+
 std::unique_ptr<DDCheckerboard> makeCheckerboard(const lsw_dd_params& p, bool device) {
   LSW_REQUIRE(p.precision == LSW_PREC_FP64 || p.precision == LSW_PREC_FP32, LSW_ERR_ARG, "unknown precision");
   LSW_REQUIRE(p.omega > 0 && p.min_wavelengths_in_pml >= 0, LSW_ERR_ARG, "bad Omega / MinWavelengthsInPML");
@@ -191,6 +199,9 @@ bool Server::admit(Entry& e, double need) {
     e.reserved += need;
     mStats.reservedBytes += need;
     mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
+    if (stepLog())
+      std::fprintf(stderr, "admit entry %p need %.0f reserved %.0f total %.0f\n", (void*)&e, need, e.reserved,
+                   mStats.reservedBytes);
   };
   if (mStats.reservedBytes + need <= mOpt.budgetBytes) {
     take();
@@ -289,13 +300,6 @@ void Server::failEntry(Entry& e, int code, const std::string& msg) {
   mDoneCv.notify_all();
 }
 
-namespace {
-// LSW_STEP_LOG=1: one line per worker step on stderr (times, allocations)
-bool stepLog() {
-  static const bool on = std::getenv("LSW_STEP_LOG") != nullptr;
-  return on;
-}
-}  // namespace
 
 void Server::workerLoop() {
   (void)hipSetDevice(0);
@@ -452,6 +456,10 @@ void Server::workerLoop() {
       mStats.reservedBytes += keep - e.reserved;
       e.reserved = keep;
     }
+    if (stepLog())
+      std::fprintf(stderr, "reserve entry %p load %d released %d actual %.0f steady %.0f krylov %.0f reserved %.0f total %.0f\n",
+                   (void*)&e, (int)load, (int)released, e.actual, e.steadyBytes, e.krylov, e.reserved,
+                   mStats.reservedBytes);
     mStats.peakReservedBytes = std::max(mStats.peakReservedBytes, mStats.reservedBytes);
     if (err != LSW_OK) {
       for (auto& t : take) e.pending.push_front(t.second);  // failed with the entry below
