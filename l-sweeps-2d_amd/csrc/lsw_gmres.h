@@ -100,6 +100,7 @@ class GlobalIterativeSolverGMRESHIP : public GlobalIterativeSolver {
   mutable double2* dBT = nullptr;
   mutable double2* dBPart = nullptr;
   mutable double2* dBH = nullptr;
+  mutable int* dBLive = nullptr;  // the live columns of an inner step
   mutable long long mBN = 0;
   mutable int mBRhs = 0, mBM = 0;
 };

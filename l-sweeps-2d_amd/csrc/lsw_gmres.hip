@@ -111,6 +111,7 @@ void norm2sq(const double2* a, long long n, double2* part, double2* out, hipStre
 
 GlobalIterativeSolverGMRESHIP::~GlobalIterativeSolverGMRESHIP() {
   for (double2* p : {dV, dZ, dT, dPart, dScal, dBV, dBZ, dBT, dBPart, dBH}) if (p) (void)devFree(p);
+  if (dBLive) (void)devFree(dBLive);
 }
 
 GlobalIterativeSolver* GlobalIterativeSolverGMRESHIP::copy() const {

@@ -12,13 +12,20 @@ namespace {
 constexpr int kBlocks = 256;
 constexpr int kThreads = 256;
 
+// Column a of the step's vectors for grid row y: live[y] while some of the
+// active right-hand sides have converged (the inner steps then run only on
+// the others), y itself otherwise.
+__device__ __forceinline__ int liveCol(const int* __restrict__ live, int y) { return live ? live[y] : y; }
+
 // part[(pair)*kBlocks + block] = sum conj(V_i^a) w^a over the block's range;
-// pair = a*nv + i, V_i^a = V + (i*nact + a)*n, w^a = W + a*n.
+// pair = y*nv + i with a = liveCol(y), V_i^a = V + (i*nact + a)*n, w^a = W + a*n.
 __global__ __launch_bounds__(kThreads) void k_bdot_part(const double2* __restrict__ V, const double2* __restrict__ W,
-                                                        int nv, int nact, long long n, double2* __restrict__ part) {
+                                                        int nv, int nact, long long n, double2* __restrict__ part,
+                                                        const int* __restrict__ live) {
   __shared__ double2 sm[kThreads];
   const int pair = blockIdx.y;
-  const int a = pair / nv, i = pair - a * nv;
+  const int y = pair / nv, i = pair - y * nv;
+  const int a = liveCol(live, y);
   const double2* v = V + ((long long)i * nact + a) * n;
   const double2* w = W + (long long)a * n;
   double2 acc = make_double2(0.0, 0.0);
@@ -33,11 +40,12 @@ __global__ __launch_bounds__(kThreads) void k_bdot_part(const double2* __restric
   if (threadIdx.x == 0) part[(long long)pair * kBlocks + blockIdx.x] = sm[0];
 }
 
-// per-vector squared norms: part[a*kBlocks + block]
+// per-vector squared norms: part[y*kBlocks + block] of column liveCol(y)
 __global__ __launch_bounds__(kThreads) void k_bnorm_part(const double2* __restrict__ W, long long n,
-                                                         double2* __restrict__ part) {
+                                                         double2* __restrict__ part,
+                                                         const int* __restrict__ live = nullptr) {
   __shared__ double sm[kThreads];
-  const double2* w = W + (long long)blockIdx.y * n;
+  const double2* w = W + (long long)liveCol(live, blockIdx.y) * n;
   double acc = 0.0;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
     acc = fma(w[q].x, w[q].x, fma(w[q].y, w[q].y, acc));
@@ -66,25 +74,37 @@ __global__ __launch_bounds__(kThreads) void k_bfinal(const double2* __restrict__
   if (threadIdx.x == 0) out[p] = accumulate ? cadd(out[p], sm[0]) : sm[0];
 }
 
-// w^a -= sum_i h[a*nv+i] V_i^a
+// w^a -= sum_i h[y*nv+i] V_i^a, a = liveCol(y)
 __global__ void k_bupdate(const double2* __restrict__ V, double2* __restrict__ W, const double2* __restrict__ h,
-                          int nv, int nact, long long n) {
-  const int a = blockIdx.y;
+                          int nv, int nact, long long n, const int* __restrict__ live) {
+  const int y = blockIdx.y, a = liveCol(live, y);
   double2* w = W + (long long)a * n;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
     double2 acc = w[q];
-    for (int i = 0; i < nv; ++i) acc = cfms(h[a * nv + i], V[((long long)i * nact + a) * n + q], acc);
+    for (int i = 0; i < nv; ++i) acc = cfms(h[y * nv + i], V[((long long)i * nact + a) * n + q], acc);
     w[q] = acc;
   }
 }
 
-// w^a *= 1/sqrt(nrm2[a])
-__global__ void k_bscale(double2* __restrict__ W, const double2* __restrict__ nrm2, long long n) {
-  const int a = blockIdx.y;
-  const double inv = 1.0 / sqrt(nrm2[a].x);
-  double2* w = W + (long long)a * n;
+// w^a *= 1/sqrt(nrm2[y]), a = liveCol(y)
+__global__ void k_bscale(double2* __restrict__ W, const double2* __restrict__ nrm2, long long n,
+                         const int* __restrict__ live) {
+  const int y = blockIdx.y;
+  const double inv = 1.0 / sqrt(nrm2[y].x);
+  double2* w = W + (long long)liveCol(live, y) * n;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
     w[q] = make_double2(w[q].x * inv, w[q].y * inv);
+}
+
+// gather (dst column y = src column live[y]) or scatter (dst column live[y] =
+// src column y) of n-vectors
+__global__ void k_cols_copy(double2* __restrict__ dst, const double2* __restrict__ src, const int* __restrict__ live,
+                            long long n, int scatter) {
+  const long long c = live[blockIdx.y];
+  double2* d = dst + (scatter ? c : (long long)blockIdx.y) * n;
+  const double2* v = src + (scatter ? (long long)blockIdx.y : c) * n;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    d[q] = v[q];
 }
 
 __global__ void k_gather_scale(double2* __restrict__ dst, const double2* __restrict__ src, double c, long long n) {
@@ -116,12 +136,15 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
   const int m = std::max(1, mRestart);
   if (n != mBN || nrhs > mBRhs || m > mBM) {
     for (double2* p : {dBV, dBZ, dBT, dBPart, dBH}) if (p) (void)devFree(p);
+    if (dBLive) (void)devFree(dBLive);
     dBV = dBZ = dBT = dBPart = dBH = nullptr;
+    dBLive = nullptr;
     LSW_HIP(devMalloc(&dBV, (size_t)(m + 1) * nrhs * n * sizeof(double2)));
     LSW_HIP(devMalloc(&dBZ, (size_t)m * nrhs * n * sizeof(double2)));
     LSW_HIP(devMalloc(&dBT, (size_t)2 * nrhs * n * sizeof(double2)));
     LSW_HIP(devMalloc(&dBPart, (size_t)(m + 1) * nrhs * kBlocks * sizeof(double2)));
     LSW_HIP(devMalloc(&dBH, (size_t)2 * (m + 2) * nrhs * sizeof(double2)));
+    LSW_HIP(devMalloc(&dBLive, (size_t)nrhs * sizeof(int)));
     mBN = n;
     mBRhs = nrhs;
     mBM = m;
@@ -181,32 +204,59 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
       double2* Vj = dBV + (long long)j * na * n;
       double2* Zj = dBZ + (long long)j * na * n;
       double2* W = dBV + (long long)(j + 1) * na * n;
-      dd->applyPrecondDev(Vj, Zj, s, na);
-      dd->applyADevBlock(Zj, W, na, s);
-      for (int a = 0; a < na; ++a) infos[act[a]].preconditionerApplies++, infos[act[a]].matvecs++;
+      // the step runs on the right-hand sides whose estimate has not
+      // converged: once some have, the others are gathered into a block of
+      // the next power of two (zero-padded; a handful of plan widths) for
+      // M^-1 and A and scattered back, and the Gram-Schmidt kernels index
+      // them through the live list
+      std::vector<int> live;
+      for (int a = 0; a < na; ++a)
+        if (!done[a]) live.push_back(a);
+      const int nl = (int)live.size();
+      const int* dl = nullptr;
+      if (nl < na) {
+        LSW_HIP(hipMemcpyAsync(dBLive, live.data(), nl * sizeof(int), hipMemcpyHostToDevice, s));
+        dl = dBLive;
+        int wb = 1;
+        while (wb < nl) wb *= 2;
+        wb = std::min(wb, na);
+        double2* G = dBT;                         // gathered V_j, then A M^-1 V_j
+        double2* GZ = dBT + (long long)nrhs * n;  // M^-1 V_j
+        const dim3 cg(gridN(n).x, nl);
+        hipLaunchKernelGGL(k_cols_copy, cg, dim3(256), 0, s, G, Vj, dl, n, 0);
+        if (wb > nl) LSW_HIP(hipMemsetAsync(G + (long long)nl * n, 0, (size_t)(wb - nl) * n * sizeof(double2), s));
+        dd->applyPrecondDev(G, GZ, s, wb);
+        hipLaunchKernelGGL(k_cols_copy, cg, dim3(256), 0, s, Zj, GZ, dl, n, 1);
+        dd->applyADevBlock(GZ, G, nl, s);
+        hipLaunchKernelGGL(k_cols_copy, cg, dim3(256), 0, s, W, G, dl, n, 1);
+      } else {
+        dd->applyPrecondDev(Vj, Zj, s, na);
+        dd->applyADevBlock(Zj, W, na, s);
+      }
+      for (int a : live) infos[act[a]].preconditionerApplies++, infos[act[a]].matvecs++;
       const int nv = j + 1;
       // two passes of classical Gram-Schmidt
       for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(k_bdot_part, dim3(kBlocks, nv * na), dim3(kThreads), 0, s, dBV, W, nv, na, n, dBPart);
-        hipLaunchKernelGGL(k_bfinal, dim3(nv * na), dim3(kThreads), 0, s, dBPart, dBH + pass * (m + 2) * nrhs, 0);
-        hipLaunchKernelGGL(k_bupdate, dim3(gridN(n).x, na), dim3(256), 0, s, dBV, W, dBH + pass * (m + 2) * nrhs,
-                           nv, na, n);
+        hipLaunchKernelGGL(k_bdot_part, dim3(kBlocks, nv * nl), dim3(kThreads), 0, s, dBV, W, nv, na, n, dBPart, dl);
+        hipLaunchKernelGGL(k_bfinal, dim3(nv * nl), dim3(kThreads), 0, s, dBPart, dBH + pass * (m + 2) * nrhs, 0);
+        hipLaunchKernelGGL(k_bupdate, dim3(gridN(n).x, nl), dim3(256), 0, s, dBV, W, dBH + pass * (m + 2) * nrhs,
+                           nv, na, n, dl);
       }
-      hipLaunchKernelGGL(k_bnorm_part, dim3(kBlocks, na), dim3(kThreads), 0, s, W, n, dBPart);
-      hipLaunchKernelGGL(k_bfinal, dim3(na), dim3(kThreads), 0, s, dBPart, dBH + 2 * (m + 2) * nrhs - na, 0);
-      hipLaunchKernelGGL(k_bscale, dim3(gridN(n).x, na), dim3(256), 0, s, W, dBH + 2 * (m + 2) * nrhs - na, n);
-      std::vector<double2> h1(nv * na), h2(nv * na), nr(na);
-      LSW_HIP(hipMemcpyAsync(h1.data(), dBH, nv * na * sizeof(double2), hipMemcpyDeviceToHost, s));
-      LSW_HIP(hipMemcpyAsync(h2.data(), dBH + (m + 2) * nrhs, nv * na * sizeof(double2), hipMemcpyDeviceToHost, s));
-      LSW_HIP(hipMemcpyAsync(nr.data(), dBH + 2 * (m + 2) * nrhs - na, na * sizeof(double2), hipMemcpyDeviceToHost, s));
+      hipLaunchKernelGGL(k_bnorm_part, dim3(kBlocks, nl), dim3(kThreads), 0, s, W, n, dBPart, dl);
+      hipLaunchKernelGGL(k_bfinal, dim3(nl), dim3(kThreads), 0, s, dBPart, dBH + 2 * (m + 2) * nrhs - nl, 0);
+      hipLaunchKernelGGL(k_bscale, dim3(gridN(n).x, nl), dim3(256), 0, s, W, dBH + 2 * (m + 2) * nrhs - nl, n, dl);
+      std::vector<double2> h1(nv * nl), h2(nv * nl), nr(nl);
+      LSW_HIP(hipMemcpyAsync(h1.data(), dBH, nv * nl * sizeof(double2), hipMemcpyDeviceToHost, s));
+      LSW_HIP(hipMemcpyAsync(h2.data(), dBH + (m + 2) * nrhs, nv * nl * sizeof(double2), hipMemcpyDeviceToHost, s));
+      LSW_HIP(hipMemcpyAsync(nr.data(), dBH + 2 * (m + 2) * nrhs - nl, nl * sizeof(double2), hipMemcpyDeviceToHost, s));
       LSW_HIP(hipStreamSynchronize(s));
       bool all = true;
-      for (int a = 0; a < na; ++a) {
-        if (done[a]) continue;
+      for (int y = 0; y < nl; ++y) {
+        const int a = live[y];
         auto& Ha = H[a];
         for (int i = 0; i < nv; ++i)
-          Ha[i][j] = Complex(h1[a * nv + i].x + h2[a * nv + i].x, h1[a * nv + i].y + h2[a * nv + i].y);
-        Ha[j + 1][j] = std::sqrt(nr[a].x);
+          Ha[i][j] = Complex(h1[y * nv + i].x + h2[y * nv + i].x, h1[y * nv + i].y + h2[y * nv + i].y);
+        Ha[j + 1][j] = std::sqrt(nr[y].x);
         LSW_REQUIRE(std::isfinite(Ha[j + 1][j].real()), LSW_ERR_NUMERIC, "GMRES breakdown: non-finite norm");
         for (int i = 0; i < j; ++i) {
           const Complex t = std::conj(cs[a][i]) * Ha[i][j] + std::conj(sn[a][i]) * Ha[i + 1][j];

@@ -84,6 +84,30 @@ def test_block_precond_wide_fronts(prec):
             assert relerr(Y[k], P.apply_global(X[k])) < 1e-9
 
 
+def test_block_gmres_converged_columns_leave_the_block():
+    """Right-hand sides that converge at different steps (here the point
+    sources take one step more than the random vectors on an 8 x 8 board at
+    tol 1e-10): once a column's estimate is below tol its inner steps stop
+    (its preconditioner applications equal its own iterations), the others
+    go on as a smaller block, and every column still equals its single-source
+    solve."""
+    cb, dd = make_pair((8, 8), (12, 12), 2 * math.pi * 9.0, precision=1)
+    dd.factorize()
+    srcs = [(0.5, 0.5), (0.2, 0.8), (0.7, 0.3), (0.05, 0.5), (0.93, 0.07), (0.31, 0.62)]
+    rng = np.random.default_rng(5)
+    B = np.stack([dd.assemble_rhs([s]) for s in srcs] +
+                 [rng.standard_normal(dd.N) + 1j * rng.standard_normal(dd.N) for _ in range(2)])
+    X, infos = dd.gmres_block(B, tol=1e-10, max_iter=60, restart=40)
+    its = [inf["iterations"] for inf in infos]
+    assert len(set(its)) > 1, its
+    for k, inf in enumerate(infos):
+        assert inf["precond_applies"] == inf["iterations"], (k, its)
+        assert inf["true_residual"] < 1e-10
+        x1, info1 = dd.gmres(B[k], mode=1, tol=1e-10, max_iter=60, restart=40)
+        assert relerr(X[k], x1) < 1e-8
+        assert abs(inf["iterations"] - info1["iterations"]) <= 1
+
+
 def test_block_gmres_returns_when_right_hand_sides_hit_the_cap():
     """Block flexible GMRES terminates when right-hand sides stop at different
     steps: an easy one converges on its estimate, a hard one runs into
