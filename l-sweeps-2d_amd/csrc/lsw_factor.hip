@@ -595,34 +595,37 @@ __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__
 // replays the swaps in order), together with the copy of B.  Same
 // arithmetic in the same order per entry as k_front_panel, and the same
 // pivots (largest modulus, lowest row on ties, an order-free choice).
+// Up to 280 rows in FP64, 576 in FP32 (157 KB of the 160 KB LDS).
 constexpr int kPanelLdsRows = 280;
+constexpr int kPanelLdsRows32 = 576;
 constexpr int kPanelLd = kPanel + 1;  // padded LDS row (bank spread)
 
-__host__ __device__ constexpr size_t panelLdsBytes(int rows) {
-  return (size_t)rows * (2 * kPanelLd + 1) * sizeof(double2);
+__host__ __device__ constexpr size_t panelLdsBytes(int rows, size_t es) {
+  return (size_t)rows * (2 * kPanelLd + 1) * es;
 }
 
+template <typename W2>
 __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restrict__ cells,
                                                          const ShapeDev* __restrict__ shapes,
                                                          const FrontTask* __restrict__ fts,
-                                                         double2* __restrict__ ws, int k0, int* __restrict__ info) {
+                                                         W2* __restrict__ ws, int k0, int* __restrict__ info) {
   extern __shared__ double2 lswDynLds[];
   __shared__ double wbest[4];
   __shared__ int wrow[4];
   __shared__ int perm[kPanel];
   const FrontTask ft = fts[blockIdx.x];
-  const FrontView<double2> f = frontView(cells, shapes, ft, ws);
+  const FrontView<W2> f = frontView(cells, shapes, ft, ws);
   const int e = f.e, R = f.R, Cc = f.Cc;
   const int nbk = min(kPanel, e - k0);
   if (nbk <= 0) return;
-  double2* sP = lswDynLds;            // R x kPanelLd: panel columns k0 .. k0+nbk-1
-  double2* sT = sP + R * kPanelLd;    // R x kPanelLd
-  double2* sC = sT + R * kPanelLd;    // R: the column being eliminated
-  double2* M = f.M;
-  double2* T = M + (long long)R * Cc;
-  double2* B = T + (long long)R * kPanel;
+  W2* sP = reinterpret_cast<W2*>(lswDynLds);  // R x kPanelLd: panel columns k0 .. k0+nbk-1
+  W2* sT = sP + R * kPanelLd;                  // R x kPanelLd
+  W2* sC = sT + R * kPanelLd;                  // R: the column being eliminated
+  W2* M = f.M;
+  W2* T = M + (long long)R * Cc;
+  W2* B = T + (long long)R * kPanel;
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
-  const double2 zero = make_double2(0.0, 0.0), one = make_double2(1.0, 0.0);
+  const W2 zero = czero<W2>(), one = CTraits<W2>::make(1, 0);
   const int pend = k0 + nbk;
   for (int idx = tid; idx < R * kPanel; idx += nt) {
     const int i = idx / kPanel, c = idx - i * kPanel;
@@ -635,8 +638,8 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
     double best = -1.0;
     int bi = e;
     for (int i = k + tid; i < e; i += nt) {
-      const double2 v = sP[i * kPanelLd + kk];
-      const double a = v.x * v.x + v.y * v.y;
+      const W2 v = sP[i * kPanelLd + kk];
+      const double a = (double)(v.x * v.x + v.y * v.y);
       if (a > best) {
         best = a;
         bi = i;
@@ -669,24 +672,24 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
     // columns before kk are unit columns of rows above k: zero in rows k, p
     if (p != k) {
       for (int j = tid; j < (nbk - kk) + kk; j += nt) {
-        double2* a = j < nbk - kk ? &sP[k * kPanelLd + kk + j] : &sT[k * kPanelLd + (j - (nbk - kk))];
-        double2* b = j < nbk - kk ? &sP[p * kPanelLd + kk + j] : &sT[p * kPanelLd + (j - (nbk - kk))];
-        const double2 t = *a;
+        W2* a = j < nbk - kk ? &sP[k * kPanelLd + kk + j] : &sT[k * kPanelLd + (j - (nbk - kk))];
+        W2* b = j < nbk - kk ? &sP[p * kPanelLd + kk + j] : &sT[p * kPanelLd + (j - (nbk - kk))];
+        const W2 t = *a;
         *a = *b;
         *b = t;
       }
     }
     __syncthreads();
     if (tid == 0) sT[k * kPanelLd + kk] = one;
-    double2 piv = sP[k * kPanelLd + kk];
+    W2 piv = sP[k * kPanelLd + kk];
     if (!(pmag > 0.0) || !isfinite(pmag)) {
       if (tid == 0) atomicMax(info, 1);
       piv = one;
     }
-    const double2 inv = cinvT(piv);
+    const W2 inv = cinvT(piv);
     __syncthreads();
     for (int j = tid; j < (nbk - kk - 1) + (kk + 1); j += nt) {
-      double2* t = j < nbk - kk - 1 ? &sP[k * kPanelLd + kk + 1 + j] : &sT[k * kPanelLd + (j - (nbk - kk - 1))];
+      W2* t = j < nbk - kk - 1 ? &sP[k * kPanelLd + kk + 1 + j] : &sT[k * kPanelLd + (j - (nbk - kk - 1))];
       *t = cmul(*t, inv);
     }
     for (int i = tid; i < R; i += nt) sC[i] = (i == k) ? zero : sP[i * kPanelLd + kk];
@@ -695,14 +698,14 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
     for (int idx = tid; idx < R * w; idx += nt) {
       const int i = idx / w, j = idx - i * w;
       if (i == k) continue;
-      const double2 fi = sC[i];
-      if (fi.x == 0.0 && fi.y == 0.0) continue;
+      const W2 fi = sC[i];
+      if (fi.x == 0 && fi.y == 0) continue;
       if (j < nbk - kk - 1) {
-        double2* t = &sP[i * kPanelLd + kk + 1 + j];
+        W2* t = &sP[i * kPanelLd + kk + 1 + j];
         *t = cfms(fi, sP[k * kPanelLd + kk + 1 + j], *t);
       } else {
         const int c = j - (nbk - kk - 1);
-        double2* t = &sT[i * kPanelLd + c];
+        W2* t = &sT[i * kPanelLd + c];
         *t = cfms(fi, sT[k * kPanelLd + c], *t);
       }
     }
@@ -720,7 +723,7 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
     for (int kk = 0; kk < nbk; ++kk) {
       const int k = k0 + kk, p = perm[kk];
       if (p != k) {
-        const double2 t = M[(long long)k * Cc + j];
+        const W2 t = M[(long long)k * Cc + j];
         M[(long long)k * Cc + j] = M[(long long)p * Cc + j];
         M[(long long)p * Cc + j] = t;
       }
@@ -1119,8 +1122,8 @@ bool trailMfma() {
   return v;
 }
 
-// LDS-resident panels for fronts of up to kPanelLdsRows rows (FP64 fronts)
-// unless LSW_PANEL_LDS=0.
+// LDS-resident panels for fronts of up to kPanelLdsRows rows (FP64) or
+// kPanelLdsRows32 rows (FP32) unless LSW_PANEL_LDS=0.
 bool panelLds() {
   static const bool v = [] {
     const char* e = std::getenv("LSW_PANEL_LDS");
@@ -1132,15 +1135,19 @@ bool panelLds() {
 void launchPanel(unsigned nf, int maxR, size_t shm, hipStream_t s, const CellDev* cells, const ShapeDev* shapes,
                  const FrontTask* fts, double2* ws, int k0, int* info) {
   if (panelLds() && maxR > 0 && maxR <= kPanelLdsRows)
-    hipLaunchKernelGGL(k_front_panel_lds, dim3(nf), dim3(256), panelLdsBytes(maxR), s, cells, shapes, fts, ws, k0,
-                       info);
+    hipLaunchKernelGGL(k_front_panel_lds<double2>, dim3(nf), dim3(256), panelLdsBytes(maxR, sizeof(double2)), s, cells,
+                       shapes, fts, ws, k0, info);
   else
     hipLaunchKernelGGL(k_front_panel<double2>, dim3(nf), dim3(256), shm, s, cells, shapes, fts, ws, k0, info);
 }
 
-void launchPanel(unsigned nf, int, size_t shm, hipStream_t s, const CellDev* cells, const ShapeDev* shapes,
+void launchPanel(unsigned nf, int maxR, size_t shm, hipStream_t s, const CellDev* cells, const ShapeDev* shapes,
                  const FrontTask* fts, float2* ws, int k0, int* info) {
-  hipLaunchKernelGGL(k_front_panel<float2>, dim3(nf), dim3(256), shm, s, cells, shapes, fts, ws, k0, info);
+  if (panelLds() && maxR > 0 && maxR <= kPanelLdsRows32)
+    hipLaunchKernelGGL(k_front_panel_lds<float2>, dim3(nf), dim3(256), panelLdsBytes(maxR, sizeof(float2)), s, cells,
+                       shapes, fts, ws, k0, info);
+  else
+    hipLaunchKernelGGL(k_front_panel<float2>, dim3(nf), dim3(256), shm, s, cells, shapes, fts, ws, k0, info);
 }
 
 void launchTrail(dim3 grid, hipStream_t s, const CellDev* cells, const ShapeDev* shapes, const FrontTask* fts,
