@@ -272,12 +272,67 @@ constexpr int kSmallRows = 96;
 constexpr int kMidRows = 128;
 constexpr int kBig32Rows = 192;  // FP32 arithmetic only
 
+// Lane exchanges without the LDS crossbar (ds_bpermute): a value of lane
+// `src` (wave-uniform) read by every lane (v_readlane per 32-bit word), and
+// a DPP move within rows of 16 lanes.
+template <typename T>
+__device__ __forceinline__ T laneRead(T v, int src) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  int w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = __builtin_amdgcn_readlane(w[i], src);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T dppMove(T v) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  int w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = __builtin_amdgcn_mov_dpp(w[i], CTRL, 0xF, 0xF, false);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+// (modulus, row) pairs: the larger modulus wins, the lower row on ties
+template <typename MT>
+__device__ __forceinline__ void pivotMerge(MT& best, int& bi, MT om, int oi) {
+  if (om > best || (om == best && oi < bi)) {
+    best = om;
+    bi = oi;
+  }
+}
+
+// The pivot of a column held by the 32 lanes of one half wavefront (`half`:
+// 0 for lanes 0-31, 1 for 32-63, uniform): butterflies within each row of 16
+// lanes (DPP quad permutations [1,0,3,2] and [2,3,0,1], half-row and row
+// mirrors), then the two rows' winners by lane reads.  The merge is a total
+// order, so the winner is the same as any other reduction order's.
+template <typename MT>
+__device__ __forceinline__ void pivotHalfWave(MT& best, int& bi, int half) {
+  pivotMerge(best, bi, dppMove<0xB1>(best), dppMove<0xB1>(bi));
+  pivotMerge(best, bi, dppMove<0x4E>(best), dppMove<0x4E>(bi));
+  pivotMerge(best, bi, dppMove<0x141>(best), dppMove<0x141>(bi));
+  pivotMerge(best, bi, dppMove<0x140>(best), dppMove<0x140>(bi));
+  MT b0 = laneRead(best, 32 * half);
+  int i0 = laneRead(bi, 32 * half);
+  pivotMerge(b0, i0, laneRead(best, 32 * half + 16), laneRead(bi, 32 * half + 16));
+  best = b0;
+  bi = i0;
+}
+
 template <typename W2, typename O2, int PR, int PC>
 __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
     const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes, const FrontTask* __restrict__ fts,
     const double2* __restrict__ coef, O2* __restrict__ factor, O2* __restrict__ schur, int* __restrict__ info) {
   constexpr int RT = 32, ROWS = RT * PR, CT = ROWS / PC, NT = RT * CT;
   static_assert(CT * PC == ROWS && CT <= 32, "thread grid");
+  static_assert(RT == 32, "a column group is one half wavefront");
   __shared__ int rowCnt[ROWS];
   __shared__ int rowCol[ROWS][5];  // a row's original entries: column
   __shared__ int rowAsm[ROWS][5];  // ... and assembly-list index
@@ -350,14 +405,17 @@ __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
   // ---- the e exchange steps (one workgroup barrier each)
   // Column k's owners (tj == k % CT: the 32 lanes of one half wavefront)
   // publish it and its pivot row p in LDS; row p's values reach the other
-  // threads of their column group by lane shuffles within the half
-  // wavefront (thread (p % RT, tj) owns them), so no second barrier.
+  // threads of their column group by lane reads within the half wavefront
+  // (thread (p % RT, tj) owns them), so no second barrier.
   unsigned usedMask = 0;  // this thread's rows already exchanged
   for (int k = 0; k < e; ++k) {
     const int kb = k / CT, kc = k % CT;
     W2* ck = colK[k & 1];
     if (tj == kc) {
-      double best = -1.0;
+      // (the modulus in the arithmetic's precision: the same values as
+      // their conversions to double, so the same pivots)
+      using MT = typename CTraits<W2>::R;
+      MT best = -1;
       int bi = e;
 #pragma unroll
       for (int c = 0; c < PC; ++c)
@@ -367,7 +425,7 @@ __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
             const int i = ti + RT * a;
             if (i < R) ck[i] = M[a][c];
             if (i < e && !((usedMask >> a) & 1u)) {
-              const double m = (double)(M[a][c].x * M[a][c].x + M[a][c].y * M[a][c].y);
+              const MT m = M[a][c].x * M[a][c].x + M[a][c].y * M[a][c].y;
               if (m > best) {
                 best = m;
                 bi = i;
@@ -375,26 +433,19 @@ __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
             }
           }
         }
-      // largest modulus over the 32 lanes (lowest row on ties)
-#pragma unroll
-      for (int o = 1; o < RT; o <<= 1) {
-        const double om = __shfl_xor(best, o, RT);
-        const int oi = __shfl_xor(bi, o, RT);
-        if (om > best || (om == best && oi < bi)) {
-          best = om;
-          bi = oi;
-        }
-      }
+      // largest modulus over the 32 lanes (lowest row on ties); this half
+      // wavefront is tj's, and tj's parity is the half
+      pivotHalfWave(best, bi, kc & 1);
       if (ti == 0) {
-        sMag[k & 1] = best;
+        sMag[k & 1] = (double)best;
         sP[k & 1] = bi;
       }
     }
     __syncthreads();
-    const int p = sP[k & 1];
+    const int p = __builtin_amdgcn_readfirstlane(sP[k & 1]);  // (uniform: the lane reads below)
     const double pmag = sMag[k & 1];
     const bool bad = !(pmag > 0.0) || !isfinite(pmag) || p >= e;
-    const int pp = bad ? 0 : p;
+    const int pp = __builtin_amdgcn_readfirstlane(bad ? 0 : p);
     const int pa = pp / RT, pr = pp % RT;
     W2 pv = ck[pp];
     if (bad) {
@@ -410,7 +461,7 @@ __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
       const int i = ti + RT * a;
       ca[a] = ck[i < R ? i : 0];
     }
-    const int srcLane = (tid & ~(RT - 1) & 63) + pr;  // lane of (pr, tj) in this wavefront
+    const bool upper = (tid & 32) != 0;  // this lane's half wavefront: thread (pr, tj) is lane pr of it
 #pragma unroll
     for (int c = 0; c < PC; ++c) {
       // row p's value in this column (before this step's update)
@@ -418,9 +469,8 @@ __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
 #pragma unroll
       for (int a = 1; a < PR; ++a)
         if (a == pa) mine = M[a][c];
-      W2 rowv;
-      rowv.x = __shfl(mine.x, srcLane, 64);
-      rowv.y = __shfl(mine.y, srcLane, 64);
+      const W2 rowLo = laneRead(mine, pr), rowHi = laneRead(mine, pr + 32);
+      const W2 rowv = upper ? rowHi : rowLo;
       const W2 rs = cmul(rowv, inv);
 #pragma unroll
       for (int a = 0; a < PR; ++a) M[a][c] = cfms(ca[a], rs, M[a][c]);
