@@ -326,6 +326,21 @@ __device__ __forceinline__ void pivotHalfWave(MT& best, int& bi, int half) {
   bi = i0;
 }
 
+// The same over the whole wavefront: the four rows' winners by lane reads.
+template <typename MT>
+__device__ __forceinline__ void pivotWave(MT& best, int& bi) {
+  pivotMerge(best, bi, dppMove<0xB1>(best), dppMove<0xB1>(bi));
+  pivotMerge(best, bi, dppMove<0x4E>(best), dppMove<0x4E>(bi));
+  pivotMerge(best, bi, dppMove<0x141>(best), dppMove<0x141>(bi));
+  pivotMerge(best, bi, dppMove<0x140>(best), dppMove<0x140>(bi));
+  MT b0 = laneRead(best, 0);
+  int i0 = laneRead(bi, 0);
+#pragma unroll
+  for (int r = 16; r < 64; r += 16) pivotMerge(b0, i0, laneRead(best, r), laneRead(bi, r));
+  best = b0;
+  bi = i0;
+}
+
 template <typename W2, typename O2, int PR, int PC>
 __global__ __launch_bounds__(32 * (32 * PR / PC)) void k_front_small(
     const CellDev* __restrict__ cells, const ShapeDev* __restrict__ shapes, const FrontTask* __restrict__ fts,
@@ -639,8 +654,8 @@ __global__ __launch_bounds__(256) void k_front_panel(const CellDev* __restrict__
 // The same panel with its columns, T and the elimination column held in
 // LDS (fronts of up to kPanelLdsRows rows): the 16 pivot steps then run on
 // LDS instead of chains of L2 round trips, and the pivot search reduces
-// within each wavefront by shuffles (one barrier instead of a 256-wide LDS
-// tree).  The row swaps of the columns after the panel are recorded and
+// within each wavefront by DPP moves and lane reads (one barrier instead of
+// a 256-wide LDS tree).  The row swaps of the columns after the panel are recorded and
 // applied at the end, column by column (each thread owns columns and
 // replays the swaps in order), together with the copy of B.  Same
 // arithmetic in the same order per entry as k_front_panel, and the same
@@ -659,8 +674,9 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
                                                          const ShapeDev* __restrict__ shapes,
                                                          const FrontTask* __restrict__ fts,
                                                          W2* __restrict__ ws, int k0, int* __restrict__ info) {
+  using MT = typename CTraits<W2>::R;  // pivot moduli (the values their conversions to double have)
   extern __shared__ double2 lswDynLds[];
-  __shared__ double wbest[4];
+  __shared__ MT wbest[4];
   __shared__ int wrow[4];
   __shared__ int perm[kPanel];
   const FrontTask ft = fts[blockIdx.x];
@@ -685,25 +701,17 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
   __syncthreads();
   for (int kk = 0; kk < nbk; ++kk) {
     const int k = k0 + kk;
-    double best = -1.0;
+    MT best = -1;
     int bi = e;
     for (int i = k + tid; i < e; i += nt) {
       const W2 v = sP[i * kPanelLd + kk];
-      const double a = (double)(v.x * v.x + v.y * v.y);
+      const MT a = v.x * v.x + v.y * v.y;
       if (a > best) {
         best = a;
         bi = i;
       }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
+    pivotWave(best, bi);
     if (lane == 0) {
       wbest[wave] = best;
       wrow[wave] = bi;
@@ -717,7 +725,7 @@ __global__ __launch_bounds__(256) void k_front_panel_lds(const CellDev* __restri
         bi = wrow[w];
       }
     const int p = bi;
-    const double pmag = best;
+    const double pmag = (double)best;
     if (tid == 0) perm[kk] = p;
     // columns before kk are unit columns of rows above k: zero in rows k, p
     if (p != k) {
