@@ -599,19 +599,27 @@ def _cpu_object(r):
 
 def pack_pass(L, probs, workers, restart, budget, slots=16):
     """One pass over the uneven pack: a new server (empty factor cache), every
-    problem registered and its request submitted at once, every result
-    waited for.  Returns (wall seconds, per-request results, server stats)."""
+    problem registered and its request submitted, every result waited for.
+    Returns (wall seconds, per-request results in pack order, server stats).
+
+    Registration (host geometry and the device-bytes estimate) runs on a few
+    threads (the library call releases the GIL), largest problems (grid nodes
+    x sources) first, and each request is submitted as soon as its problem is
+    registered: the workers start on the largest problems while the rest
+    register (the server itself then orders by estimated work)."""
     from concurrent.futures import ThreadPoolExecutor
+    order = sorted(range(len(probs)), key=lambda k: -probs[k]["grid"][0] * probs[k]["grid"][1] * len(probs[k]["sources"]))
     t0 = time.perf_counter()
     srv = L.Server(budget_bytes=budget, slots=slots, restart=restart, tol=1e-6, max_iter=200,
                    precision=L.LSW_PREC_FP32, workers=workers, batch_order=True)
     try:
-        # registration (host geometry and the device-bytes estimate) on a few
-        # threads (the library call releases the GIL), then every request
+        def register(k):
+            pid = srv.add_problem(_problem_params(L, probs[k], L.LSW_PREC_FP32))
+            return srv.submit(pid, probs[k]["sources"], want_solution=False)
+
         with ThreadPoolExecutor(8) as ex:
-            pids = list(ex.map(lambda p: srv.add_problem(_problem_params(L, p, L.LSW_PREC_FP32)), probs))
-        ids = [srv.submit(pids[k], p["sources"], want_solution=False) for k, p in enumerate(probs)]
-        res = [srv.wait(i) for i in ids]
+            ids = dict(zip(order, ex.map(register, order)))
+        res = [srv.wait(ids[k]) for k in range(len(probs))]
         wall = time.perf_counter() - t0
         st = srv.stats()
     finally:

@@ -108,12 +108,17 @@ int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
     e->params.velocity = e->velocity.data();
   }
   // host geometry: grid size and the device-bytes estimate
+  const double t0 = now();
   auto geo = makeCheckerboard(e->params, false);
   e->Gx = geo->Gx();
   e->Gy = geo->Gy();
+  const double t1 = now();
   // factors and sweep buffers (steady) and the factorisation peak; the Krylov
   // state is reserved on top when work arrives
   geo->deviceBytesEstimate(mOpt.precision, planWidth(mOpt.slots), &e->steadyBytes, &e->peakBytes, &e->trees);
+  if (stepLog())
+    std::fprintf(stderr, "register N %lld geometry_ms %.2f estimate_ms %.2f\n", (long long)geo->N(), 1e3 * (t1 - t0),
+                 1e3 * (now() - t1));
   e->N = geo->N();
   const double full = e->steadyBytes + (double)GmresSlots::bytesFor(e->N, mOpt.slots, mOpt.restart);
   LSW_REQUIRE(std::max(e->peakBytes, full) <= mOpt.budgetBytes, LSW_ERR_CAPACITY,
@@ -427,9 +432,9 @@ void Server::workerLoop() {
       long long ups = 0, allocs = 0;
       threadMemTimes(&upS, &ups, &allocS, &allocs);
       std::fprintf(stderr,
-                   "step entry %p N %lld load %d slots %d admitted %zu done %zu setup_ms %.2f factor_ms %.2f "
+                   "step t %.4f entry %p N %lld load %d slots %d admitted %zu done %zu setup_ms %.2f factor_ms %.2f "
                    "block_ms %.2f step_ms %.2f allocs %lld alloc_ms %.2f uploads %lld upload_ms %.2f\n",
-                   (void*)&e, e.dd ? (long long)e.dd->N() : 0LL, (int)load, (int)makeSlots, admitted.size(),
+                   now(), (void*)&e, e.dd ? (long long)e.dd->N() : 0LL, (int)load, (int)makeSlots, admitted.size(),
                    finished.size(), 1e3 * tSetup, 1e3 * tFactor, 1e3 * tBlock, 1e3 * tStep, allocs - allocs0,
                    1e3 * (allocS - allocS0), ups - ups0, 1e3 * (upS - upS0));
     }
