@@ -737,7 +737,7 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip timing alternative launch shapes per DD and block width before the timed solves")
     ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
-    ap.add_argument("--pack-workers", type=int, default=8)
+    ap.add_argument("--pack-workers", type=int, default=12)
     ap.add_argument("--serve-models", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-load-requests", type=int, default=32)

@@ -98,6 +98,9 @@ class DDCheckerboard : public DD {
   void setUpGeometry(const Assembler* assembler, const LocalLinearSolver* solver,
                      std::vector<DoubleVector> domain, std::vector<UInt> nCells,
                      std::vector<UInt> nElsInCell);
+  // The device half of setUp after setUpGeometry (the assembler, its PML and
+  // medium given there must still be alive).
+  void setUpDevice();
   bool deviceReady() const { return mFactors != nullptr; }
   // Replay applyPrecondDev from captured HIP graphs (off by default; the
   // LSW_GRAPHS environment variable turns it on).  Capture needs the stream

@@ -556,6 +556,12 @@ void DDCheckerboard::setUp(const Assembler* inputAssembler, const LocalLinearSol
   }
 }
 
+void DDCheckerboard::setUpDevice() {
+  LSW_REQUIRE(!mCells.empty(), LSW_ERR_STATE, "setUpDevice before setUpGeometry");
+  LSW_REQUIRE(mFactors == nullptr, LSW_ERR_STATE, "setUpDevice: already set up");
+  initDevice();
+}
+
 void DDCheckerboard::setUpGeometry(const Assembler* inputAssembler, const LocalLinearSolver* solver,
                                    std::vector<DoubleVector> domain, std::vector<UInt> nCells,
                                    std::vector<UInt> nElsInCell) {

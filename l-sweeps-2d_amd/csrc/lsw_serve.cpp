@@ -23,39 +23,42 @@ bool stepLog() {
 }
 }  // namespace
 
-std::unique_ptr<DDCheckerboard> makeCheckerboard(const lsw_dd_params& p, bool device) {
+std::unique_ptr<DDCheckerboard> makeCheckerboard(const lsw_dd_params& p, bool device,
+                                                 std::unique_ptr<ModelSetup>* keep) {
   LSW_REQUIRE(p.precision == LSW_PREC_FP64 || p.precision == LSW_PREC_FP32, LSW_ERR_ARG, "unknown precision");
   LSW_REQUIRE(p.omega > 0 && p.min_wavelengths_in_pml >= 0, LSW_ERR_ARG, "bad Omega / MinWavelengthsInPML");
   LSW_REQUIRE(p.order == 2, LSW_ERR_UNSUPPORTED, "only Order 2 is supported");
   LSW_REQUIRE(p.ncells_x >= 1 && p.ncells_y >= 1 && p.nx >= 1 && p.ny >= 1, LSW_ERR_ARG, "bad cell counts");
-  std::vector<DoubleVector> dom = {{p.domain[0], p.domain[1]}, {p.domain[2], p.domain[3]}};
+  auto ms = std::make_unique<ModelSetup>();
+  ms->domain = {{p.domain[0], p.domain[1]}, {p.domain[2], p.domain[3]}};
+  const auto& dom = ms->domain;
   LSW_REQUIRE(dom[0][1] > dom[0][0] && dom[1][1] > dom[1][0], LSW_ERR_ARG, "empty domain");
-  std::unique_ptr<Function> m;
   if (p.medium_kind == 0) {
-    m.reset(new FunctionConstant(1.0));
+    ms->medium.reset(new FunctionConstant(1.0));
   } else if (p.medium_kind == 1) {
     LSW_REQUIRE(p.velocity && p.vel_rows > 0 && p.vel_cols > 0, LSW_ERR_ARG, "missing velocity grid");
     auto* f = new FunctionCSV();
     f->setData(p.velocity, p.vel_rows, p.vel_cols,
                {{p.vel_domain[0], p.vel_domain[1]}, {p.vel_domain[2], p.vel_domain[3]}});
-    m.reset(f);
+    ms->medium.reset(f);
   } else {
     throw Error(LSW_ERR_ARG, "unknown medium kind");
   }
+  Function* m = ms->medium.get();
   // Input::readAssemblerHelmholtz2DFD (src/Input.cpp:795-857): C = ln(omega),
   // width = MinWavelengthsInPML * min(Lx, Ly) * 2 pi / omega / sqrt(min m)
-  PMLCubic2D pml;
+  PMLCubic2D& pml = ms->pml;
   pml.setAbsorptionCoeff(std::log(p.omega));
   const double wl1 = (dom[0][1] - dom[0][0]) * 2 * kPi / p.omega;
   const double wl2 = (dom[1][1] - dom[1][0]) * 2 * kPi / p.omega;
   double wl = std::min(wl1, wl2);
   wl /= std::sqrt(m->getMinReal());
   pml.setWidth(p.min_wavelengths_in_pml * wl);
-  AssemblerHelmholtz2DFD as;
+  AssemblerHelmholtz2DFD& as = ms->assembler;
   as.setPML(&pml);
   as.setOmega(p.omega);
   m->setDomain(dom);
-  as.setM(m.get());
+  as.setM(m);
   as.setOrder((UInt)p.order);
   LocalLinearSolverHIP proto(p.precision);
   auto dd = std::make_unique<DDCheckerboard>();
@@ -65,6 +68,7 @@ std::unique_ptr<DDCheckerboard> makeCheckerboard(const lsw_dd_params& p, bool de
     dd->setUp(&as, &proto, dom, {(UInt)p.ncells_x, (UInt)p.ncells_y}, {(UInt)p.nx, (UInt)p.ny});
   else
     dd->setUpGeometry(&as, &proto, dom, {(UInt)p.ncells_x, (UInt)p.ncells_y}, {(UInt)p.nx, (UInt)p.ny});
+  if (keep) *keep = std::move(ms);
   return dd;
 }
 
@@ -109,7 +113,8 @@ int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
   }
   // host geometry: grid size and the device-bytes estimate
   const double t0 = now();
-  auto geo = makeCheckerboard(e->params, false);
+  std::unique_ptr<ModelSetup> model;
+  auto geo = makeCheckerboard(e->params, false, &model);
   e->Gx = geo->Gx();
   e->Gy = geo->Gy();
   const double t1 = now();
@@ -120,6 +125,8 @@ int Server::addProblem(const lsw_dd_params& p, int* Gx, int* Gy) {
     std::fprintf(stderr, "register N %lld geometry_ms %.2f estimate_ms %.2f\n", (long long)geo->N(), 1e3 * (t1 - t0),
                  1e3 * (now() - t1));
   e->N = geo->N();
+  e->geometry = std::move(geo);
+  e->model = std::move(model);
   const double full = e->steadyBytes + (double)GmresSlots::bytesFor(e->N, mOpt.slots, mOpt.restart);
   LSW_REQUIRE(std::max(e->peakBytes, full) <= mOpt.budgetBytes, LSW_ERR_CAPACITY,
               "serve: problem needs more device memory than the budget");
@@ -358,7 +365,16 @@ void Server::workerLoop() {
     try {
       if (load) {
         const double t0 = now();
-        auto dd = makeCheckerboard(e.params, true);
+        std::unique_ptr<DDCheckerboard> dd;
+        if (e.geometry) {
+          // the first load finishes the registration's host geometry (after
+          // that nothing reads its medium, PML or assembler again)
+          dd = std::move(e.geometry);
+          dd->setUpDevice();
+          e.model.reset();
+        } else {
+          dd = makeCheckerboard(e.params, true);
+        }
         dd->setGraphs(false);  // workers run concurrently: no stream capture
         const double t1 = now();
         dd->factorize();

@@ -42,10 +42,22 @@
 
 namespace lsw {
 
+// The medium, PML and assembler of a parameter block, following
+// Input::readAssemblerHelmholtz2DFD (src/Input.cpp:795-857).
+struct ModelSetup {
+  std::unique_ptr<Function> medium;
+  PMLCubic2D pml;
+  AssemblerHelmholtz2DFD assembler;
+  std::vector<DoubleVector> domain;
+};
+
 // DDCheckerboard from the C parameter block, following
-// Input::readCheckerboardDD / readAssemblerHelmholtz2DFD (src/Input.cpp:470-623,
-// 795-857).  device = false stops after the host geometry.
-std::unique_ptr<DDCheckerboard> makeCheckerboard(const lsw_dd_params& p, bool device);
+// Input::readCheckerboardDD (src/Input.cpp:470-623).  device = false stops
+// after the host geometry.  With `keep`, the medium, PML and assembler are
+// handed to the caller, who keeps them alive for a later setUpDevice();
+// otherwise they end with the call.
+std::unique_ptr<DDCheckerboard> makeCheckerboard(const lsw_dd_params& p, bool device,
+                                                 std::unique_ptr<ModelSetup>* keep = nullptr);
 
 struct ServeOptions {
   double budgetBytes = 250e9;
@@ -130,6 +142,11 @@ class Server {
     double krylov = 0;   // its reserved bytes
     std::unique_ptr<DDCheckerboard> dd;
     std::unique_ptr<GmresSlots> slots;
+    // the host geometry built at registration and what its device set-up
+    // reads (medium, PML, assembler): the first load finishes it instead of
+    // building it again
+    std::unique_ptr<DDCheckerboard> geometry;
+    std::unique_ptr<ModelSetup> model;
     // the ND trees of its cell shapes, built for the memory estimate at
     // registration and held so that every load reuses them
     std::vector<std::shared_ptr<const NdTree>> trees;
