@@ -643,15 +643,17 @@ def north_star_bench(args):
     # then each source on its own), one after the other
     t0 = time.perf_counter()
     pt, pres, pits = 0.0, 0.0, []
-    for p in probs:
+    for p in ([] if args.pack_passes_only else probs):
         t, worst, it = plain_request(L, torch, _problem_params(L, p, L.LSW_PREC_FP64), p["sources"])
         pt += t
         pres = max(pres, worst)
         pits.append(it)
     torch.cuda.empty_cache()
-    plain = dict(solves_per_s=nsrc / pt, seconds=pt, problems=len(probs), solves=nsrc, worst_true_residual=pres,
-                 max_iterations=max(pits), gmres="flexible FP64, restart 30, one source at a time")
-    log("plain", plain, f"({time.perf_counter() - t0:.1f} s)")
+    if not args.pack_passes_only:
+        plain = dict(solves_per_s=nsrc / pt, seconds=pt, problems=len(probs), solves=nsrc,
+                     worst_true_residual=pres, max_iterations=max(pits),
+                     gmres="flexible FP64, restart 30, one source at a time")
+        log("plain", plain, f"({time.perf_counter() - t0:.1f} s)")
     # mixed, packed: W untimed passes, then K timed passes
     for w in range(args.warmup):
         wall, _, _ = pack_pass(L, probs, args.pack_workers, args.serve_restart, args.serve_budget)
@@ -673,6 +675,11 @@ def north_star_bench(args):
                   worker_s={k: round(st[k], 3) for k in ("setup_s", "factor_s", "block_s", "step_s")},
                   workers=args.pack_workers, restart=args.serve_restart, budget_gb=args.serve_budget / 1e9)
     log("packed", packed)
+    if args.pack_passes_only:
+        # (a profiling run of the packed passes alone: not the bench line)
+        print(json.dumps({"workload": f"pack{len(probs)}_seed0", "packed": packed,
+                          "p50_ms": round(1e3 * pct(lat, 50), 1), "p99_ms": round(1e3 * pct(lat, 99), 1)}))
+        return
     # the 1600^2 lens workload: single-problem figures and the solve roofline
     cfg = config_lens_1600()
     out, xs, prof, tune = lens_section(args, L, torch, cfg, args.paths.split(","))
@@ -738,6 +745,8 @@ def main():
                     help="skip timing alternative launch shapes per DD and block width before the timed solves")
     ap.add_argument("--paths", default="plain_ref,plain,mixed,batched")
     ap.add_argument("--pack-workers", type=int, default=12)
+    ap.add_argument("--pack-passes-only", action="store_true",
+                    help="default workload: only the packed passes (for profiling; prints no bench line)")
     ap.add_argument("--serve-models", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-load-requests", type=int, default=32)
