@@ -173,6 +173,7 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
   // inner step, so the loop ends after at most nrhs * mMaxIter steps.
   std::vector<int> iters(nrhs, 0);
   bool first = true;
+  bool fresh = false;  // res holds the true residuals of the final X
   while (true) {
     if (!first) {
       dd->applyADevBlock(X, T, nrhs, s);
@@ -187,7 +188,10 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
       infos[r].estimate = rel;
       if (rel > mTol && iters[r] < mMaxIter) act.push_back(r);
     }
-    if (act.empty()) break;
+    if (act.empty()) {
+      fresh = true;
+      break;
+    }
     const int na = (int)act.size();
     // V_0 = r / ||r|| for the active right-hand sides (compacted)
     for (int a = 0; a < na; ++a)
@@ -302,10 +306,13 @@ void GlobalIterativeSolverGMRESHIP::solveBlockDev(const double2* B, double2* X, 
     for (int r = 0; r < nrhs; ++r) anyLeft |= iters[r] < mMaxIter;
     if (!anyLeft) break;
   }
-  // true FP64 residuals
-  dd->applyADevBlock(X, T, nrhs, s);
-  hipLaunchKernelGGL(k_sub2, gridN(nrhs * n), dim3(256), 0, s, R, B, T, nrhs * n);
-  norms(R, nrhs, res);
+  // true FP64 residuals (computed at the top of the last pass when the loop
+  // ended there)
+  if (!fresh) {
+    dd->applyADevBlock(X, T, nrhs, s);
+    hipLaunchKernelGGL(k_sub2, gridN(nrhs * n), dim3(256), 0, s, R, B, T, nrhs * n);
+    norms(R, nrhs, res);
+  }
   for (int r = 0; r < nrhs; ++r) {
     infos[r].iterations = iters[r];
     infos[r].trueResidual = bn[r] > 0 ? res[r] / bn[r] : res[r];
