@@ -200,7 +200,9 @@ int lsw_gmres_solve_dev(lsw_dd* dd, const lsw_gmres_params* p, const void* b, vo
                         lsw_gmres_result* res);
 /* Flexible GMRES for nrhs right-hand sides sharing the factors (B, X: nrhs
  * consecutive global vectors; res: nrhs results).  Every Arnoldi step applies
- * the preconditioner to the whole block of active right-hand sides.  No
+ * the preconditioner to the block of right-hand sides whose estimate has not
+ * converged yet (a converged one leaves the block: its precond_applies equal
+ * its Arnoldi steps).  No
  * counterpart in the reference, which sums all sources into one right-hand
  * side (src/RHSSrcPts.cpp:93-94). */
 int lsw_gmres_solve_block(lsw_dd* dd, const lsw_gmres_params* p, int nrhs, const double* B, double* X,
